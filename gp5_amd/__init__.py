@@ -1,0 +1,13 @@
+"""gp5_amd: MI355X (gfx950) Gross-Pitaevskii pseudo-spectral core.
+
+Layers:
+  include/gp5.h, gp5_amd/csrc/*.hip   HIP kernels + C ABI (libgp5.so)
+  gp5_amd/csrc/gp5pp.hpp              C++ facade: gp::functional, gp::operators::laplacian,
+                                      gp::stepper, gp::constraint, ... over the C ABI
+  gp5_amd/csrc/gpCpp.cpp              pybind11 module gpCpp (reference module surface)
+  gp5_amd/capi.py                     ctypes view of the C ABI
+  gp5_amd/gp.py                       Python API with the reference's names (gp.geometry, ...)
+"""
+from . import capi  # noqa: F401
+
+__all__ = ["capi"]

@@ -1,0 +1,317 @@
+"""ctypes binding of the C ABI in include/gp5.h (libgp5.so).
+
+This is the thinnest host layer: it exposes the C entry points one to one,
+with numpy arrays in the reference layout (complex128, shape (Nx,Ny,Nz,nC),
+Fortran order; src/traits.h:12-20) on the host side.  The reference-shaped
+API (geometry / field / LHY / stepper, the gpCpp module) sits on top.
+
+Loading fails loudly when libgp5.so has not been built: there is no CPU
+fallback for any operation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgp5.so")
+
+GP5_OK = 0
+GP5_ERR_ARG = 1
+GP5_ERR_HIP = 2
+GP5_ERR_UNSUPPORTED = 3
+GP5_ERR_ALLOC = 4
+GP5_ERR_NONFINITE = 5
+
+GP5_FORWARD, GP5_BACKWARD = 0, 1
+GP5_FUNC_GP, GP5_FUNC_LHY, GP5_FUNC_LHY_UNLOCKED = 0, 1, 2
+GP5_STEP_EULERO, GP5_STEP_RK4 = 0, 1
+
+# every symbol include/gp5.h declares
+EXPORTS = [
+    "gp5_last_error", "gp5_version", "gp5_ctx_create", "gp5_ctx_destroy", "gp5_ctx_sync",
+    "gp5_ctx_points", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
+    "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
+    "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
+    "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed",
+]
+
+
+class GP5Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class FunctionalDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("has_couplings", ctypes.c_int),
+        ("couplings", ctypes.c_double * 4), ("masses", ctypes.c_double * 2),
+        ("lhy_a", ctypes.c_double), ("lhy_b", ctypes.c_double),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("eta", ctypes.c_double),
+        ("potential", ctypes.c_void_p),
+    ]
+
+
+class StepperDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("tau_re", ctypes.c_double), ("tau_im", ctypes.c_double),
+        ("renormalize", ctypes.c_int), ("normalizations", ctypes.c_double * 2),
+        ("phase", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libgp5.so; raises if it is missing (build it with python -m gp5_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m gp5_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_size_t
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    DP = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "gp5_last_error": ([], ctypes.c_char_p),
+        "gp5_version": ([], I),
+        "gp5_ctx_create": ([PP, ctypes.POINTER(ctypes.c_int), DP, DP, I], I),
+        "gp5_ctx_destroy": ([P], I),
+        "gp5_ctx_sync": ([P], I),
+        "gp5_ctx_points": ([P], S),
+        "gp5_buf_alloc": ([P, S, PP], I),
+        "gp5_buf_free": ([P], I),
+        "gp5_buf_size": ([P], S),
+        "gp5_buf_upload": ([P, P, S], I),
+        "gp5_buf_download": ([P, P, S], I),
+        "gp5_buf_copy": ([P, P], I),
+        "gp5_buf_zero": ([P], I),
+        "gp5_fft": ([P, P, P, I], I),
+        "gp5_laplacian": ([P, P, P], I),
+        "gp5_functional_apply": ([P, ctypes.POINTER(FunctionalDesc), P, P, D], I),
+        "gp5_norm": ([P, P, I, DP], I),
+        "gp5_normalize": ([P, P, I, D], I),
+        "gp5_max_density": ([P, P, I, DP], I),
+        "gp5_energy": ([P, ctypes.POINTER(FunctionalDesc), P, DP], I),
+        "gp5_phase_constraint": ([P, P, P], I),
+        "gp5_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, P, P], I),
+        "gp5_advance_timed": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != GP5_OK:
+        msg = load().gp5_last_error().decode(errors="replace")
+        raise GP5Error(rc, msg)
+
+
+class Context:
+    """gp5_ctx: grid, FFT plans, stream and scratch of one field shape."""
+
+    def __init__(self, shape: Sequence[int], left: Sequence[float], right: Sequence[float], nc: int = 1):
+        L = load()
+        self.shape = tuple(int(s) for s in shape)
+        self.left = tuple(float(x) for x in left)
+        self.right = tuple(float(x) for x in right)
+        self.nc = int(nc)
+        h = ctypes.c_void_p()
+        sh = (ctypes.c_int * 3)(*self.shape)
+        lf = (ctypes.c_double * 3)(*self.left)
+        rt = (ctypes.c_double * 3)(*self.right)
+        check(L.gp5_ctx_create(ctypes.byref(h), sh, lf, rt, self.nc))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def M(self) -> int:
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            check(load().gp5_ctx_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                load().gp5_ctx_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def field(self, data: Optional[np.ndarray] = None) -> "Buffer":
+        b = Buffer(self, 2 * self.M * self.nc)
+        if data is not None:
+            b.upload_complex(data)
+        return b
+
+    def real(self, data: Optional[np.ndarray] = None) -> "Buffer":
+        b = Buffer(self, self.M * self.nc)
+        if data is not None:
+            b.upload_real(data)
+        return b
+
+    def sync(self) -> None:
+        check(load().gp5_ctx_sync(self._h))
+
+
+class Buffer:
+    """gp5_buf: device array of doubles owned by a context."""
+
+    def __init__(self, ctx: Context, n_doubles: int):
+        self.ctx = ctx
+        self.n = int(n_doubles)
+        h = ctypes.c_void_p()
+        check(load().gp5_buf_alloc(ctx.handle, self.n, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            check(load().gp5_buf_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                load().gp5_buf_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def _shape(self):
+        s = self.ctx.shape
+        return (s[0], s[1], s[2], self.ctx.nc)
+
+    def upload_complex(self, a: np.ndarray) -> None:
+        a = np.asarray(a)
+        if a.ndim == 3 and self.ctx.nc == 1:
+            a = a[..., None]
+        if a.shape != self._shape():
+            raise GP5Error(GP5_ERR_ARG, f"array of shape {a.shape} for a field of shape {self._shape()}")
+        flat = np.ascontiguousarray(np.asarray(a, dtype=np.complex128).ravel(order="F"))
+        check(load().gp5_buf_upload(self._h, flat.ctypes.data_as(ctypes.c_void_p), self.n))
+
+    def upload_real(self, a: np.ndarray) -> None:
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim == 3 and self.ctx.nc == 1:
+            a = a[..., None]
+        if a.shape != self._shape():
+            raise GP5Error(GP5_ERR_ARG, f"array of shape {a.shape} for a real field of shape {self._shape()}")
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        check(load().gp5_buf_upload(self._h, flat.ctypes.data_as(ctypes.c_void_p), self.n))
+
+    def download_complex(self) -> np.ndarray:
+        flat = np.empty(self.n // 2, dtype=np.complex128)
+        check(load().gp5_buf_download(self._h, flat.ctypes.data_as(ctypes.c_void_p), self.n))
+        return flat.reshape(self._shape(), order="F")
+
+
+def functional_desc(kind: int = GP5_FUNC_LHY, *, couplings=None, masses=(1.0, 1.0), lhy_a: float = -3.0,
+                    lhy_b: float = 2.5, alpha: float = 0.0, beta: float = 0.0, eta: float = 0.0,
+                    potential: Optional[Buffer] = None) -> FunctionalDesc:
+    f = FunctionalDesc()
+    f.kind = kind
+    f.has_couplings = 0 if couplings is None else 1
+    if couplings is not None:
+        g = np.asarray(couplings, dtype=float)
+        for i in range(g.shape[0]):
+            for j in range(g.shape[1]):
+                f.couplings[i + 2 * j] = g[i, j]
+    f.masses[0] = masses[0]
+    f.masses[1] = masses[1] if len(masses) > 1 else 1.0
+    f.lhy_a, f.lhy_b = lhy_a, lhy_b
+    f.alpha, f.beta, f.eta = alpha, beta, eta
+    f.potential = potential.handle.value if potential is not None else None
+    return f
+
+
+def stepper_desc(kind: int = GP5_STEP_RK4, *, dt: float, imaginary: bool, renormalize=None,
+                 phase: Optional[Buffer] = None) -> StepperDesc:
+    s = StepperDesc()
+    s.kind = kind
+    s.tau_re, s.tau_im = (dt, 0.0) if imaginary else (0.0, dt)
+    s.renormalize = 0 if renormalize is None else 1
+    if renormalize is not None:
+        for c, N in enumerate(renormalize):
+            s.normalizations[c] = float(N)
+    s.phase = phase.handle.value if phase is not None else None
+    return s
+
+
+def fft(ctx: Context, src: Buffer, dst: Buffer, direction: int) -> None:
+    check(load().gp5_fft(ctx.handle, src.handle, dst.handle, int(direction)))
+
+
+def laplacian(ctx: Context, src: Buffer, dst: Buffer) -> None:
+    check(load().gp5_laplacian(ctx.handle, src.handle, dst.handle))
+
+
+def functional_apply(ctx: Context, f: FunctionalDesc, src: Buffer, dst: Buffer, time: float = 0.0) -> None:
+    check(load().gp5_functional_apply(ctx.handle, ctypes.byref(f), src.handle, dst.handle, float(time)))
+
+
+def norm(ctx: Context, f: Buffer, c: int = 0) -> float:
+    out = ctypes.c_double()
+    check(load().gp5_norm(ctx.handle, f.handle, int(c), ctypes.byref(out)))
+    return out.value
+
+
+def normalize(ctx: Context, f: Buffer, c: int, N: float) -> None:
+    check(load().gp5_normalize(ctx.handle, f.handle, int(c), float(N)))
+
+
+def max_density(ctx: Context, f: Buffer, c: int = 0) -> float:
+    out = ctypes.c_double()
+    check(load().gp5_max_density(ctx.handle, f.handle, int(c), ctypes.byref(out)))
+    return out.value
+
+
+def energy(ctx: Context, f: FunctionalDesc, psi: Buffer) -> float:
+    out = ctypes.c_double()
+    check(load().gp5_energy(ctx.handle, ctypes.byref(f), psi.handle, ctypes.byref(out)))
+    return out.value
+
+
+def phase_constraint(ctx: Context, psi: Buffer, phase: Buffer) -> None:
+    check(load().gp5_phase_constraint(ctx.handle, psi.handle, phase.handle))
+
+
+def advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int, *,
+            log: bool = False):
+    """Advance psi in place; with log=True returns (norms[nsteps, nC], energies[nsteps])."""
+    L = load()
+    if log:
+        norms = np.zeros((int(nsteps), ctx.nc), dtype=np.float64)
+        ens = np.zeros(int(nsteps), dtype=np.float64)
+        check(L.gp5_advance(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps),
+                            norms.ctypes.data_as(ctypes.c_void_p), ens.ctypes.data_as(ctypes.c_void_p)))
+        return norms, ens
+    check(L.gp5_advance(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps), None, None))
+    return None
+
+
+def advance_timed(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int) -> float:
+    """Run nsteps and return the device time between HIP events on the context stream (ms)."""
+    ms = ctypes.c_double()
+    check(load().gp5_advance_timed(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps),
+                                   ctypes.byref(ms)))
+    return ms.value

@@ -1,0 +1,715 @@
+// C ABI of the gfx950 GP core (declared in include/gp5.h).
+//
+// A context owns the grid, the per-axis Stockham plans and twiddle tables,
+// one HIP stream and the scratch fields of the fused passes.  Every spectral
+// operation is five axis passes over a column-major complex128 field:
+//   x forward, y forward, z forward * k-space multiplier * z inverse,
+//   y inverse, x inverse + epilogue
+// where the epilogue is a store (laplacian), the functional H(phi), or one
+// Euler/RK4 stage update with fused norm/energy reductions.
+#include "../../include/gp5.h"
+#include "gp5_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace gp5 {
+hipError_t launch_axis(hipStream_t st, int mode, const double2 *src, double2 *dst, const AxisPlan &P,
+                       const AxisGeom &G, int sign, const double *scale_in, long long Opc, const MidArgs &mid,
+                       const EpiArgs &epi);
+unsigned axis_grid(const AxisGeom &G);
+hipError_t launch_norm(hipStream_t st, const double2 *f, long long len, unsigned blocks, const EpiArgs &E);
+hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp);
+hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax);
+hipError_t launch_phase(hipStream_t st, double2 *f, const double *phase, long long len);
+}  // namespace gp5
+
+using namespace gp5;
+
+struct gp5_buf {
+    gp5_ctx *ctx;
+    double *d;
+    size_t n;  // doubles
+};
+
+struct gp5_ctx {
+    int n[3];
+    double left[3], right[3];
+    int nc;
+    long long M;
+    double dV;
+    double dk[3];
+    int device;
+    hipStream_t st;
+    AxisPlan plan[3];
+    AxisGeom geom[3];
+    long long Opc[3];
+    double2 *tw[3];
+    // scratch fields (allocated on first use)
+    double2 *T, *phiA, *phiB, *ksum;
+    // reduction scratch
+    double *partials;
+    size_t partials_cap;  // blocks
+    unsigned int *counter;
+    double *results;      // kRedQ
+    double *scales;       // 8 slots x 2 components
+    double *blockmax;
+    double *dlog;
+    size_t dlog_cap;      // steps
+    hipEvent_t ev0, ev1;
+};
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(GP5_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+#define RCHK(expr)               \
+    do {                         \
+        int r_ = (expr);         \
+        if (r_ != GP5_OK) return r_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+static int make_plan(int n, AxisPlan &P)
+{
+    P.n = n;
+    P.nrad = 0;
+    int m = n;
+    auto push = [&](int r) -> bool {
+        if (P.nrad >= kMaxRadices) return false;
+        P.rad[P.nrad++] = r;
+        return true;
+    };
+    while (m % 8 == 0) { if (!push(8)) return -1; m /= 8; }
+    while (m % 4 == 0) { if (!push(4)) return -1; m /= 4; }
+    while (m % 2 == 0) { if (!push(2)) return -1; m /= 2; }
+    while (m % 3 == 0) { if (!push(3)) return -1; m /= 3; }
+    while (m % 5 == 0) { if (!push(5)) return -1; m /= 5; }
+    for (int r = 7; m > 1; r += 2) {
+        while (m % r == 0) { if (!push(r)) return -1; m /= r; }
+        if ((long long)r * r > m && m > 1) { if (!push(m)) return -1; m = 1; }
+    }
+    return 0;
+}
+
+static std::vector<double2> twiddles(int n)
+{
+    // e^{-2 pi i t / n} from the exact integer phase, evaluated in long double
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    std::vector<double2> tw(n);
+    for (int t = 0; t < n; ++t) {
+        long double a = -two_pi * (long double)t / (long double)n;
+        tw[t] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    return tw;
+}
+
+static int ensure_partials(gp5_ctx *c, size_t blocks)
+{
+    if (blocks <= c->partials_cap) return GP5_OK;
+    if (c->partials) HIPCHK(hipFree(c->partials));
+    c->partials = nullptr;
+    size_t cap = std::max<size_t>(blocks, 4096);
+    HIPCHK(hipMalloc(&c->partials, cap * kRedQ * sizeof(double)));
+    c->partials_cap = cap;
+    return GP5_OK;
+}
+
+static int ensure_scratch(gp5_ctx *c, bool rk)
+{
+    const size_t bytes = (size_t)c->M * c->nc * sizeof(double2);
+    if (!c->T) {
+        if (hipMalloc(&c->T, bytes) != hipSuccess) { c->T = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
+    }
+    if (!c->phiA) {
+        if (hipMalloc(&c->phiA, bytes) != hipSuccess) { c->phiA = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
+    }
+    if (rk) {
+        if (!c->phiB) {
+            if (hipMalloc(&c->phiB, bytes) != hipSuccess) { c->phiB = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
+        }
+        if (!c->ksum) {
+            if (hipMalloc(&c->ksum, bytes) != hipSuccess) { c->ksum = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
+        }
+    }
+    return GP5_OK;
+}
+
+static int check_field(const gp5_ctx *c, const gp5_buf *b, const char *what)
+{
+    if (!b) return fail(GP5_ERR_ARG, "%s is NULL", what);
+    if (b->ctx != c) return fail(GP5_ERR_ARG, "%s belongs to another context", what);
+    if (b->n != (size_t)c->M * c->nc * 2)
+        return fail(GP5_ERR_ARG, "%s holds %zu doubles, the field needs %lld", what, b->n, c->M * c->nc * 2);
+    return GP5_OK;
+}
+
+static int check_real(const gp5_ctx *c, const gp5_buf *b, const char *what)
+{
+    if (b->ctx != c) return fail(GP5_ERR_ARG, "%s belongs to another context", what);
+    if (b->n != (size_t)c->M * c->nc)
+        return fail(GP5_ERR_ARG, "%s holds %zu doubles, a real field needs %lld", what, b->n, c->M * c->nc);
+    return GP5_OK;
+}
+
+static EpiArgs base_epi(gp5_ctx *c)
+{
+    EpiArgs E;
+    memset(&E, 0, sizeof E);
+    E.M = c->M;
+    E.nc = c->nc;
+    E.nx = c->n[0];
+    E.dV = c->dV;
+    E.partials = c->partials;
+    E.counter = c->counter;
+    E.results = c->results;
+    return E;
+}
+
+static int fill_functional(gp5_ctx *c, const gp5_functional_desc *f, EpiArgs &E, double coef[2])
+{
+    if (!f) return fail(GP5_ERR_ARG, "functional descriptor is NULL");
+    E.fkind = f->kind;
+    if (f->kind == GP5_FUNC_GP) {
+        E.has_couplings = f->has_couplings;
+        if (f->has_couplings && c->nc > 2) return fail(GP5_ERR_UNSUPPORTED, "couplings support 1 or 2 components");
+        E.g11 = f->couplings[0];
+        E.g12 = f->couplings[2];
+        E.g22 = f->couplings[3];
+        for (int k = 0; k < 2; ++k) {
+            double m = k < c->nc ? f->masses[k] : 1.0;
+            if (m == 0.0) m = 1.0;
+            coef[k] = 0.5 * (1.0 / m) / (double)c->M;
+        }
+    } else if (f->kind == GP5_FUNC_LHY) {
+        E.lhy_a = f->lhy_a;
+        E.lhy_b = f->lhy_b;
+        coef[0] = coef[1] = 0.5 / (double)c->M;
+    } else if (f->kind == GP5_FUNC_LHY_UNLOCKED) {
+        if (c->nc != 2) return fail(GP5_ERR_ARG, "Incompatible number of components");
+        E.alpha = f->alpha;
+        E.beta = f->beta;
+        E.eta = f->eta;
+        coef[0] = coef[1] = 0.5 / (double)c->M;
+    } else {
+        return fail(GP5_ERR_ARG, " Unkown functional");
+    }
+    if (f->potential) {
+        RCHK(check_real(c, f->potential, "potential"));
+        E.V = f->potential->d;
+    }
+    return GP5_OK;
+}
+
+static MidArgs make_mid(gp5_ctx *c, const double coef[2], const double *scale)
+{
+    MidArgs m;
+    m.nx = c->n[0];
+    m.ny = c->n[1];
+    m.nz = c->n[2];
+    for (int d = 0; d < 3; ++d) m.dk[d] = c->dk[d];
+    m.coef[0] = coef[0];
+    m.coef[1] = coef[1];
+    m.M = c->M;
+    m.scale = scale;
+    return m;
+}
+
+// Five passes: src (scaled by scale_src) -> T -> ... -> epilogue.
+static int spectral(gp5_ctx *c, const double2 *src, const double *scale_src, const double coef[2], const EpiArgs &E)
+{
+    MidArgs mid = make_mid(c, coef, nullptr);
+    EpiArgs none;
+    memset(&none, 0, sizeof none);
+    HIPCHK(launch_axis(c->st, 0, src, c->T, c->plan[0], c->geom[0], -1, scale_src, c->Opc[0], mid, none));
+    HIPCHK(launch_axis(c->st, 0, c->T, c->T, c->plan[1], c->geom[1], -1, nullptr, c->Opc[1], mid, none));
+    HIPCHK(launch_axis(c->st, 1, c->T, c->T, c->plan[2], c->geom[2], -1, nullptr, c->Opc[2], mid, none));
+    HIPCHK(launch_axis(c->st, 0, c->T, c->T, c->plan[1], c->geom[1], +1, nullptr, c->Opc[1], mid, none));
+    RCHK(ensure_partials(c, axis_grid(c->geom[0])));
+    EpiArgs e = E;
+    e.partials = c->partials;
+    HIPCHK(launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr, c->Opc[0], mid, e));
+    return GP5_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *gp5_last_error(void) { return g_err.c_str(); }
+
+int gp5_version(void) { return 1; }
+
+int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], const double right[3], int nc)
+{
+    if (!out || !shape || !left || !right) return fail(GP5_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (nc < 1 || nc > 2) return fail(GP5_ERR_UNSUPPORTED, "nComponents must be 1 or 2 (got %d)", nc);
+    for (int d = 0; d < 3; ++d) {
+        if (shape[d] < 1) return fail(GP5_ERR_ARG, "shape[%d] = %d", d, shape[d]);
+        if (shape[d] > kMaxLen)
+            return fail(GP5_ERR_UNSUPPORTED, "axis %d has %d points; the LDS FFT supports up to %d", d, shape[d], kMaxLen);
+        if (!(right[d] > left[d])) return fail(GP5_ERR_ARG, "domain axis %d is empty", d);
+    }
+    gp5_ctx *c = new gp5_ctx;
+    memset(c, 0, sizeof *c);
+    for (int d = 0; d < 3; ++d) {
+        c->n[d] = shape[d];
+        c->left[d] = left[d];
+        c->right[d] = right[d];
+        c->dk[d] = 2 * M_PI / (right[d] - left[d]);  // discretization::getFourierStep
+    }
+    c->nc = nc;
+    c->M = (long long)shape[0] * shape[1] * shape[2];
+    c->dV = 1.0;
+    for (int d = 0; d < 3; ++d) c->dV *= (right[d] - left[d]) / shape[d];  // getSpaceStep
+    int rc = GP5_OK;
+    auto bail = [&](int code) { gp5_ctx_destroy(c); return code; };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(GP5_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(GP5_ERR_HIP, "hipStreamCreate failed"));
+    long long I = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (make_plan(shape[d], c->plan[d])) return bail(fail(GP5_ERR_UNSUPPORTED, "too many radices for %d", shape[d]));
+        std::vector<double2> tw = twiddles(shape[d]);
+        if (hipMalloc(&c->tw[d], sizeof(double2) * shape[d]) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "twiddles"));
+        if (hipMemcpy(c->tw[d], tw.data(), sizeof(double2) * shape[d], hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(GP5_ERR_HIP, "twiddle upload"));
+        c->plan[d].tw = c->tw[d];
+        AxisGeom &G = c->geom[d];
+        G.I = I;
+        G.O = c->M / (I * shape[d]) * nc;
+        c->Opc[d] = c->M / (I * shape[d]);
+        int B = std::max(1, kTileElems / shape[d]);
+        if (I == 1) B = (int)std::min<long long>(B, G.O);
+        else B = (int)std::min<long long>(std::min(B, 64), I);
+        G.B = std::max(1, B);
+        I *= shape[d];
+    }
+    if (hipMalloc(&c->counter, sizeof(unsigned int)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "counter"));
+    if (hipMemset(c->counter, 0, sizeof(unsigned int)) != hipSuccess) return bail(fail(GP5_ERR_HIP, "memset"));
+    if (hipMalloc(&c->results, kRedQ * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "results"));
+    if (hipMalloc(&c->scales, 16 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "scales"));
+    if (hipMalloc(&c->blockmax, 8192 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "blockmax"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(GP5_ERR_HIP, "events"));
+    rc = ensure_partials(c, 4096);
+    if (rc != GP5_OK) return bail(rc);
+    *out = c;
+    return GP5_OK;
+}
+
+int gp5_ctx_destroy(gp5_ctx *c)
+{
+    if (!c) return GP5_OK;
+    // teardown continues past individual failures; the first one is reported
+    hipError_t first = hipSuccess;
+    auto note = [&](hipError_t e) { if (first == hipSuccess) first = e; };
+    if (c->st) note(hipStreamSynchronize(c->st));
+    for (int d = 0; d < 3; ++d) if (c->tw[d]) note(hipFree(c->tw[d]));
+    void *ptrs[] = {c->T, c->phiA, c->phiB, c->ksum, c->partials, c->counter, c->results, c->scales, c->blockmax, c->dlog};
+    for (void *p : ptrs) if (p) note(hipFree(p));
+    if (c->ev0) note(hipEventDestroy(c->ev0));
+    if (c->ev1) note(hipEventDestroy(c->ev1));
+    if (c->st) note(hipStreamDestroy(c->st));
+    delete c;
+    if (first != hipSuccess) return fail(GP5_ERR_HIP, "context teardown: %s", hipGetErrorString(first));
+    return GP5_OK;
+}
+
+int gp5_ctx_sync(gp5_ctx *c)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+size_t gp5_ctx_points(const gp5_ctx *c) { return c ? (size_t)c->M : 0; }
+
+int gp5_buf_alloc(gp5_ctx *c, size_t n, gp5_buf **out)
+{
+    if (!c || !out) return fail(GP5_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    gp5_buf *b = new gp5_buf{c, nullptr, n};
+    if (n) {
+        if (hipMalloc(&b->d, n * sizeof(double)) != hipSuccess) {
+            delete b;
+            return fail(GP5_ERR_ALLOC, "cannot allocate %zu B on the device", n * sizeof(double));
+        }
+        if (hipMemsetAsync(b->d, 0, n * sizeof(double), c->st) != hipSuccess ||
+            hipStreamSynchronize(c->st) != hipSuccess) {
+            (void)hipFree(b->d);
+            delete b;
+            return fail(GP5_ERR_HIP, "memset of a new buffer failed");
+        }
+    }
+    *out = b;
+    return GP5_OK;
+}
+
+int gp5_buf_free(gp5_buf *b)
+{
+    if (!b) return GP5_OK;
+    hipError_t e = hipSuccess;
+    if (b->d) {
+        e = hipStreamSynchronize(b->ctx->st);
+        hipError_t e2 = hipFree(b->d);
+        if (e == hipSuccess) e = e2;
+    }
+    delete b;
+    if (e != hipSuccess) return fail(GP5_ERR_HIP, "buffer free: %s", hipGetErrorString(e));
+    return GP5_OK;
+}
+
+size_t gp5_buf_size(const gp5_buf *b) { return b ? b->n : 0; }
+
+int gp5_buf_upload(gp5_buf *b, const double *host, size_t n)
+{
+    if (!b || (!host && n)) return fail(GP5_ERR_ARG, "NULL argument");
+    if (n != b->n) return fail(GP5_ERR_ARG, "upload of %zu doubles into a buffer of %zu", n, b->n);
+    HIPCHK(hipMemcpyAsync(b->d, host, n * sizeof(double), hipMemcpyHostToDevice, b->ctx->st));
+    HIPCHK(hipStreamSynchronize(b->ctx->st));
+    return GP5_OK;
+}
+
+int gp5_buf_download(const gp5_buf *b, double *host, size_t n)
+{
+    if (!b || (!host && n)) return fail(GP5_ERR_ARG, "NULL argument");
+    if (n != b->n) return fail(GP5_ERR_ARG, "download of %zu doubles from a buffer of %zu", n, b->n);
+    HIPCHK(hipMemcpyAsync(host, b->d, n * sizeof(double), hipMemcpyDeviceToHost, b->ctx->st));
+    HIPCHK(hipStreamSynchronize(b->ctx->st));
+    return GP5_OK;
+}
+
+int gp5_buf_copy(gp5_buf *dst, const gp5_buf *src)
+{
+    if (!dst || !src) return fail(GP5_ERR_ARG, "NULL argument");
+    if (dst->n != src->n) return fail(GP5_ERR_ARG, "copy between buffers of %zu and %zu doubles", src->n, dst->n);
+    if (dst == src) return GP5_OK;
+    HIPCHK(hipMemcpyAsync(dst->d, src->d, src->n * sizeof(double), hipMemcpyDeviceToDevice, dst->ctx->st));
+    HIPCHK(hipStreamSynchronize(dst->ctx->st));
+    return GP5_OK;
+}
+
+int gp5_buf_zero(gp5_buf *b)
+{
+    if (!b) return fail(GP5_ERR_ARG, "NULL argument");
+    HIPCHK(hipMemsetAsync(b->d, 0, b->n * sizeof(double), b->ctx->st));
+    HIPCHK(hipStreamSynchronize(b->ctx->st));
+    return GP5_OK;
+}
+
+int gp5_fft(gp5_ctx *c, const gp5_buf *src, gp5_buf *dst, int direction)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, src, "src"));
+    RCHK(check_field(c, dst, "dst"));
+    int sign;
+    if (direction == GP5_FORWARD) sign = -1;
+    else if (direction == GP5_BACKWARD) sign = +1;
+    else return fail(GP5_ERR_ARG, "Unkown direction in fourier transform");
+    MidArgs mid;
+    memset(&mid, 0, sizeof mid);
+    EpiArgs none;
+    memset(&none, 0, sizeof none);
+    double2 *d = (double2 *)dst->d;
+    HIPCHK(launch_axis(c->st, 0, (const double2 *)src->d, d, c->plan[0], c->geom[0], sign, nullptr, c->Opc[0], mid, none));
+    HIPCHK(launch_axis(c->st, 0, d, d, c->plan[1], c->geom[1], sign, nullptr, c->Opc[1], mid, none));
+    HIPCHK(launch_axis(c->st, 0, d, d, c->plan[2], c->geom[2], sign, nullptr, c->Opc[2], mid, none));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+int gp5_laplacian(gp5_ctx *c, const gp5_buf *src, gp5_buf *dst)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, src, "src"));
+    RCHK(check_field(c, dst, "dst"));
+    RCHK(ensure_scratch(c, false));
+    EpiArgs E = base_epi(c);
+    E.mode = EPI_STORE;
+    E.out = (double2 *)dst->d;
+    double coef[2] = {-1.0 / (double)c->M, -1.0 / (double)c->M};
+    RCHK(spectral(c, (const double2 *)src->d, nullptr, coef, E));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+int gp5_functional_apply(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf *src, gp5_buf *dst, double time)
+{
+    (void)time;
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, src, "src"));
+    RCHK(check_field(c, dst, "dst"));
+    if (src == dst || src->d == dst->d) return fail(GP5_ERR_ARG, "functional::apply needs distinct source and destination");
+    RCHK(ensure_scratch(c, false));
+    EpiArgs E = base_epi(c);
+    double coef[2];
+    RCHK(fill_functional(c, f, E, coef));
+    E.mode = EPI_H;
+    E.phi = (const double2 *)src->d;
+    E.out = (double2 *)dst->d;
+    RCHK(spectral(c, (const double2 *)src->d, nullptr, coef, E));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+static int norm_impl(gp5_ctx *c, const double2 *f, int comp, double N, double *out, double *scale_out)
+{
+    const long long M = c->M;
+    unsigned blocks = (unsigned)std::min<long long>((M + kThreads - 1) / kThreads, 2048);
+    RCHK(ensure_partials(c, blocks));
+    EpiArgs E = base_epi(c);
+    E.partials = c->partials;
+    E.nc = 1;
+    E.Nnorm[0] = N;
+    E.scale_out = scale_out;
+    HIPCHK(launch_norm(c->st, f + M * comp, M, blocks, E));
+    if (out) {
+        double r[kRedQ];
+        HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        *out = r[0];
+    }
+    return GP5_OK;
+}
+
+int gp5_norm(gp5_ctx *c, const gp5_buf *f, int comp, double *out)
+{
+    if (!c || !out) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, f, "field"));
+    if (comp < 0 || comp >= c->nc) return fail(GP5_ERR_ARG, "component %d out of range", comp);
+    return norm_impl(c, (const double2 *)f->d, comp, 1.0, out, nullptr);
+}
+
+int gp5_normalize(gp5_ctx *c, gp5_buf *f, int comp, double N)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, f, "field"));
+    if (comp < 0 || comp >= c->nc) return fail(GP5_ERR_ARG, "component %d out of range", comp);
+    double nrm = 0.0;
+    RCHK(norm_impl(c, (const double2 *)f->d, comp, N, &nrm, c->scales + 14));
+    if (!std::isfinite(nrm) || nrm <= 0.0) return fail(GP5_ERR_NONFINITE, "norm of component %d is %g", comp, nrm);
+    HIPCHK(launch_scale(c->st, (double2 *)f->d, c->M, c->nc, comp, 0.0, c->scales + 14));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+int gp5_max_density(gp5_ctx *c, const gp5_buf *f, int comp, double *out)
+{
+    if (!c || !out) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, f, "field"));
+    if (comp < 0 || comp >= c->nc) return fail(GP5_ERR_ARG, "component %d out of range", comp);
+    unsigned blocks = (unsigned)std::min<long long>((c->M + kThreads - 1) / kThreads, 8192);
+    HIPCHK(launch_maxdens(c->st, (const double2 *)f->d + c->M * comp, c->M, blocks, c->blockmax));
+    std::vector<double> bm(blocks);
+    HIPCHK(hipMemcpyAsync(bm.data(), c->blockmax, blocks * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double m = 0.0;
+    for (double v : bm) m = std::max(m, v);
+    *out = m;
+    return GP5_OK;
+}
+
+int gp5_energy(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf *psi, double *out)
+{
+    if (!c || !out) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, psi, "psi"));
+    if (f && f->kind == GP5_FUNC_LHY_UNLOCKED)
+        return fail(GP5_ERR_UNSUPPORTED, "the unlocked LHY functional has no energy for general alpha, beta, eta");
+    RCHK(ensure_scratch(c, false));
+    EpiArgs E = base_epi(c);
+    double coef[2];
+    RCHK(fill_functional(c, f, E, coef));
+    E.mode = EPI_ENERGY;
+    E.phi = (const double2 *)psi->d;
+    E.want_red = 1;
+    RCHK(spectral(c, (const double2 *)psi->d, nullptr, coef, E));
+    double r[kRedQ];
+    HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *out = r[2] + (c->nc > 1 ? r[3] : 0.0);
+    return GP5_OK;
+}
+
+int gp5_phase_constraint(gp5_ctx *c, gp5_buf *psi, const gp5_buf *phase)
+{
+    if (!c || !phase) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, psi, "psi"));
+    RCHK(check_real(c, phase, "phase"));
+    HIPCHK(launch_phase(c->st, (double2 *)psi->d, phase->d, c->M * c->nc));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+// One step of Euler or RK4 on c's stream.  psi_slot: scale slot of psi (-1 = none).
+static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                     int &psi_slot, int step, double *dlog)
+{
+    EpiArgs E = base_epi(c);
+    double coef[2];
+    RCHK(fill_functional(c, f, E, coef));
+    E.want_red = 1;
+    E.Nnorm[0] = s->normalizations[0];
+    E.Nnorm[1] = s->normalizations[1];
+    if (s->phase) {
+        RCHK(check_real(c, s->phase, "phase"));
+        E.phase = s->phase->d;
+    }
+    const bool rn = s->renormalize != 0;
+    double *slot_psi = psi_slot >= 0 ? c->scales + 2 * psi_slot : nullptr;
+    const int next_psi_slot = psi_slot == 0 ? 1 : 0;
+    double *slotA = c->scales + 4, *slotB = c->scales + 6;
+    double2 *P = (double2 *)psi->d;
+    const double tr = s->tau_re, ti = s->tau_im;
+    if (s->kind == GP5_STEP_EULERO) {
+        E.mode = EPI_EULER;
+        E.phi = P;
+        E.sc_phi = slot_psi;
+        E.cr = tr;
+        E.ci = ti;
+        E.out = c->phiA;
+        E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
+        E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
+        RCHK(spectral(c, P, slot_psi, coef, E));
+        // the new state lives in phiA: swap device pointers with the caller's buffer
+        double2 *fresh = c->phiA;
+        c->phiA = (double2 *)psi->d;
+        psi->d = (double *)fresh;
+        psi_slot = rn ? next_psi_slot : -1;
+        return GP5_OK;
+    }
+    // RK4 (src/stepper.cpp:46-70)
+    E.mode = EPI_RK1;
+    E.phi = P;
+    E.sc_phi = slot_psi;
+    E.cr = tr * 0.5;
+    E.ci = ti * 0.5;
+    E.ksum = c->ksum;
+    E.out = c->phiA;
+    E.scale_out = rn ? slotA : nullptr;
+    E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
+    RCHK(spectral(c, P, slot_psi, coef, E));
+    E.log = nullptr;
+
+    E.mode = EPI_RK2;
+    E.phi = c->phiA;
+    E.sc_phi = rn ? slotA : nullptr;
+    E.psi = P;
+    E.sc_psi = slot_psi;
+    E.out = c->phiB;
+    E.scale_out = rn ? slotB : nullptr;
+    RCHK(spectral(c, c->phiA, E.sc_phi, coef, E));
+
+    E.mode = EPI_RK3;
+    E.phi = c->phiB;
+    E.sc_phi = rn ? slotB : nullptr;
+    E.cr = tr;
+    E.ci = ti;
+    E.out = c->phiA;
+    E.scale_out = rn ? slotA : nullptr;
+    RCHK(spectral(c, c->phiB, E.sc_phi, coef, E));
+
+    E.mode = EPI_RK4;
+    E.phi = c->phiA;
+    E.sc_phi = rn ? slotA : nullptr;
+    E.cr = (1. / 6) * tr;
+    E.ci = (1. / 6) * ti;
+    E.out = P;
+    E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
+    RCHK(spectral(c, c->phiA, E.sc_phi, coef, E));
+    psi_slot = rn ? next_psi_slot : -1;
+    return GP5_OK;
+}
+
+static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                        int nsteps, double *norm_log, double *energy_log, double *elapsed_ms)
+{
+    if (!c || !f || !s) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, psi, "psi"));
+    if (s->kind != GP5_STEP_EULERO && s->kind != GP5_STEP_RK4) return fail(GP5_ERR_ARG, "unknown stepper kind %d", s->kind);
+    if (nsteps < 0) return fail(GP5_ERR_ARG, "nsteps < 0");
+    if (s->renormalize) {
+        for (int k = 0; k < c->nc; ++k)
+            if (!(s->normalizations[k] > 0.0)) return fail(GP5_ERR_ARG, "normalization %d must be > 0", k);
+    }
+    if (nsteps == 0) return GP5_OK;
+    RCHK(ensure_scratch(c, s->kind == GP5_STEP_RK4));
+    RCHK(ensure_partials(c, axis_grid(c->geom[0])));
+    const bool logging = norm_log || energy_log;
+    if (logging && (size_t)nsteps > c->dlog_cap) {
+        if (c->dlog) HIPCHK(hipFree(c->dlog));
+        c->dlog = nullptr;
+        HIPCHK(hipMalloc(&c->dlog, (size_t)nsteps * kRedQ * sizeof(double)));
+        c->dlog_cap = nsteps;
+    }
+    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev0, c->st));
+    int psi_slot = -1;
+    for (int st = 0; st < nsteps; ++st) RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr));
+    if (psi_slot >= 0) HIPCHK(launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, c->scales + 2 * psi_slot));
+    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev1, c->st));
+    double r[kRedQ];
+    HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+    std::vector<double> lg;
+    if (logging) {
+        lg.resize((size_t)nsteps * kRedQ);
+        HIPCHK(hipMemcpyAsync(lg.data(), c->dlog, lg.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (elapsed_ms) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *elapsed_ms = ms;
+    }
+    for (int st = 0; logging && st < nsteps; ++st) {
+        const double *q = &lg[(size_t)st * kRedQ];
+        if (norm_log)
+            for (int k = 0; k < c->nc; ++k) norm_log[(size_t)st * c->nc + k] = q[4 + k];
+        if (energy_log) energy_log[st] = f->kind == GP5_FUNC_LHY_UNLOCKED ? NAN : q[2] + (c->nc > 1 ? q[3] : 0.0);
+    }
+    for (int k = 0; k < c->nc; ++k)
+        if (!std::isfinite(r[k])) return fail(GP5_ERR_NONFINITE, "norm of component %d became %g during the step", k, r[k]);
+    return GP5_OK;
+}
+
+int gp5_advance(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi, int nsteps,
+                double *norm_log, double *energy_log)
+{
+    return advance_impl(c, f, s, psi, nsteps, norm_log, energy_log, nullptr);
+}
+
+int gp5_advance_timed(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi, int nsteps,
+                      double *elapsed_ms)
+{
+    if (!elapsed_ms) return fail(GP5_ERR_ARG, "NULL elapsed_ms");
+    return advance_impl(c, f, s, psi, nsteps, nullptr, nullptr, elapsed_ms);
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+"""Seeded synthetic initial states of the benchmark workloads.
+
+One counter-based stream: u_i = splitmix64(seed * 2^40 + i) mapped to
+[-1, 1) from its top 53 bits.  The same stream is restated in the oracle's C
+code, so CPU and GPU runs start from bit-identical inputs.  Arrays are
+complex128 in the reference layout (Nx, Ny, Nz, nC), Fortran order.
+"""
+from __future__ import annotations
+
+from typing import Sequence, Tuple
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def uniform(seed: int, count: int, start: int = 0) -> np.ndarray:
+    """splitmix64 counter stream mapped to [-1, 1)."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) << np.uint64(40)) + np.uint64(start) + np.arange(count, dtype=np.uint64)
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return 2.0 * ((x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) - 1.0
+
+
+def axes(shape: Sequence[int], left: Sequence[float], right: Sequence[float]):
+    """Cell-centred coordinates x_i = left + (i + 1/2) dx (reference src/geometry.cpp:165-188)."""
+    return [left[d] + (np.arange(shape[d]) + 0.5) * ((right[d] - left[d]) / shape[d]) for d in range(3)]
+
+
+def r2_grid(shape, left, right, center=(0.0, 0.0, 0.0)) -> np.ndarray:
+    x, y, z = axes(shape, left, right)
+    return ((x - center[0])[:, None, None] ** 2 + (y - center[1])[None, :, None] ** 2
+            + (z - center[2])[None, None, :] ** 2)
+
+
+def dV(shape, left, right) -> float:
+    v = 1.0
+    for d in range(3):
+        v *= (right[d] - left[d]) / shape[d]
+    return v
+
+
+def gaussian_noisy(shape, left, right, sigma: float, noise: float, seed: int, N: float) -> np.ndarray:
+    """exp(-r^2/(2 sigma^2)) + noise*(u1 + i u2), normalised to N (harmonic-trap workload)."""
+    M = int(np.prod(shape))
+    g = np.exp(-r2_grid(shape, left, right) / (2 * sigma * sigma))
+    u = uniform(seed, 2 * M)
+    re = u[0::2].reshape(shape, order="F")
+    im = u[1::2].reshape(shape, order="F")
+    psi = (g + noise * re) + 1j * (noise * im)
+    psi *= np.sqrt(N / (np.sum(np.abs(psi) ** 2) * dV(shape, left, right)))
+    return np.asfortranarray(psi[..., None])
+
+
+def flat_top_droplet(shape, left, right, radius: float, width: float, phase_noise: float, seed: int,
+                     N: float) -> np.ndarray:
+    """A * s(r) * exp(i * phase_noise * u), s = (1 - tanh((r - R)/w))/2, A fixed by N (LHY droplet workload)."""
+    M = int(np.prod(shape))
+    r = np.sqrt(r2_grid(shape, left, right))
+    s = 0.5 * (1.0 - np.tanh((r - radius) / width))
+    A = np.sqrt(N / (np.sum(s * s) * dV(shape, left, right)))
+    u = uniform(seed, M).reshape(shape, order="F")
+    psi = A * s * np.exp(1j * phase_noise * u)
+    return np.asfortranarray(psi[..., None])
+
+
+def harmonic_potential(shape, left, right, omegas: Sequence[Sequence[float]]) -> np.ndarray:
+    """V_c = 1/2 sum_d omega_{c,d}^2 x_d^2 (reference src/externalPotential.cpp:11-32), real FP64."""
+    x, y, z = axes(shape, left, right)
+    out = np.empty(tuple(shape) + (len(omegas),), order="F")
+    for c, w in enumerate(omegas):
+        out[..., c] = 0.5 * (w[0] ** 2 * x[:, None, None] ** 2 + w[1] ** 2 * y[None, :, None] ** 2
+                             + w[2] ** 2 * z[None, None, :] ** 2)
+    return out
+
+
+# Workload definitions ------------------------------------------------------
+
+def droplet_256(n: int = 256) -> Tuple[Tuple[int, int, int], Tuple[float, ...], Tuple[float, ...], np.ndarray]:
+    """Untrapped LHY droplet, n^3 periodic box L = 40, flat top r = 6 (width 0.5) scaled
+    to N = 3000 with seeded (seed 2) 1e-4 phase noise."""
+    shape = (n, n, n)
+    left, right = (-20.0,) * 3, (20.0,) * 3
+    return shape, left, right, flat_top_droplet(shape, left, right, 6.0, 0.5, 1e-4, 2, 3000.0)
+
+
+def harmonic_64(n: int = 64):
+    """Single-component GP, n^3 on [-10, 10]^3, Gaussian sigma = 1.5 plus seeded (seed 1)
+    1e-3 complex uniform noise, normalised to N = 1; harmonic trap omega = (1,1,1)."""
+    shape = (n, n, n)
+    left, right = (-10.0,) * 3, (10.0,) * 3
+    psi = gaussian_noisy(shape, left, right, 1.5, 1e-3, 1, 1.0)
+    V = harmonic_potential(shape, left, right, [(1.0, 1.0, 1.0)])
+    return shape, left, right, psi, V

@@ -1,0 +1,145 @@
+/*
+ * gp5.h -- C ABI of the MI355X (gfx950) Gross-Pitaevskii pseudo-spectral core.
+ *
+ * Every entry point is a plain C function over opaque handles, plain pointers
+ * and sizes.  Host buffers use the reference's memory layout
+ * (src/traits.h:12-20): complex128 interleaved (re, im), column-major
+ * (x fastest, then y, z, component), so an Eigen::Tensor<complex<double>,4>
+ * data() pointer or a Fortran-ordered numpy complex128 array passes through
+ * unchanged.  Device state stays resident between calls; only scalars come back
+ * per step.
+ *
+ * All functions return GP5_OK (0) or a GP5_ERR_* code; gp5_last_error() holds
+ * a thread-local message for the last failure.  Calls are synchronous unless
+ * stated otherwise.  One HIP stream per context; contexts are not thread-safe.
+ */
+#ifndef GP5_H
+#define GP5_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP5_OK 0
+#define GP5_ERR_ARG 1          /* bad argument / shape mismatch / unknown name */
+#define GP5_ERR_HIP 2          /* HIP runtime error (message carries hipGetErrorString) */
+#define GP5_ERR_UNSUPPORTED 3  /* e.g. axis length beyond the FFT's LDS tile */
+#define GP5_ERR_ALLOC 4        /* device allocation failed */
+#define GP5_ERR_NONFINITE 5    /* NaN/Inf detected by a fused reduction */
+
+/* FFT_DIRECTION (reference src/fourierTransform.h:12) */
+#define GP5_FORWARD 0
+#define GP5_BACKWARD 1
+
+/* functional kinds (reference src/functional.h:51-113) */
+#define GP5_FUNC_GP 0            /* gpFunctional */
+#define GP5_FUNC_LHY 1           /* LHYDropletFunctional */
+#define GP5_FUNC_LHY_UNLOCKED 2  /* LHYDropletUnlockedFunctional */
+
+/* stepper kinds (reference src/stepper.h:70-97) */
+#define GP5_STEP_EULERO 0
+#define GP5_STEP_RK4 1
+
+typedef struct gp5_ctx gp5_ctx; /* grid + FFT plans + stream + scratch */
+typedef struct gp5_buf gp5_buf; /* device array of doubles */
+
+/* Parameters of a functional; plain data, copied at each call.
+ * Reference: functional / gpFunctional / LHYDropletFunctional /
+ * LHYDropletUnlockedFunctional state (src/functional.h:9-113). */
+typedef struct gp5_functional_desc {
+    int kind;                 /* GP5_FUNC_* */
+    int has_couplings;        /* gpFunctional::_setCouplings */
+    double couplings[4];      /* couplings(i,j) at [i + 2*j] */
+    double masses[2];         /* gpFunctional::_masses (default 1) */
+    double lhy_a, lhy_b;      /* LHY: a*rho + b*rho^{3/2}; reference values -3, 2.5 */
+    double alpha, beta, eta;  /* UnlockedLHYDroplet parameters */
+    const gp5_buf *potential; /* real FP64 V, M*nC doubles, or NULL (addPotential) */
+} gp5_functional_desc;
+
+/* Parameters of a stepper (reference src/stepper.h:8-67). */
+typedef struct gp5_stepper_desc {
+    int kind;                   /* GP5_STEP_* */
+    double tau_re, tau_im;      /* complex time step: (dt,0) imaginary, (0,dt) real */
+    int renormalize;            /* stepper::_reNormalize */
+    double normalizations[2];   /* stepper::_normalizations */
+    const gp5_buf *phase;       /* phaseConstraint phase (real, M*nC) or NULL */
+} gp5_stepper_desc;
+
+const char *gp5_last_error(void);
+int gp5_version(void);
+
+/* Grid, plans and stream.  Replaces fourierTransformCreator::create +
+ * createUniformDiscretization + the fftwFourierTransform constructor
+ * (src/fourierTransform.cpp:228-249, :129-143; src/geometry.cpp:56-105).
+ * One GPU, so the processor grid is {1,1,1}. */
+int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3],
+                   const double right[3], int n_components);
+int gp5_ctx_destroy(gp5_ctx *ctx);
+int gp5_ctx_sync(gp5_ctx *ctx);
+/* Points per component (Nx*Ny*Nz). */
+size_t gp5_ctx_points(const gp5_ctx *ctx);
+
+/* Device buffers (the Eigen tensors of the reference live here). */
+int gp5_buf_alloc(gp5_ctx *ctx, size_t n_doubles, gp5_buf **out);
+int gp5_buf_free(gp5_buf *buf);
+size_t gp5_buf_size(const gp5_buf *buf);
+int gp5_buf_upload(gp5_buf *buf, const double *host, size_t n_doubles);
+int gp5_buf_download(const gp5_buf *buf, double *host, size_t n_doubles);
+int gp5_buf_copy(gp5_buf *dst, const gp5_buf *src);
+int gp5_buf_zero(gp5_buf *buf);
+
+/* Unnormalised 3-D c2c DFT of every component, out-of-place with src
+ * preserved (src == dst allowed).  Replaces fftwFourierTransform::apply
+ * (src/fourierTransform.cpp:184-214). */
+int gp5_fft(gp5_ctx *ctx, const gp5_buf *src, gp5_buf *dst, int direction);
+
+/* dst = IFFT(-K^2 FFT(src)) / M.  Replaces operators::laplacian::apply
+ * (src/operators.cpp:28-36). */
+int gp5_laplacian(gp5_ctx *ctx, const gp5_buf *src, gp5_buf *dst);
+
+/* dst = H(src).  Replaces functional::apply for gpFunctional
+ * (src/functional.cpp:13-43,54-102), LHYDropletFunctional (:105-122) and
+ * LHYDropletUnlockedFunctional (:125-150), including addPotential (:46-53).
+ * `time` is accepted and unused, as in the reference. */
+int gp5_functional_apply(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_buf *src,
+                         gp5_buf *dst, double time);
+
+/* norm_c = sum |psi_c|^2 dV.  Replaces gp::norm (src/geometry.cpp:190-214). */
+int gp5_norm(gp5_ctx *ctx, const gp5_buf *f, int component, double *out);
+
+/* psi_c *= sqrt(N / norm_c).  Replaces gp::normalize (src/geometry.cpp:217-229). */
+int gp5_normalize(gp5_ctx *ctx, gp5_buf *f, int component, double N);
+
+/* max_x |psi_c(x)|^2.  Replaces the local part of gp.maxDensity
+ * (python/gp/gp.py:210-222). */
+int gp5_max_density(gp5_ctx *ctx, const gp5_buf *f, int component, double *out);
+
+/* E[psi] = sum_c int psi_c^* (-1/2 m_c^-1 lap) psi_c + V rho + e(rho).  No
+ * reference counterpart (the reference computes no energy); GP5_ERR_UNSUPPORTED
+ * for the unlocked functional, which is not a gradient for general alpha, beta, eta. */
+int gp5_energy(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_buf *psi, double *out);
+
+/* psi <- |psi| e^{i phase}.  Replaces phaseConstraint::apply (src/constraint.cpp:6-9). */
+int gp5_phase_constraint(gp5_ctx *ctx, gp5_buf *psi, const gp5_buf *phase);
+
+/* Advance psi in place by nsteps of euleroStepper / RK4Stepper
+ * (src/stepper.cpp:30-81) as driven by pyInterface::timeStepper::advance
+ * (src/gpCpp.cpp:255-269).  Optional per-step logs (host arrays, may be NULL):
+ * norm_log[s*nC + c] = norm_c and energy_log[s] = E of the state at the start
+ * of step s (computed inside the fused stage-1 epilogue at no extra pass). */
+int gp5_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
+                gp5_buf *psi, int nsteps, double *norm_log, double *energy_log);
+
+/* Time the hot loop: run nsteps of gp5_advance with no logs and report the
+ * elapsed device time between HIP events recorded on the context's stream,
+ * in milliseconds. */
+int gp5_advance_timed(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
+                      gp5_buf *psi, int nsteps, double *elapsed_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GP5_H */
