@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Headline benchmark: FP64 real-time RK4 steps/s of the LHY droplet at 256^3 on one MI355X.
+
+Workload: untrapped LHY droplet (-3|psi|^2 psi + 2.5|psi|^3 psi), 256^3 periodic box
+L = 40, complex128, flat-top initial profile of radius 6 scaled to N = 3000 with seeded
+1e-4 phase noise, real-time RK4 dt = 5e-4 (the reference's RK4Stepper algebra,
+src/stepper.cpp:46-70).  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line.  `roofline` prices the dominant kernel against the 8 TB/s HBM
+peak from its algorithmic bytes per launch and its average launch duration measured
+here with HIP events on the context's stream; `cpu_baseline` times the CPU oracle
+(the C restatement of the reference path) on this host's cores for a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--n 256] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 steps/s at 256³ FP64 + ground-state time, surrogate-seeded vs cold start"
+HBM_PEAK_GBPS = 8000.0
+
+# Algorithmic HBM bytes per launch of each kernel kind, in units of F = 16*M bytes
+# (one complex128 field): an axis pass reads and writes the field once; the x-inverse
+# epilogue of RK4 stage s reads T, phi_s, psi, ksum and writes ksum/phi_{s+1} or psi'
+# (stage 1: 4F, stages 2-3: 6F, stage 4: 5F; average 5.25F); the fused variant also
+# writes the x transform of its output (one more F: average 6.25F).
+KIND_BYTES_F = {
+    "fft_x_forward": 2.0,
+    "fft_y_pass": 2.0,
+    "fft_z_fwd_mul_inv": 2.0,
+    "fft_x_inverse_epilogue": 5.25,
+    "fft_x_inv_epilogue_fwd": 6.25,
+}
+STEP_MODEL_F = 29.0  # ideal compulsory traffic of one RK4 step (fused, one r/w per FFT)
+
+
+def _affinity() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(psi0, shape, left, right, budget_s: float = 25.0):
+    """Time the CPU oracle (restatement of GP5's FFTW path) on this host: 1 RK4 step of the
+    same workload with OpenMP over the available cores (capped at 16)."""
+    from oracle import oracle as O
+
+    O.build()
+    cores = min(_affinity(), 16)
+    O.set_threads(cores)
+    geo = O.Geometry(shape, left, right, 1)
+    F = O.Functional(kind=O.OR_LHY)
+    steps = 0
+    t0 = time.perf_counter()
+    psi = psi0
+    while True:
+        psi = O.advance(F, psi, geo, stepper="RK4", dt=5e-4, imaginary=False, nsteps=1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el * (steps + 1) / steps > budget_s or steps >= 3:
+            break
+    return {
+        "value": steps / el,
+        "unit": "steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} RK4 step(s) of the {shape[0]}^3 droplet workload, CPU oracle "
+                  f"(C restatement of the reference FFTW path) with OpenMP over {cores} threads",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    from gp5_amd import capi, synth
+
+    n = args.n
+    shape, left, right, psi0 = synth.droplet_256(n)
+    ctx = capi.Context(shape, left, right, 1)
+    psi = ctx.field(psi0)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=5e-4, imaginary=False)
+
+    # norm / energy at the start (fused reductions of stage 1)
+    norms0, ens0 = capi.advance(ctx, f, s, psi, 1, log=True)
+    if args.warmup > 1:
+        capi.advance(ctx, f, s, psi, args.warmup - 1)
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    dev_ms = capi.advance_timed(ctx, f, s, psi, args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+
+    steps_per_s = args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+
+    # per-kernel timing (HIP events around each launch) for the roofline object
+    prof = capi.profile_advance(ctx, f, s, psi, args.profile_steps)
+    norms1, ens1 = capi.advance(ctx, f, s, psi, 1, log=True)
+    F = 16.0 * shape[0] * shape[1] * shape[2]
+    dom = max((k for k in prof if k in KIND_BYTES_F), key=lambda k: prof[k][0])
+    avg_ms = prof[dom][0] / prof[dom][1]
+    bytes_launch = KIND_BYTES_F[dom] * F
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(dom)
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"LHY droplet real-time RK4, {n}^3 periodic box L=40, complex128, "
+                               f"flat top r=6 N=3000, dt=5e-4", "grid": n, "components": 1,
+                   "stepper": "RK4", "time": "real", "dt": 5e-4},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_launch},
+        "step_roofline": {"model_bytes_per_step": STEP_MODEL_F * F,
+                          "achieved_GBps": STEP_MODEL_F * F * steps_per_s / 1e9,
+                          "frac": STEP_MODEL_F * F * steps_per_s / 1e9 / HBM_PEAK_GBPS},
+        "device_ms_per_step": dev_ms / args.steps,
+        "kernels": {k: {"avg_ms": v[0] / v[1], "launches": v[1]} for k, v in prof.items()},
+        "checks": {"norm_start": float(norms0[0, 0]), "norm_end": float(norms1[0, 0]),
+                   "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
+                   "steps_between": args.warmup + args.steps + args.profile_steps},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -17,7 +17,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgp5.so")
+# GP5_LIB_PATH selects another build of the same ABI (A/B timing of build options)
+LIB_PATH = os.environ.get("GP5_LIB_PATH") or os.path.join(_HERE, "libgp5.so")
 
 GP5_OK = 0
 GP5_ERR_ARG = 1
@@ -36,8 +37,10 @@ EXPORTS = [
     "gp5_ctx_points", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
-    "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed",
+    "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
+    "gp5_kernel_kind_name",
 ]
+GP5_NKINDS = 8
 
 
 class GP5Error(RuntimeError):
@@ -102,6 +105,9 @@ def load() -> ctypes.CDLL:
         "gp5_phase_constraint": ([P, P, P], I),
         "gp5_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, P, P], I),
         "gp5_advance_timed": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP], I),
+        "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
+                                 ctypes.POINTER(ctypes.c_int)], I),
+        "gp5_kernel_kind_name": ([I], ctypes.c_char_p),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -315,3 +321,17 @@ def advance_timed(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, 
     check(load().gp5_advance_timed(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps),
                                    ctypes.byref(ms)))
     return ms.value
+
+
+def profile_advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int) -> dict:
+    """Per-kernel-kind device time of nsteps (HIP events around each launch on the context stream).
+    Returns {kind_name: (total_ms, launches)} for kinds that ran."""
+    L = load()
+    ms = (ctypes.c_double * GP5_NKINDS)()
+    cnt = (ctypes.c_int * GP5_NKINDS)()
+    check(L.gp5_profile_advance(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps), ms, cnt))
+    out = {}
+    for k in range(GP5_NKINDS):
+        if cnt[k]:
+            out[L.gp5_kernel_kind_name(k).decode()] = (ms[k], cnt[k])
+    return out

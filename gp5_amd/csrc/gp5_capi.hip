@@ -28,6 +28,8 @@ hipError_t launch_norm(hipStream_t st, const double2 *f, long long len, unsigned
 hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp);
 hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax);
 hipError_t launch_phase(hipStream_t st, double2 *f, const double *phase, long long len);
+hipError_t launch_phase_exp(hipStream_t st, double2 *ex, const double *phase, long long len);
+hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks);
 }  // namespace gp5
 
 using namespace gp5;
@@ -47,23 +49,78 @@ struct gp5_ctx {
     double dk[3];
     int device;
     hipStream_t st;
+    bool fast[3];        // axis runs on the register-resident kernels (gp5_fast.hip)
     AxisPlan plan[3];
     AxisGeom geom[3];
     long long Opc[3];
     double2 *tw[3];
     // scratch fields (allocated on first use)
     double2 *T, *phiA, *phiB, *ksum;
+    double2 *phase_ex;   // e^{i phase} of the stepper's phase constraint
+    bool fuse_x;         // epilogue pass also forward-transforms its output along x
     // reduction scratch
     double *partials;
     size_t partials_cap;  // blocks
-    unsigned int *counter;
     double *results;      // kRedQ
     double *scales;       // 8 slots x 2 components
     double *blockmax;
     double *dlog;
     size_t dlog_cap;      // steps
     hipEvent_t ev0, ev1;
+    struct ProfState *prof;  // non-null while gp5_profile_advance runs
 };
+
+// Event pairs around every launch while profiling.
+enum KernelKind {
+    K_X_FWD = 0,      // x forward pass (first stage of an advance, or unfused stages)
+    K_Y = 1,          // y pass, either direction
+    K_Z_MID = 2,      // z forward * multiplier * z inverse
+    K_X_EPI = 3,      // x inverse + stage epilogue
+    K_FFT = 4,        // passes of the standalone transform
+    K_NORM = 5,       // norm partials / finalize
+    K_SCALE = 6,      // materialised lazy scale
+    K_X_EPI_FWD = 7,  // x inverse + stage epilogue + x forward of the new stage vector
+};
+static const char *kKindNames[GP5_NKINDS] = {"fft_x_forward", "fft_y_pass", "fft_z_fwd_mul_inv", "fft_x_inverse_epilogue",
+                                              "fft_plain", "norm_reduce", "scale", "fft_x_inv_epilogue_fwd"};
+struct ProfState {
+    std::vector<hipEvent_t> ev;  // pairs
+    std::vector<int> kind;
+    size_t used = 0;
+};
+
+static hipError_t prof_begin(gp5_ctx *c, int kind)
+{
+    ProfState *p = c->prof;
+    if (!p) return hipSuccess;
+    if (2 * (p->used + 1) > p->ev.size()) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            p->ev.push_back(e);
+        }
+    }
+    if (p->kind.size() < p->used + 1) p->kind.push_back(kind);
+    else p->kind[p->used] = kind;
+    return hipEventRecord(p->ev[2 * p->used], c->st);
+}
+
+static hipError_t prof_end(gp5_ctx *c)
+{
+    ProfState *p = c->prof;
+    if (!p) return hipSuccess;
+    hipError_t r = hipEventRecord(p->ev[2 * p->used + 1], c->st);
+    p->used++;
+    return r;
+}
+
+#define PLAUNCH(c, kind, call)                   \
+    do {                                         \
+        HIPCHK(prof_begin((c), (kind)));         \
+        HIPCHK(call);                            \
+        HIPCHK(prof_end(c));                     \
+    } while (0)
 
 // ---------------------------------------------------------------------------
 // errors
@@ -187,7 +244,6 @@ static EpiArgs base_epi(gp5_ctx *c)
     E.nx = c->n[0];
     E.dV = c->dV;
     E.partials = c->partials;
-    E.counter = c->counter;
     E.results = c->results;
     return E;
 }
@@ -241,21 +297,82 @@ static MidArgs make_mid(gp5_ctx *c, const double coef[2], const double *scale)
     return m;
 }
 
-// Five passes: src (scaled by scale_src) -> T -> ... -> epilogue.
-static int spectral(gp5_ctx *c, const double2 *src, const double *scale_src, const double coef[2], const EpiArgs &E)
+static FastArgs fast_args(gp5_ctx *c, int axis, const double2 *src, double2 *dst)
 {
-    MidArgs mid = make_mid(c, coef, nullptr);
+    FastArgs a;
+    memset(&a, 0, sizeof a);
+    a.src = src;
+    a.dst = dst;
+    a.tw = c->tw[axis];
+    a.I = c->geom[axis].I;
+    a.O = c->geom[axis].O;
+    a.Opc = c->Opc[axis];
+    return a;
+}
+
+// One FFT pass along `axis` (sign -1 forward, +1 inverse), src -> dst.
+static int pass_plain(gp5_ctx *c, int kind, int axis, const double2 *src, double2 *dst, int sign, const double *scale_in)
+{
+    if (c->fast[axis]) {
+        FastArgs a = fast_args(c, axis, src, dst);
+        a.scale_in = scale_in;
+        PLAUNCH(c, kind, launch_fast(c->st, c->n[axis], 0, sign, a));
+    } else {
+        MidArgs mid;
+        memset(&mid, 0, sizeof mid);
+        EpiArgs none;
+        memset(&none, 0, sizeof none);
+        PLAUNCH(c, kind, launch_axis(c->st, 0, src, dst, c->plan[axis], c->geom[axis], sign, scale_in, c->Opc[axis], mid, none));
+    }
+    return GP5_OK;
+}
+
+// Passes of one H application.  Plain mode: x forward of src (scaled by
+// in_scale), y, z (forward * multiplier * inverse), y inverse, x inverse +
+// epilogue.  Fused mode (x axis on the register kernels): T already holds the
+// raw x transform of the stage input whose lazy scale is in_scale, the scale is
+// applied in the z pass, and the epilogue pass leaves the x transform of its own
+// output in T for the next stage.
+static int stage_passes(gp5_ctx *c, const double2 *src, const double *in_scale, const double coef[2],
+                        const EpiArgs &E, bool fused)
+{
+    MidArgs mid = make_mid(c, coef, fused ? in_scale : nullptr);
     EpiArgs none;
     memset(&none, 0, sizeof none);
-    HIPCHK(launch_axis(c->st, 0, src, c->T, c->plan[0], c->geom[0], -1, scale_src, c->Opc[0], mid, none));
-    HIPCHK(launch_axis(c->st, 0, c->T, c->T, c->plan[1], c->geom[1], -1, nullptr, c->Opc[1], mid, none));
-    HIPCHK(launch_axis(c->st, 1, c->T, c->T, c->plan[2], c->geom[2], -1, nullptr, c->Opc[2], mid, none));
-    HIPCHK(launch_axis(c->st, 0, c->T, c->T, c->plan[1], c->geom[1], +1, nullptr, c->Opc[1], mid, none));
-    RCHK(ensure_partials(c, axis_grid(c->geom[0])));
+    if (!fused) RCHK(pass_plain(c, K_X_FWD, 0, src, c->T, -1, in_scale));
+    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, -1, nullptr));
+    if (c->fast[2]) {
+        FastArgs a = fast_args(c, 2, c->T, c->T);
+        a.mid = mid;
+        PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, a));
+    } else {
+        PLAUNCH(c, K_Z_MID, launch_axis(c->st, 1, c->T, c->T, c->plan[2], c->geom[2], -1, nullptr, c->Opc[2], mid, none));
+    }
+    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, +1, nullptr));
+    unsigned nblocks;
     EpiArgs e = E;
-    e.partials = c->partials;
-    HIPCHK(launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr, c->Opc[0], mid, e));
+    if (c->fast[0]) {
+        nblocks = fast_grid(c->n[0], c->geom[0].I, c->geom[0].O);
+        RCHK(ensure_partials(c, nblocks));
+        FastArgs a = fast_args(c, 0, c->T, nullptr);
+        e.partials = c->partials;
+        a.epi = e;
+        a.t_out = fused ? c->T : nullptr;
+        PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], 2, +1, a));
+    } else {
+        if (fused) return fail(GP5_ERR_UNSUPPORTED, "fused stages need a register-kernel x length");
+        nblocks = axis_grid(c->geom[0]);
+        RCHK(ensure_partials(c, nblocks));
+        e.partials = c->partials;
+        PLAUNCH(c, K_X_EPI, launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr, c->Opc[0], mid, e));
+    }
+    if (e.want_red) PLAUNCH(c, K_NORM, launch_finalize(c->st, e, nblocks));
     return GP5_OK;
+}
+
+static int spectral(gp5_ctx *c, const double2 *src, const double *scale_src, const double coef[2], const EpiArgs &E)
+{
+    return stage_passes(c, src, scale_src, coef, E, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -303,6 +420,12 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
         if (hipMemcpy(c->tw[d], tw.data(), sizeof(double2) * shape[d], hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(GP5_ERR_HIP, "twiddle upload"));
         c->plan[d].tw = c->tw[d];
+        // GP5_GENERIC_FFT=1 forces the generic LDS kernels (used to cross-check paths)
+        const char *gen = getenv("GP5_GENERIC_FFT");
+        c->fast[d] = fast_supported(shape[d]) && !(gen && gen[0] == '1');
+        // GP5_NO_FUSE=1 runs the x forward transform as its own pass (A/B timing)
+        const char *nf = getenv("GP5_NO_FUSE");
+        c->fuse_x = !(nf && nf[0] == '1');
         AxisGeom &G = c->geom[d];
         G.I = I;
         G.O = c->M / (I * shape[d]) * nc;
@@ -313,8 +436,6 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
         G.B = std::max(1, B);
         I *= shape[d];
     }
-    if (hipMalloc(&c->counter, sizeof(unsigned int)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "counter"));
-    if (hipMemset(c->counter, 0, sizeof(unsigned int)) != hipSuccess) return bail(fail(GP5_ERR_HIP, "memset"));
     if (hipMalloc(&c->results, kRedQ * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "results"));
     if (hipMalloc(&c->scales, 16 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "scales"));
     if (hipMalloc(&c->blockmax, 8192 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "blockmax"));
@@ -334,7 +455,8 @@ int gp5_ctx_destroy(gp5_ctx *c)
     auto note = [&](hipError_t e) { if (first == hipSuccess) first = e; };
     if (c->st) note(hipStreamSynchronize(c->st));
     for (int d = 0; d < 3; ++d) if (c->tw[d]) note(hipFree(c->tw[d]));
-    void *ptrs[] = {c->T, c->phiA, c->phiB, c->ksum, c->partials, c->counter, c->results, c->scales, c->blockmax, c->dlog};
+    void *ptrs[] = {c->T,       c->phiA,    c->phiB,   c->ksum,   c->phase_ex, c->partials,
+                    c->results, c->scales, c->blockmax, c->dlog};
     for (void *p : ptrs) if (p) note(hipFree(p));
     if (c->ev0) note(hipEventDestroy(c->ev0));
     if (c->ev1) note(hipEventDestroy(c->ev1));
@@ -377,12 +499,10 @@ int gp5_buf_alloc(gp5_ctx *c, size_t n, gp5_buf **out)
 int gp5_buf_free(gp5_buf *b)
 {
     if (!b) return GP5_OK;
+    // hipFree synchronises with the device itself, so the owning context (which
+    // may already be destroyed) is not touched here.
     hipError_t e = hipSuccess;
-    if (b->d) {
-        e = hipStreamSynchronize(b->ctx->st);
-        hipError_t e2 = hipFree(b->d);
-        if (e == hipSuccess) e = e2;
-    }
+    if (b->d) e = hipFree(b->d);
     delete b;
     if (e != hipSuccess) return fail(GP5_ERR_HIP, "buffer free: %s", hipGetErrorString(e));
     return GP5_OK;
@@ -435,14 +555,10 @@ int gp5_fft(gp5_ctx *c, const gp5_buf *src, gp5_buf *dst, int direction)
     if (direction == GP5_FORWARD) sign = -1;
     else if (direction == GP5_BACKWARD) sign = +1;
     else return fail(GP5_ERR_ARG, "Unkown direction in fourier transform");
-    MidArgs mid;
-    memset(&mid, 0, sizeof mid);
-    EpiArgs none;
-    memset(&none, 0, sizeof none);
     double2 *d = (double2 *)dst->d;
-    HIPCHK(launch_axis(c->st, 0, (const double2 *)src->d, d, c->plan[0], c->geom[0], sign, nullptr, c->Opc[0], mid, none));
-    HIPCHK(launch_axis(c->st, 0, d, d, c->plan[1], c->geom[1], sign, nullptr, c->Opc[1], mid, none));
-    HIPCHK(launch_axis(c->st, 0, d, d, c->plan[2], c->geom[2], sign, nullptr, c->Opc[2], mid, none));
+    RCHK(pass_plain(c, K_FFT, 0, (const double2 *)src->d, d, sign, nullptr));
+    RCHK(pass_plain(c, K_FFT, 1, d, d, sign, nullptr));
+    RCHK(pass_plain(c, K_FFT, 2, d, d, sign, nullptr));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -492,6 +608,7 @@ static int norm_impl(gp5_ctx *c, const double2 *f, int comp, double N, double *o
     E.Nnorm[0] = N;
     E.scale_out = scale_out;
     HIPCHK(launch_norm(c->st, f + M * comp, M, blocks, E));
+    HIPCHK(launch_finalize(c->st, E, blocks));
     if (out) {
         double r[kRedQ];
         HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
@@ -571,7 +688,7 @@ int gp5_phase_constraint(gp5_ctx *c, gp5_buf *psi, const gp5_buf *phase)
 
 // One step of Euler or RK4 on c's stream.  psi_slot: scale slot of psi (-1 = none).
 static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
-                     int &psi_slot, int step, double *dlog)
+                     int &psi_slot, int step, double *dlog, bool fused)
 {
     EpiArgs E = base_epi(c);
     double coef[2];
@@ -579,10 +696,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.want_red = 1;
     E.Nnorm[0] = s->normalizations[0];
     E.Nnorm[1] = s->normalizations[1];
-    if (s->phase) {
-        RCHK(check_real(c, s->phase, "phase"));
-        E.phase = s->phase->d;
-    }
+    if (s->phase) E.phase_exp = c->phase_ex;  // filled by advance_impl
     const bool rn = s->renormalize != 0;
     double *slot_psi = psi_slot >= 0 ? c->scales + 2 * psi_slot : nullptr;
     const int next_psi_slot = psi_slot == 0 ? 1 : 0;
@@ -598,7 +712,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
         E.out = c->phiA;
         E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
         E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
-        RCHK(spectral(c, P, slot_psi, coef, E));
+        RCHK(stage_passes(c, P, slot_psi, coef, E, fused));
         // the new state lives in phiA: swap device pointers with the caller's buffer
         double2 *fresh = c->phiA;
         c->phiA = (double2 *)psi->d;
@@ -616,7 +730,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.out = c->phiA;
     E.scale_out = rn ? slotA : nullptr;
     E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
-    RCHK(spectral(c, P, slot_psi, coef, E));
+    RCHK(stage_passes(c, P, slot_psi, coef, E, fused));
     E.log = nullptr;
 
     E.mode = EPI_RK2;
@@ -626,7 +740,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.sc_psi = slot_psi;
     E.out = c->phiB;
     E.scale_out = rn ? slotB : nullptr;
-    RCHK(spectral(c, c->phiA, E.sc_phi, coef, E));
+    RCHK(stage_passes(c, c->phiA, E.sc_phi, coef, E, fused));
 
     E.mode = EPI_RK3;
     E.phi = c->phiB;
@@ -635,7 +749,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.ci = ti;
     E.out = c->phiA;
     E.scale_out = rn ? slotA : nullptr;
-    RCHK(spectral(c, c->phiB, E.sc_phi, coef, E));
+    RCHK(stage_passes(c, c->phiB, E.sc_phi, coef, E, fused));
 
     E.mode = EPI_RK4;
     E.phi = c->phiA;
@@ -644,7 +758,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.ci = (1. / 6) * ti;
     E.out = P;
     E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
-    RCHK(spectral(c, c->phiA, E.sc_phi, coef, E));
+    RCHK(stage_passes(c, c->phiA, E.sc_phi, coef, E, fused));
     psi_slot = rn ? next_psi_slot : -1;
     return GP5_OK;
 }
@@ -662,6 +776,11 @@ static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     }
     if (nsteps == 0) return GP5_OK;
     RCHK(ensure_scratch(c, s->kind == GP5_STEP_RK4));
+    if (s->phase) {
+        RCHK(check_real(c, s->phase, "phase"));
+        if (!c->phase_ex) HIPCHK(hipMalloc(&c->phase_ex, (size_t)c->M * c->nc * sizeof(double2)));
+        HIPCHK(launch_phase_exp(c->st, c->phase_ex, s->phase->d, c->M * c->nc));
+    }
     RCHK(ensure_partials(c, axis_grid(c->geom[0])));
     const bool logging = norm_log || energy_log;
     if (logging && (size_t)nsteps > c->dlog_cap) {
@@ -672,8 +791,14 @@ static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     }
     if (elapsed_ms) HIPCHK(hipEventRecord(c->ev0, c->st));
     int psi_slot = -1;
-    for (int st = 0; st < nsteps; ++st) RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr));
-    if (psi_slot >= 0) HIPCHK(launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, c->scales + 2 * psi_slot));
+    // fused stages: the epilogue pass of each stage also forward-transforms its
+    // output along x, so only the first stage needs a separate x pass
+    const bool fused = c->fast[0] && c->fuse_x;
+    if (fused) RCHK(pass_plain(c, K_X_FWD, 0, (const double2 *)psi->d, c->T, -1, nullptr));
+    for (int st = 0; st < nsteps; ++st)
+        RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr, fused));
+    if (psi_slot >= 0)
+        PLAUNCH(c, K_SCALE, launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, c->scales + 2 * psi_slot));
     if (elapsed_ms) HIPCHK(hipEventRecord(c->ev1, c->st));
     double r[kRedQ];
     HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
@@ -710,6 +835,38 @@ int gp5_advance_timed(gp5_ctx *c, const gp5_functional_desc *f, const gp5_steppe
 {
     if (!elapsed_ms) return fail(GP5_ERR_ARG, "NULL elapsed_ms");
     return advance_impl(c, f, s, psi, nsteps, nullptr, nullptr, elapsed_ms);
+}
+
+const char *gp5_kernel_kind_name(int kind)
+{
+    if (kind < 0 || kind >= GP5_NKINDS) return "";
+    return kKindNames[kind];
+}
+
+int gp5_profile_advance(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                        int nsteps, double *kind_ms, int *kind_launches)
+{
+    if (!c || !kind_ms || !kind_launches) return fail(GP5_ERR_ARG, "NULL argument");
+    ProfState p;
+    c->prof = &p;
+    int rc = advance_impl(c, f, s, psi, nsteps, nullptr, nullptr, nullptr);
+    c->prof = nullptr;
+    for (int k = 0; k < GP5_NKINDS; ++k) {
+        kind_ms[k] = 0.0;
+        kind_launches[k] = 0;
+    }
+    if (rc == GP5_OK) {
+        hipError_t e = hipStreamSynchronize(c->st);
+        for (size_t i = 0; e == hipSuccess && i < p.used; ++i) {
+            float ms = 0.f;
+            e = hipEventElapsedTime(&ms, p.ev[2 * i], p.ev[2 * i + 1]);
+            kind_ms[p.kind[i]] += ms;
+            kind_launches[p.kind[i]] += 1;
+        }
+        if (e != hipSuccess) rc = fail(GP5_ERR_HIP, "profile events: %s", hipGetErrorString(e));
+    }
+    for (hipEvent_t ev : p.ev) (void)hipEventDestroy(ev);
+    return rc;
 }
 
 }  // extern "C"

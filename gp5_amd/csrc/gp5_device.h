@@ -1,0 +1,394 @@
+// Device helpers shared by the generic LDS FFT kernels (gp5_kernels.hip) and the
+// register-resident fast kernels (gp5_fast.hip): complex arithmetic, small DFTs,
+// the reference k-grid rule, the deterministic block reduction and the fused
+// functional / stepper epilogue.
+#pragma once
+
+#include "gp5_internal.h"
+
+#include <math.h>
+
+namespace gp5 {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// a * (i * s) for s = +-1
+__device__ __forceinline__ double2 cmul_is(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+
+__device__ __forceinline__ double2 twiddle(const double2 *tw, int t, int sign)
+{
+    double2 w = tw[t];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// small DFTs in registers; sign = -1 forward (e^{-2 pi i/R}), +1 inverse
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void dft2(double2 *v)
+{
+    double2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+__device__ __forceinline__ void dft3(double2 *v, double s)
+{
+    const double c = -0.5, sn = 0.86602540378443864676372317075294;  // sin(2 pi/3)
+    double2 t1 = cadd(v[1], v[2]);
+    double2 t2 = make_double2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    double2 d = csub(v[1], v[2]);
+    double2 t3 = cmul_is(d, s * sn);
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+}
+
+__device__ __forceinline__ void dft4(double2 *v, double s)
+{
+    double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    double2 b0 = cadd(v[1], v[3]), b1 = cmul_is(csub(v[1], v[3]), s);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+}
+
+__device__ __forceinline__ void dft5(double2 *v, double s)
+{
+    const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)
+    const double c2 = -0.80901699437494742410229341718282;  // cos(4 pi/5)
+    const double s1 = 0.95105651629515357211643933337938;   // sin(2 pi/5)
+    const double s2 = 0.58778525229247312916870595463907;   // sin(4 pi/5)
+    double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    double2 a0 = v[0];
+    double2 b1 = make_double2(a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y);
+    double2 b2 = make_double2(a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y);
+    double2 d1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    double2 d2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    double2 e1 = cmul_is(d1, s), e2 = cmul_is(d2, s);
+    v[0] = cadd(a0, cadd(t1, t2));
+    v[1] = cadd(b1, e1);
+    v[4] = csub(b1, e1);
+    v[2] = cadd(b2, e2);
+    v[3] = csub(b2, e2);
+}
+
+__device__ __forceinline__ void dft8(double2 *v, double s)
+{
+    const double h = 0.70710678118654752440084436210485;
+    double2 e[4] = {v[0], v[2], v[4], v[6]};
+    double2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft4(e, s);
+    dft4(o, s);
+    // w8^k = cos(pi k/4) + i s sin(pi k/4)
+    double2 o1 = cmul(o[1], make_double2(h, s * h));
+    double2 o2 = cmul_is(o[2], s);
+    double2 o3 = cmul(o[3], make_double2(-h, s * h));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+}
+
+// 16-point DFT as 4 x 4: r = 4a + b, k = c + 4d;
+// X[c + 4d] = sum_b w4^{bd} w16^{bc} sum_a w4^{ac} v[4a + b]
+__device__ __forceinline__ void dft16(double2 *v, double s)
+{
+    const double c1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+    const double s1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+    const double h = 0.70710678118654752440084436210485;
+    double2 y[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double2 t[4] = {v[b], v[b + 4], v[b + 8], v[b + 12]};
+        dft4(t, s);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[b][c] = t[c];
+    }
+    // w16^{bc} = cos(pi bc/8) + i s sin(pi bc/8), bc in {1,2,3,4,6,9}
+    y[1][1] = cmul(y[1][1], make_double2(c1, s * s1));
+    y[1][2] = cmul(y[1][2], make_double2(h, s * h));
+    y[1][3] = cmul(y[1][3], make_double2(s1, s * c1));
+    y[2][1] = cmul(y[2][1], make_double2(h, s * h));
+    y[2][2] = cmul_is(y[2][2], s);
+    y[2][3] = cmul(y[2][3], make_double2(-h, s * h));
+    y[3][1] = cmul(y[3][1], make_double2(s1, s * c1));
+    y[3][2] = cmul(y[3][2], make_double2(-h, s * h));
+    y[3][3] = cmul(y[3][3], make_double2(-c1, -s * s1));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double2 t[4] = {y[0][c], y[1][c], y[2][c], y[3][c]};
+        dft4(t, s);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) v[c + 4 * d] = t[d];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k-space wavenumber (src/geometry.cpp:127-148)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double momentum(int i, int n, double dk)
+{
+    if ((n & 1) == 0) return i < n / 2 ? dk * i : (double)(i - n) * dk;
+    return i <= (n - 1) / 2 ? dk * i : (double)(i - n) * dk;
+}
+
+// ---------------------------------------------------------------------------
+// block reduction of kRedQ partial sums; the last block to finish sums the
+// per-block partials in a fixed order (deterministic) and publishes results.
+// ---------------------------------------------------------------------------
+// Blocks are multiples of 64 threads, at most 512; `sred` is LDS scratch of at
+// least kRedScratch doubles.  Each block writes its kRedQ partial sums (wave
+// shuffles, then the waves in order); k_finalize, one block launched right
+// after on the same stream, sums the partials in a fixed order.  No atomics:
+// a single global counter hit by every block serialises tens of thousands of
+// blocks at the L2.
+constexpr int kRedScratch = kRedQ * 8 + 1;
+
+__device__ __forceinline__ void block_sum(double (&v)[kRedQ], double *sred, double (&out)[kRedQ])
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int NW = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kRedQ; ++q) {
+        double x = v[q];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) sred[w * kRedQ + q] = x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kRedQ; ++q) {
+        double x = 0.0;
+        for (int i = 0; i < NW; ++i) x += sred[i * kRedQ + q];
+        out[q] = x;
+    }
+}
+
+__device__ __forceinline__ void reduce_block(double (&v)[kRedQ], double *partials, double *sred)
+{
+    double s[kRedQ];
+    block_sum(v, sred, s);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < kRedQ; ++q) partials[(size_t)blockIdx.x * kRedQ + q] = s[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// epilogue: one element of the last inverse pass
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 load_scaled(const double2 *p, long long g, const double *sc, int c)
+{
+    double2 v = p[g];
+    if (sc) { double s = sc[c]; v.x *= s; v.y *= s; }
+    return v;
+}
+
+// Batched epilogue over CH elements of one line (one component c) at
+// gbase + pos[k]: every global load of the batch is issued before the
+// arithmetic, with the mode / functional tests uniform across the wave, so each
+// operand stream has CH independent requests in flight per thread.  On return
+// v[k] holds the value written to E.out (the kinetic term for EPI_STORE, H for
+// EPI_H, the new state for the stepping modes).  Same arithmetic as epilogue().
+template <int CH>
+__device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase, const int (&pos)[CH], int c,
+                                               double2 (&v)[CH], double (&red)[kRedQ])
+{
+    const int mode = E.mode;
+    if (mode == EPI_STORE) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) E.out[gbase + pos[k]] = v[k];
+        return;
+    }
+    const bool rk_mid = mode == EPI_RK2 || mode == EPI_RK3 || mode == EPI_RK4;
+    const bool coupled = E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2);
+    const bool constrain = E.phase_exp && (mode == EPI_EULER || mode == EPI_RK4);
+    double2 phi[CH], psi[CH], ks[CH], po[CH], ph[CH];
+    double V[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) phi[k] = E.phi[gbase + pos[k]];
+    if (rk_mid) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) psi[k] = E.psi[gbase + pos[k]];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) ks[k] = E.ksum[gbase + pos[k]];
+    }
+    if (E.V) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) V[k] = E.V[gbase + pos[k]];
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) V[k] = 0.0;
+    }
+    if (coupled) {
+        const long long off = c == 0 ? E.M : -E.M;
+#pragma unroll
+        for (int k = 0; k < CH; ++k) po[k] = E.phi[gbase + off + pos[k]];
+    }
+    if (constrain) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) ph[k] = E.phase_exp[gbase + pos[k]];
+    }
+    const double sphi = E.sc_phi ? E.sc_phi[c] : 1.0;
+    const double soth = (coupled && E.sc_phi) ? E.sc_phi[1 - c] : 1.0;
+    const double spsi = (rk_mid && E.sc_psi) ? E.sc_psi[c] : 1.0;
+    const double2 cf = make_double2(E.cr, E.ci);
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+        const long long g = gbase + pos[k];
+        const double2 p = cscale(phi[k], sphi);
+        const double2 kin = v[k];
+        const double rho = p.x * p.x + p.y * p.y;
+        double nl = 0.0, enl = 0.0;
+        if (E.fkind == 1) {
+            const double sr = sqrt(rho);
+            nl = E.lhy_a * rho + E.lhy_b * rho * sr;
+            enl = 0.5 * E.lhy_a * rho * rho + 0.4 * E.lhy_b * rho * rho * sr;
+        } else if (E.fkind == 0) {
+            if (E.has_couplings) {
+                if (E.nc == 1) {
+                    nl = E.g11 * rho;
+                    enl = 0.5 * E.g11 * rho * rho;
+                } else {
+                    const double2 q = cscale(po[k], soth);
+                    const double ro = q.x * q.x + q.y * q.y;
+                    const double gcc = c == 0 ? E.g11 : E.g22;
+                    nl = gcc * rho + E.g12 * ro;
+                    enl = 0.5 * gcc * rho * rho + 0.5 * E.g12 * rho * ro;
+                }
+            }
+        } else {
+            const double2 q = cscale(po[k], soth);
+            const double ro = q.x * q.x + q.y * q.y;
+            const double r1 = c == 0 ? rho : ro, r2 = c == 0 ? ro : rho;
+            const double base = r1 + E.beta * r2;
+            const double s = base * sqrt(base);
+            nl = c == 0 ? r1 + E.eta * r2 + E.alpha * s
+                        : E.eta * E.beta * r1 + E.beta * r2 + E.alpha * E.beta * E.beta * s;
+        }
+        const double a = nl + V[k];
+        const double2 h = make_double2(kin.x + a * p.x, kin.y + a * p.y);
+        red[2 + c] += p.x * kin.x + p.y * kin.y + V[k] * rho + enl;
+        red[4 + c] += rho;
+        if (mode == EPI_H) {
+            E.out[g] = h;
+            v[k] = h;
+            continue;
+        }
+        if (mode == EPI_ENERGY) {
+            v[k] = h;
+            continue;
+        }
+        double2 o;
+        if (mode == EPI_EULER) {
+            o = csub(p, cmul(cf, h));
+        } else if (mode == EPI_RK1) {
+            E.ksum[g] = h;
+            o = csub(p, cmul(cf, h));
+        } else if (mode == EPI_RK2 || mode == EPI_RK3) {
+            E.ksum[g] = make_double2(ks[k].x + 2.0 * h.x, ks[k].y + 2.0 * h.y);
+            o = csub(cscale(psi[k], spsi), cmul(cf, h));
+        } else {
+            o = csub(cscale(psi[k], spsi), cmul(cf, cadd(ks[k], h)));
+        }
+        if (constrain) {
+            const double amp = sqrt(o.x * o.x + o.y * o.y);
+            o = make_double2(amp * ph[k].x, amp * ph[k].y);
+        }
+        E.out[g] = o;
+        red[c] += o.x * o.x + o.y * o.y;
+        v[k] = o;
+    }
+}
+
+// Returns the value written to E.out (the new state for the stepping modes).
+__device__ __forceinline__ double2 epilogue(const EpiArgs &E, long long g, int c, double2 kin, double (&red)[kRedQ])
+{
+    if (E.mode == EPI_STORE) {
+        E.out[g] = kin;
+        return kin;
+    }
+    const long long M = E.M;
+    const double2 phi = load_scaled(E.phi, g, E.sc_phi, c);
+    const double rho = phi.x * phi.x + phi.y * phi.y;
+    double nl = 0.0, enl = 0.0;
+    const double V = E.V ? E.V[g] : 0.0;
+    if (E.fkind == 1) {  // LHY droplet: a rho + b rho^{3/2}
+        const double sr = sqrt(rho);
+        nl = E.lhy_a * rho + E.lhy_b * rho * sr;
+        enl = 0.5 * E.lhy_a * rho * rho + 0.4 * E.lhy_b * rho * rho * sr;
+    } else if (E.fkind == 0) {  // gpFunctional contact couplings
+        if (E.has_couplings) {
+            if (E.nc == 1) {
+                nl = E.g11 * rho;
+                enl = 0.5 * E.g11 * rho * rho;
+            } else {
+                const long long go = c == 0 ? g + M : g - M;
+                const double2 po = load_scaled(E.phi, go, E.sc_phi, 1 - c);
+                const double ro = po.x * po.x + po.y * po.y;
+                const double gcc = c == 0 ? E.g11 : E.g22;
+                nl = gcc * rho + E.g12 * ro;
+                enl = 0.5 * gcc * rho * rho + 0.5 * E.g12 * rho * ro;
+            }
+        }
+    } else {  // unlocked two-component LHY
+        const long long go = c == 0 ? g + M : g - M;
+        const double2 po = load_scaled(E.phi, go, E.sc_phi, 1 - c);
+        const double ro = po.x * po.x + po.y * po.y;
+        const double r1 = c == 0 ? rho : ro, r2 = c == 0 ? ro : rho;
+        const double base = r1 + E.beta * r2;
+        const double s = base * sqrt(base);
+        nl = c == 0 ? r1 + E.eta * r2 + E.alpha * s
+                    : E.eta * E.beta * r1 + E.beta * r2 + E.alpha * E.beta * E.beta * s;
+    }
+    const double a = nl + V;
+    const double2 h = make_double2(kin.x + a * phi.x, kin.y + a * phi.y);
+    // energy density of the stage input and its norm
+    red[2 + c] += phi.x * kin.x + phi.y * kin.y + V * rho + enl;
+    red[4 + c] += rho;
+    if (E.mode == EPI_H) {
+        E.out[g] = h;
+        return h;
+    }
+    if (E.mode == EPI_ENERGY) return h;
+    double2 o;
+    if (E.mode == EPI_EULER) {
+        o = csub(phi, cmul(make_double2(E.cr, E.ci), h));
+    } else if (E.mode == EPI_RK1) {
+        E.ksum[g] = h;
+        o = csub(phi, cmul(make_double2(E.cr, E.ci), h));
+    } else if (E.mode == EPI_RK2 || E.mode == EPI_RK3) {
+        const double2 psi = load_scaled(E.psi, g, E.sc_psi, c);
+        const double2 ks = E.ksum[g];
+        E.ksum[g] = make_double2(ks.x + 2.0 * h.x, ks.y + 2.0 * h.y);
+        o = csub(psi, cmul(make_double2(E.cr, E.ci), h));
+    } else {  // EPI_RK4: psi - (tau/6) ((k1 + 2k2 + 2k3) + k4)
+        const double2 psi = load_scaled(E.psi, g, E.sc_psi, c);
+        const double2 ks = E.ksum[g];
+        o = csub(psi, cmul(make_double2(E.cr, E.ci), cadd(ks, h)));
+    }
+    if (E.phase_exp && (E.mode == EPI_EULER || E.mode == EPI_RK4)) {
+        const double amp = sqrt(o.x * o.x + o.y * o.y);
+        const double2 ex = E.phase_exp[g];
+        o = make_double2(amp * ex.x, amp * ex.y);
+    }
+    E.out[g] = o;
+    red[c] += o.x * o.x + o.y * o.y;
+    return o;
+}
+
+}  // namespace gp5

@@ -1,0 +1,411 @@
+// Register-resident Stockham FFT passes for power-of-two axis lengths 16..512.
+//
+// Each thread keeps E complex values of one line in registers; a length-N
+// transform runs as 1-3 radix stages (16x16 for 256, 16x8 for 128, 8x8 for 64,
+// 8x8x8 for 512) with one LDS exchange between stages.  Global loads and stores
+// go straight between HBM and registers: element (t + TL*i) + (N/R)*r of a line
+// belongs to thread t, so for fixed (i, r) consecutive threads touch consecutive
+// elements (x lines) or consecutive columns (y/z axes, columns interleaved).
+//
+// Forward transforms run the radix list in order and inverse transforms in
+// reverse, so the register layout after an inverse transform equals the input
+// layout of a forward transform: the fused x pass (inverse x FFT, functional and
+// RK4 stage epilogue, forward x FFT of the new stage vector) and the fused z pass
+// (forward z FFT, k-space multiplier, inverse z FFT) never leave registers
+// between their transforms.
+#include "gp5_device.h"
+
+#include <algorithm>
+
+namespace gp5 {
+
+template <int R>
+__device__ __forceinline__ void dftR(double2 *v, double s)
+{
+    if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 4) dft4(v, s);
+    else if constexpr (R == 8) dft8(v, s);
+    else if constexpr (R == 16) dft16(v, s);
+}
+
+// Radix plans per layout (ROWS: x lines; else interleaved y/z columns): R0
+// first for forward transforms; 0 = no stage; E = complex values per thread.
+// The x-line kernels carry the stage epilogue, so they keep E at 8 to leave
+// registers for it (16x16 would hold 16 values per thread).
+template <int N, bool ROWS> struct FPlan;
+#define GP5_PLAN(NN, RW, A, B, C, EE) \
+    template <> struct FPlan<NN, RW> { static constexpr int R0 = A, R1 = B, R2 = C, E = EE; }
+GP5_PLAN(16, false, 16, 0, 0, 16);
+GP5_PLAN(32, false, 8, 4, 0, 8);
+GP5_PLAN(64, false, 8, 8, 0, 8);
+GP5_PLAN(128, false, 16, 8, 0, 16);
+GP5_PLAN(256, false, 16, 16, 0, 16);
+GP5_PLAN(512, false, 8, 8, 8, 8);
+GP5_PLAN(16, true, 4, 4, 0, 4);
+GP5_PLAN(32, true, 8, 4, 0, 8);
+GP5_PLAN(64, true, 8, 8, 0, 8);
+#ifdef GP5_ROWS_E16
+GP5_PLAN(128, true, 16, 8, 0, 16);
+GP5_PLAN(256, true, 16, 16, 0, 16);
+#else
+GP5_PLAN(128, true, 8, 4, 4, 8);
+GP5_PLAN(256, true, 8, 8, 4, 8);
+#endif
+GP5_PLAN(512, true, 8, 8, 8, 8);
+#undef GP5_PLAN
+
+// Block shapes.  rows: x lines, LPB lines per 256-thread block, LDS line stride
+// N + N/16 (one pad element every 16 keeps stride-16 accesses conflict free).
+// cols: B interleaved columns of TL threads each.
+template <int N, bool ROWS> struct FShape {
+    static constexpr int E = FPlan<N, ROWS>::E;
+    static constexpr int TL = N / E;
+    static constexpr int ROW_THREADS = TL > 256 ? TL : 256;
+    static constexpr int LPB = ROW_THREADS / TL;
+    static constexpr int ROW_STRIDE = N + N / 16;
+    static constexpr int COLS = TL >= 64 ? 8 : 16;
+    static constexpr int COL_THREADS = COLS * TL;
+};
+
+struct RowMap {
+    int base;
+    __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+};
+struct ColMap {
+    int b, B;
+    __device__ __forceinline__ int operator()(int pos) const { return b + B * pos; }
+};
+
+// One radix-R stage with L already-combined points: E/R butterflies per thread.
+template <int N, int E, int L, int R>
+__device__ __forceinline__ void stage_compute(double2 (&v)[E], int t, const double2 *__restrict__ tw, double s)
+{
+    constexpr int TL = N / E;
+    constexpr int m = N / (L * R);
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+        if constexpr (L > 1) {
+            // w_{LR}^{j r}, r = 1..R-1, by binary powering of one table entry
+            // (at most 4 products deep for R = 16): one load instead of R-1
+            // keeps the twiddles out of the register file until they are used
+            const int bf = t + TL * i;
+            const int j = bf / m;
+            double2 w1 = tw[j * m];
+            w1.y *= -s;  // table holds e^{-2 pi i t/N}; s = -1 forward keeps it
+            double2 wp[R];
+            wp[1] = w1;
+#pragma unroll
+            for (int r = 2; r < R; ++r) wp[r] = (r & 1) ? cmul(wp[r - 1], w1) : cmul(wp[r / 2], wp[r / 2]);
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[i * R + r] = cmul(v[i * R + r], wp[r]);
+        }
+        dftR<R>(&v[i * R], s);
+    }
+}
+
+// Stage outputs -> LDS, then next stage inputs <- LDS (in place, two barriers).
+template <int N, int E, int L, int R, int RN, class Map>
+__device__ __forceinline__ void exchange(double2 (&v)[E], int t, double2 *lds, const Map &map)
+{
+    constexpr int TL = N / E;
+    constexpr int m = N / (L * R);
+#pragma unroll
+    for (int i = 0; i < E / R; ++i) {
+        const int bf = t + TL * i;
+        const int q = bf % m, j = bf / m;
+#pragma unroll
+        for (int k = 0; k < R; ++k) lds[map(q + m * (j + L * k))] = v[i * R + k];
+    }
+    __syncthreads();
+    constexpr int L2 = L * R;
+    constexpr int m2 = N / (L2 * RN);
+#pragma unroll
+    for (int i = 0; i < E / RN; ++i) {
+        const int bf = t + TL * i;
+        const int q = bf % m2, j = bf / m2;
+#pragma unroll
+        for (int r = 0; r < RN; ++r) v[i * RN + r] = lds[map(q + m2 * (r + RN * j))];
+    }
+    __syncthreads();
+}
+
+// Full transform over the stages Ra, Rb, Rc (0 = absent), sign s.
+// In:  v[i*Ra + r] = x[(t + TL i) + (N/Ra) r];  out: v[i*Rl + k] = X[(t + TL i) + (N/Rl) k], Rl last radix.
+template <int N, int E, int Ra, int Rb, int Rc, class Map>
+__device__ __forceinline__ void fft_run(double2 (&v)[E], int t, const double2 *__restrict__ tw, double s,
+                                        double2 *lds, const Map &map)
+{
+    stage_compute<N, E, 1, Ra>(v, t, tw, s);
+    if constexpr (Rb > 0) {
+        exchange<N, E, 1, Ra, Rb>(v, t, lds, map);
+        stage_compute<N, E, Ra, Rb>(v, t, tw, s);
+        if constexpr (Rc > 0) {
+            exchange<N, E, Ra, Rb, Rc>(v, t, lds, map);
+            stage_compute<N, E, Ra * Rb, Rc>(v, t, tw, s);
+        }
+    }
+}
+
+template <int N, bool ROWS, class Map>
+__device__ __forceinline__ void fft_forward(double2 (&v)[(FPlan<N, ROWS>::E)], int t, const double2 *tw, double2 *lds,
+                                            const Map &map)
+{
+    using P = FPlan<N, ROWS>;
+    fft_run<N, P::E, P::R0, P::R1, P::R2>(v, t, tw, -1.0, lds, map);
+}
+
+template <int N, bool ROWS, class Map>
+__device__ __forceinline__ void fft_inverse(double2 (&v)[(FPlan<N, ROWS>::E)], int t, const double2 *tw, double2 *lds,
+                                            const Map &map)
+{
+    using P = FPlan<N, ROWS>;
+    if constexpr (P::R2 > 0) fft_run<N, P::E, P::R2, P::R1, P::R0>(v, t, tw, 1.0, lds, map);
+    else if constexpr (P::R1 > 0) fft_run<N, P::E, P::R1, P::R0, 0>(v, t, tw, 1.0, lds, map);
+    else fft_run<N, P::E, P::R0, 0, 0>(v, t, tw, 1.0, lds, map);
+}
+
+template <int N, bool ROWS> struct FirstLast {
+    using P = FPlan<N, ROWS>;
+    static constexpr int FWD_FIRST = P::R0;
+    static constexpr int FWD_LAST = P::R2 > 0 ? P::R2 : (P::R1 > 0 ? P::R1 : P::R0);
+    static constexpr int INV_FIRST = FWD_LAST;
+    static constexpr int INV_LAST = P::R0;
+};
+
+// MODE 0: one transform (sign from DIR: -1 forward, +1 inverse) src -> dst.
+// MODE 1: forward, k-space multiplier, inverse (the last axis).
+// MODE 2: rows only: inverse, epilogue, optional forward of the epilogue output.
+template <int N, bool ROWS, int MODE, int DIR>
+__global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<N, ROWS>::COL_THREADS))
+    k_fast(FastArgs a)
+{
+    using S = FShape<N, ROWS>;
+    using FL = FirstLast<N, ROWS>;
+    constexpr int E = S::E, TL = S::TL;
+    extern __shared__ double2 lds[];
+    const int tid = threadIdx.x;
+    int t;
+    long long gbase, gstride;
+    bool valid;
+    int c;
+    long long colidx = 0;
+    RowMap rmap{0};
+    ColMap cmap{0, S::COLS};
+    if constexpr (ROWS) {
+        const int l = tid / TL;
+        t = tid - l * TL;
+        const long long line = (long long)blockIdx.x * S::LPB + l;
+        valid = line < a.O;
+        gbase = line * N;
+        gstride = 1;
+        c = valid ? (int)(line / a.Opc) : 0;
+        rmap.base = l * S::ROW_STRIDE;
+    } else {
+        const int b = tid % S::COLS;
+        t = tid / S::COLS;
+        const long long tpo = (a.I + S::COLS - 1) / S::COLS;
+        const long long o = (long long)blockIdx.x / tpo;
+        colidx = ((long long)blockIdx.x - o * tpo) * S::COLS + b;
+        valid = colidx < a.I;
+        gbase = colidx + a.I * (long long)N * o;
+        gstride = a.I;
+        c = (int)(o / a.Opc);
+        cmap.b = b;
+    }
+    double2 v[E];
+    // ---- load: inputs of the first stage of the first transform ----
+    constexpr bool first_fwd = (MODE == 1) || (MODE == 0 && DIR < 0);
+    constexpr int R_IN = first_fwd ? FL::FWD_FIRST : FL::INV_FIRST;
+    {
+        const double sc = (a.scale_in && valid) ? a.scale_in[c] : 1.0;
+#pragma unroll
+        for (int i = 0; i < E / R_IN; ++i) {
+#pragma unroll
+            for (int r = 0; r < R_IN; ++r) {
+                const int pos = (t + TL * i) + (N / R_IN) * r;
+                double2 x = valid ? a.src[gbase + gstride * pos] : make_double2(0.0, 0.0);
+                v[i * R_IN + r] = make_double2(x.x * sc, x.y * sc);
+            }
+        }
+    }
+    if constexpr (MODE == 0) {
+        if constexpr (DIR < 0) {
+            if constexpr (ROWS) fft_forward<N, ROWS>(v, t, a.tw, lds, rmap);
+            else fft_forward<N, ROWS>(v, t, a.tw, lds, cmap);
+        } else {
+            if constexpr (ROWS) fft_inverse<N, ROWS>(v, t, a.tw, lds, rmap);
+            else fft_inverse<N, ROWS>(v, t, a.tw, lds, cmap);
+        }
+        constexpr int RL = DIR < 0 ? FL::FWD_LAST : FL::INV_LAST;
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+                for (int k = 0; k < RL; ++k) a.dst[gbase + gstride * ((t + TL * i) + (N / RL) * k)] = v[i * RL + k];
+        }
+        return;
+    }
+    if constexpr (MODE == 1) {
+        if constexpr (ROWS) fft_forward<N, ROWS>(v, t, a.tw, lds, rmap);
+        else fft_forward<N, ROWS>(v, t, a.tw, lds, cmap);
+        // multiplier coef[c] * |k|^2 (* lazy scale); this pass runs on z, so the
+        // column index carries (x, y)
+        const MidArgs &md = a.mid;
+        double kxy2 = 0.0;
+        if constexpr (!ROWS) {
+            const int x = (int)(colidx % md.nx);
+            const int y = (int)(colidx / md.nx);
+            const double kx = momentum(x, md.nx, md.dk[0]);
+            const double ky = momentum(y, md.ny, md.dk[1]);
+            kxy2 = kx * kx + ky * ky;
+        }
+        double f0 = valid ? md.coef[c] : 0.0;
+        if (md.scale && valid) f0 *= md.scale[c];
+        constexpr int RL = FL::FWD_LAST;
+#pragma unroll
+        for (int i = 0; i < E / RL; ++i)
+#pragma unroll
+            for (int k = 0; k < RL; ++k) {
+                const int pos = (t + TL * i) + (N / RL) * k;
+                const double kz = momentum(pos, N, md.dk[2]);
+                v[i * RL + k] = cscale(v[i * RL + k], f0 * (kxy2 + kz * kz));
+            }
+        if constexpr (ROWS) fft_inverse<N, ROWS>(v, t, a.tw, lds, rmap);
+        else fft_inverse<N, ROWS>(v, t, a.tw, lds, cmap);
+        constexpr int RO = FL::INV_LAST;
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < E / RO; ++i)
+#pragma unroll
+                for (int k = 0; k < RO; ++k) a.dst[gbase + gstride * ((t + TL * i) + (N / RO) * k)] = v[i * RO + k];
+        }
+        return;
+    }
+    if constexpr (MODE == 2) {
+        static_assert(ROWS, "the epilogue pass runs on x lines");
+        fft_inverse<N, ROWS>(v, t, a.tw, lds, rmap);
+        double red[kRedQ];
+#pragma unroll
+        for (int q = 0; q < kRedQ; ++q) red[q] = 0.0;
+        constexpr int RL = FL::INV_LAST;  // == FWD_FIRST
+#ifndef GP5_EPI_CHUNK
+#define GP5_EPI_CHUNK 4
+#endif
+        constexpr int CH = GP5_EPI_CHUNK < E ? GP5_EPI_CHUNK : E;
+        static_assert(E % CH == 0, "epilogue chunking");
+        if (valid) {
+#pragma unroll
+            for (int e0 = 0; e0 < E; e0 += CH) {
+                // keep each chunk's loads inside its chunk: hoisting all chunks
+                // at once exhausts the VGPR file and drops occupancy to 1 wave
+                __builtin_amdgcn_sched_barrier(0);
+                int pos[CH];
+                double2 w[CH];
+#pragma unroll
+                for (int u = 0; u < CH; ++u) {
+                    const int e = e0 + u, i = e / RL, k = e % RL;
+                    pos[u] = (t + TL * i) + (N / RL) * k;
+                    w[u] = v[e];
+                }
+                epilogue_batch<CH>(a.epi, gbase, pos, c, w, red);
+#pragma unroll
+                for (int u = 0; u < CH; ++u) v[e0 + u] = w[u];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = make_double2(0.0, 0.0);
+        }
+        if (a.t_out) {
+            fft_forward<N, ROWS>(v, t, a.tw, lds, rmap);
+            constexpr int RF = FL::FWD_LAST;
+            if (valid) {
+#pragma unroll
+                for (int i = 0; i < E / RF; ++i)
+#pragma unroll
+                    for (int k = 0; k < RF; ++k) a.t_out[gbase + (t + TL * i) + (N / RF) * k] = v[i * RF + k];
+            }
+        }
+        if (a.epi.want_red) {
+            __syncthreads();
+            reduce_block(red, a.epi.partials, reinterpret_cast<double *>(lds));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+bool fast_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+template <int N>
+static void fast_shape(bool rows, long long I, long long O, unsigned &grid, unsigned &threads, size_t &lds)
+{
+    if (rows) {
+        using S = FShape<N, true>;
+        grid = (unsigned)((O + S::LPB - 1) / S::LPB);
+        threads = S::ROW_THREADS;
+        lds = sizeof(double2) * (size_t)S::LPB * S::ROW_STRIDE;
+    } else {
+        using S = FShape<N, false>;
+        grid = (unsigned)(((I + S::COLS - 1) / S::COLS) * O);
+        threads = S::COL_THREADS;
+        lds = sizeof(double2) * (size_t)S::COLS * N;
+    }
+    lds = std::max(lds, sizeof(double) * (size_t)kRedScratch);
+}
+
+template <int N>
+static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArgs &a)
+{
+    const bool rows = (a.I == 1);
+    unsigned grid, threads;
+    size_t lds;
+    fast_shape<N>(rows, a.I, a.O, grid, threads, lds);
+    dim3 g(grid), b(threads);
+#define GP5_FAST_LAUNCH(R, M, D) hipLaunchKernelGGL((k_fast<N, R, M, D>), g, b, lds, st, a)
+    if (rows) {
+        if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(true, 0, -1);
+        else if (mode == 0) GP5_FAST_LAUNCH(true, 0, 1);
+        else if (mode == 1) GP5_FAST_LAUNCH(true, 1, -1);
+        else GP5_FAST_LAUNCH(true, 2, 1);
+    } else {
+        if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(false, 0, -1);
+        else if (mode == 0) GP5_FAST_LAUNCH(false, 0, 1);
+        else if (mode == 1) GP5_FAST_LAUNCH(false, 1, -1);
+        else return hipErrorInvalidValue;  // the epilogue pass needs x lines
+    }
+#undef GP5_FAST_LAUNCH
+    return hipGetLastError();
+}
+
+unsigned fast_grid(int n, long long I, long long O)
+{
+    unsigned grid = 0, threads;
+    size_t lds;
+    const bool rows = (I == 1);
+    switch (n) {
+        case 16: fast_shape<16>(rows, I, O, grid, threads, lds); break;
+        case 32: fast_shape<32>(rows, I, O, grid, threads, lds); break;
+        case 64: fast_shape<64>(rows, I, O, grid, threads, lds); break;
+        case 128: fast_shape<128>(rows, I, O, grid, threads, lds); break;
+        case 256: fast_shape<256>(rows, I, O, grid, threads, lds); break;
+        case 512: fast_shape<512>(rows, I, O, grid, threads, lds); break;
+        default: break;
+    }
+    return grid;
+}
+
+hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a)
+{
+    switch (n) {
+        case 16: return launch_fast_n<16>(st, mode, dir, a);
+        case 32: return launch_fast_n<32>(st, mode, dir, a);
+        case 64: return launch_fast_n<64>(st, mode, dir, a);
+        case 128: return launch_fast_n<128>(st, mode, dir, a);
+        case 256: return launch_fast_n<256>(st, mode, dir, a);
+        case 512: return launch_fast_n<512>(st, mode, dir, a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace gp5

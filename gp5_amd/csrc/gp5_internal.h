@@ -69,19 +69,34 @@ struct EpiArgs {
     double2 *ksum;      // RK accumulator
     double2 *out;
     double cr, ci;      // stage coefficient (tau/2, tau or tau/6)
-    const double *phase;
+    const double2 *phase_exp;  // e^{i phase} of phaseConstraint (M*nc) or null
     long long M;
     int nc;
     int nx;
     // fused reductions: q = 0,1 norm of out (c=0,1); 2,3 energy of phi; 4,5 norm of phi
     int want_red;
     double *partials;       // [gridDim][kRedQ]
-    unsigned int *counter;  // zero on entry, reset by the last block
     double *results;        // [kRedQ], final sums times dV
     double *log;            // optional copy of results (per-step log) or null
     double *scale_out;      // if non-null: scale_out[c] = sqrt(N[c] / norm_out[c])
     double Nnorm[2];
     double dV;
 };
+
+// Arguments of the register-resident passes (gp5_fast.hip).
+struct FastArgs {
+    const double2 *src;
+    double2 *dst;
+    const double2 *tw;
+    long long I, O, Opc;   // axis geometry as in AxisGeom
+    const double *scale_in;
+    MidArgs mid;
+    EpiArgs epi;
+    double2 *t_out;  // epilogue pass: forward x FFT of its output goes here (null: skip)
+};
+
+bool fast_supported(int n);
+unsigned fast_grid(int n, long long I, long long O);
+hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a);
 
 }  // namespace gp5

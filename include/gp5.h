@@ -138,6 +138,17 @@ int gp5_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_de
 int gp5_advance_timed(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
                       gp5_buf *psi, int nsteps, double *elapsed_ms);
 
+/* Per-kernel device time of the hot loop (the role of the reference's
+ * START_TIMER/STOP_TIMER labels, src/timers.h:69-113, at kernel granularity):
+ * runs nsteps exactly as gp5_advance does, with a HIP event pair on the
+ * context's stream around every kernel launch, and accumulates per kernel
+ * kind.  kind_ms[k] / kind_launches[k] is the average launch duration of kind k
+ * (k < GP5_NKINDS, named by gp5_kernel_kind_name). */
+#define GP5_NKINDS 8
+int gp5_profile_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
+                        gp5_buf *psi, int nsteps, double *kind_ms, int *kind_launches);
+const char *gp5_kernel_kind_name(int kind);
+
 #ifdef __cplusplus
 }
 #endif

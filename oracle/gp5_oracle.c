@@ -85,6 +85,54 @@ static void or_stockham(const double *a, double *b, int n, int L, int R, int sig
         hi = hr + R;
         vr = hr; vi = hi;
     }
+    if (R == 2 || R == 4) {
+        /* radix 2 / 4 butterflies written out (same algebra as the direct DFT
+         * below, with w_4 = -i forward, +i backward) */
+        for (int j = 0; j < L; ++j) {
+            double wr[4] = {1, 0, 0, 0}, wi[4] = {0, 0, 0, 0};
+            for (int r = 1; r < R; ++r) {
+                size_t t = ((size_t)j * (size_t)r * (size_t)m) % (size_t)n;
+                wr[r] = tw[2 * t];
+                wi[r] = sign > 0 ? -tw[2 * t + 1] : tw[2 * t + 1];
+            }
+            for (int q = 0; q < m; ++q) {
+                double ar[4], ai[4];
+                for (int r = 0; r < R; ++r) {
+                    size_t idx = (size_t)q + (size_t)m * ((size_t)r + (size_t)R * j);
+                    double xr = a[2 * idx], xi = a[2 * idx + 1];
+                    ar[r] = xr * wr[r] - xi * wi[r];
+                    ai[r] = xr * wi[r] + xi * wr[r];
+                }
+                if (R == 2) {
+                    size_t o0 = (size_t)q + (size_t)m * (size_t)j;
+                    size_t o1 = (size_t)q + (size_t)m * ((size_t)j + (size_t)L);
+                    b[2 * o0] = ar[0] + ar[1];
+                    b[2 * o0 + 1] = ai[0] + ai[1];
+                    b[2 * o1] = ar[0] - ar[1];
+                    b[2 * o1 + 1] = ai[0] - ai[1];
+                } else {
+                    double s = sign > 0 ? 1.0 : -1.0; /* w_4 = s*i */
+                    double t0r = ar[0] + ar[2], t0i = ai[0] + ai[2];
+                    double t1r = ar[0] - ar[2], t1i = ai[0] - ai[2];
+                    double t2r = ar[1] + ar[3], t2i = ai[1] + ai[3];
+                    double dr = ar[1] - ar[3], di = ai[1] - ai[3];
+                    double t3r = -s * di, t3i = s * dr; /* (s i) * d */
+                    size_t o[4];
+                    for (int k = 0; k < 4; ++k) o[k] = (size_t)q + (size_t)m * ((size_t)j + (size_t)L * k);
+                    b[2 * o[0]] = t0r + t2r;
+                    b[2 * o[0] + 1] = t0i + t2i;
+                    b[2 * o[2]] = t0r - t2r;
+                    b[2 * o[2] + 1] = t0i - t2i;
+                    b[2 * o[1]] = t1r + t3r;
+                    b[2 * o[1] + 1] = t1i + t3i;
+                    b[2 * o[3]] = t1r - t3r;
+                    b[2 * o[3] + 1] = t1i - t3i;
+                }
+            }
+        }
+        free(hr);
+        return;
+    }
     for (int j = 0; j < L; ++j) {
         for (int q = 0; q < m; ++q) {
             /* gather the R inputs of this butterfly: a[q + m*(j + L*r)]?  We use

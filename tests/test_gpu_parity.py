@@ -26,7 +26,8 @@ def _ctx(C, shape, left, right, nc=1):
 
 
 FFT_CASES = [((16, 16, 16), 1), ((8, 12, 10), 2), ((30, 30, 30), 1), ((7, 11, 17), 1), ((64, 1, 5), 1),
-             ((96, 24, 40), 1)]
+             ((96, 24, 40), 1), ((256, 16, 8), 1), ((512, 4, 8), 2), ((32, 128, 64), 1), ((20, 64, 512), 1),
+             ((16, 256, 32), 2), ((128, 3, 256), 1)]
 
 
 @pytest.mark.parametrize("shape,nc", FFT_CASES)
@@ -180,10 +181,11 @@ STEP_CASES = [
 ]
 
 
+@pytest.mark.parametrize("shape", [(24, 20, 16), (32, 16, 20)], ids=["generic_x", "fused_x"])
 @pytest.mark.parametrize("name,imag,renorm,case", STEP_CASES)
-def test_advance_matches_oracle(C, oracle_mod, name, imag, renorm, case):
-    print(f"advance {name} imag={imag} renorm={renorm} {case}", flush=True)
-    shape, left, right = (24, 20, 16), (-5.0, -5.0, -4.0), (5.0, 5.0, 4.0)
+def test_advance_matches_oracle(C, oracle_mod, name, imag, renorm, case, shape):
+    print(f"advance {name} imag={imag} renorm={renorm} {case} {shape}", flush=True)
+    left, right = (-5.0, -5.0, -4.0), (5.0, 5.0, 4.0)
     nc, gspec, ospec = _functional_cases(oracle_mod, shape, left, right)[case]
     a = _smooth(shape, left, right, nc, seed=31)
     dt, nsteps = 2e-3, 12
