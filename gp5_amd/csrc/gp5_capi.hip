@@ -21,8 +21,8 @@
 
 namespace gp5 {
 hipError_t launch_axis(hipStream_t st, int mode, const double2 *src, double2 *dst, const AxisPlan &P,
-                       const AxisGeom &G, int sign, const double *scale_in, long long Opc, const MidArgs &mid,
-                       const EpiArgs &epi);
+                       const AxisGeom &G, int sign, const double *scale_in, const LoadPre &pre, long long Opc,
+                       const MidArgs &mid, const EpiArgs &epi);
 unsigned axis_grid(const AxisGeom &G);
 hipError_t launch_norm(hipStream_t st, const double2 *f, long long len, unsigned blocks, const EpiArgs &E);
 hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp);
@@ -55,9 +55,13 @@ struct gp5_ctx {
     long long Opc[3];
     double2 *tw[3];
     // scratch fields (allocated on first use)
-    double2 *T, *phiA, *phiB, *ksum;
+    double2 *T;          // x/y/z transforms of the current stage input
+    double2 *kA, *kB;    // RK4: K = k_s, alternating between the two
+    double2 *sbuf;       // RK4: S = k1 + 2 k2
+    double2 *eout;       // new state, swapped with the caller's buffer after each step
     double2 *phase_ex;   // e^{i phase} of the stepper's phase constraint
     bool fuse_x;         // epilogue pass also forward-transforms its output along x
+    int mall_chunks;     // x slabs of the y/z passes (1 = whole field per pass)
     // reduction scratch
     double *partials;
     size_t partials_cap;  // blocks
@@ -198,23 +202,25 @@ static int ensure_partials(gp5_ctx *c, size_t blocks)
     return GP5_OK;
 }
 
-static int ensure_scratch(gp5_ctx *c, bool rk)
+// Scratch fields: T always; a stepper adds the output field, RK4 also K (x2) and S.
+static int ensure_scratch(gp5_ctx *c, bool rk, bool stepper)
 {
     const size_t bytes = (size_t)c->M * c->nc * sizeof(double2);
-    if (!c->T) {
-        if (hipMalloc(&c->T, bytes) != hipSuccess) { c->T = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
-    }
-    if (!c->phiA) {
-        if (hipMalloc(&c->phiA, bytes) != hipSuccess) { c->phiA = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
-    }
+    auto need = [&](double2 *&p) -> int {
+        if (p) return GP5_OK;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes);
+        }
+        return GP5_OK;
+    };
+    RCHK(need(c->T));
     if (rk) {
-        if (!c->phiB) {
-            if (hipMalloc(&c->phiB, bytes) != hipSuccess) { c->phiB = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
-        }
-        if (!c->ksum) {
-            if (hipMalloc(&c->ksum, bytes) != hipSuccess) { c->ksum = nullptr; return fail(GP5_ERR_ALLOC, "cannot allocate %zu B of scratch", bytes); }
-        }
+        RCHK(need(c->kA));
+        RCHK(need(c->kB));
+        RCHK(need(c->sbuf));
     }
+    if (stepper) RCHK(need(c->eout));
     return GP5_OK;
 }
 
@@ -307,48 +313,110 @@ static FastArgs fast_args(gp5_ctx *c, int axis, const double2 *src, double2 *dst
     a.I = c->geom[axis].I;
     a.O = c->geom[axis].O;
     a.Opc = c->Opc[axis];
+    a.nx = c->n[0];
     return a;
 }
 
-// One FFT pass along `axis` (sign -1 forward, +1 inverse), src -> dst.
-static int pass_plain(gp5_ctx *c, int kind, int axis, const double2 *src, double2 *dst, int sign, const double *scale_in)
+static int pass_plain(gp5_ctx *c, int kind, int axis, const double2 *src, double2 *dst, int sign,
+                      const double *scale_in, const LoadPre *pre);
+
+// y forward, z forward * multiplier * z inverse, y inverse on T.  With
+// register kernels on y and z the three passes run slab by slab over x
+// (mall_chunks slabs): a slab written by one pass is read back by the next
+// while it is still resident in the memory-side Infinity Cache, so HBM sees
+// each slab read once and written once instead of three times each.
+static int middle_passes(gp5_ctx *c, const MidArgs &mid)
 {
+    const int nx = c->n[0];
+    const int nch = (c->fast[1] && c->fast[2]) ? c->mall_chunks : 1;
+    if (nch > 1) {
+        int step = (nx + nch - 1) / nch;
+        step = (step + 15) / 16 * 16;
+        for (int x0 = 0; x0 < nx; x0 += step) {
+            const int xc = std::min(step, nx - x0);
+            FastArgs ay = fast_args(c, 1, c->T, c->T);
+            ay.x0 = x0;
+            ay.xcount = xc;
+            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, -1, ay));
+            FastArgs az = fast_args(c, 2, c->T, c->T);
+            az.x0 = x0;
+            az.xcount = xc;
+            az.mid = mid;
+            PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, az));
+            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, +1, ay));
+        }
+        return GP5_OK;
+    }
+    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, -1, nullptr, nullptr));
+    if (c->fast[2]) {
+        FastArgs a = fast_args(c, 2, c->T, c->T);
+        a.mid = mid;
+        PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, a));
+    } else {
+        EpiArgs none;
+        memset(&none, 0, sizeof none);
+        PLAUNCH(c, K_Z_MID, launch_axis(c->st, 1, c->T, c->T, c->plan[2], c->geom[2], -1, nullptr,
+                                        LoadPre{nullptr, 0.0, 0.0}, c->Opc[2], mid, none));
+    }
+    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, +1, nullptr, nullptr));
+    return GP5_OK;
+}
+
+// One FFT pass along `axis` (sign -1 forward, +1 inverse), src -> dst, loading
+// src * scale_in - cf * src2 when `pre` carries src2.
+static int pass_plain(gp5_ctx *c, int kind, int axis, const double2 *src, double2 *dst, int sign,
+                      const double *scale_in, const LoadPre *pre)
+{
+    LoadPre none_pre = {nullptr, 0.0, 0.0};
+    if (!pre) pre = &none_pre;
     if (c->fast[axis]) {
         FastArgs a = fast_args(c, axis, src, dst);
         a.scale_in = scale_in;
+        a.src2 = pre->src2;
+        a.cfr = pre->cfr;
+        a.cfi = pre->cfi;
         PLAUNCH(c, kind, launch_fast(c->st, c->n[axis], 0, sign, a));
     } else {
         MidArgs mid;
         memset(&mid, 0, sizeof mid);
         EpiArgs none;
         memset(&none, 0, sizeof none);
-        PLAUNCH(c, kind, launch_axis(c->st, 0, src, dst, c->plan[axis], c->geom[axis], sign, scale_in, c->Opc[axis], mid, none));
+        PLAUNCH(c, kind, launch_axis(c->st, 0, src, dst, c->plan[axis], c->geom[axis], sign, scale_in, *pre,
+                                     c->Opc[axis], mid, none));
     }
     return GP5_OK;
 }
 
-// Passes of one H application.  Plain mode: x forward of src (scaled by
-// in_scale), y, z (forward * multiplier * inverse), y inverse, x inverse +
-// epilogue.  Fused mode (x axis on the register kernels): T already holds the
-// raw x transform of the stage input whose lazy scale is in_scale, the scale is
-// applied in the z pass, and the epilogue pass leaves the x transform of its own
-// output in T for the next stage.
-static int stage_passes(gp5_ctx *c, const double2 *src, const double *in_scale, const double coef[2],
-                        const EpiArgs &E, bool fused)
+// Input of one H application: what an unfused first x pass loads
+// (src * src_scale - cf * src2) and the lazy scale the z pass applies.
+struct StageIn {
+    const double2 *src;
+    const double *src_scale;
+    LoadPre pre;
+    const double *mid_scale;
+};
+
+static StageIn plain_input(const double2 *src)
 {
-    MidArgs mid = make_mid(c, coef, fused ? in_scale : nullptr);
-    EpiArgs none;
-    memset(&none, 0, sizeof none);
-    if (!fused) RCHK(pass_plain(c, K_X_FWD, 0, src, c->T, -1, in_scale));
-    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, -1, nullptr));
-    if (c->fast[2]) {
-        FastArgs a = fast_args(c, 2, c->T, c->T);
-        a.mid = mid;
-        PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, a));
-    } else {
-        PLAUNCH(c, K_Z_MID, launch_axis(c->st, 1, c->T, c->T, c->plan[2], c->geom[2], -1, nullptr, c->Opc[2], mid, none));
-    }
-    RCHK(pass_plain(c, K_Y, 1, c->T, c->T, +1, nullptr));
+    StageIn in;
+    in.src = src;
+    in.src_scale = nullptr;
+    in.pre = {nullptr, 0.0, 0.0};
+    in.mid_scale = nullptr;
+    return in;
+}
+
+// Passes of one H application: x forward, y, z (forward * multiplier *
+// inverse), y inverse, x inverse + epilogue.  Fused mode (x axis on the
+// register kernels): T already holds the raw x transform of the stage input,
+// and the epilogue pass leaves the x transform of its own output in T for the
+// next stage, so the x forward pass is skipped.  Either way T carries the raw
+// transform and the z pass applies in.mid_scale.
+static int stage_passes(gp5_ctx *c, const StageIn &in, const double coef[2], const EpiArgs &E, bool fused)
+{
+    MidArgs mid = make_mid(c, coef, in.mid_scale);
+    if (!fused) RCHK(pass_plain(c, K_X_FWD, 0, in.src, c->T, -1, in.src_scale, &in.pre));
+    RCHK(middle_passes(c, mid));
     unsigned nblocks;
     EpiArgs e = E;
     if (c->fast[0]) {
@@ -364,15 +432,16 @@ static int stage_passes(gp5_ctx *c, const double2 *src, const double *in_scale, 
         nblocks = axis_grid(c->geom[0]);
         RCHK(ensure_partials(c, nblocks));
         e.partials = c->partials;
-        PLAUNCH(c, K_X_EPI, launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr, c->Opc[0], mid, e));
+        PLAUNCH(c, K_X_EPI, launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr,
+                                        LoadPre{nullptr, 0.0, 0.0}, c->Opc[0], mid, e));
     }
     if (e.want_red) PLAUNCH(c, K_NORM, launch_finalize(c->st, e, nblocks));
     return GP5_OK;
 }
 
-static int spectral(gp5_ctx *c, const double2 *src, const double *scale_src, const double coef[2], const EpiArgs &E)
+static int spectral(gp5_ctx *c, const double2 *src, const double coef[2], const EpiArgs &E)
 {
-    return stage_passes(c, src, scale_src, coef, E, false);
+    return stage_passes(c, plain_input(src), coef, E, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -426,6 +495,15 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
         // GP5_NO_FUSE=1 runs the x forward transform as its own pass (A/B timing)
         const char *nf = getenv("GP5_NO_FUSE");
         c->fuse_x = !(nf && nf[0] == '1');
+        // x slabs for the y/z passes: off by default.  Measured at 256^3 on
+        // MI355X, 2/4/8/16 slabs of the three passes were slower in sum than
+        // whole-field passes (the Infinity Cache did not return the slab
+        // bandwidth, and the smaller launches lose tail efficiency);
+        // GP5_MALL_CHUNKS=n turns it on for experiments.
+        int chunks = 1;
+        const char *mc = getenv("GP5_MALL_CHUNKS");
+        if (mc && mc[0]) chunks = atoi(mc);
+        c->mall_chunks = std::max(1, std::min(chunks, shape[0] / 16));
         AxisGeom &G = c->geom[d];
         G.I = I;
         G.O = c->M / (I * shape[d]) * nc;
@@ -455,7 +533,7 @@ int gp5_ctx_destroy(gp5_ctx *c)
     auto note = [&](hipError_t e) { if (first == hipSuccess) first = e; };
     if (c->st) note(hipStreamSynchronize(c->st));
     for (int d = 0; d < 3; ++d) if (c->tw[d]) note(hipFree(c->tw[d]));
-    void *ptrs[] = {c->T,       c->phiA,    c->phiB,   c->ksum,   c->phase_ex, c->partials,
+    void *ptrs[] = {c->T,       c->kA,      c->kB,     c->sbuf,   c->eout,     c->phase_ex, c->partials,
                     c->results, c->scales, c->blockmax, c->dlog};
     for (void *p : ptrs) if (p) note(hipFree(p));
     if (c->ev0) note(hipEventDestroy(c->ev0));
@@ -556,9 +634,9 @@ int gp5_fft(gp5_ctx *c, const gp5_buf *src, gp5_buf *dst, int direction)
     else if (direction == GP5_BACKWARD) sign = +1;
     else return fail(GP5_ERR_ARG, "Unkown direction in fourier transform");
     double2 *d = (double2 *)dst->d;
-    RCHK(pass_plain(c, K_FFT, 0, (const double2 *)src->d, d, sign, nullptr));
-    RCHK(pass_plain(c, K_FFT, 1, d, d, sign, nullptr));
-    RCHK(pass_plain(c, K_FFT, 2, d, d, sign, nullptr));
+    RCHK(pass_plain(c, K_FFT, 0, (const double2 *)src->d, d, sign, nullptr, nullptr));
+    RCHK(pass_plain(c, K_FFT, 1, d, d, sign, nullptr, nullptr));
+    RCHK(pass_plain(c, K_FFT, 2, d, d, sign, nullptr, nullptr));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -568,12 +646,12 @@ int gp5_laplacian(gp5_ctx *c, const gp5_buf *src, gp5_buf *dst)
     if (!c) return fail(GP5_ERR_ARG, "NULL context");
     RCHK(check_field(c, src, "src"));
     RCHK(check_field(c, dst, "dst"));
-    RCHK(ensure_scratch(c, false));
+    RCHK(ensure_scratch(c, false, false));
     EpiArgs E = base_epi(c);
     E.mode = EPI_STORE;
     E.out = (double2 *)dst->d;
     double coef[2] = {-1.0 / (double)c->M, -1.0 / (double)c->M};
-    RCHK(spectral(c, (const double2 *)src->d, nullptr, coef, E));
+    RCHK(spectral(c, (const double2 *)src->d, coef, E));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -585,14 +663,14 @@ int gp5_functional_apply(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf
     RCHK(check_field(c, src, "src"));
     RCHK(check_field(c, dst, "dst"));
     if (src == dst || src->d == dst->d) return fail(GP5_ERR_ARG, "functional::apply needs distinct source and destination");
-    RCHK(ensure_scratch(c, false));
+    RCHK(ensure_scratch(c, false, false));
     EpiArgs E = base_epi(c);
     double coef[2];
     RCHK(fill_functional(c, f, E, coef));
     E.mode = EPI_H;
     E.phi = (const double2 *)src->d;
     E.out = (double2 *)dst->d;
-    RCHK(spectral(c, (const double2 *)src->d, nullptr, coef, E));
+    RCHK(spectral(c, (const double2 *)src->d, coef, E));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -661,14 +739,14 @@ int gp5_energy(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf *psi, dou
     RCHK(check_field(c, psi, "psi"));
     if (f && f->kind == GP5_FUNC_LHY_UNLOCKED)
         return fail(GP5_ERR_UNSUPPORTED, "the unlocked LHY functional has no energy for general alpha, beta, eta");
-    RCHK(ensure_scratch(c, false));
+    RCHK(ensure_scratch(c, false, false));
     EpiArgs E = base_epi(c);
     double coef[2];
     RCHK(fill_functional(c, f, E, coef));
     E.mode = EPI_ENERGY;
     E.phi = (const double2 *)psi->d;
     E.want_red = 1;
-    RCHK(spectral(c, (const double2 *)psi->d, nullptr, coef, E));
+    RCHK(spectral(c, (const double2 *)psi->d, coef, E));
     double r[kRedQ];
     HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -687,13 +765,13 @@ int gp5_phase_constraint(gp5_ctx *c, gp5_buf *psi, const gp5_buf *phase)
 }
 
 // One step of Euler or RK4 on c's stream.  psi_slot: scale slot of psi (-1 = none).
+// `last`: final step of this advance (its norm feeds the non-finite check).
 static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
-                     int &psi_slot, int step, double *dlog, bool fused)
+                     int &psi_slot, int step, double *dlog, bool fused, bool last)
 {
     EpiArgs E = base_epi(c);
     double coef[2];
     RCHK(fill_functional(c, f, E, coef));
-    E.want_red = 1;
     E.Nnorm[0] = s->normalizations[0];
     E.Nnorm[1] = s->normalizations[1];
     if (s->phase) E.phase_exp = c->phase_ex;  // filled by advance_impl
@@ -703,62 +781,82 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     double *slotA = c->scales + 4, *slotB = c->scales + 6;
     double2 *P = (double2 *)psi->d;
     const double tr = s->tau_re, ti = s->tau_im;
+    // reductions only where a result is consumed: lazy renormalisation scales,
+    // the per-step log (first stage), and the non-finite check after the last step
+    E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
+    // stage 1 input: psi itself; its lazy scale is applied in the z pass
+    StageIn in = plain_input(P);
+    in.mid_scale = slot_psi;
     if (s->kind == GP5_STEP_EULERO) {
+        // euleroStepper::advance (src/stepper.cpp:30-43)
         E.mode = EPI_EULER;
         E.phi = P;
         E.sc_phi = slot_psi;
         E.cr = tr;
         E.ci = ti;
-        E.out = c->phiA;
+        E.out = c->eout;
         E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
-        E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
-        RCHK(stage_passes(c, P, slot_psi, coef, E, fused));
-        // the new state lives in phiA: swap device pointers with the caller's buffer
-        double2 *fresh = c->phiA;
-        c->phiA = (double2 *)psi->d;
+        E.want_red = rn || dlog || last;
+        RCHK(stage_passes(c, in, coef, E, fused));
+        // the new state lives in eout: swap device pointers with the caller's buffer
+        double2 *fresh = c->eout;
+        c->eout = (double2 *)psi->d;
         psi->d = (double *)fresh;
         psi_slot = rn ? next_psi_slot : -1;
         return GP5_OK;
     }
-    // RK4 (src/stepper.cpp:46-70)
-    E.mode = EPI_RK1;
-    E.phi = P;
-    E.sc_phi = slot_psi;
-    E.cr = tr * 0.5;
-    E.ci = ti * 0.5;
-    E.ksum = c->ksum;
-    E.out = c->phiA;
-    E.scale_out = rn ? slotA : nullptr;
-    E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
-    RCHK(stage_passes(c, P, slot_psi, coef, E, fused));
-    E.log = nullptr;
-
-    E.mode = EPI_RK2;
-    E.phi = c->phiA;
-    E.sc_phi = rn ? slotA : nullptr;
+    // RK4Stepper::advance (src/stepper.cpp:46-70); see EpiMode for the K/S form
     E.psi = P;
     E.sc_psi = slot_psi;
-    E.out = c->phiB;
-    E.scale_out = rn ? slotB : nullptr;
-    RCHK(stage_passes(c, c->phiA, E.sc_phi, coef, E, fused));
+    E.sbuf = c->sbuf;
+    E.mode = EPI_RK1;
+    E.kout = c->kA;  // k1
+    E.cr = tr * 0.5;
+    E.ci = ti * 0.5;
+    E.scale_out = rn ? slotA : nullptr;
+    E.want_red = rn || dlog;
+    RCHK(stage_passes(c, in, coef, E, fused));
+    E.log = nullptr;
+    E.want_red = rn;
 
-    E.mode = EPI_RK3;
-    E.phi = c->phiB;
-    E.sc_phi = rn ? slotB : nullptr;
+    // later stages: input u_s = psi - c_in K (psi scaled), renormalised by sc_in
+    in.src = P;
+    in.src_scale = slot_psi;
+
+    E.mode = EPI_RK2;  // u2 = psi - (tau/2) k1
+    E.kbuf = in.pre.src2 = c->kA;
+    E.kout = c->kB;  // k2
+    E.cr_in = in.pre.cfr = tr * 0.5;
+    E.ci_in = in.pre.cfi = ti * 0.5;
+    E.sc_in = in.mid_scale = rn ? slotA : nullptr;
+    E.scale_out = rn ? slotB : nullptr;
+    RCHK(stage_passes(c, in, coef, E, fused));
+
+    E.mode = EPI_RK3;  // u3 = psi - (tau/2) k2
+    E.kbuf = in.pre.src2 = c->kB;
+    E.kout = c->kA;  // k3
+    E.sc_in = in.mid_scale = rn ? slotB : nullptr;
     E.cr = tr;
     E.ci = ti;
-    E.out = c->phiA;
     E.scale_out = rn ? slotA : nullptr;
-    RCHK(stage_passes(c, c->phiB, E.sc_phi, coef, E, fused));
+    RCHK(stage_passes(c, in, coef, E, fused));
 
-    E.mode = EPI_RK4;
-    E.phi = c->phiA;
-    E.sc_phi = rn ? slotA : nullptr;
+    E.mode = EPI_RK4;  // u4 = psi - tau k3
+    E.kbuf = in.pre.src2 = c->kA;
+    E.kout = nullptr;
+    E.cr_in = in.pre.cfr = tr;
+    E.ci_in = in.pre.cfi = ti;
+    E.sc_in = in.mid_scale = rn ? slotA : nullptr;
     E.cr = (1. / 6) * tr;
     E.ci = (1. / 6) * ti;
-    E.out = P;
+    E.out = c->eout;
     E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
-    RCHK(stage_passes(c, c->phiA, E.sc_phi, coef, E, fused));
+    E.want_red = rn || last;
+    RCHK(stage_passes(c, in, coef, E, fused));
+    // psi' lives in eout: swap device pointers with the caller's buffer
+    double2 *fresh = c->eout;
+    c->eout = (double2 *)psi->d;
+    psi->d = (double *)fresh;
     psi_slot = rn ? next_psi_slot : -1;
     return GP5_OK;
 }
@@ -775,7 +873,7 @@ static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
             if (!(s->normalizations[k] > 0.0)) return fail(GP5_ERR_ARG, "normalization %d must be > 0", k);
     }
     if (nsteps == 0) return GP5_OK;
-    RCHK(ensure_scratch(c, s->kind == GP5_STEP_RK4));
+    RCHK(ensure_scratch(c, s->kind == GP5_STEP_RK4, true));
     if (s->phase) {
         RCHK(check_real(c, s->phase, "phase"));
         if (!c->phase_ex) HIPCHK(hipMalloc(&c->phase_ex, (size_t)c->M * c->nc * sizeof(double2)));
@@ -794,9 +892,9 @@ static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     // fused stages: the epilogue pass of each stage also forward-transforms its
     // output along x, so only the first stage needs a separate x pass
     const bool fused = c->fast[0] && c->fuse_x;
-    if (fused) RCHK(pass_plain(c, K_X_FWD, 0, (const double2 *)psi->d, c->T, -1, nullptr));
+    if (fused) RCHK(pass_plain(c, K_X_FWD, 0, (const double2 *)psi->d, c->T, -1, nullptr, nullptr));
     for (int st = 0; st < nsteps; ++st)
-        RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr, fused));
+        RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr, fused, st == nsteps - 1));
     if (psi_slot >= 0)
         PLAUNCH(c, K_SCALE, launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, c->scales + 2 * psi_slot));
     if (elapsed_ms) HIPCHK(hipEventRecord(c->ev1, c->st));

@@ -148,8 +148,7 @@ __device__ __forceinline__ double momentum(int i, int n, double dk)
 }
 
 // ---------------------------------------------------------------------------
-// block reduction of kRedQ partial sums; the last block to finish sums the
-// per-block partials in a fixed order (deterministic) and publishes results.
+// block reduction of kRedQ partial sums
 // ---------------------------------------------------------------------------
 // Blocks are multiples of 64 threads, at most 512; `sred` is LDS scratch of at
 // least kRedScratch doubles.  Each block writes its kRedQ partial sums (wave
@@ -191,19 +190,34 @@ __device__ __forceinline__ void reduce_block(double (&v)[kRedQ], double *partial
 // ---------------------------------------------------------------------------
 // epilogue: one element of the last inverse pass
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 load_scaled(const double2 *p, long long g, const double *sc, int c)
+// v = src[g] * sc[c] - cf * src2[g]: the pre-operation of a first (x) pass
+// (terms absent when the pointer is null)
+__device__ __forceinline__ double2 load_pre(const double2 *src, long long g, const double *sc, int c,
+                                            const double2 *src2, double2 cf)
 {
-    double2 v = p[g];
-    if (sc) { double s = sc[c]; v.x *= s; v.y *= s; }
+    double2 v = src[g];
+    if (sc) {
+        const double s = sc[c];
+        v.x *= s;
+        v.y *= s;
+    }
+    if (src2) v = csub(v, cmul(cf, src2[g]));
     return v;
 }
 
-// Batched epilogue over CH elements of one line (one component c) at
-// gbase + pos[k]: every global load of the batch is issued before the
-// arithmetic, with the mode / functional tests uniform across the wave, so each
-// operand stream has CH independent requests in flight per thread.  On return
-// v[k] holds the value written to E.out (the kinetic term for EPI_STORE, H for
-// EPI_H, the new state for the stepping modes).  Same arithmetic as epilogue().
+// Epilogue of the last inverse x pass over CH elements of one line (one
+// component c) at gbase + pos[k].  Every global load of the batch is issued
+// before the arithmetic, with the mode / functional tests uniform across the
+// wave, so each operand stream has CH independent requests in flight per
+// thread.  kin = -1/2 m_c^-1 lap(phi) arrives in v[k]; on return v[k] holds the
+// value the pass hands on: the kinetic term (STORE), H (H, ENERGY), or o.
+//
+// Functionals (src/functional.cpp): H phi = kin + (V + nl) phi with
+//   LHYDroplet (:105-122)         nl = a rho + b rho^{3/2}
+//   gpFunctional (:13-43,54-89)   nl = g11 rho (1 comp), gcc rho_c + g12 rho_o (2 comps)
+//   UnlockedLHYDroplet (:125-150) nl_0 = rho1 + eta rho2 + alpha (rho1 + beta rho2)^{3/2},
+//                                 nl_1 = eta beta rho1 + beta rho2 + alpha beta^2 (...)^{3/2}
+// Reductions: red[c] |o|^2, red[2+c] energy density of phi, red[4+c] |phi|^2.
 template <int CH>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase, const int (&pos)[CH], int c,
                                                double2 (&v)[CH], double (&red)[kRedQ])
@@ -214,18 +228,23 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
         for (int k = 0; k < CH; ++k) E.out[gbase + pos[k]] = v[k];
         return;
     }
-    const bool rk_mid = mode == EPI_RK2 || mode == EPI_RK3 || mode == EPI_RK4;
+    const bool rk = mode >= EPI_RK1 && mode <= EPI_RK4;
+    const bool rk_k = mode == EPI_RK2 || mode == EPI_RK3 || mode == EPI_RK4;
     const bool coupled = E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2);
     const bool constrain = E.phase_exp && (mode == EPI_EULER || mode == EPI_RK4);
-    double2 phi[CH], psi[CH], ks[CH], po[CH], ph[CH];
+    const double2 *src = rk ? E.psi : E.phi;
+    const double *ssc = rk ? E.sc_psi : E.sc_phi;
+    double2 x[CH], kk[CH], ss[CH], xo[CH], ko[CH], ph[CH];
     double V[CH];
 #pragma unroll
-    for (int k = 0; k < CH; ++k) phi[k] = E.phi[gbase + pos[k]];
-    if (rk_mid) {
+    for (int k = 0; k < CH; ++k) x[k] = src[gbase + pos[k]];
+    if (rk_k) {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) psi[k] = E.psi[gbase + pos[k]];
+        for (int k = 0; k < CH; ++k) kk[k] = E.kbuf[gbase + pos[k]];
+    }
+    if (mode == EPI_RK4) {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) ks[k] = E.ksum[gbase + pos[k]];
+        for (int k = 0; k < CH; ++k) ss[k] = E.sbuf[gbase + pos[k]];
     }
     if (E.V) {
 #pragma unroll
@@ -237,20 +256,28 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
     if (coupled) {
         const long long off = c == 0 ? E.M : -E.M;
 #pragma unroll
-        for (int k = 0; k < CH; ++k) po[k] = E.phi[gbase + off + pos[k]];
+        for (int k = 0; k < CH; ++k) xo[k] = src[gbase + off + pos[k]];
+        if (rk_k) {
+#pragma unroll
+            for (int k = 0; k < CH; ++k) ko[k] = E.kbuf[gbase + off + pos[k]];
+        }
     }
     if (constrain) {
 #pragma unroll
         for (int k = 0; k < CH; ++k) ph[k] = E.phase_exp[gbase + pos[k]];
     }
-    const double sphi = E.sc_phi ? E.sc_phi[c] : 1.0;
-    const double soth = (coupled && E.sc_phi) ? E.sc_phi[1 - c] : 1.0;
-    const double spsi = (rk_mid && E.sc_psi) ? E.sc_psi[c] : 1.0;
+    const double sx = ssc ? ssc[c] : 1.0;
+    const double sxo = (coupled && ssc) ? ssc[1 - c] : 1.0;
+    const double sin_ = (rk_k && E.sc_in) ? E.sc_in[c] : 1.0;
+    const double sino = (rk_k && coupled && E.sc_in) ? E.sc_in[1 - c] : 1.0;
+    const double2 cin = make_double2(E.cr_in, E.ci_in);
     const double2 cf = make_double2(E.cr, E.ci);
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
         const long long g = gbase + pos[k];
-        const double2 p = cscale(phi[k], sphi);
+        const double2 xs = cscale(x[k], sx);  // psi (RK) or phi
+        // stage input phi: psi for RK1, norm(u_s) = sc_in (psi - c_in K) for RK2-4
+        const double2 p = rk_k ? cscale(csub(xs, cmul(cin, kk[k])), sin_) : xs;
         const double2 kin = v[k];
         const double rho = p.x * p.x + p.y * p.y;
         double nl = 0.0, enl = 0.0;
@@ -258,27 +285,25 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
             const double sr = sqrt(rho);
             nl = E.lhy_a * rho + E.lhy_b * rho * sr;
             enl = 0.5 * E.lhy_a * rho * rho + 0.4 * E.lhy_b * rho * rho * sr;
-        } else if (E.fkind == 0) {
-            if (E.has_couplings) {
-                if (E.nc == 1) {
-                    nl = E.g11 * rho;
-                    enl = 0.5 * E.g11 * rho * rho;
-                } else {
-                    const double2 q = cscale(po[k], soth);
-                    const double ro = q.x * q.x + q.y * q.y;
-                    const double gcc = c == 0 ? E.g11 : E.g22;
-                    nl = gcc * rho + E.g12 * ro;
-                    enl = 0.5 * gcc * rho * rho + 0.5 * E.g12 * rho * ro;
-                }
-            }
+        } else if (E.fkind == 0 && !E.has_couplings) {
+        } else if (E.fkind == 0 && E.nc == 1) {
+            nl = E.g11 * rho;
+            enl = 0.5 * E.g11 * rho * rho;
         } else {
-            const double2 q = cscale(po[k], soth);
+            const double2 xso = cscale(xo[k], sxo);
+            const double2 q = rk_k ? cscale(csub(xso, cmul(cin, ko[k])), sino) : xso;
             const double ro = q.x * q.x + q.y * q.y;
-            const double r1 = c == 0 ? rho : ro, r2 = c == 0 ? ro : rho;
-            const double base = r1 + E.beta * r2;
-            const double s = base * sqrt(base);
-            nl = c == 0 ? r1 + E.eta * r2 + E.alpha * s
-                        : E.eta * E.beta * r1 + E.beta * r2 + E.alpha * E.beta * E.beta * s;
+            if (E.fkind == 0) {
+                const double gcc = c == 0 ? E.g11 : E.g22;
+                nl = gcc * rho + E.g12 * ro;
+                enl = 0.5 * gcc * rho * rho + 0.5 * E.g12 * rho * ro;
+            } else {
+                const double r1 = c == 0 ? rho : ro, r2 = c == 0 ? ro : rho;
+                const double base = r1 + E.beta * r2;
+                const double s = base * sqrt(base);
+                nl = c == 0 ? r1 + E.eta * r2 + E.alpha * s
+                            : E.eta * E.beta * r1 + E.beta * r2 + E.alpha * E.beta * E.beta * s;
+            }
         }
         const double a = nl + V[k];
         const double2 h = make_double2(kin.x + a * p.x, kin.y + a * p.y);
@@ -294,101 +319,32 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
             continue;
         }
         double2 o;
-        if (mode == EPI_EULER) {
-            o = csub(p, cmul(cf, h));
-        } else if (mode == EPI_RK1) {
-            E.ksum[g] = h;
-            o = csub(p, cmul(cf, h));
-        } else if (mode == EPI_RK2 || mode == EPI_RK3) {
-            E.ksum[g] = make_double2(ks[k].x + 2.0 * h.x, ks[k].y + 2.0 * h.y);
-            o = csub(cscale(psi[k], spsi), cmul(cf, h));
+        if (mode == EPI_RK4) {
+            // psi - ((1/6) tau) (((k1 + 2 k2) + 2 k3) + k4), S = k1 + 2 k2, K = k3
+            const double2 sum = cadd(make_double2(ss[k].x + 2.0 * kk[k].x, ss[k].y + 2.0 * kk[k].y), h);
+            o = csub(xs, cmul(cf, sum));
         } else {
-            o = csub(cscale(psi[k], spsi), cmul(cf, cadd(ks[k], h)));
+            o = csub(xs, cmul(cf, h));
+            if (mode == EPI_RK2) E.sbuf[g] = make_double2(kk[k].x + 2.0 * h.x, kk[k].y + 2.0 * h.y);
+            if (rk) E.kout[g] = h;
         }
         if (constrain) {
             const double amp = sqrt(o.x * o.x + o.y * o.y);
             o = make_double2(amp * ph[k].x, amp * ph[k].y);
         }
-        E.out[g] = o;
+        if (mode == EPI_EULER || mode == EPI_RK4) E.out[g] = o;
         red[c] += o.x * o.x + o.y * o.y;
         v[k] = o;
     }
 }
 
-// Returns the value written to E.out (the new state for the stepping modes).
+// Single-element form used by the generic LDS kernels.
 __device__ __forceinline__ double2 epilogue(const EpiArgs &E, long long g, int c, double2 kin, double (&red)[kRedQ])
 {
-    if (E.mode == EPI_STORE) {
-        E.out[g] = kin;
-        return kin;
-    }
-    const long long M = E.M;
-    const double2 phi = load_scaled(E.phi, g, E.sc_phi, c);
-    const double rho = phi.x * phi.x + phi.y * phi.y;
-    double nl = 0.0, enl = 0.0;
-    const double V = E.V ? E.V[g] : 0.0;
-    if (E.fkind == 1) {  // LHY droplet: a rho + b rho^{3/2}
-        const double sr = sqrt(rho);
-        nl = E.lhy_a * rho + E.lhy_b * rho * sr;
-        enl = 0.5 * E.lhy_a * rho * rho + 0.4 * E.lhy_b * rho * rho * sr;
-    } else if (E.fkind == 0) {  // gpFunctional contact couplings
-        if (E.has_couplings) {
-            if (E.nc == 1) {
-                nl = E.g11 * rho;
-                enl = 0.5 * E.g11 * rho * rho;
-            } else {
-                const long long go = c == 0 ? g + M : g - M;
-                const double2 po = load_scaled(E.phi, go, E.sc_phi, 1 - c);
-                const double ro = po.x * po.x + po.y * po.y;
-                const double gcc = c == 0 ? E.g11 : E.g22;
-                nl = gcc * rho + E.g12 * ro;
-                enl = 0.5 * gcc * rho * rho + 0.5 * E.g12 * rho * ro;
-            }
-        }
-    } else {  // unlocked two-component LHY
-        const long long go = c == 0 ? g + M : g - M;
-        const double2 po = load_scaled(E.phi, go, E.sc_phi, 1 - c);
-        const double ro = po.x * po.x + po.y * po.y;
-        const double r1 = c == 0 ? rho : ro, r2 = c == 0 ? ro : rho;
-        const double base = r1 + E.beta * r2;
-        const double s = base * sqrt(base);
-        nl = c == 0 ? r1 + E.eta * r2 + E.alpha * s
-                    : E.eta * E.beta * r1 + E.beta * r2 + E.alpha * E.beta * E.beta * s;
-    }
-    const double a = nl + V;
-    const double2 h = make_double2(kin.x + a * phi.x, kin.y + a * phi.y);
-    // energy density of the stage input and its norm
-    red[2 + c] += phi.x * kin.x + phi.y * kin.y + V * rho + enl;
-    red[4 + c] += rho;
-    if (E.mode == EPI_H) {
-        E.out[g] = h;
-        return h;
-    }
-    if (E.mode == EPI_ENERGY) return h;
-    double2 o;
-    if (E.mode == EPI_EULER) {
-        o = csub(phi, cmul(make_double2(E.cr, E.ci), h));
-    } else if (E.mode == EPI_RK1) {
-        E.ksum[g] = h;
-        o = csub(phi, cmul(make_double2(E.cr, E.ci), h));
-    } else if (E.mode == EPI_RK2 || E.mode == EPI_RK3) {
-        const double2 psi = load_scaled(E.psi, g, E.sc_psi, c);
-        const double2 ks = E.ksum[g];
-        E.ksum[g] = make_double2(ks.x + 2.0 * h.x, ks.y + 2.0 * h.y);
-        o = csub(psi, cmul(make_double2(E.cr, E.ci), h));
-    } else {  // EPI_RK4: psi - (tau/6) ((k1 + 2k2 + 2k3) + k4)
-        const double2 psi = load_scaled(E.psi, g, E.sc_psi, c);
-        const double2 ks = E.ksum[g];
-        o = csub(psi, cmul(make_double2(E.cr, E.ci), cadd(ks, h)));
-    }
-    if (E.phase_exp && (E.mode == EPI_EULER || E.mode == EPI_RK4)) {
-        const double amp = sqrt(o.x * o.x + o.y * o.y);
-        const double2 ex = E.phase_exp[g];
-        o = make_double2(amp * ex.x, amp * ex.y);
-    }
-    E.out[g] = o;
-    red[c] += o.x * o.x + o.y * o.y;
-    return o;
+    const int pos[1] = {0};
+    double2 v[1] = {kin};
+    epilogue_batch<1>(E, g, pos, c, v, red);
+    return v[0];
 }
 
 }  // namespace gp5

@@ -54,13 +54,16 @@ GP5_PLAN(256, true, 8, 8, 4, 8);
 GP5_PLAN(512, true, 8, 8, 8, 8);
 #undef GP5_PLAN
 
-// Block shapes.  rows: x lines, LPB lines per 256-thread block, LDS line stride
+// Block shapes.  rows: x lines, LPB lines per 128-thread block, LDS line stride
 // N + N/16 (one pad element every 16 keeps stride-16 accesses conflict free).
 // cols: B interleaved columns of TL threads each.
 template <int N, bool ROWS> struct FShape {
     static constexpr int E = FPlan<N, ROWS>::E;
     static constexpr int TL = N / E;
-    static constexpr int ROW_THREADS = TL > 256 ? TL : 256;
+#ifndef GP5_ROW_THREADS
+#define GP5_ROW_THREADS 128
+#endif
+    static constexpr int ROW_THREADS = TL > GP5_ROW_THREADS ? TL : GP5_ROW_THREADS;
     static constexpr int LPB = ROW_THREADS / TL;
     static constexpr int ROW_STRIDE = N + N / 16;
     static constexpr int COLS = TL >= 64 ? 8 : 16;
@@ -176,8 +179,13 @@ template <int N, bool ROWS> struct FirstLast {
 // MODE 1: forward, k-space multiplier, inverse (the last axis).
 // MODE 2: rows only: inverse, epilogue, optional forward of the epilogue output.
 template <int N, bool ROWS, int MODE, int DIR>
+#ifdef GP5_WAVES_PER_EU
+#define GP5_KATTR __attribute__((amdgpu_waves_per_eu(GP5_WAVES_PER_EU)))
+#else
+#define GP5_KATTR
+#endif
 __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<N, ROWS>::COL_THREADS))
-    k_fast(FastArgs a)
+    GP5_KATTR k_fast(FastArgs a)
 {
     using S = FShape<N, ROWS>;
     using FL = FirstLast<N, ROWS>;
@@ -203,10 +211,24 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
     } else {
         const int b = tid % S::COLS;
         t = tid / S::COLS;
-        const long long tpo = (a.I + S::COLS - 1) / S::COLS;
-        const long long o = (long long)blockIdx.x / tpo;
-        colidx = ((long long)blockIdx.x - o * tpo) * S::COLS + b;
-        valid = colidx < a.I;
+        long long o;
+        if (a.xcount == 0) {
+            const long long tpo = (a.I + S::COLS - 1) / S::COLS;
+            o = (long long)blockIdx.x / tpo;
+            colidx = ((long long)blockIdx.x - o * tpo) * S::COLS + b;
+            valid = colidx < a.I;
+        } else {
+            // x slab [x0, x0 + xcount) of every row r of columns i = x + nx*r
+            const long long tpr = (a.xcount + S::COLS - 1) / S::COLS;
+            const long long rows_per_o = a.I / a.nx;
+            const long long per_o = tpr * rows_per_o;
+            o = (long long)blockIdx.x / per_o;
+            const long long rem = (long long)blockIdx.x - o * per_o;
+            const long long r = rem / tpr;
+            const int xo = (int)(rem - r * tpr) * S::COLS + b;
+            valid = xo < a.xcount;
+            colidx = a.x0 + xo + a.nx * r;
+        }
         gbase = colidx + a.I * (long long)N * o;
         gstride = a.I;
         c = (int)(o / a.Opc);
@@ -217,14 +239,14 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
     constexpr bool first_fwd = (MODE == 1) || (MODE == 0 && DIR < 0);
     constexpr int R_IN = first_fwd ? FL::FWD_FIRST : FL::INV_FIRST;
     {
-        const double sc = (a.scale_in && valid) ? a.scale_in[c] : 1.0;
+        const double2 cf = make_double2(a.cfr, a.cfi);
 #pragma unroll
         for (int i = 0; i < E / R_IN; ++i) {
 #pragma unroll
             for (int r = 0; r < R_IN; ++r) {
                 const int pos = (t + TL * i) + (N / R_IN) * r;
-                double2 x = valid ? a.src[gbase + gstride * pos] : make_double2(0.0, 0.0);
-                v[i * R_IN + r] = make_double2(x.x * sc, x.y * sc);
+                v[i * R_IN + r] = valid ? load_pre(a.src, gbase + gstride * pos, a.scale_in, c, a.src2, cf)
+                                        : make_double2(0.0, 0.0);
             }
         }
     }
@@ -338,7 +360,8 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 bool fast_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
 
 template <int N>
-static void fast_shape(bool rows, long long I, long long O, unsigned &grid, unsigned &threads, size_t &lds)
+static void fast_shape(bool rows, long long I, long long O, int nx, int xcount, unsigned &grid, unsigned &threads,
+                       size_t &lds)
 {
     if (rows) {
         using S = FShape<N, true>;
@@ -347,7 +370,8 @@ static void fast_shape(bool rows, long long I, long long O, unsigned &grid, unsi
         lds = sizeof(double2) * (size_t)S::LPB * S::ROW_STRIDE;
     } else {
         using S = FShape<N, false>;
-        grid = (unsigned)(((I + S::COLS - 1) / S::COLS) * O);
+        if (xcount == 0) grid = (unsigned)(((I + S::COLS - 1) / S::COLS) * O);
+        else grid = (unsigned)(((xcount + S::COLS - 1) / S::COLS) * (I / nx) * O);
         threads = S::COL_THREADS;
         lds = sizeof(double2) * (size_t)S::COLS * N;
     }
@@ -360,7 +384,7 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
     const bool rows = (a.I == 1);
     unsigned grid, threads;
     size_t lds;
-    fast_shape<N>(rows, a.I, a.O, grid, threads, lds);
+    fast_shape<N>(rows, a.I, a.O, a.nx, a.xcount, grid, threads, lds);
     dim3 g(grid), b(threads);
 #define GP5_FAST_LAUNCH(R, M, D) hipLaunchKernelGGL((k_fast<N, R, M, D>), g, b, lds, st, a)
     if (rows) {
@@ -384,12 +408,12 @@ unsigned fast_grid(int n, long long I, long long O)
     size_t lds;
     const bool rows = (I == 1);
     switch (n) {
-        case 16: fast_shape<16>(rows, I, O, grid, threads, lds); break;
-        case 32: fast_shape<32>(rows, I, O, grid, threads, lds); break;
-        case 64: fast_shape<64>(rows, I, O, grid, threads, lds); break;
-        case 128: fast_shape<128>(rows, I, O, grid, threads, lds); break;
-        case 256: fast_shape<256>(rows, I, O, grid, threads, lds); break;
-        case 512: fast_shape<512>(rows, I, O, grid, threads, lds); break;
+        case 16: fast_shape<16>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 32: fast_shape<32>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 64: fast_shape<64>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 128: fast_shape<128>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 256: fast_shape<256>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 512: fast_shape<512>(rows, I, O, 1, 0, grid, threads, lds); break;
         default: break;
     }
     return grid;

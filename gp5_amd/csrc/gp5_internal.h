@@ -43,14 +43,20 @@ struct MidArgs {
     const double *scale;   // lazy per-component scale of the transformed field, or null
 };
 
+// RK4 of src/stepper.cpp:46-70 with stage vectors that are never stored:
+// phi_{s+1} = norm(u_{s+1}), u_{s+1} = psi - c_{s+1} tau k_s, is rebuilt from
+// psi and K = k_s wherever it is needed.  Fields between stages: psi, K, S
+// (S = k1 + 2 k2, written by stage 2, read by stage 4) and the x transform of
+// the raw u_{s+1} (T).  `o` below is the value the pass hands on (transformed
+// along x in the fused pass, renormalisation partials taken of it).
 enum EpiMode {
     EPI_STORE = 0,   // out = multiplier result (laplacian / kinetic term)
     EPI_H = 1,       // out = H(phi)
-    EPI_EULER = 2,   // out = psi - tau*H(psi)
-    EPI_RK1 = 3,     // ksum = k1;          out = psi - (tau/2) k1
-    EPI_RK2 = 4,     // ksum += 2 k2;       out = psi - (tau/2) k2
-    EPI_RK3 = 5,     // ksum += 2 k3;       out = psi - tau k3
-    EPI_RK4 = 6,     // out = psi - (tau/6) (ksum + k4)
+    EPI_EULER = 2,   // out = o = psi - tau H(psi)
+    EPI_RK1 = 3,     // k1 = H(psi);            K = k1;                  o = u2 = psi - (tau/2) k1
+    EPI_RK2 = 4,     // k2 = H(norm(u2(K)));    S = K + 2 k2;  K = k2;   o = u3 = psi - (tau/2) k2
+    EPI_RK3 = 5,     // k3 = H(norm(u3(K)));    K = k3;                  o = u4 = psi - tau k3
+    EPI_RK4 = 6,     // k4 = H(norm(u4(K)));    out = o = psi - (tau/6) (((S + 2 K) + k4)
     EPI_ENERGY = 7,  // reductions only (energy and norm of phi)
 };
 
@@ -62,13 +68,20 @@ struct EpiArgs {
     double lhy_a, lhy_b;
     double alpha, beta, eta;
     const double *V;    // real potential M*nc or null
-    const double2 *phi; // stage input (raw), scaled by *sc_phi
+    const double2 *phi; // H / ENERGY / EULER: input field (raw), scaled by *sc_phi
     const double *sc_phi;
-    const double2 *psi; // step start state (raw), scaled by *sc_psi
+    const double2 *psi; // RK: step start state (raw), scaled by *sc_psi
     const double *sc_psi;
-    double2 *ksum;      // RK accumulator
+    // RK: K read (k_{s-1}) and written (k_s) through different buffers, as
+    // the coupled functionals read the other component's K and psi at the
+    // same point; for the same reason psi' goes to `out`, never onto psi
+    const double2 *kbuf;
+    double2 *kout;
+    double2 *sbuf;      // RK: S = k1 + 2 k2
+    const double *sc_in;  // RK2-4: renormalisation scale of u_s (null: 1)
+    double cr_in, ci_in;  // RK2-4: u_s = psi - c_in K
     double2 *out;
-    double cr, ci;      // stage coefficient (tau/2, tau or tau/6)
+    double cr, ci;      // output coefficient (tau/2, tau/2, tau or tau/6; tau for Euler)
     const double2 *phase_exp;  // e^{i phase} of phaseConstraint (M*nc) or null
     long long M;
     int nc;
@@ -83,16 +96,29 @@ struct EpiArgs {
     double dV;
 };
 
+// Load pre-operation of a first pass: v = src * scale - (cfr + i cfi) * src2.
+struct LoadPre {
+    const double2 *src2;
+    double cfr, cfi;
+};
+
 // Arguments of the register-resident passes (gp5_fast.hip).
 struct FastArgs {
     const double2 *src;
     double2 *dst;
     const double2 *tw;
     long long I, O, Opc;   // axis geometry as in AxisGeom
+    // load: v = src * scale_in[c] - (cfr + i cfi) * src2 (terms absent when null);
+    // rebuilds an RK stage vector u_s = psi - c_in K for an unfused x pass
     const double *scale_in;
+    const double2 *src2;
+    double cfr, cfi;
     MidArgs mid;
     EpiArgs epi;
     double2 *t_out;  // epilogue pass: forward x FFT of its output goes here (null: skip)
+    // column kernels restricted to x in [x0, x0 + xcount) (columns i = x + nx*r);
+    // xcount == 0: all columns, packed COLS at a time
+    int nx, x0, xcount;
 };
 
 bool fast_supported(int n);

@@ -115,7 +115,7 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, const AxisPlan &P, int B, bo
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_axis(const double2 *__restrict__ src, double2 *dst, AxisPlan P,
-                                                   AxisGeom G, int sign, const double *scale_in,
+                                                   AxisGeom G, int sign, const double *scale_in, LoadPre pre,
                                                    long long Opc, MidArgs mid, EpiArgs epi)
 {
     extern __shared__ double2 smem[];
@@ -142,13 +142,8 @@ __global__ __launch_bounds__(kThreads) void k_axis(const double2 *__restrict__ s
         else      { b = e % B; j = e / B; gi = (i0 + b) + G.I * (j + (long long)n * ofix); }
         double2 v = make_double2(0.0, 0.0);
         if (b < count) {
-            v = src[gi];
-            if (scale_in) {
-                const int c = (int)((rows ? o0 + b : ofix) / Opc);
-                const double s = scale_in[c];
-                v.x *= s;
-                v.y *= s;
-            }
+            const int c = (int)((rows ? o0 + b : ofix) / Opc);
+            v = load_pre(src, gi, scale_in, c, pre.src2, make_double2(pre.cfr, pre.cfi));
         }
         bufA[lidx(b, j, n, B, rows)] = v;
     }
@@ -339,14 +334,17 @@ static unsigned tiles_of(const AxisGeom &G)
 }
 
 hipError_t launch_axis(hipStream_t st, int mode, const double2 *src, double2 *dst, const AxisPlan &P,
-                       const AxisGeom &G, int sign, const double *scale_in, long long Opc, const MidArgs &mid,
-                       const EpiArgs &epi)
+                       const AxisGeom &G, int sign, const double *scale_in, const LoadPre &pre, long long Opc,
+                       const MidArgs &mid, const EpiArgs &epi)
 {
     dim3 grid(tiles_of(G)), block(kThreads);
     size_t sh = lds_bytes(P, G);
-    if (mode == 0) hipLaunchKernelGGL(k_axis<0>, grid, block, sh, st, src, dst, P, G, sign, scale_in, Opc, mid, epi);
-    else if (mode == 1) hipLaunchKernelGGL(k_axis<1>, grid, block, sh, st, src, dst, P, G, sign, scale_in, Opc, mid, epi);
-    else hipLaunchKernelGGL(k_axis<2>, grid, block, sh, st, src, dst, P, G, sign, scale_in, Opc, mid, epi);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_axis<0>, grid, block, sh, st, src, dst, P, G, sign, scale_in, pre, Opc, mid, epi);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_axis<1>, grid, block, sh, st, src, dst, P, G, sign, scale_in, pre, Opc, mid, epi);
+    else
+        hipLaunchKernelGGL(k_axis<2>, grid, block, sh, st, src, dst, P, G, sign, scale_in, pre, Opc, mid, epi);
     return hipGetLastError();
 }
 
