@@ -86,6 +86,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with one normalize of the field: its scale kernel reads and writes the field "
+                         "once, the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -159,6 +162,9 @@ def main() -> int:
                    "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
                    "steps_between": args.warmup + args.steps + args.profile_steps},
     }
+    if args.calibrate:
+        capi.normalize(ctx, psi, 0, 3000.0)
+        ctx.sync()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)
     print(json.dumps(out), flush=True)

@@ -426,7 +426,10 @@ static int stage_passes(gp5_ctx *c, const StageIn &in, const double coef[2], con
         e.partials = c->partials;
         a.epi = e;
         a.t_out = fused ? c->T : nullptr;
-        PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], 2, +1, a));
+        // component-local epilogue (fewer registers) unless components couple or a constraint applies
+        const bool coupled = e.fkind == GP5_FUNC_LHY_UNLOCKED || (e.fkind == GP5_FUNC_GP && e.has_couplings && e.nc == 2);
+        const int mode = (coupled || e.phase_exp) ? 2 : 3;
+        PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], mode, +1, a));
     } else {
         if (fused) return fail(GP5_ERR_UNSUPPORTED, "fused stages need a register-kernel x length");
         nblocks = axis_grid(c->geom[0]);

@@ -218,7 +218,11 @@ __device__ __forceinline__ double2 load_pre(const double2 *src, long long g, con
 //   UnlockedLHYDroplet (:125-150) nl_0 = rho1 + eta rho2 + alpha (rho1 + beta rho2)^{3/2},
 //                                 nl_1 = eta beta rho1 + beta rho2 + alpha beta^2 (...)^{3/2}
 // Reductions: red[c] |o|^2, red[2+c] energy density of phi, red[4+c] |phi|^2.
-template <int CH>
+// LOCAL = true compiles the form without cross-component reads and without the
+// phase constraint (one component, or two uncoupled ones, no phaseConstraint):
+// registers are allocated for the longest path of a kernel, so the general
+// form costs occupancy even when those paths are never taken.
+template <int CH, bool LOCAL = false>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase, const int (&pos)[CH], int c,
                                                double2 (&v)[CH], double (&red)[kRedQ])
 {
@@ -230,8 +234,8 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
     }
     const bool rk = mode >= EPI_RK1 && mode <= EPI_RK4;
     const bool rk_k = mode == EPI_RK2 || mode == EPI_RK3 || mode == EPI_RK4;
-    const bool coupled = E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2);
-    const bool constrain = E.phase_exp && (mode == EPI_EULER || mode == EPI_RK4);
+    const bool coupled = !LOCAL && (E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2));
+    const bool constrain = !LOCAL && E.phase_exp && (mode == EPI_EULER || mode == EPI_RK4);
     const double2 *src = rk ? E.psi : E.phi;
     const double *ssc = rk ? E.sc_psi : E.sc_phi;
     double2 x[CH], kk[CH], ss[CH], xo[CH], ko[CH], ph[CH];

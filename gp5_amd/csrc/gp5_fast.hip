@@ -66,7 +66,10 @@ template <int N, bool ROWS> struct FShape {
     static constexpr int ROW_THREADS = TL > GP5_ROW_THREADS ? TL : GP5_ROW_THREADS;
     static constexpr int LPB = ROW_THREADS / TL;
     static constexpr int ROW_STRIDE = N + N / 16;
-    static constexpr int COLS = TL >= 64 ? 8 : 16;
+#ifndef GP5_COLS
+#define GP5_COLS 16
+#endif
+    static constexpr int COLS = TL >= 64 ? 8 : GP5_COLS;
     static constexpr int COL_THREADS = COLS * TL;
 };
 
@@ -178,6 +181,7 @@ template <int N, bool ROWS> struct FirstLast {
 // MODE 0: one transform (sign from DIR: -1 forward, +1 inverse) src -> dst.
 // MODE 1: forward, k-space multiplier, inverse (the last axis).
 // MODE 2: rows only: inverse, epilogue, optional forward of the epilogue output.
+// MODE 3: MODE 2 with the component-local epilogue (no coupling, no constraint).
 template <int N, bool ROWS, int MODE, int DIR>
 #ifdef GP5_WAVES_PER_EU
 #define GP5_KATTR __attribute__((amdgpu_waves_per_eu(GP5_WAVES_PER_EU)))
@@ -303,7 +307,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
         }
         return;
     }
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 2 || MODE == 3) {
         static_assert(ROWS, "the epilogue pass runs on x lines");
         fft_inverse<N, ROWS>(v, t, a.tw, lds, rmap);
         double red[kRedQ];
@@ -329,7 +333,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
                     pos[u] = (t + TL * i) + (N / RL) * k;
                     w[u] = v[e];
                 }
-                epilogue_batch<CH>(a.epi, gbase, pos, c, w, red);
+                epilogue_batch<CH, MODE == 3>(a.epi, gbase, pos, c, w, red);
 #pragma unroll
                 for (int u = 0; u < CH; ++u) v[e0 + u] = w[u];
             }
@@ -391,7 +395,8 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
         if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(true, 0, -1);
         else if (mode == 0) GP5_FAST_LAUNCH(true, 0, 1);
         else if (mode == 1) GP5_FAST_LAUNCH(true, 1, -1);
-        else GP5_FAST_LAUNCH(true, 2, 1);
+        else if (mode == 2) GP5_FAST_LAUNCH(true, 2, 1);
+        else GP5_FAST_LAUNCH(true, 3, 1);
     } else {
         if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(false, 0, -1);
         else if (mode == 0) GP5_FAST_LAUNCH(false, 0, 1);
