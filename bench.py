@@ -28,16 +28,18 @@ METRIC = "RK4 steps/s at 256³ FP64 + ground-state time, surrogate-seeded vs col
 HBM_PEAK_GBPS = 8000.0
 
 # Algorithmic HBM bytes per launch of each kernel kind, in units of F = 16*M bytes
-# (one complex128 field): an axis pass reads and writes the field once; the x-inverse
-# epilogue of RK4 stage s reads T, phi_s, psi, ksum and writes ksum/phi_{s+1} or psi'
-# (stage 1: 4F, stages 2-3: 6F, stage 4: 5F; average 5.25F); the fused variant also
-# writes the x transform of its output (one more F: average 6.25F).
+# (one complex128 field): an axis pass reads and writes the field once.  The x-inverse
+# epilogue of RK4 stage s never stores the stage vector u_s: it rebuilds it from psi and
+# the previous slope K, so stage 1 reads T, psi and writes K1 (3F); stage 2 reads T, psi,
+# K1 and writes K2, S = K1 + 2K2 (5F); stage 3 reads T, psi, K2 and writes K3 (4F);
+# stage 4 reads T, psi, K3, S and writes psi' (5F): 4.25F on average.  The fused variant
+# also writes the x transform of the next stage's input (one more F: 5.25F on average).
 KIND_BYTES_F = {
     "fft_x_forward": 2.0,
     "fft_y_pass": 2.0,
     "fft_z_fwd_mul_inv": 2.0,
-    "fft_x_inverse_epilogue": 5.25,
-    "fft_x_inv_epilogue_fwd": 6.25,
+    "fft_x_inverse_epilogue": 4.25,
+    "fft_x_inv_epilogue_fwd": 5.25,
 }
 STEP_MODEL_F = 29.0  # ideal compulsory traffic of one RK4 step (fused, one r/w per FFT)
 

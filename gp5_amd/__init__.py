@@ -10,4 +10,4 @@ Layers:
 """
 from . import capi  # noqa: F401
 
-__all__ = ["capi"]
+__all__ = ["capi", "gp"]

@@ -8,8 +8,9 @@ closed forms and an independent numpy restatement.  The fixtures freeze those
 outputs so the GPU path and later changes of the oracle are checked against the
 same numbers.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]
 """
+import json
 import os
 import sys
 
@@ -62,6 +63,31 @@ def two_component_rk4_imag():
                 renorm=np.array([2.0, 0.5]))
 
 
+def mu_lhy_imaginary():
+    """The reference's ground-state regression protocol (python/test/regressionTests/LHY_imaginary.py:8-55):
+    LHY droplet N = 370 on 30^3 over [-10, 10]^3, psi0 = exp(-r^2 / 50), RK4 imaginary time with
+    dt = 0.01 dx^2; blocks of 1000 (normalize, step) pairs; after each block
+    mu = ln(N / norm) / (2 dt), normalize; stop when |mu - mu_previous| <= 1e-9."""
+    n, L, N = 30, 10.0, 370.0
+    geo = O.Geometry((n, n, n), (-L, -L, -L), (L, L, L), 1)
+    X, Y, Z = geo.grid()
+    psi = (np.exp(-(X**2 + Y**2 + Z**2) / (2 * 5**2)) + 0j)[..., None]
+    psi = O.normalize(psi, geo, 0, N)
+    F = O.Functional(kind=O.OR_LHY)
+    dt = 0.01 * geo.dx[0] ** 2
+    mu, old, history = 1.0e18, 0.0, []
+    while abs(mu - old) > 1e-9:
+        for _ in range(1000):
+            psi = O.normalize(psi, geo, 0, N)
+            psi = O.advance(F, psi, geo, stepper="RK4", dt=dt, imaginary=True, nsteps=1)
+        old = mu
+        mu = 0.5 * np.log(N / O.norm(psi, geo, 0)) / dt
+        psi = O.normalize(psi, geo, 0, N)
+        history.append(float(mu))
+    return dict(mu=float(mu), blocks=len(history), history=history, dt=dt, N=N, shape=[n, n, n],
+                left=[-L] * 3, right=[L] * 3, max_density=float(np.max(np.abs(psi)) ** 2))
+
+
 FIXTURES = {
     "droplet_rk4_real_16": droplet_rk4_real,
     "harmonic_euler_imag_16": harmonic_euler_imag,
@@ -69,9 +95,24 @@ FIXTURES = {
 }
 
 
-def main():
+JSON_FIXTURES = {
+    "mu_lhy_imaginary_30": mu_lhy_imaginary,
+}
+
+
+def main(names=None):
     O.build()
+    O.set_threads(min(8, os.cpu_count() or 1))
+    for name, fn in JSON_FIXTURES.items():
+        if names and name not in names:
+            continue
+        d = fn()
+        with open(os.path.join(HERE, name + ".json"), "w") as fh:
+            json.dump(d, fh, indent=1)
+        print(name, d["mu"], d["blocks"])
     for name, fn in FIXTURES.items():
+        if names and name not in names:
+            continue
         d = fn()
         arrays = {k: np.asarray(v) for k, v in d.items()}
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
@@ -79,4 +120,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])
