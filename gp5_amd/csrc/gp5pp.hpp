@@ -26,8 +26,10 @@
 #include <complex>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gp5.h"
@@ -93,6 +95,22 @@ private:
     int _nc;
 };
 
+inline std::shared_ptr<device_context> shared_context(const intDVec_t &shape, const realDVec_t &left,
+                                                     const realDVec_t &right, int nc)
+{
+    using key_t = std::tuple<intDVec_t, realDVec_t, realDVec_t, int>;
+    static std::mutex mtx;
+    static std::map<key_t, std::weak_ptr<device_context>> live;
+    std::lock_guard<std::mutex> lock(mtx);
+    auto &w = live[key_t(shape, left, right, nc)];
+    auto p = w.lock();
+    if (!p) {
+        p = std::make_shared<device_context>(shape, left, right, nc);
+        w = p;
+    }
+    return p;
+}
+
 class discretization {
 public:
     discretization() = default;
@@ -105,10 +123,14 @@ public:
     // geometry.h:88-90
     real_t getSpaceStep(int d) const { return _domain->getLBox()[d] / _globalMesh->shape()[d]; }
     real_t getFourierStep(int d) const { return 2 * M_PI / _domain->getLBox()[d]; }
+    // Every discretisation of the same grid and box shares one device context per
+    // component count, so tensors of different fields can meet in one operator call
+    // (model.apply(psi1, psi2)) and their work is ordered on one stream.  Calls that
+    // share a context must come from one host thread at a time (its scratch is shared).
     std::shared_ptr<device_context> context(int nc)
     {
         auto &p = _ctx[nc];
-        if (!p) p = std::make_shared<device_context>(_globalMesh->shape(), _domain->getLeft(), _domain->getRight(), nc);
+        if (!p) p = shared_context(_globalMesh->shape(), _domain->getLeft(), _domain->getRight(), nc);
         return p;
     }
 
