@@ -164,6 +164,17 @@ def main() -> int:
                    "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
                    "steps_between": args.warmup + args.steps + args.profile_steps},
     }
+    # host <-> HBM copies of the field through the boundary (gp5_buf_upload / download of
+    # a pageable numpy array); outside the timed region, reported for the PCIe-inclusive rate
+    t0 = time.perf_counter()
+    host = psi.download_complex()
+    t_d2h = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    psi.upload_complex(host)
+    ctx.sync()
+    t_h2d = time.perf_counter() - t0
+    out["host_transfer"] = {"field_bytes": F, "d2h_ms": 1e3 * t_d2h, "h2d_ms": 1e3 * t_h2d,
+                            "steps_per_s_incl_one_roundtrip": args.steps / (wall + t_d2h + t_h2d)}
     if args.calibrate:
         capi.normalize(ctx, psi, 0, 3000.0)
         ctx.sync()

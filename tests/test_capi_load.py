@@ -3,6 +3,11 @@ include/gp5.h declares.  No compute call is made (no GPU here)."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
 
 from conftest import ROOT
 
@@ -34,3 +39,42 @@ def test_library_reports_version_and_error_text():
     # NULL context is rejected on the host without touching a device
     assert L.gp5_ctx_sync(None) == capi.GP5_ERR_ARG
     assert b"NULL" in L.gp5_last_error()
+
+
+def _build_facade_example(out_dir):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = os.path.join(str(out_dir), "facade_example")
+    cmd = [gxx, "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "gp5_amd", "csrc"),
+           os.path.join(ROOT, "tests", "cpp", "facade_example.cpp"), "-L", os.path.join(ROOT, "gp5_amd"), "-lgp5",
+           "-Wl,-rpath," + os.path.join(ROOT, "gp5_amd"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def test_cpp_facade_example_compiles_and_links(tmp_path):
+    """The reference-style C++ API (gp5pp.hpp) builds against libgp5.so."""
+    assert os.path.exists(_build_facade_example(tmp_path))
+
+
+@pytest.mark.gpu
+def test_cpp_facade_example_runs(tmp_path, oracle_mod):
+    """Runs the C++ example (64^3 LHY Gaussian, N = 500, 10 real-time RK4 steps) and checks
+    its printed norm and energy against the oracle."""
+    exe = _build_facade_example(tmp_path)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    parts = r.stdout.split()
+    norm, energy = float(parts[1]), float(parts[3])
+
+    O = oracle_mod
+    geo = O.Geometry((64, 64, 64), (-10, -10, -10), (10, 10, 10), 1)
+    X, Y, Z = geo.grid()
+    a = (np.exp(-(X**2 + Y**2 + Z**2) / 8.0) + 0j)[..., None]
+    a = O.normalize(a, geo, 0, 500.0)
+    F = O.Functional(kind=O.OR_LHY)
+    a = O.advance(F, a, geo, stepper="RK4", dt=1e-3, imaginary=False, nsteps=10)
+    assert norm == pytest.approx(O.norm(a, geo, 0), rel=1e-11)
+    assert energy == pytest.approx(O.energy(F, a, geo), rel=1e-10)
