@@ -205,6 +205,25 @@ __device__ __forceinline__ double2 load_pre(const double2 *src, long long g, con
     return v;
 }
 
+// Per-thread reduction sums of the epilogue for the thread's component c; they
+// are placed into the component slots of the block reduction once per thread
+// (acc_scatter), not per element: a runtime index into the register array
+// would cost a compare-and-select chain over every slot at each update.
+struct EpiAcc {
+    double o2 = 0.0, en = 0.0, rho = 0.0;
+};
+
+__device__ __forceinline__ void acc_scatter(const EpiAcc &a, int c, double (&red)[kRedQ])
+{
+    const bool c0 = c == 0;
+    red[0] += c0 ? a.o2 : 0.0;
+    red[1] += c0 ? 0.0 : a.o2;
+    red[2] += c0 ? a.en : 0.0;
+    red[3] += c0 ? 0.0 : a.en;
+    red[4] += c0 ? a.rho : 0.0;
+    red[5] += c0 ? 0.0 : a.rho;
+}
+
 // Epilogue of the last inverse x pass over CH elements of one line (one
 // component c) at gbase + pos[k].  Every global load of the batch is issued
 // before the arithmetic, with the mode / functional tests uniform across the
@@ -224,7 +243,7 @@ __device__ __forceinline__ double2 load_pre(const double2 *src, long long g, con
 // form costs occupancy even when those paths are never taken.
 template <int CH, bool LOCAL = false>
 __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase, const int (&pos)[CH], int c,
-                                               double2 (&v)[CH], double (&red)[kRedQ])
+                                               double2 (&v)[CH], EpiAcc &acc)
 {
     const int mode = E.mode;
     if (mode == EPI_STORE) {
@@ -311,8 +330,8 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
         }
         const double a = nl + V[k];
         const double2 h = make_double2(kin.x + a * p.x, kin.y + a * p.y);
-        red[2 + c] += p.x * kin.x + p.y * kin.y + V[k] * rho + enl;
-        red[4 + c] += rho;
+        acc.en += p.x * kin.x + p.y * kin.y + V[k] * rho + enl;
+        acc.rho += rho;
         if (mode == EPI_H) {
             E.out[g] = h;
             v[k] = h;
@@ -337,7 +356,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
             o = make_double2(amp * ph[k].x, amp * ph[k].y);
         }
         if (mode == EPI_EULER || mode == EPI_RK4) E.out[g] = o;
-        red[c] += o.x * o.x + o.y * o.y;
+        acc.o2 += o.x * o.x + o.y * o.y;
         v[k] = o;
     }
 }
@@ -347,7 +366,9 @@ __device__ __forceinline__ double2 epilogue(const EpiArgs &E, long long g, int c
 {
     const int pos[1] = {0};
     double2 v[1] = {kin};
-    epilogue_batch<1>(E, g, pos, c, v, red);
+    EpiAcc acc;
+    epilogue_batch<1>(E, g, pos, c, v, acc);
+    acc_scatter(acc, c, red);
     return v[0];
 }
 

@@ -310,9 +310,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
     if constexpr (MODE == 2 || MODE == 3) {
         static_assert(ROWS, "the epilogue pass runs on x lines");
         fft_inverse<N, ROWS>(v, t, a.tw, lds, rmap);
-        double red[kRedQ];
-#pragma unroll
-        for (int q = 0; q < kRedQ; ++q) red[q] = 0.0;
+        EpiAcc acc;
         constexpr int RL = FL::INV_LAST;  // == FWD_FIRST
 #ifndef GP5_EPI_CHUNK
 #define GP5_EPI_CHUNK 4
@@ -333,7 +331,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
                     pos[u] = (t + TL * i) + (N / RL) * k;
                     w[u] = v[e];
                 }
-                epilogue_batch<CH, MODE == 3>(a.epi, gbase, pos, c, w, red);
+                epilogue_batch<CH, MODE == 3>(a.epi, gbase, pos, c, w, acc);
 #pragma unroll
                 for (int u = 0; u < CH; ++u) v[e0 + u] = w[u];
             }
@@ -352,6 +350,10 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
             }
         }
         if (a.epi.want_red) {
+            double red[kRedQ];
+#pragma unroll
+            for (int q = 0; q < kRedQ; ++q) red[q] = 0.0;
+            acc_scatter(acc, c, red);
             __syncthreads();
             reduce_block(red, a.epi.partials, reinterpret_cast<double *>(lds));
         }
