@@ -278,6 +278,10 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
     // ---- load: inputs of the first stage of the first transform ----
     constexpr bool first_fwd = (MODE == 1) || (MODE == 0 && DIR < 0);
     constexpr int R_IN = first_fwd ? FL::FWD_FIRST : FL::INV_FIRST;
+    // streaming hints: epilogue passes follow GP5_NT_EPI (bit 0 loads, bit 1
+    // stores), column passes GP5_NT_COLS
+    constexpr bool NTL = (MODE >= 2) ? (GP5_NT_EPI & 1) != 0 : (!ROWS && GP5_NT_COLS != 0);
+    constexpr bool NTS = (MODE >= 2) ? (GP5_NT_EPI & 2) != 0 : (!ROWS && GP5_NT_COLS != 0);
     {
         const double2 cf = make_double2(a.cfr, a.cfi);
 #pragma unroll
@@ -285,7 +289,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 #pragma unroll
             for (int r = 0; r < R_IN; ++r) {
                 const int pos = (t + TL * i) + (N / R_IN) * r;
-                v[i * R_IN + r] = valid ? load_pre(a.src, gbase + gstride * pos, a.scale_in, c, a.src2, cf)
+                v[i * R_IN + r] = valid ? load_pre<NTL>(a.src, gbase + gstride * pos, a.scale_in, c, a.src2, cf)
                                         : make_double2(0.0, 0.0);
             }
         }
@@ -303,7 +307,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 #pragma unroll
             for (int i = 0; i < E / RL; ++i)
 #pragma unroll
-                for (int k = 0; k < RL; ++k) a.dst[gbase + gstride * ((t + TL * i) + (N / RL) * k)] = v[i * RL + k];
+                for (int k = 0; k < RL; ++k) stg<NTS>(a.dst + gbase + gstride * ((t + TL * i) + (N / RL) * k), v[i * RL + k]);
         }
         return;
     }
@@ -339,7 +343,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 #pragma unroll
             for (int i = 0; i < E / RO; ++i)
 #pragma unroll
-                for (int k = 0; k < RO; ++k) a.dst[gbase + gstride * ((t + TL * i) + (N / RO) * k)] = v[i * RO + k];
+                for (int k = 0; k < RO; ++k) stg<NTS>(a.dst + gbase + gstride * ((t + TL * i) + (N / RO) * k), v[i * RO + k]);
         }
         return;
     }
@@ -382,7 +386,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 #pragma unroll
                 for (int i = 0; i < E / RF; ++i)
 #pragma unroll
-                    for (int k = 0; k < RF; ++k) a.t_out[gbase + (t + TL * i) + (N / RF) * k] = v[i * RF + k];
+                    for (int k = 0; k < RF; ++k) stg<NTS>(a.t_out + gbase + (t + TL * i) + (N / RF) * k, v[i * RF + k]);
             }
         }
         if (a.epi.want_red) {
