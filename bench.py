@@ -70,6 +70,12 @@ def cpu_baseline(psi0, shape, left, right, budget_s: float = 25.0):
         el = time.perf_counter() - t0
         if el * (steps + 1) / steps > budget_s or steps >= 3:
             break
+    # the reference as shipped runs one rank without threads: one step on one core
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.advance(F, psi0, geo, stepper="RK4", dt=5e-4, imaginary=False, nsteps=1)
+    one = time.perf_counter() - t0
+    O.set_threads(cores)
     return {
         "value": steps / el,
         "unit": "steps/s",
@@ -77,6 +83,7 @@ def cpu_baseline(psi0, shape, left, right, budget_s: float = 25.0):
         "kind": "port",
         "sample": f"{steps} RK4 step(s) of the {shape[0]}^3 droplet workload, CPU oracle "
                   f"(C restatement of the reference FFTW path) with OpenMP over {cores} threads",
+        "one_thread": {"value": 1.0 / one, "unit": "steps/s", "cores": 1, "sample": "1 RK4 step, 1 thread"},
     }
 
 
