@@ -39,6 +39,9 @@ EXPORTS = [
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
     "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
     "gp5_kernel_kind_name",
+    "gp5_batch_create", "gp5_batch_destroy", "gp5_batch_classes", "gp5_batch_class_of",
+    "gp5_batch_upload", "gp5_batch_download", "gp5_batch_set_lhy", "gp5_batch_set_potential",
+    "gp5_batch_advance", "gp5_batch_tune", "gp5_batch_ground_state",
 ]
 GP5_NKINDS = 8
 
@@ -108,6 +111,18 @@ def load() -> ctypes.CDLL:
         "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
                                  ctypes.POINTER(ctypes.c_int)], I),
         "gp5_kernel_kind_name": ([I], ctypes.c_char_p),
+        "gp5_batch_create": ([PP, I, ctypes.POINTER(ctypes.c_int), DP, DP], I),
+        "gp5_batch_destroy": ([P], I),
+        "gp5_batch_classes": ([P], I),
+        "gp5_batch_class_of": ([P, I], I),
+        "gp5_batch_upload": ([P, I, P], I),
+        "gp5_batch_download": ([P, I, P], I),
+        "gp5_batch_set_lhy": ([P, I, D, D], I),
+        "gp5_batch_set_potential": ([P, I, P], I),
+        "gp5_batch_advance": ([P, ctypes.POINTER(FunctionalDesc), D, I], I),
+        "gp5_batch_tune": ([P, ctypes.POINTER(FunctionalDesc), D, I, ctypes.c_char_p, S], I),
+        "gp5_batch_ground_state": ([P, ctypes.POINTER(FunctionalDesc), D, D, I, ctypes.c_longlong,
+                                    ctypes.POINTER(ctypes.c_longlong), DP, DP], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -335,3 +350,88 @@ def profile_advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer
         if cnt[k]:
             out[L.gp5_kernel_kind_name(k).decode()] = (ms[k], cnt[k])
     return out
+
+
+class Batch:
+    """gp5_batch: many single-component instances (cube grid n_i on [-L_i/2, L_i/2]^3,
+    normalisation N_i) stepped together, grouped by (n, L) into one context each."""
+
+    def __init__(self, n, L, N):
+        L_ = load()
+        self.n = [int(x) for x in n]
+        self.L = [float(x) for x in L]
+        self.N = [float(x) for x in N]
+        k = len(self.n)
+        if not (len(self.L) == k and len(self.N) == k):
+            raise GP5Error(GP5_ERR_ARG, "n, L and N need one entry per instance")
+        h = ctypes.c_void_p()
+        check(L_.gp5_batch_create(ctypes.byref(h), k, (ctypes.c_int * k)(*self.n), (ctypes.c_double * k)(*self.L),
+                                  (ctypes.c_double * k)(*self.N)))
+        self.handle = h
+        self.count = k
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            check(load().gp5_batch_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def classes(self) -> int:
+        return load().gp5_batch_classes(self.handle)
+
+    def class_of(self, i: int) -> int:
+        return load().gp5_batch_class_of(self.handle, i)
+
+    def upload(self, i: int, a: np.ndarray) -> None:
+        a = np.asarray(a, dtype=np.complex128)
+        n = self.n[i]
+        if a.ndim == 4:
+            a = a[..., 0]
+        if a.shape != (n, n, n):
+            raise GP5Error(GP5_ERR_ARG, f"instance {i} holds a {n}^3 field, got {a.shape}")
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        check(load().gp5_batch_upload(self.handle, i, flat.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self, i: int) -> np.ndarray:
+        n = self.n[i]
+        flat = np.empty(n ** 3, dtype=np.complex128)
+        check(load().gp5_batch_download(self.handle, i, flat.ctypes.data_as(ctypes.c_void_p)))
+        return flat.reshape((n, n, n), order="F")
+
+    def set_lhy(self, i: int, a: float, b: float) -> None:
+        check(load().gp5_batch_set_lhy(self.handle, i, float(a), float(b)))
+
+    def set_potential(self, i: int, V: np.ndarray) -> None:
+        V = np.asarray(V, dtype=np.float64)
+        n = self.n[i]
+        if V.ndim == 4:
+            V = V[..., 0]
+        if V.shape != (n, n, n):
+            raise GP5Error(GP5_ERR_ARG, f"instance {i} holds a {n}^3 potential, got {V.shape}")
+        flat = np.ascontiguousarray(V.ravel(order="F"))
+        check(load().gp5_batch_set_potential(self.handle, i, flat.ctypes.data_as(ctypes.c_void_p)))
+
+    def advance(self, f: FunctionalDesc, dt_factor: float, nsteps: int) -> None:
+        check(load().gp5_batch_advance(self.handle, ctypes.byref(f), float(dt_factor), int(nsteps)))
+
+    def tune(self, f: FunctionalDesc, dt_factor: float, steps: int = 2) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        check(load().gp5_batch_tune(self.handle, ctypes.byref(f), float(dt_factor), int(steps), buf, len(buf)))
+        return buf.value.decode()
+
+    def ground_state(self, f: FunctionalDesc, dt_factor: float, tol: float, check_every: int = 100,
+                     max_steps: int = 0):
+        """Returns (steps[count], energy[count], elapsed_ms); steps < 0: not converged."""
+        k = self.count
+        steps = (ctypes.c_longlong * k)()
+        en = (ctypes.c_double * k)()
+        ms = ctypes.c_double()
+        check(load().gp5_batch_ground_state(self.handle, ctypes.byref(f), float(dt_factor), float(tol),
+                                            int(check_every), int(max_steps), steps, en, ctypes.byref(ms)))
+        return np.array(steps[:], dtype=np.int64), np.array(en[:]), ms.value

@@ -16,6 +16,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -29,7 +31,7 @@ hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int com
 hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax);
 hipError_t launch_phase(hipStream_t st, double2 *f, const double *phase, long long len);
 hipError_t launch_phase_exp(hipStream_t st, double2 *ex, const double *phase, long long len);
-hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks);
+hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks, long long lpb, long long lpc, int ncomp);
 }  // namespace gp5
 
 using namespace gp5;
@@ -43,7 +45,9 @@ struct gp5_buf {
 struct gp5_ctx {
     int n[3];
     double left[3], right[3];
-    int nc;
+    int nc;              // components of the field (active components of a batch class)
+    int nc_cap;          // components the scratch and the scale slots are sized for
+    bool uncoupled;      // components are independent instances (more than two, or a batch class)
     long long M;
     double dV;
     double dk[3];
@@ -65,8 +69,11 @@ struct gp5_ctx {
     // reduction scratch
     double *partials;
     size_t partials_cap;  // blocks
-    double *results;      // kRedQ
-    double *scales;       // 8 slots x 2 components
+    double *results;      // [kRedN][nc]
+    double *scales;       // 8 slots x nc_cap components (slot k at scales + k * nc_cap)
+    double *nnorm;        // nc_cap normalisation targets of a batch (device), or null
+    double *lhy_ab;       // nc_cap (a, b) LHY coefficient pairs of a batch (device), or null
+    double *Vdev;         // M * nc_cap potential of a batch (device), or null
     double *blockmax;
     double *dlog;
     size_t dlog_cap;      // steps
@@ -157,7 +164,12 @@ static int fail(int code, const char *fmt, ...)
 // ---------------------------------------------------------------------------
 // plans
 // ---------------------------------------------------------------------------
-static int make_plan(int n, AxisPlan &P)
+// Radix plans of the generic LDS kernels.  order 0: 8s, 4s, 2, 3s, 5s, other
+// primes; order 1: 4s and 2 before 3s and 5s (no radix 8); order 2: 3s and 5s
+// first, then 8s, 4s, 2; order 3: 2s only for the power-of-two part.  The
+// autotuner picks per axis; the default is order 0.
+constexpr int kPlanOrders = 4;
+static int make_plan(int n, AxisPlan &P, int order = 0)
 {
     P.n = n;
     P.nrad = 0;
@@ -167,11 +179,26 @@ static int make_plan(int n, AxisPlan &P)
         P.rad[P.nrad++] = r;
         return true;
     };
-    while (m % 8 == 0) { if (!push(8)) return -1; m /= 8; }
-    while (m % 4 == 0) { if (!push(4)) return -1; m /= 4; }
-    while (m % 2 == 0) { if (!push(2)) return -1; m /= 2; }
-    while (m % 3 == 0) { if (!push(3)) return -1; m /= 3; }
-    while (m % 5 == 0) { if (!push(5)) return -1; m /= 5; }
+    auto pow2 = [&]() -> bool {
+        if (order == 0 || order == 2) {
+            while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
+        }
+        if (order != 3) {
+            while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+        }
+        while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+        return true;
+    };
+    auto small_odd = [&]() -> bool {
+        while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
+        while (m % 5 == 0) { if (!push(5)) return false; m /= 5; }
+        return true;
+    };
+    if (order == 2) {
+        if (!small_odd() || !pow2()) return -1;
+    } else {
+        if (!pow2() || !small_odd()) return -1;
+    }
     for (int r = 7; m > 1; r += 2) {
         while (m % r == 0) { if (!push(r)) return -1; m /= r; }
         if ((long long)r * r > m && m > 1) { if (!push(m)) return -1; m = 1; }
@@ -191,6 +218,36 @@ static std::vector<double2> twiddles(int n)
     return tw;
 }
 
+// Axis geometry of the passes for the context's current component count.
+// Generic-kernel tiles (lines per block) default to kTileElems / n; with more
+// than two components a row tile never spans more than one component
+// boundary (the partial sums carry two component slots).  A nonzero geom[d].B
+// (set by the autotuner) is kept, clipped to the same limits.
+static void set_geometry(gp5_ctx *c)
+{
+    long long I = 1;
+    for (int d = 0; d < 3; ++d) {
+        AxisGeom &G = c->geom[d];
+        const int n = c->n[d];
+        G.I = I;
+        c->Opc[d] = c->M / (I * n);
+        G.O = c->Opc[d] * c->nc;
+        int B = G.B > 0 ? G.B : std::max(1, kTileElems / n);
+        B = std::min(B, std::max(1, kTileElems / n));
+        if (I == 1) {
+            B = (int)std::min<long long>(B, G.O);
+            if (c->nc_cap > 2) B = (int)std::min<long long>(B, c->Opc[d]);
+        } else {
+            B = (int)std::min<long long>(std::min(B, 64), I);
+        }
+        G.B = std::max(1, B);
+        I *= n;
+    }
+}
+
+// Lazy-scale slot k (per component): 0/1 psi, 2/3 RK stage inputs, 7 normalize.
+static double *scale_slot(gp5_ctx *c, int k) { return c->scales + (size_t)k * c->nc_cap; }
+
 static int ensure_partials(gp5_ctx *c, size_t blocks)
 {
     if (blocks <= c->partials_cap) return GP5_OK;
@@ -205,7 +262,7 @@ static int ensure_partials(gp5_ctx *c, size_t blocks)
 // Scratch fields: T always; a stepper adds the output field, RK4 also K (x2) and S.
 static int ensure_scratch(gp5_ctx *c, bool rk, bool stepper)
 {
-    const size_t bytes = (size_t)c->M * c->nc * sizeof(double2);
+    const size_t bytes = (size_t)c->M * c->nc_cap * sizeof(double2);
     auto need = [&](double2 *&p) -> int {
         if (p) return GP5_OK;
         if (hipMalloc(&p, bytes) != hipSuccess) {
@@ -258,23 +315,26 @@ static int fill_functional(gp5_ctx *c, const gp5_functional_desc *f, EpiArgs &E,
 {
     if (!f) return fail(GP5_ERR_ARG, "functional descriptor is NULL");
     E.fkind = f->kind;
+    E.uncoupled = c->uncoupled;
     if (f->kind == GP5_FUNC_GP) {
+        // more than two components: uncoupled instances, each with g = couplings[0]
         E.has_couplings = f->has_couplings;
-        if (f->has_couplings && c->nc > 2) return fail(GP5_ERR_UNSUPPORTED, "couplings support 1 or 2 components");
         E.g11 = f->couplings[0];
         E.g12 = f->couplings[2];
         E.g22 = f->couplings[3];
+        // more than two (uncoupled) components share masses[0]
         for (int k = 0; k < 2; ++k) {
-            double m = k < c->nc ? f->masses[k] : 1.0;
+            double m = c->uncoupled ? f->masses[0] : (k < c->nc ? f->masses[k] : 1.0);
             if (m == 0.0) m = 1.0;
             coef[k] = 0.5 * (1.0 / m) / (double)c->M;
         }
     } else if (f->kind == GP5_FUNC_LHY) {
         E.lhy_a = f->lhy_a;
         E.lhy_b = f->lhy_b;
+        E.lhy_ab = c->lhy_ab;  // batch classes: per-instance coefficients
         coef[0] = coef[1] = 0.5 / (double)c->M;
     } else if (f->kind == GP5_FUNC_LHY_UNLOCKED) {
-        if (c->nc != 2) return fail(GP5_ERR_ARG, "Incompatible number of components");
+        if (c->nc != 2 || c->uncoupled) return fail(GP5_ERR_ARG, "Incompatible number of components");
         E.alpha = f->alpha;
         E.beta = f->beta;
         E.eta = f->eta;
@@ -285,6 +345,8 @@ static int fill_functional(gp5_ctx *c, const gp5_functional_desc *f, EpiArgs &E,
     if (f->potential) {
         RCHK(check_real(c, f->potential, "potential"));
         E.V = f->potential->d;
+    } else if (c->Vdev) {
+        E.V = c->Vdev;  // batch classes: per-instance potentials
     }
     return GP5_OK;
 }
@@ -418,27 +480,31 @@ static int stage_passes(gp5_ctx *c, const StageIn &in, const double coef[2], con
     if (!fused) RCHK(pass_plain(c, K_X_FWD, 0, in.src, c->T, -1, in.src_scale, &in.pre));
     RCHK(middle_passes(c, mid));
     unsigned nblocks;
+    long long lpb;  // lines per block of the epilogue pass (component slots of its partials)
     EpiArgs e = E;
     if (c->fast[0]) {
         nblocks = fast_grid(c->n[0], c->geom[0].I, c->geom[0].O);
+        lpb = fast_rows_per_block(c->n[0]);
         RCHK(ensure_partials(c, nblocks));
         FastArgs a = fast_args(c, 0, c->T, nullptr);
         e.partials = c->partials;
         a.epi = e;
         a.t_out = fused ? c->T : nullptr;
         // component-local epilogue (fewer registers) unless components couple or a constraint applies
-        const bool coupled = e.fkind == GP5_FUNC_LHY_UNLOCKED || (e.fkind == GP5_FUNC_GP && e.has_couplings && e.nc == 2);
+        const bool coupled = e.fkind == GP5_FUNC_LHY_UNLOCKED ||
+                             (e.fkind == GP5_FUNC_GP && e.has_couplings && e.nc == 2 && !e.uncoupled);
         const int mode = (coupled || e.phase_exp) ? 2 : 3;
         PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], mode, +1, a));
     } else {
         if (fused) return fail(GP5_ERR_UNSUPPORTED, "fused stages need a register-kernel x length");
         nblocks = axis_grid(c->geom[0]);
+        lpb = c->geom[0].B;
         RCHK(ensure_partials(c, nblocks));
         e.partials = c->partials;
         PLAUNCH(c, K_X_EPI, launch_axis(c->st, 2, c->T, nullptr, c->plan[0], c->geom[0], +1, nullptr,
                                         LoadPre{nullptr, 0.0, 0.0}, c->Opc[0], mid, e));
     }
-    if (e.want_red) PLAUNCH(c, K_NORM, launch_finalize(c->st, e, nblocks));
+    if (e.want_red) PLAUNCH(c, K_NORM, launch_finalize(c->st, e, nblocks, lpb, c->Opc[0], c->nc));
     return GP5_OK;
 }
 
@@ -460,7 +526,7 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
 {
     if (!out || !shape || !left || !right) return fail(GP5_ERR_ARG, "NULL argument");
     *out = nullptr;
-    if (nc < 1 || nc > 2) return fail(GP5_ERR_UNSUPPORTED, "nComponents must be 1 or 2 (got %d)", nc);
+    if (nc < 1 || nc > kMaxComp) return fail(GP5_ERR_UNSUPPORTED, "nComponents must be 1..%d (got %d)", kMaxComp, nc);
     for (int d = 0; d < 3; ++d) {
         if (shape[d] < 1) return fail(GP5_ERR_ARG, "shape[%d] = %d", d, shape[d]);
         if (shape[d] > kMaxLen)
@@ -476,6 +542,8 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
         c->dk[d] = 2 * M_PI / (right[d] - left[d]);  // discretization::getFourierStep
     }
     c->nc = nc;
+    c->nc_cap = nc;
+    c->uncoupled = nc > 2;
     c->M = (long long)shape[0] * shape[1] * shape[2];
     c->dV = 1.0;
     for (int d = 0; d < 3; ++d) c->dV *= (right[d] - left[d]) / shape[d];  // getSpaceStep
@@ -484,7 +552,6 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(GP5_ERR_HIP, "no HIP device"));
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(GP5_ERR_HIP, "hipStreamCreate failed"));
-    long long I = 1;
     for (int d = 0; d < 3; ++d) {
         if (make_plan(shape[d], c->plan[d])) return bail(fail(GP5_ERR_UNSUPPORTED, "too many radices for %d", shape[d]));
         std::vector<double2> tw = twiddles(shape[d]);
@@ -507,18 +574,14 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
         const char *mc = getenv("GP5_MALL_CHUNKS");
         if (mc && mc[0]) chunks = atoi(mc);
         c->mall_chunks = std::max(1, std::min(chunks, shape[0] / 16));
-        AxisGeom &G = c->geom[d];
-        G.I = I;
-        G.O = c->M / (I * shape[d]) * nc;
-        c->Opc[d] = c->M / (I * shape[d]);
-        int B = std::max(1, kTileElems / shape[d]);
-        if (I == 1) B = (int)std::min<long long>(B, G.O);
-        else B = (int)std::min<long long>(std::min(B, 64), I);
-        G.B = std::max(1, B);
-        I *= shape[d];
+        c->geom[d].B = 0;  // default tile
     }
-    if (hipMalloc(&c->results, kRedQ * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "results"));
-    if (hipMalloc(&c->scales, 16 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "scales"));
+    // the epilogue's partial sums carry two component slots: with more than two
+    // components an x-line block must not span more than one boundary
+    if (nc > 2 && c->fast[0] && fast_rows_per_block(shape[0]) > (long long)shape[1] * shape[2]) c->fast[0] = false;
+    set_geometry(c);
+    if (hipMalloc(&c->results, kRedN * (size_t)nc * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "results"));
+    if (hipMalloc(&c->scales, 8 * (size_t)nc * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "scales"));
     if (hipMalloc(&c->blockmax, 8192 * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "blockmax"));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
         return bail(fail(GP5_ERR_HIP, "events"));
@@ -536,8 +599,8 @@ int gp5_ctx_destroy(gp5_ctx *c)
     auto note = [&](hipError_t e) { if (first == hipSuccess) first = e; };
     if (c->st) note(hipStreamSynchronize(c->st));
     for (int d = 0; d < 3; ++d) if (c->tw[d]) note(hipFree(c->tw[d]));
-    void *ptrs[] = {c->T,       c->kA,      c->kB,     c->sbuf,   c->eout,     c->phase_ex, c->partials,
-                    c->results, c->scales, c->blockmax, c->dlog};
+    void *ptrs[] = {c->T,       c->kA,      c->kB,       c->sbuf, c->eout,  c->phase_ex, c->partials,
+                    c->results, c->scales, c->blockmax, c->dlog, c->nnorm, c->lhy_ab, c->Vdev};
     for (void *p : ptrs) if (p) note(hipFree(p));
     if (c->ev0) note(hipEventDestroy(c->ev0));
     if (c->ev1) note(hipEventDestroy(c->ev1));
@@ -689,12 +752,12 @@ static int norm_impl(gp5_ctx *c, const double2 *f, int comp, double N, double *o
     E.Nnorm[0] = N;
     E.scale_out = scale_out;
     HIPCHK(launch_norm(c->st, f + M * comp, M, blocks, E));
-    HIPCHK(launch_finalize(c->st, E, blocks));
+    HIPCHK(launch_finalize(c->st, E, blocks, 0, 1, 1));
     if (out) {
-        double r[kRedQ];
-        HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+        double r = 0.0;
+        HIPCHK(hipMemcpyAsync(&r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
         HIPCHK(hipStreamSynchronize(c->st));
-        *out = r[0];
+        *out = r;
     }
     return GP5_OK;
 }
@@ -713,9 +776,9 @@ int gp5_normalize(gp5_ctx *c, gp5_buf *f, int comp, double N)
     RCHK(check_field(c, f, "field"));
     if (comp < 0 || comp >= c->nc) return fail(GP5_ERR_ARG, "component %d out of range", comp);
     double nrm = 0.0;
-    RCHK(norm_impl(c, (const double2 *)f->d, comp, N, &nrm, c->scales + 14));
+    RCHK(norm_impl(c, (const double2 *)f->d, comp, N, &nrm, scale_slot(c, 7)));
     if (!std::isfinite(nrm) || nrm <= 0.0) return fail(GP5_ERR_NONFINITE, "norm of component %d is %g", comp, nrm);
-    HIPCHK(launch_scale(c->st, (double2 *)f->d, c->M, c->nc, comp, 0.0, c->scales + 14));
+    HIPCHK(launch_scale(c->st, (double2 *)f->d, c->M, c->nc, comp, 0.0, scale_slot(c, 7)));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -750,10 +813,12 @@ int gp5_energy(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf *psi, dou
     E.phi = (const double2 *)psi->d;
     E.want_red = 1;
     RCHK(spectral(c, (const double2 *)psi->d, coef, E));
-    double r[kRedQ];
-    HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+    std::vector<double> r((size_t)kRedN * c->nc);
+    HIPCHK(hipMemcpyAsync(r.data(), c->results, r.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    *out = r[2] + (c->nc > 1 ? r[3] : 0.0);
+    double e = 0.0;
+    for (int k = 0; k < c->nc; ++k) e += r[(size_t)c->nc + k];
+    *out = e;
     return GP5_OK;
 }
 
@@ -777,16 +842,17 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     RCHK(fill_functional(c, f, E, coef));
     E.Nnorm[0] = s->normalizations[0];
     E.Nnorm[1] = s->normalizations[1];
+    E.Nnorm_dev = c->nnorm;  // batch classes: one target per component
     if (s->phase) E.phase_exp = c->phase_ex;  // filled by advance_impl
     const bool rn = s->renormalize != 0;
-    double *slot_psi = psi_slot >= 0 ? c->scales + 2 * psi_slot : nullptr;
+    double *slot_psi = psi_slot >= 0 ? scale_slot(c, psi_slot) : nullptr;
     const int next_psi_slot = psi_slot == 0 ? 1 : 0;
-    double *slotA = c->scales + 4, *slotB = c->scales + 6;
+    double *slotA = scale_slot(c, 2), *slotB = scale_slot(c, 3);
     double2 *P = (double2 *)psi->d;
     const double tr = s->tau_re, ti = s->tau_im;
     // reductions only where a result is consumed: lazy renormalisation scales,
     // the per-step log (first stage), and the non-finite check after the last step
-    E.log = dlog ? dlog + (size_t)step * kRedQ : nullptr;
+    E.log = dlog ? dlog + (size_t)step * kRedN * c->nc : nullptr;
     // stage 1 input: psi itself; its lazy scale is applied in the z pass
     StageIn in = plain_input(P);
     in.mid_scale = slot_psi;
@@ -798,7 +864,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
         E.cr = tr;
         E.ci = ti;
         E.out = c->eout;
-        E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
+        E.scale_out = rn ? scale_slot(c, next_psi_slot) : nullptr;
         E.want_red = rn || dlog || last;
         RCHK(stage_passes(c, in, coef, E, fused));
         // the new state lives in eout: swap device pointers with the caller's buffer
@@ -853,7 +919,7 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     E.cr = (1. / 6) * tr;
     E.ci = (1. / 6) * ti;
     E.out = c->eout;
-    E.scale_out = rn ? c->scales + 2 * next_psi_slot : nullptr;
+    E.scale_out = rn ? scale_slot(c, next_psi_slot) : nullptr;
     E.want_red = rn || last;
     RCHK(stage_passes(c, in, coef, E, fused));
     // psi' lives in eout: swap device pointers with the caller's buffer
@@ -864,33 +930,42 @@ static int step_once(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper
     return GP5_OK;
 }
 
-static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
-                        int nsteps, double *norm_log, double *energy_log, double *elapsed_ms)
+// advance = prepare (checks, allocations: nothing of it may run under graph
+// capture) + enqueue (kernels only; capturable) + collect (read-back, checks).
+static int advance_prepare(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                           int nsteps, bool logging)
 {
     if (!c || !f || !s) return fail(GP5_ERR_ARG, "NULL argument");
     RCHK(check_field(c, psi, "psi"));
     if (s->kind != GP5_STEP_EULERO && s->kind != GP5_STEP_RK4) return fail(GP5_ERR_ARG, "unknown stepper kind %d", s->kind);
     if (nsteps < 0) return fail(GP5_ERR_ARG, "nsteps < 0");
     if (s->renormalize) {
-        for (int k = 0; k < c->nc; ++k)
-            if (!(s->normalizations[k] > 0.0)) return fail(GP5_ERR_ARG, "normalization %d must be > 0", k);
+        if (c->nc > 2 && !c->nnorm)
+            return fail(GP5_ERR_UNSUPPORTED, "renormalisation of more than two components needs per-component targets (batch API)");
+        for (int k = 0; k < std::min(c->nc, 2); ++k)
+            if (!c->nnorm && !(s->normalizations[k] > 0.0)) return fail(GP5_ERR_ARG, "normalization %d must be > 0", k);
     }
-    if (nsteps == 0) return GP5_OK;
     RCHK(ensure_scratch(c, s->kind == GP5_STEP_RK4, true));
     if (s->phase) {
         RCHK(check_real(c, s->phase, "phase"));
-        if (!c->phase_ex) HIPCHK(hipMalloc(&c->phase_ex, (size_t)c->M * c->nc * sizeof(double2)));
+        if (!c->phase_ex) HIPCHK(hipMalloc(&c->phase_ex, (size_t)c->M * c->nc_cap * sizeof(double2)));
         HIPCHK(launch_phase_exp(c->st, c->phase_ex, s->phase->d, c->M * c->nc));
     }
-    RCHK(ensure_partials(c, axis_grid(c->geom[0])));
-    const bool logging = norm_log || energy_log;
+    size_t blocks = axis_grid(c->geom[0]);
+    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O));
+    RCHK(ensure_partials(c, blocks));
     if (logging && (size_t)nsteps > c->dlog_cap) {
         if (c->dlog) HIPCHK(hipFree(c->dlog));
         c->dlog = nullptr;
-        HIPCHK(hipMalloc(&c->dlog, (size_t)nsteps * kRedQ * sizeof(double)));
+        HIPCHK(hipMalloc(&c->dlog, (size_t)nsteps * kRedN * c->nc_cap * sizeof(double)));
         c->dlog_cap = nsteps;
     }
-    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev0, c->st));
+    return GP5_OK;
+}
+
+static int advance_enqueue(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                           int nsteps, bool logging)
+{
     int psi_slot = -1;
     // fused stages: the epilogue pass of each stage also forward-transforms its
     // output along x, so only the first stage needs a separate x pass
@@ -899,29 +974,56 @@ static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     for (int st = 0; st < nsteps; ++st)
         RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr, fused, st == nsteps - 1));
     if (psi_slot >= 0)
-        PLAUNCH(c, K_SCALE, launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, c->scales + 2 * psi_slot));
-    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev1, c->st));
-    double r[kRedQ];
-    HIPCHK(hipMemcpyAsync(r, c->results, sizeof r, hipMemcpyDeviceToHost, c->st));
+        PLAUNCH(c, K_SCALE, launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, scale_slot(c, psi_slot)));
+    return GP5_OK;
+}
+
+// Reads the last step's output norms (non-finite check) and the per-step logs:
+// norm_log[s*nc + c] and energy_log[s] (total), energy_comp_log[s*nc + c].
+static int advance_collect(gp5_ctx *c, const gp5_functional_desc *f, int nsteps, double *norm_log, double *energy_log,
+                           double *energy_comp_log)
+{
+    const int nc = c->nc;
+    std::vector<double> r((size_t)nc);
+    HIPCHK(hipMemcpyAsync(r.data(), c->results, r.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    const bool logging = norm_log || energy_log || energy_comp_log;
     std::vector<double> lg;
     if (logging) {
-        lg.resize((size_t)nsteps * kRedQ);
+        lg.resize((size_t)nsteps * kRedN * nc);
         HIPCHK(hipMemcpyAsync(lg.data(), c->dlog, lg.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     }
     HIPCHK(hipStreamSynchronize(c->st));
+    const bool no_energy = f->kind == GP5_FUNC_LHY_UNLOCKED;
+    for (int st = 0; logging && st < nsteps; ++st) {
+        const double *q = &lg[(size_t)st * kRedN * nc];  // [quantity][component]
+        double e = 0.0;
+        for (int k = 0; k < nc; ++k) {
+            if (norm_log) norm_log[(size_t)st * nc + k] = q[2 * nc + k];
+            if (energy_comp_log) energy_comp_log[(size_t)st * nc + k] = no_energy ? NAN : q[nc + k];
+            e += q[nc + k];
+        }
+        if (energy_log) energy_log[st] = no_energy ? NAN : e;
+    }
+    for (int k = 0; k < nc; ++k)
+        if (!std::isfinite(r[k])) return fail(GP5_ERR_NONFINITE, "norm of component %d became %g during the step", k, r[k]);
+    return GP5_OK;
+}
+
+static int advance_impl(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi,
+                        int nsteps, double *norm_log, double *energy_log, double *elapsed_ms)
+{
+    const bool logging = norm_log || energy_log;
+    RCHK(advance_prepare(c, f, s, psi, nsteps, logging));
+    if (nsteps == 0) return GP5_OK;
+    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev0, c->st));
+    RCHK(advance_enqueue(c, f, s, psi, nsteps, logging));
+    if (elapsed_ms) HIPCHK(hipEventRecord(c->ev1, c->st));
+    RCHK(advance_collect(c, f, nsteps, norm_log, energy_log, nullptr));
     if (elapsed_ms) {
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         *elapsed_ms = ms;
     }
-    for (int st = 0; logging && st < nsteps; ++st) {
-        const double *q = &lg[(size_t)st * kRedQ];
-        if (norm_log)
-            for (int k = 0; k < c->nc; ++k) norm_log[(size_t)st * c->nc + k] = q[4 + k];
-        if (energy_log) energy_log[st] = f->kind == GP5_FUNC_LHY_UNLOCKED ? NAN : q[2] + (c->nc > 1 ? q[3] : 0.0);
-    }
-    for (int k = 0; k < c->nc; ++k)
-        if (!std::isfinite(r[k])) return fail(GP5_ERR_NONFINITE, "norm of component %d became %g during the step", k, r[k]);
     return GP5_OK;
 }
 
@@ -968,6 +1070,505 @@ int gp5_profile_advance(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     }
     for (hipEvent_t ev : p.ev) (void)hipEventDestroy(ev);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches of independent instances (no counterpart in the reference)
+//
+// The instances that share a grid and box form one class: they are the
+// components of one context, so every pass is one launch over all of them.
+// Ground states run the reference's imaginary-time RK4 with per-stage
+// renormalisation (src/stepper.cpp:46-70 as src/main.cpp:131-145 sets it up)
+// in hipGraph replays of an even number of steps (the state buffer pointers
+// return to their places after an even number of steps).  After each replay
+// the per-instance energies are read; an instance whose energy moved by at
+// most tol relative since the previous replay is retired: its field is copied
+// out and the class's last active instance takes its slot, so the class's
+// launches shrink to the active instances and the graph is captured again.
+// ---------------------------------------------------------------------------
+struct BatchClass {
+    int n = 0;
+    double L = 0.0;
+    gp5_ctx *ctx = nullptr;
+    gp5_buf *psi = nullptr;      // 2*M*cap doubles; ->n tracks the active slots
+    std::vector<int> slot_inst;  // slot -> instance; slots < active are live
+    int active = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_nc = -1;
+    int graph_steps = 0;
+    long long steps_done = 0;
+    std::vector<double> prevE;   // per slot: energy at the previous check
+    std::vector<int> checks;     // per slot: checks done
+    std::vector<double> host_log;
+};
+
+struct gp5_batch {
+    int count = 0;
+    std::vector<int> inst_class, inst_slot;  // instance -> class, slot (-1: retired)
+    std::vector<double> N;
+    std::vector<BatchClass> cls;
+    std::vector<double *> result;            // retired instances: their field (device)
+    std::vector<long long> steps;
+    std::vector<double> energy;
+};
+
+static void batch_drop_graph(BatchClass &k)
+{
+    if (k.exec) (void)hipGraphExecDestroy(k.exec);
+    if (k.graph) (void)hipGraphDestroy(k.graph);
+    k.exec = nullptr;
+    k.graph = nullptr;
+    k.graph_nc = -1;
+}
+
+static int batch_check_inst(const gp5_batch *b, int i)
+{
+    if (!b) return fail(GP5_ERR_ARG, "NULL batch");
+    if (i < 0 || i >= b->count) return fail(GP5_ERR_ARG, "instance %d out of range (batch of %d)", i, b->count);
+    return GP5_OK;
+}
+
+// Retire slot s of class k: copy its field out, move the last active slot in.
+static int batch_retire(gp5_batch *b, BatchClass &k, int s, long long steps, double energy)
+{
+    gp5_ctx *c = k.ctx;
+    const long long M = c->M;
+    const int inst = k.slot_inst[s];
+    double *dst = nullptr;
+    HIPCHK(hipMalloc(&dst, (size_t)M * sizeof(double2)));
+    HIPCHK(hipMemcpyAsync(dst, k.psi->d + (size_t)2 * M * s, (size_t)M * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
+    b->result[inst] = dst;
+    b->steps[inst] = steps;
+    b->energy[inst] = energy;
+    b->inst_slot[inst] = -1;
+    const int a = k.active - 1;
+    if (s != a) {
+        HIPCHK(hipMemcpyAsync(k.psi->d + (size_t)2 * M * s, k.psi->d + (size_t)2 * M * a, (size_t)M * sizeof(double2),
+                              hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->nnorm + s, c->nnorm + a, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        if (c->lhy_ab)
+            HIPCHK(hipMemcpyAsync(c->lhy_ab + 2 * s, c->lhy_ab + 2 * a, 2 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        if (c->Vdev)
+            HIPCHK(hipMemcpyAsync(c->Vdev + (size_t)M * s, c->Vdev + (size_t)M * a, (size_t)M * sizeof(double),
+                                  hipMemcpyDeviceToDevice, c->st));
+        k.slot_inst[s] = k.slot_inst[a];
+        b->inst_slot[k.slot_inst[s]] = s;
+        k.prevE[s] = k.prevE[a];
+        k.checks[s] = k.checks[a];
+    }
+    k.active = a;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+// The class's context now runs its active slots only.
+static void batch_shrink(BatchClass &k)
+{
+    gp5_ctx *c = k.ctx;
+    c->nc = std::max(1, k.active);
+    set_geometry(c);
+    k.psi->n = (size_t)2 * c->M * c->nc;
+}
+
+static gp5_stepper_desc batch_stepper(const BatchClass &k, double dt_factor)
+{
+    gp5_stepper_desc s;
+    memset(&s, 0, sizeof s);
+    s.kind = GP5_STEP_RK4;
+    const double dx = k.L / k.n;
+    s.tau_re = dt_factor * dx * dx;  // imaginary time: tau = (dt, 0)
+    s.tau_im = 0.0;
+    s.renormalize = 1;
+    s.normalizations[0] = s.normalizations[1] = 1.0;  // the class's device targets apply
+    return s;
+}
+
+// Median-free timing of `steps` steps of a class on its own stream (ms).
+static int batch_time(BatchClass &k, const gp5_functional_desc *f, const gp5_stepper_desc *s, int steps, float *ms)
+{
+    gp5_ctx *c = k.ctx;
+    RCHK(advance_prepare(c, f, s, k.psi, steps + 1, false));
+    RCHK(advance_enqueue(c, f, s, k.psi, 1, false));  // warm-up
+    HIPCHK(hipEventRecord(c->ev0, c->st));
+    RCHK(advance_enqueue(c, f, s, k.psi, steps, false));
+    HIPCHK(hipEventRecord(c->ev1, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return GP5_OK;
+}
+
+extern "C" {
+
+int gp5_batch_create(gp5_batch **out, int count, const int *n, const double *L, const double *N)
+{
+    if (!out || !n || !L || !N) return fail(GP5_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (count < 1) return fail(GP5_ERR_ARG, "a batch needs at least one instance");
+    for (int i = 0; i < count; ++i) {
+        if (n[i] < 1 || n[i] > kMaxLen) return fail(GP5_ERR_UNSUPPORTED, "instance %d: grid %d", i, n[i]);
+        if (!(L[i] > 0.0)) return fail(GP5_ERR_ARG, "instance %d: box length %g", i, L[i]);
+        if (!(N[i] > 0.0)) return fail(GP5_ERR_ARG, "instance %d: normalization %g", i, N[i]);
+    }
+    gp5_batch *b = new gp5_batch;
+    b->count = count;
+    b->inst_class.assign(count, -1);
+    b->inst_slot.assign(count, -1);
+    b->N.assign(N, N + count);
+    b->result.assign(count, nullptr);
+    b->steps.assign(count, 0);
+    b->energy.assign(count, NAN);
+    // classes: instances with the same (n, L), at most kMaxComp per class
+    for (int i = 0; i < count; ++i) {
+        int k = -1;
+        for (size_t j = 0; j < b->cls.size(); ++j)
+            if (b->cls[j].n == n[i] && b->cls[j].L == L[i] && (int)b->cls[j].slot_inst.size() < kMaxComp) k = (int)j;
+        if (k < 0) {
+            b->cls.emplace_back();
+            k = (int)b->cls.size() - 1;
+            b->cls[k].n = n[i];
+            b->cls[k].L = L[i];
+        }
+        b->inst_class[i] = k;
+        b->inst_slot[i] = (int)b->cls[k].slot_inst.size();
+        b->cls[k].slot_inst.push_back(i);
+    }
+    for (BatchClass &k : b->cls) {
+        const int nc = (int)k.slot_inst.size();
+        const int shape[3] = {k.n, k.n, k.n};
+        const double left[3] = {-0.5 * k.L, -0.5 * k.L, -0.5 * k.L}, right[3] = {0.5 * k.L, 0.5 * k.L, 0.5 * k.L};
+        int rc = gp5_ctx_create(&k.ctx, shape, left, right, nc);
+        if (rc == GP5_OK) k.ctx->uncoupled = true;
+        if (rc == GP5_OK) rc = gp5_buf_alloc(k.ctx, (size_t)2 * k.ctx->M * nc, &k.psi);
+        if (rc == GP5_OK && hipMalloc(&k.ctx->nnorm, nc * sizeof(double)) != hipSuccess)
+            rc = fail(GP5_ERR_ALLOC, "batch normalisations");
+        if (rc == GP5_OK) {
+            std::vector<double> h(nc);
+            for (int sl = 0; sl < nc; ++sl) h[sl] = N[k.slot_inst[sl]];
+            if (hipMemcpy(k.ctx->nnorm, h.data(), nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(GP5_ERR_HIP, "batch normalisations upload");
+        }
+        if (rc != GP5_OK) {
+            gp5_batch_destroy(b);
+            return rc;
+        }
+        k.active = nc;
+        k.prevE.assign(nc, NAN);
+        k.checks.assign(nc, 0);
+    }
+    *out = b;
+    return GP5_OK;
+}
+
+int gp5_batch_destroy(gp5_batch *b)
+{
+    if (!b) return GP5_OK;
+    int rc = GP5_OK;
+    for (BatchClass &k : b->cls) {
+        batch_drop_graph(k);
+        if (k.psi) {
+            int r = gp5_buf_free(k.psi);
+            if (rc == GP5_OK) rc = r;
+        }
+        if (k.ctx) {
+            int r = gp5_ctx_destroy(k.ctx);
+            if (rc == GP5_OK) rc = r;
+        }
+    }
+    for (double *p : b->result)
+        if (p) (void)hipFree(p);
+    delete b;
+    return rc;
+}
+
+int gp5_batch_classes(const gp5_batch *b) { return b ? (int)b->cls.size() : 0; }
+
+int gp5_batch_class_of(const gp5_batch *b, int i)
+{
+    if (batch_check_inst(b, i) != GP5_OK) return -1;
+    return b->inst_class[i];
+}
+
+int gp5_batch_upload(gp5_batch *b, int i, const double *host)
+{
+    RCHK(batch_check_inst(b, i));
+    if (!host) return fail(GP5_ERR_ARG, "NULL host array");
+    const int s = b->inst_slot[i];
+    if (s < 0) return fail(GP5_ERR_ARG, "instance %d is retired (its ground state is final)", i);
+    BatchClass &k = b->cls[b->inst_class[i]];
+    const long long M = k.ctx->M;
+    HIPCHK(hipMemcpy(k.psi->d + (size_t)2 * M * s, host, (size_t)M * sizeof(double2), hipMemcpyHostToDevice));
+    return GP5_OK;
+}
+
+int gp5_batch_download(const gp5_batch *b, int i, double *host)
+{
+    RCHK(batch_check_inst(b, i));
+    if (!host) return fail(GP5_ERR_ARG, "NULL host array");
+    const BatchClass &k = b->cls[b->inst_class[i]];
+    const long long M = k.ctx->M;
+    const int s = b->inst_slot[i];
+    const double *src = s >= 0 ? k.psi->d + (size_t)2 * M * s : b->result[i];
+    HIPCHK(hipMemcpy(host, src, (size_t)M * sizeof(double2), hipMemcpyDeviceToHost));
+    return GP5_OK;
+}
+
+int gp5_batch_set_lhy(gp5_batch *b, int i, double a, double bcoef)
+{
+    RCHK(batch_check_inst(b, i));
+    const int s = b->inst_slot[i];
+    if (s < 0) return fail(GP5_ERR_ARG, "instance %d is retired", i);
+    BatchClass &k = b->cls[b->inst_class[i]];
+    gp5_ctx *c = k.ctx;
+    if (!c->lhy_ab) {
+        std::vector<double> h(2 * (size_t)c->nc_cap);
+        for (int sl = 0; sl < c->nc_cap; ++sl) {
+            h[2 * sl] = -3.0;  // LHYDropletFunctional's coefficients (src/functional.cpp:117-118)
+            h[2 * sl + 1] = 2.5;
+        }
+        HIPCHK(hipMalloc(&c->lhy_ab, h.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(c->lhy_ab, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const double ab[2] = {a, bcoef};
+    HIPCHK(hipMemcpy(c->lhy_ab + 2 * s, ab, sizeof ab, hipMemcpyHostToDevice));
+    batch_drop_graph(k);
+    return GP5_OK;
+}
+
+int gp5_batch_set_potential(gp5_batch *b, int i, const double *V)
+{
+    RCHK(batch_check_inst(b, i));
+    if (!V) return fail(GP5_ERR_ARG, "NULL potential");
+    const int s = b->inst_slot[i];
+    if (s < 0) return fail(GP5_ERR_ARG, "instance %d is retired", i);
+    BatchClass &k = b->cls[b->inst_class[i]];
+    gp5_ctx *c = k.ctx;
+    if (!c->Vdev) {
+        HIPCHK(hipMalloc(&c->Vdev, (size_t)c->M * c->nc_cap * sizeof(double)));
+        HIPCHK(hipMemset(c->Vdev, 0, (size_t)c->M * c->nc_cap * sizeof(double)));
+    }
+    HIPCHK(hipMemcpy(c->Vdev + (size_t)c->M * s, V, (size_t)c->M * sizeof(double), hipMemcpyHostToDevice));
+    batch_drop_graph(k);
+    return GP5_OK;
+}
+
+int gp5_batch_advance(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, int nsteps)
+{
+    if (!b || !f) return fail(GP5_ERR_ARG, "NULL argument");
+    if (f->potential) return fail(GP5_ERR_ARG, "batch potentials are set per instance (gp5_batch_set_potential)");
+    for (BatchClass &k : b->cls) {
+        if (k.active == 0) continue;
+        gp5_stepper_desc s = batch_stepper(k, dt_factor);
+        RCHK(advance_impl(k.ctx, f, &s, k.psi, nsteps, nullptr, nullptr, nullptr));
+        k.steps_done += nsteps;
+    }
+    return GP5_OK;
+}
+
+int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, int steps, char *report, size_t report_len)
+{
+    if (!b || !f) return fail(GP5_ERR_ARG, "NULL argument");
+    if (steps < 1) return fail(GP5_ERR_ARG, "steps < 1");
+    std::string rep;
+    for (BatchClass &k : b->cls) {
+        if (k.active == 0) continue;
+        gp5_ctx *c = k.ctx;
+        batch_drop_graph(k);
+        gp5_stepper_desc s = batch_stepper(k, dt_factor);
+        // tune on a copy: the class state is restored afterwards
+        const size_t bytes = (size_t)c->M * c->nc * sizeof(double2);
+        double *save = nullptr;
+        HIPCHK(hipMalloc(&save, bytes));
+        HIPCHK(hipMemcpy(save, k.psi->d, bytes, hipMemcpyDeviceToDevice));
+        float best = 0.f;
+        int rc = batch_time(k, f, &s, steps, &best);
+        char line[256];
+        snprintf(line, sizeof line, "class n=%d L=%g instances=%d: default %.3f ms/step", k.n, k.L, k.active, best / steps);
+        rep += line;
+        for (int d = 0; rc == GP5_OK && d < 3; ++d) {
+            if (c->fast[d]) continue;
+            // radix order of the generic plan
+            AxisPlan keep = c->plan[d];
+            int best_order = -1;
+            for (int o = 0; rc == GP5_OK && o < kPlanOrders; ++o) {
+                AxisPlan P;
+                if (make_plan(c->n[d], P, o)) continue;
+                P.tw = c->tw[d];
+                if (P.nrad == keep.nrad && std::equal(P.rad, P.rad + P.nrad, keep.rad)) continue;
+                c->plan[d] = P;
+                float t = 0.f;
+                rc = batch_time(k, f, &s, steps, &t);
+                if (rc == GP5_OK && t < best) {
+                    best = t;
+                    best_order = o;
+                    keep = P;
+                }
+            }
+            c->plan[d] = keep;
+            // tile: lines per block
+            const int b0 = c->geom[d].B;
+            int best_B = b0;
+            for (int B = 1; rc == GP5_OK && B <= 64; B *= 2) {
+                if (B == b0 || B * c->n[d] > kTileElems) continue;
+                c->geom[d].B = B;
+                set_geometry(c);
+                if (c->geom[d].B != B) continue;
+                float t = 0.f;
+                rc = batch_time(k, f, &s, steps, &t);
+                if (rc == GP5_OK && t < best) {
+                    best = t;
+                    best_B = B;
+                }
+            }
+            c->geom[d].B = best_B;
+            set_geometry(c);
+            std::string radices;
+            for (int r = 0; r < c->plan[d].nrad; ++r) radices += (r ? "x" : "") + std::to_string(c->plan[d].rad[r]);
+            snprintf(line, sizeof line, "; axis %d plan %s%s tile %d", d, radices.c_str(), best_order >= 0 ? " (tuned)" : "",
+                     c->geom[d].B);
+            rep += line;
+        }
+        if (rc == GP5_OK && c->fast[0]) {
+            const bool f0 = c->fuse_x;
+            c->fuse_x = !f0;
+            float t = 0.f;
+            rc = batch_time(k, f, &s, steps, &t);
+            if (rc == GP5_OK && t < best) best = t;
+            else c->fuse_x = f0;
+            snprintf(line, sizeof line, "; fused x %s", c->fuse_x ? "on" : "off");
+            rep += line;
+        }
+        snprintf(line, sizeof line, "; tuned %.3f ms/step\n", best / steps);
+        rep += line;
+        hipError_t e = hipMemcpy(k.psi->d, save, bytes, hipMemcpyDeviceToDevice);
+        (void)hipFree(save);
+        RCHK(rc);
+        HIPCHK(e);
+    }
+    if (report && report_len) {
+        strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return GP5_OK;
+}
+
+int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, double tol, int check_every,
+                           long long max_steps, long long *steps_out, double *energy_out, double *elapsed_ms)
+{
+    if (!b || !f) return fail(GP5_ERR_ARG, "NULL argument");
+    if (f->potential) return fail(GP5_ERR_ARG, "batch potentials are set per instance (gp5_batch_set_potential)");
+    if (f->kind == GP5_FUNC_LHY_UNLOCKED) return fail(GP5_ERR_UNSUPPORTED, "batches hold uncoupled instances");
+    if (check_every < 2 || check_every % 2) return fail(GP5_ERR_ARG, "check_every must be even and >= 2 (got %d)", check_every);
+    if (!(tol > 0.0)) return fail(GP5_ERR_ARG, "tol must be > 0");
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventRecord(t0, b->cls[0].ctx->st));
+    HIPCHK(hipStreamSynchronize(b->cls[0].ctx->st));
+    const auto wall0 = std::chrono::steady_clock::now();
+    int rc = GP5_OK;
+    for (;;) {
+        bool any = false;
+        // launch every class's replay on its own stream, then collect
+        for (BatchClass &k : b->cls) {
+            if (k.active == 0 || rc != GP5_OK) continue;
+            any = true;
+            gp5_ctx *c = k.ctx;
+            gp5_stepper_desc s = batch_stepper(k, dt_factor);
+            if (!k.exec || k.graph_nc != k.active || k.graph_steps != check_every) {
+                batch_drop_graph(k);
+                rc = advance_prepare(c, f, &s, k.psi, check_every, true);
+                if (rc != GP5_OK) break;
+                if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    rc = fail(GP5_ERR_HIP, "stream capture failed to start");
+                    break;
+                }
+                double *d0 = k.psi->d;
+                rc = advance_enqueue(c, f, &s, k.psi, check_every, true);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(c->st, &g);
+                if (rc == GP5_OK && e != hipSuccess) rc = fail(GP5_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+                if (rc == GP5_OK && k.psi->d != d0) rc = fail(GP5_ERR_HIP, "graph does not return the state to its buffer");
+                if (rc == GP5_OK && hipGraphInstantiate(&k.exec, g, nullptr, nullptr, 0) != hipSuccess)
+                    rc = fail(GP5_ERR_HIP, "graph instantiation failed");
+                k.graph = g;
+                if (rc != GP5_OK) break;
+                k.graph_nc = k.active;
+                k.graph_steps = check_every;
+            }
+            if (hipGraphLaunch(k.exec, c->st) != hipSuccess) {
+                rc = fail(GP5_ERR_HIP, "graph launch failed");
+                break;
+            }
+        }
+        if (rc != GP5_OK || !any) break;
+        for (BatchClass &k : b->cls) {
+            if (k.active == 0 || k.graph_nc != k.active) continue;
+            gp5_ctx *c = k.ctx;
+            const int nc = k.active;
+            k.host_log.resize((size_t)kRedN * nc + nc);
+            // last step's row of the log: [quantity][slot]; then the last output norms
+            const double *row = c->dlog + (size_t)(check_every - 1) * kRedN * nc;
+            hipError_t e = hipMemcpyAsync(k.host_log.data(), row, (size_t)kRedN * nc * sizeof(double),
+                                          hipMemcpyDeviceToHost, c->st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(k.host_log.data() + (size_t)kRedN * nc, c->results, nc * sizeof(double),
+                                   hipMemcpyDeviceToHost, c->st);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            if (e != hipSuccess) {
+                rc = fail(GP5_ERR_HIP, "batch read-back: %s", hipGetErrorString(e));
+                break;
+            }
+            k.steps_done += check_every;
+            std::vector<int> done;
+            std::vector<double> done_e;
+            for (int sl = 0; sl < nc; ++sl) {
+                const double E = k.host_log[(size_t)nc + sl];
+                const double nrm = k.host_log[(size_t)kRedN * nc + sl];
+                if (!std::isfinite(E) || !std::isfinite(nrm)) {
+                    rc = fail(GP5_ERR_NONFINITE, "instance %d: energy %g, norm %g", k.slot_inst[sl], E, nrm);
+                    break;
+                }
+                if (k.checks[sl] > 0 && std::fabs(E - k.prevE[sl]) <= tol * std::fabs(E)) {
+                    done.push_back(sl);
+                    done_e.push_back(E);
+                }
+                k.prevE[sl] = E;
+                k.checks[sl]++;
+            }
+            if (rc != GP5_OK) break;
+            const bool out_of_steps = max_steps > 0 && k.steps_done >= max_steps;
+            if (out_of_steps) {
+                // retire everything left; unconverged instances report negative steps
+                for (int sl = k.active - 1; sl >= 0 && rc == GP5_OK; --sl) {
+                    const bool conv = std::find(done.begin(), done.end(), sl) != done.end();
+                    rc = batch_retire(b, k, sl, conv ? k.steps_done : -k.steps_done, k.prevE[sl]);
+                }
+            } else {
+                // retire from the highest slot down so moved slots are not revisited
+                for (int j = (int)done.size() - 1; j >= 0 && rc == GP5_OK; --j)
+                    rc = batch_retire(b, k, done[j], k.steps_done, done_e[j]);
+            }
+            if (rc != GP5_OK) break;
+            if (k.active != k.graph_nc) {
+                batch_drop_graph(k);
+                batch_shrink(k);
+            }
+        }
+        if (rc != GP5_OK) break;
+    }
+    const double wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    RCHK(rc);
+    for (int i = 0; i < b->count; ++i) {
+        if (steps_out) steps_out[i] = b->steps[i];
+        if (energy_out) energy_out[i] = b->energy[i];
+    }
+    if (elapsed_ms) *elapsed_ms = wall_ms;
+    return GP5_OK;
 }
 
 }  // extern "C"

@@ -241,23 +241,24 @@ __device__ __forceinline__ double2 load_pre(const double2 *src, long long g, con
     return v;
 }
 
-// Per-thread reduction sums of the epilogue for the thread's component c; they
-// are placed into the component slots of the block reduction once per thread
-// (acc_scatter), not per element: a runtime index into the register array
-// would cost a compare-and-select chain over every slot at each update.
+// Per-thread reduction sums of the epilogue for the thread's component; they
+// are placed into the block's component slot once per thread (acc_scatter),
+// not per element: a runtime index into the register array would cost a
+// compare-and-select chain over every slot at each update.
 struct EpiAcc {
     double o2 = 0.0, en = 0.0, rho = 0.0;
 };
 
-__device__ __forceinline__ void acc_scatter(const EpiAcc &a, int c, double (&red)[kRedQ])
+// slot: the thread's component minus the block's first component (0 or 1)
+__device__ __forceinline__ void acc_scatter(const EpiAcc &a, int slot, double (&red)[kRedQ])
 {
-    const bool c0 = c == 0;
-    red[0] += c0 ? a.o2 : 0.0;
-    red[1] += c0 ? 0.0 : a.o2;
-    red[2] += c0 ? a.en : 0.0;
-    red[3] += c0 ? 0.0 : a.en;
-    red[4] += c0 ? a.rho : 0.0;
-    red[5] += c0 ? 0.0 : a.rho;
+    const bool s0 = slot == 0;
+    red[0] += s0 ? a.o2 : 0.0;
+    red[1] += s0 ? a.en : 0.0;
+    red[2] += s0 ? a.rho : 0.0;
+    red[3] += s0 ? 0.0 : a.o2;
+    red[4] += s0 ? 0.0 : a.en;
+    red[5] += s0 ? 0.0 : a.rho;
 }
 
 // Epilogue of the last inverse x pass over CH elements of one line (one
@@ -272,7 +273,7 @@ __device__ __forceinline__ void acc_scatter(const EpiAcc &a, int c, double (&red
 //   gpFunctional (:13-43,54-89)   nl = g11 rho (1 comp), gcc rho_c + g12 rho_o (2 comps)
 //   UnlockedLHYDroplet (:125-150) nl_0 = rho1 + eta rho2 + alpha (rho1 + beta rho2)^{3/2},
 //                                 nl_1 = eta beta rho1 + beta rho2 + alpha beta^2 (...)^{3/2}
-// Reductions: red[c] |o|^2, red[2+c] energy density of phi, red[4+c] |phi|^2.
+// Reductions (acc): |o|^2, energy density of phi, |phi|^2 of the thread's component.
 // LOCAL = true compiles the form without cross-component reads and without the
 // phase constraint (one component, or two uncoupled ones, no phaseConstraint):
 // registers are allocated for the longest path of a kernel, so the general
@@ -290,7 +291,7 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
     }
     const bool rk = mode >= EPI_RK1 && mode <= EPI_RK4;
     const bool rk_k = mode == EPI_RK2 || mode == EPI_RK3 || mode == EPI_RK4;
-    const bool coupled = !LOCAL && (E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2));
+    const bool coupled = !LOCAL && (E.fkind == 2 || (E.fkind == 0 && E.has_couplings && E.nc == 2 && !E.uncoupled));
     const bool constrain = !LOCAL && E.phase_exp && (mode == EPI_EULER || mode == EPI_RK4);
     const double2 *src = rk ? E.psi : E.phi;
     const double *ssc = rk ? E.sc_psi : E.sc_phi;
@@ -332,6 +333,8 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
     const double sino = (rk_k && coupled && E.sc_in) ? E.sc_in[1 - c] : 1.0;
     const double2 cin = make_double2(E.cr_in, E.ci_in);
     const double2 cf = make_double2(E.cr, E.ci);
+    const double la = E.lhy_ab ? E.lhy_ab[2 * c] : E.lhy_a;
+    const double lb = E.lhy_ab ? E.lhy_ab[2 * c + 1] : E.lhy_b;
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
         const long long g = gbase + pos[k];
@@ -343,10 +346,11 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
         double nl = 0.0, enl = 0.0;
         if (E.fkind == 1) {
             const double sr = sqrt(rho);
-            nl = E.lhy_a * rho + E.lhy_b * rho * sr;
-            enl = 0.5 * E.lhy_a * rho * rho + 0.4 * E.lhy_b * rho * rho * sr;
+            nl = la * rho + lb * rho * sr;
+            enl = 0.5 * la * rho * rho + 0.4 * lb * rho * rho * sr;
         } else if (E.fkind == 0 && !E.has_couplings) {
-        } else if (E.fkind == 0 && E.nc == 1) {
+        } else if (E.fkind == 0 && (E.nc == 1 || E.uncoupled)) {
+            // one component, or independent instances sharing g11
             nl = E.g11 * rho;
             enl = 0.5 * E.g11 * rho * rho;
         } else {
@@ -398,14 +402,16 @@ __device__ __forceinline__ void epilogue_batch(const EpiArgs &E, long long gbase
     }
 }
 
-// Single-element form used by the generic LDS kernels.
-__device__ __forceinline__ double2 epilogue(const EpiArgs &E, long long g, int c, double2 kin, double (&red)[kRedQ])
+// Single-element form used by the generic LDS kernels (slot: c minus the
+// block's first component).
+__device__ __forceinline__ double2 epilogue(const EpiArgs &E, long long g, int c, int slot, double2 kin,
+                                           double (&red)[kRedQ])
 {
     const int pos[1] = {0};
     double2 v[1] = {kin};
     EpiAcc acc;
     epilogue_batch<1>(E, g, pos, c, v, acc);
-    acc_scatter(acc, c, red);
+    acc_scatter(acc, slot, red);
     return v[0];
 }
 

@@ -325,7 +325,7 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
             const double ky = momentum(y, md.ny, md.dk[1]);
             kxy2 = kx * kx + ky * ky;
         }
-        double f0 = valid ? md.coef[c] : 0.0;
+        double f0 = valid ? md.coef[c == 0 ? 0 : 1] : 0.0;
         if (md.scale && valid) f0 *= md.scale[c];
         constexpr int RL = FL::FWD_LAST;
 #pragma unroll
@@ -393,7 +393,8 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
             double red[kRedQ];
 #pragma unroll
             for (int q = 0; q < kRedQ; ++q) red[q] = 0.0;
-            acc_scatter(acc, c, red);
+            const int cb = (int)((long long)blockIdx.x * S::LPB / a.Opc);  // block's first component
+            acc_scatter(acc, c - cb, red);
             __syncthreads();
             reduce_block(red, a.epi.partials, reinterpret_cast<double *>(lds));
         }
@@ -447,6 +448,19 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
     }
 #undef GP5_FAST_LAUNCH
     return hipGetLastError();
+}
+
+long long fast_rows_per_block(int n)
+{
+    switch (n) {
+        case 16: return FShape<16, true>::LPB;
+        case 32: return FShape<32, true>::LPB;
+        case 64: return FShape<64, true>::LPB;
+        case 128: return FShape<128, true>::LPB;
+        case 256: return FShape<256, true>::LPB;
+        case 512: return FShape<512, true>::LPB;
+        default: return 1;
+    }
 }
 
 unsigned fast_grid(int n, long long I, long long O)

@@ -11,7 +11,9 @@ constexpr int kMaxRadices = 24;
 constexpr int kThreads = 256;          // threads per FFT workgroup (4 wave64)
 constexpr int kTileElems = 2048;       // complex elements per generic FFT tile
 constexpr int kMaxLen = 2048;          // longest axis the generic LDS FFT accepts
-constexpr int kRedQ = 6;               // reduction quantities per block (3 x 2 components)
+constexpr int kRedQ = 6;               // reduction partials per block: 3 quantities x 2 component slots
+constexpr int kRedN = 3;               // quantities: |out|^2, energy density of phi, |phi|^2
+constexpr int kMaxComp = 256;          // components of one context (more than 2: uncoupled only)
 
 // Stockham plan of one axis.  rad[] lists radices in execution order;
 // tw points to n twiddles e^{-2 pi i t / n} (device memory).
@@ -38,7 +40,7 @@ struct AxisGeom {
 struct MidArgs {
     int nx, ny, nz;
     double dk[3];
-    double coef[2];
+    double coef[2];        // component 0, components >= 1 (more than 2 components share one mass)
     long long M;
     const double *scale;   // lazy per-component scale of the transformed field, or null
 };
@@ -64,8 +66,10 @@ struct EpiArgs {
     int mode;
     int fkind;          // GP5_FUNC_*
     int has_couplings;
+    int uncoupled;      // components are independent instances sharing g11 (batches, nc > 2)
     double g11, g12, g22;
     double lhy_a, lhy_b;
+    const double *lhy_ab;  // per-component (a, b) pairs overriding lhy_a / lhy_b, or null
     double alpha, beta, eta;
     const double *V;    // real potential M*nc or null
     const double2 *phi; // H / ENERGY / EULER: input field (raw), scaled by *sc_phi
@@ -86,13 +90,16 @@ struct EpiArgs {
     long long M;
     int nc;
     int nx;
-    // fused reductions: q = 0,1 norm of out (c=0,1); 2,3 energy of phi; 4,5 norm of phi
+    // fused reductions of quantity q (0: |out|^2, 1: energy density of phi,
+    // 2: |phi|^2) per component.  A block's lines span at most two components:
+    // slot 0 holds its first line's component cb, slot 1 the component cb + 1.
     int want_red;
-    double *partials;       // [gridDim][kRedQ]
-    double *results;        // [kRedQ], final sums times dV
+    double *partials;       // [gridDim][2 slots][kRedN]
+    double *results;        // [kRedN][nc], final sums times dV
     double *log;            // optional copy of results (per-step log) or null
-    double *scale_out;      // if non-null: scale_out[c] = sqrt(N[c] / norm_out[c])
-    double Nnorm[2];
+    double *scale_out;      // if non-null: scale_out[c] = sqrt(N[c] / results[0][c])
+    double Nnorm[2];        // N[c] for up to two components ...
+    const double *Nnorm_dev;  // ... or a device array of nc values when non-null
     double dV;
 };
 
@@ -123,6 +130,7 @@ struct FastArgs {
 
 bool fast_supported(int n);
 unsigned fast_grid(int n, long long I, long long O);
+long long fast_rows_per_block(int n);  // x lines per block of the row kernels
 hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a);
 
 }  // namespace gp5

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_axis(const double2 *__restrict__ s
             const double kx = momentum(x, mid.nx, mid.dk[0]);
             const double ky = momentum(y, mid.ny, mid.dk[1]);
             const double kz = momentum(z, mid.nz, mid.dk[2]);
-            double f = mid.coef[c] * (kx * kx + ky * ky + kz * kz);
+            double f = mid.coef[c == 0 ? 0 : 1] * (kx * kx + ky * ky + kz * kz);
             if (mid.scale) f *= mid.scale[c];
             const int li = lidx(b, j, n, B, rows);
             res[li] = cscale(res[li], f);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_axis(const double2 *__restrict__ s
             if (b >= count) continue;
             const long long gi = n * o0 + e;
             const int c = (int)((o0 + b) / Opc);
-            epilogue(epi, gi, c, res[lidx(b, j, n, B, true)], red);
+            epilogue(epi, gi, c, c - (int)(o0 / Opc), res[lidx(b, j, n, B, true)], red);
         }
         if (epi.want_red) {
             __syncthreads();  // the tile in LDS is consumed; reuse it as reduction scratch
@@ -219,38 +219,51 @@ __global__ __launch_bounds__(kThreads) void k_norm(const double2 *__restrict__ f
 }
 
 // Sum the per-block partials of the preceding kernel in a fixed order and
-// publish results[q] = sum * dV, the optional per-step log copy and, for lazy
-// renormalisation, scale_out[c] = sqrt(N_c / norm_c) (src/geometry.cpp:217-229).
+// publish results[q][c] = sum * dV, the optional per-step log copy and, for
+// lazy renormalisation, scale_out[c] = sqrt(N_c / norm_c)
+// (src/geometry.cpp:217-229).  One block per component c: the partial blocks
+// whose lines [b*lpb, (b+1)*lpb) meet c's lines [c*lpc, (c+1)*lpc) hold c in
+// slot c - b*lpb/lpc (lpb = 0: every block belongs to component 0).
 constexpr int kFinalThreads = 512;
-__global__ __launch_bounds__(kFinalThreads) void k_finalize(EpiArgs E, unsigned nblocks)
+__global__ __launch_bounds__(kFinalThreads) void k_finalize(EpiArgs E, unsigned nblocks, long long lpb, long long lpc,
+                                                           int ncomp)
 {
     __shared__ double sred[kRedScratch];
+    const int c = blockIdx.x;
+    long long b0 = 0, b1 = nblocks;
+    if (lpb > 0) {
+        b0 = (long long)c * lpc / lpb;
+        b1 = min((long long)nblocks, ((long long)(c + 1) * lpc - 1) / lpb + 1);
+    }
     double acc[kRedQ];
 #pragma unroll
     for (int q = 0; q < kRedQ; ++q) acc[q] = 0.0;
-    for (unsigned b = threadIdx.x; b < nblocks; b += kFinalThreads) {
+    for (long long b = b0 + threadIdx.x; b < b1; b += kFinalThreads) {
+        const int slot = lpb > 0 ? c - (int)(b * lpb / lpc) : 0;
+        const double *p = E.partials + (size_t)b * kRedQ + kRedN * slot;
 #pragma unroll
-        for (int q = 0; q < kRedQ; ++q) acc[q] += E.partials[(size_t)b * kRedQ + q];
+        for (int q = 0; q < kRedN; ++q) acc[q] += p[q];
     }
     double s[kRedQ];
     block_sum(acc, sred, s);
     if (threadIdx.x == 0) {
-        double r[kRedQ];
+        double r[kRedN];
 #pragma unroll
-        for (int q = 0; q < kRedQ; ++q) {
+        for (int q = 0; q < kRedN; ++q) {
             r[q] = s[q] * E.dV;
-            E.results[q] = r[q];
-            if (E.log) E.log[q] = r[q];
+            E.results[q * ncomp + c] = r[q];
+            if (E.log) E.log[q * ncomp + c] = r[q];
         }
         if (E.scale_out) {
-            for (int c = 0; c < E.nc; ++c) E.scale_out[c] = sqrt(E.Nnorm[c] * (1.0 / r[c]));
+            const double N = E.Nnorm_dev ? E.Nnorm_dev[c] : E.Nnorm[c < 2 ? c : 1];
+            E.scale_out[c] = sqrt(N * (1.0 / r[0]));
         }
     }
 }
 
-hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks)
+hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks, long long lpb, long long lpc, int ncomp)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinalThreads), 0, st, E, nblocks);
+    hipLaunchKernelGGL(k_finalize, dim3(ncomp), dim3(kFinalThreads), 0, st, E, nblocks, lpb, lpc, ncomp);
     return hipGetLastError();
 }
 

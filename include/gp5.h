@@ -149,6 +149,46 @@ int gp5_profile_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_st
                         gp5_buf *psi, int nsteps, double *kind_ms, int *kind_launches);
 const char *gp5_kernel_kind_name(int kind);
 
+/* ---- batches of independent instances ------------------------------------
+ * No counterpart in the reference, which evolves one field per process: these
+ * run many single-component instances of uneven cubic grids at once, each
+ * with the reference's imaginary-time RK4 and per-stage renormalisation
+ * (RK4Stepper::advance, src/stepper.cpp:46-70, set up as src/main.cpp:131-145
+ * does), for callers that need many ground states (parameter scans, training
+ * data).  Instance i has grid n[i]^3 on the box [-L[i]/2, L[i]/2]^3 and
+ * normalisation N[i]; instances of one (n, L) share one context as its
+ * components.  Host arrays are one instance's column-major complex128 field
+ * (2 n^3 doubles) or real potential (n^3 doubles). */
+typedef struct gp5_batch gp5_batch;
+int gp5_batch_create(gp5_batch **out, int count, const int *n, const double *L, const double *N);
+int gp5_batch_destroy(gp5_batch *b);
+int gp5_batch_classes(const gp5_batch *b);    /* number of (n, L) classes */
+int gp5_batch_class_of(const gp5_batch *b, int i);
+int gp5_batch_upload(gp5_batch *b, int i, const double *host);
+int gp5_batch_download(const gp5_batch *b, int i, double *host);
+/* LHYDropletFunctional coefficients a, b of instance i (default -3, 2.5,
+ * src/functional.cpp:117-118) and its external potential (addPotential,
+ * src/functional.cpp:46-53); both apply to the active instances only. */
+int gp5_batch_set_lhy(gp5_batch *b, int i, double a, double bcoef);
+int gp5_batch_set_potential(gp5_batch *b, int i, const double *V);
+/* nsteps of imaginary-time RK4 with tau = dt_factor * dx^2 and per-stage
+ * renormalisation to N[i], on every active instance. */
+int gp5_batch_advance(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, int nsteps);
+/* Autotuner: per class, times `steps` steps of each candidate radix plan and
+ * tile (lines per block) of every generic-kernel axis and the fused x pass,
+ * keeps the fastest; the instance states are restored.  A text summary goes
+ * to report (may be NULL). */
+int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, int steps, char *report,
+                   size_t report_len);
+/* Ground states: hipGraph replays of check_every (even) steps per class until
+ * each instance's energy changes by at most tol relative between two
+ * consecutive checks; converged instances retire and their class shrinks.
+ * steps_out[i]: steps taken (negative: max_steps reached unconverged);
+ * energy_out[i]: energy at the last check; elapsed_ms: wall time. */
+int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, double tol,
+                           int check_every, long long max_steps, long long *steps_out, double *energy_out,
+                           double *elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif
