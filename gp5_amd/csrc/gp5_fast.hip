@@ -1,8 +1,10 @@
-// Register-resident Stockham FFT passes for power-of-two axis lengths 16..512.
+// Register-resident Stockham FFT passes for axis lengths 16, 32, 64, 96, 128,
+// 192, 256 and 512.
 //
 // Each thread keeps E complex values of one line in registers; a length-N
-// transform runs as 1-3 radix stages (16x16 for 256, 16x8 for 128, 8x8 for 64,
-// 8x8x8 for 512) with one LDS exchange between stages.  Global loads and stores
+// transform runs as 1-4 radix stages (16x16 for 256, 16x8 for 128, 8x8 for 64,
+// 8x8x8 for 512, 4x4x2x3 for 96, 4x4x4x3 for 192) with one LDS exchange
+// between stages.  Global loads and stores
 // go straight between HBM and registers: element (t + TL*i) + (N/R)*r of a line
 // belongs to thread t, so for fixed (i, r) consecutive threads touch consecutive
 // elements (x lines) or consecutive columns (y/z axes, columns interleaved).
@@ -23,6 +25,7 @@ template <int R>
 __device__ __forceinline__ void dftR(double2 *v, double s)
 {
     if constexpr (R == 2) dft2(v);
+    else if constexpr (R == 3) dft3(v, s);
     else if constexpr (R == 4) dft4(v, s);
     else if constexpr (R == 8) dft8(v, s);
     else if constexpr (R == 16) dft16(v, s);
@@ -33,25 +36,29 @@ __device__ __forceinline__ void dftR(double2 *v, double s)
 // The x-line kernels carry the stage epilogue, so they keep E at 8 to leave
 // registers for it (16x16 would hold 16 values per thread).
 template <int N, bool ROWS> struct FPlan;
-#define GP5_PLAN(NN, RW, A, B, C, EE) \
-    template <> struct FPlan<NN, RW> { static constexpr int R0 = A, R1 = B, R2 = C, E = EE; }
-GP5_PLAN(16, false, 16, 0, 0, 16);
-GP5_PLAN(32, false, 8, 4, 0, 8);
-GP5_PLAN(64, false, 8, 8, 0, 8);
-GP5_PLAN(128, false, 16, 8, 0, 16);
-GP5_PLAN(256, false, 16, 16, 0, 16);
-GP5_PLAN(512, false, 8, 8, 8, 8);
-GP5_PLAN(16, true, 4, 4, 0, 4);
-GP5_PLAN(32, true, 8, 4, 0, 8);
-GP5_PLAN(64, true, 8, 8, 0, 8);
+#define GP5_PLAN(NN, RW, A, B, C, D, EE) \
+    template <> struct FPlan<NN, RW> { static constexpr int R0 = A, R1 = B, R2 = C, R3 = D, E = EE; }
+GP5_PLAN(16, false, 16, 0, 0, 0, 16);
+GP5_PLAN(32, false, 8, 4, 0, 0, 8);
+GP5_PLAN(64, false, 8, 8, 0, 0, 8);
+GP5_PLAN(96, false, 4, 4, 2, 3, 12);
+GP5_PLAN(128, false, 16, 8, 0, 0, 16);
+GP5_PLAN(192, false, 4, 4, 4, 3, 12);
+GP5_PLAN(256, false, 16, 16, 0, 0, 16);
+GP5_PLAN(512, false, 8, 8, 8, 0, 8);
+GP5_PLAN(16, true, 4, 4, 0, 0, 4);
+GP5_PLAN(32, true, 8, 4, 0, 0, 8);
+GP5_PLAN(64, true, 8, 8, 0, 0, 8);
+GP5_PLAN(96, true, 4, 4, 2, 3, 12);
+GP5_PLAN(192, true, 4, 4, 4, 3, 12);
 #ifdef GP5_ROWS_E16
-GP5_PLAN(128, true, 16, 8, 0, 16);
-GP5_PLAN(256, true, 16, 16, 0, 16);
+GP5_PLAN(128, true, 16, 8, 0, 0, 16);
+GP5_PLAN(256, true, 16, 16, 0, 0, 16);
 #else
-GP5_PLAN(128, true, 8, 4, 4, 8);
-GP5_PLAN(256, true, 8, 8, 4, 8);
+GP5_PLAN(128, true, 8, 4, 4, 0, 8);
+GP5_PLAN(256, true, 8, 8, 4, 0, 8);
 #endif
-GP5_PLAN(512, true, 8, 8, 8, 8);
+GP5_PLAN(512, true, 8, 8, 8, 0, 8);
 #undef GP5_PLAN
 
 // Block shapes.  rows: x lines, LPB lines per 128-thread block, LDS line stride
@@ -171,9 +178,9 @@ __device__ __forceinline__ void exchange(double2 (&v)[E], int t, double2 *lds, c
     __syncthreads();
 }
 
-// Full transform over the stages Ra, Rb, Rc (0 = absent), sign s.
+// Full transform over the stages Ra, Rb, Rc, Rd (0 = absent, trailing), sign s.
 // In:  v[i*Ra + r] = x[(t + TL i) + (N/Ra) r];  out: v[i*Rl + k] = X[(t + TL i) + (N/Rl) k], Rl last radix.
-template <int N, int E, int Ra, int Rb, int Rc, class Map>
+template <int N, int E, int Ra, int Rb, int Rc, int Rd, class Map>
 __device__ __forceinline__ void fft_run(double2 (&v)[E], int t, const double2 *__restrict__ tw, double s,
                                         double2 *lds, const Map &map)
 {
@@ -184,6 +191,10 @@ __device__ __forceinline__ void fft_run(double2 (&v)[E], int t, const double2 *_
         if constexpr (Rc > 0) {
             exchange<N, E, Ra, Rb, Rc>(v, t, lds, map);
             stage_compute<N, E, Ra * Rb, Rc>(v, t, tw, s);
+            if constexpr (Rd > 0) {
+                exchange<N, E, Ra * Rb, Rc, Rd>(v, t, lds, map);
+                stage_compute<N, E, Ra * Rb * Rc, Rd>(v, t, tw, s);
+            }
         }
     }
 }
@@ -193,7 +204,7 @@ __device__ __forceinline__ void fft_forward(double2 (&v)[(FPlan<N, ROWS>::E)], i
                                             const Map &map)
 {
     using P = FPlan<N, ROWS>;
-    fft_run<N, P::E, P::R0, P::R1, P::R2>(v, t, tw, -1.0, lds, map);
+    fft_run<N, P::E, P::R0, P::R1, P::R2, P::R3>(v, t, tw, -1.0, lds, map);
 }
 
 template <int N, bool ROWS, class Map>
@@ -201,15 +212,16 @@ __device__ __forceinline__ void fft_inverse(double2 (&v)[(FPlan<N, ROWS>::E)], i
                                             const Map &map)
 {
     using P = FPlan<N, ROWS>;
-    if constexpr (P::R2 > 0) fft_run<N, P::E, P::R2, P::R1, P::R0>(v, t, tw, 1.0, lds, map);
-    else if constexpr (P::R1 > 0) fft_run<N, P::E, P::R1, P::R0, 0>(v, t, tw, 1.0, lds, map);
-    else fft_run<N, P::E, P::R0, 0, 0>(v, t, tw, 1.0, lds, map);
+    if constexpr (P::R3 > 0) fft_run<N, P::E, P::R3, P::R2, P::R1, P::R0>(v, t, tw, 1.0, lds, map);
+    else if constexpr (P::R2 > 0) fft_run<N, P::E, P::R2, P::R1, P::R0, 0>(v, t, tw, 1.0, lds, map);
+    else if constexpr (P::R1 > 0) fft_run<N, P::E, P::R1, P::R0, 0, 0>(v, t, tw, 1.0, lds, map);
+    else fft_run<N, P::E, P::R0, 0, 0, 0>(v, t, tw, 1.0, lds, map);
 }
 
 template <int N, bool ROWS> struct FirstLast {
     using P = FPlan<N, ROWS>;
     static constexpr int FWD_FIRST = P::R0;
-    static constexpr int FWD_LAST = P::R2 > 0 ? P::R2 : (P::R1 > 0 ? P::R1 : P::R0);
+    static constexpr int FWD_LAST = P::R3 > 0 ? P::R3 : (P::R2 > 0 ? P::R2 : (P::R1 > 0 ? P::R1 : P::R0));
     static constexpr int INV_FIRST = FWD_LAST;
     static constexpr int INV_LAST = P::R0;
 };
@@ -404,7 +416,10 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-bool fast_supported(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+bool fast_supported(int n)
+{
+    return n == 16 || n == 32 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 512;
+}
 
 template <int N>
 static void fast_shape(bool rows, long long I, long long O, int nx, int xcount, unsigned &grid, unsigned &threads,
@@ -456,7 +471,9 @@ long long fast_rows_per_block(int n)
         case 16: return FShape<16, true>::LPB;
         case 32: return FShape<32, true>::LPB;
         case 64: return FShape<64, true>::LPB;
+        case 96: return FShape<96, true>::LPB;
         case 128: return FShape<128, true>::LPB;
+        case 192: return FShape<192, true>::LPB;
         case 256: return FShape<256, true>::LPB;
         case 512: return FShape<512, true>::LPB;
         default: return 1;
@@ -472,7 +489,9 @@ unsigned fast_grid(int n, long long I, long long O)
         case 16: fast_shape<16>(rows, I, O, 1, 0, grid, threads, lds); break;
         case 32: fast_shape<32>(rows, I, O, 1, 0, grid, threads, lds); break;
         case 64: fast_shape<64>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 96: fast_shape<96>(rows, I, O, 1, 0, grid, threads, lds); break;
         case 128: fast_shape<128>(rows, I, O, 1, 0, grid, threads, lds); break;
+        case 192: fast_shape<192>(rows, I, O, 1, 0, grid, threads, lds); break;
         case 256: fast_shape<256>(rows, I, O, 1, 0, grid, threads, lds); break;
         case 512: fast_shape<512>(rows, I, O, 1, 0, grid, threads, lds); break;
         default: break;
@@ -486,7 +505,9 @@ hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs 
         case 16: return launch_fast_n<16>(st, mode, dir, a);
         case 32: return launch_fast_n<32>(st, mode, dir, a);
         case 64: return launch_fast_n<64>(st, mode, dir, a);
+        case 96: return launch_fast_n<96>(st, mode, dir, a);
         case 128: return launch_fast_n<128>(st, mode, dir, a);
+        case 192: return launch_fast_n<192>(st, mode, dir, a);
         case 256: return launch_fast_n<256>(st, mode, dir, a);
         case 512: return launch_fast_n<512>(st, mode, dir, a);
         default: return hipErrorInvalidValue;

@@ -27,7 +27,8 @@ def _ctx(C, shape, left, right, nc=1):
 
 FFT_CASES = [((16, 16, 16), 1), ((8, 12, 10), 2), ((30, 30, 30), 1), ((7, 11, 17), 1), ((64, 1, 5), 1),
              ((96, 24, 40), 1), ((256, 16, 8), 1), ((512, 4, 8), 2), ((32, 128, 64), 1), ((20, 64, 512), 1),
-             ((16, 256, 32), 2), ((128, 3, 256), 1)]
+             ((16, 256, 32), 2), ((128, 3, 256), 1), ((192, 12, 8), 1), ((16, 96, 20), 2), ((12, 10, 192), 1),
+             ((96, 192, 4), 1)]
 
 
 @pytest.mark.parametrize("shape,nc", FFT_CASES)
@@ -181,7 +182,7 @@ STEP_CASES = [
 ]
 
 
-@pytest.mark.parametrize("shape", [(24, 20, 16), (32, 16, 20)], ids=["generic_x", "fused_x"])
+@pytest.mark.parametrize("shape", [(24, 20, 16), (32, 16, 20), (96, 12, 24)], ids=["generic_x", "fused_x", "radix3_x"])
 @pytest.mark.parametrize("name,imag,renorm,case", STEP_CASES)
 def test_advance_matches_oracle(C, oracle_mod, name, imag, renorm, case, shape):
     print(f"advance {name} imag={imag} renorm={renorm} {case} {shape}", flush=True)
@@ -198,6 +199,23 @@ def test_advance_matches_oracle(C, oracle_mod, name, imag, renorm, case, shape):
     kind = C.GP5_STEP_RK4 if name == "RK4" else C.GP5_STEP_EULERO
     s = C.stepper_desc(kind, dt=dt, imaginary=imag, renormalize=renorm)
     C.advance(ctx, f, s, psi, nsteps)
+    assert rel_l2(psi.download_complex(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 192), (192, 8, 12), (12, 96, 96)])
+def test_radix3_register_passes_rk4(C, oracle_mod, shape):
+    """RK4 real time through the register passes of lengths 96 and 192 on every axis role
+    (fused x epilogue, y pass, fused z multiplier pass); dt below the explicit limit of the
+    finest axis (k_max^2 / 2 ~ 2800 at 192 points over 8)."""
+    left, right = (-4.0, -4.0, -4.0), (4.0, 4.0, 4.0)
+    a = _smooth(shape, left, right, 1, seed=33)
+    geo = oracle_mod.Geometry(shape, left, right, 1)
+    F = oracle_mod.Functional(kind=oracle_mod.OR_LHY)
+    ref = oracle_mod.advance(F, a, geo, stepper="RK4", dt=2e-4, imaginary=False, nsteps=10)
+    ctx = _ctx(C, shape, left, right)
+    psi = ctx.field(a)
+    s = C.stepper_desc(C.GP5_STEP_RK4, dt=2e-4, imaginary=False)
+    C.advance(ctx, C.functional_desc(C.GP5_FUNC_LHY), s, psi, 10)
     assert rel_l2(psi.download_complex(), ref) <= 1e-12
 
 
