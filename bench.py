@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -87,6 +89,51 @@ def cpu_baseline(psi0, shape, left, right, budget_s: float = 25.0):
     }
 
 
+def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: float = 1e-10,
+                       check_every: int = 100, max_steps: int = 200000):
+    """Cold-started batched LHY droplet ground states: `count` instances with N log-uniform
+    in [50, 5000] (seeded), grid per instance from {64, 96, 128, 192}^3 by the predicted
+    radius (box = 6 radii at dx = 0.35), Gaussian seeds, imaginary-time RK4 with per-stage
+    renormalisation (dt = 0.1 dx^2), until each instance's energy changes by at most 1e-10
+    relative between checks 100 steps apart.  Timed: gp5_batch_ground_state (wall clock,
+    graph capture and host-side convergence checks included); the autotuner runs before."""
+    from gp5_amd import capi, synth
+
+    n, L, N, psi0 = synth.droplet_batch(count, seed)
+    b = capi.Batch(n, L, N)
+    for i, a in enumerate(psi0):
+        b.upload(i, a)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    t0 = time.perf_counter()
+    report = b.tune(f, dt_factor, 2)
+    tune_s = time.perf_counter() - t0
+    steps, en, ms = b.ground_state(f, dt_factor, tol, check_every, max_steps)
+    conv = steps > 0
+    inst_steps = int(np.abs(steps).sum())
+    pts = np.array([k ** 3 for k in n], dtype=np.float64)
+    out = {
+        "workload": f"{count} LHY droplets, N log-uniform [50, 5000] (seed {seed}), grids "
+                    + ", ".join(f"{g}^3 x{int(sum(1 for k in n if k == g))}" for g in synth.BATCH_GRIDS)
+                    + f", dx = {synth.BATCH_DX}, imaginary-time RK4 dt = {dt_factor} dx^2 with per-stage "
+                      f"renormalisation, converged to {tol:g} relative energy change per {check_every} steps",
+        "seconds": ms / 1e3,
+        "converged": int(conv.sum()),
+        "instances": count,
+        "steps_min": int(np.abs(steps).min()), "steps_median": float(np.median(np.abs(steps))),
+        "steps_max": int(np.abs(steps).max()),
+        "instance_steps": inst_steps,
+        "point_steps_per_s": float((np.abs(steps) * pts).sum() / (ms / 1e3)),
+        "energy_per_atom_min": float((en / np.array(N)).min()),
+        "energy_per_atom_max": float((en / np.array(N)).max()),
+        "tune_seconds": tune_s,
+        "tuning": report.strip().splitlines(),
+        "start": "cold (Gaussian seeds)",
+        "surrogate_seeded": None,
+    }
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +142,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gs-count", type=int, default=64,
+                    help="instances of the batched ground-state workload (0 skips it)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one normalize of the field: its scale kernel reads and writes the field "
                          "once, the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -185,6 +234,8 @@ def main() -> int:
     if args.calibrate:
         capi.normalize(ctx, psi, 0, 3000.0)
         ctx.sync()
+    if args.gs_count > 0:
+        out["ground_state"] = ground_state_batch(args.gs_count)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)
     print(json.dumps(out), flush=True)

@@ -95,3 +95,44 @@ def harmonic_64(n: int = 64):
     psi = gaussian_noisy(shape, left, right, 1.5, 1e-3, 1, 1.0)
     V = harmonic_potential(shape, left, right, [(1.0, 1.0, 1.0)])
     return shape, left, right, psi, V
+
+
+# ---------------------------------------------------------------------------
+# batches of droplet ground states
+# ---------------------------------------------------------------------------
+BATCH_GRIDS = (64, 96, 128, 192)
+BATCH_DX = 0.35
+
+
+def droplet_radius(N: float) -> float:
+    """Radius of an LHY droplet of N atoms at the saturation density 1 of
+    -3 rho + 2.5 rho^{3/2} (zero pressure: rho = 1)."""
+    return (3.0 * N / (4.0 * np.pi)) ** (1.0 / 3.0)
+
+
+def batch_grid(N: float, dx: float = BATCH_DX, grids: Sequence[int] = BATCH_GRIDS) -> int:
+    """Smallest grid whose box n*dx spans six droplet radii (the largest grid otherwise)."""
+    R = droplet_radius(N)
+    for n in grids:
+        if n * dx >= 6.0 * R:
+            return int(n)
+    return int(grids[-1])
+
+
+def droplet_batch(count: int = 64, seed: int = 3, Nmin: float = 50.0, Nmax: float = 5000.0,
+                  dx: float = BATCH_DX):
+    """Instances of the batched ground-state workload: N log-uniform in [Nmin, Nmax]
+    from the seeded stream, grid n from the predicted radius, box L = n dx centred on
+    the origin, Gaussian seed of width 0.6 R (amplitude set by the first renormalisation).
+    Returns (n, L, N, psi0) lists."""
+    u = 0.5 * (uniform(seed, count) + 1.0)
+    N = np.exp(np.log(Nmin) + u * (np.log(Nmax) - np.log(Nmin)))
+    n = [batch_grid(v, dx) for v in N]
+    L = [k * dx for k in n]
+    psi0 = []
+    for i in range(count):
+        x = -0.5 * L[i] + (np.arange(n[i]) + 0.5) * dx
+        r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+        s = 0.6 * droplet_radius(N[i])
+        psi0.append(np.asfortranarray(np.exp(-r2 / (2.0 * s * s)).astype(np.complex128)))
+    return n, L, [float(v) for v in N], psi0
