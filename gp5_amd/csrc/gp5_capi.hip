@@ -149,6 +149,15 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
+// for the other translation units of the library (gp5_nn.hip)
+namespace gp5 {
+int set_error(int code, const char *msg)
+{
+    g_err = msg;
+    return code;
+}
+}  // namespace gp5
+
 #define HIPCHK(expr)                                                                         \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \

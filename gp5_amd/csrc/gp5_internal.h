@@ -128,6 +128,7 @@ struct FastArgs {
     int nx, x0, xcount;
 };
 
+int set_error(int code, const char *msg);  // sets gp5_last_error(), returns code
 bool fast_supported(int n);
 unsigned fast_grid(int n, long long I, long long O);
 long long fast_rows_per_block(int n);  // x lines per block of the row kernels

@@ -1,0 +1,332 @@
+"""The warm-start network on the GPU: a 4-level 3D convolutional encoder-decoder
+(16 -> 32 -> 64 -> 128 channels, bfloat16 weights and activations, float
+accumulation and gradients), run and trained entirely by the kernels of
+include/gp5_nn.h (libgp5.so).  No counterpart in the reference.
+
+Graph (channels-last cubes of side D, D divisible by 8):
+    e1 = lrelu(c1(X))            D     16
+    e2 = lrelu(c2(pool(e1)))     D/2   32
+    e3 = lrelu(c3(pool(e2)))     D/4   64
+    e4 = lrelu(c4(pool(e3)))     D/8   128
+    d3 = lrelu(c5(up(e4))) + e3  D/4   64
+    d2 = lrelu(c6(up(d3))) + e2  D/2   32
+    d1 = lrelu(c7(up(d2))) + e1  D     16
+    y  = base + c8(d1)[..., 0]   D     (float)
+Loss: mean squared error of y against the target field.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import capi
+
+SHAPES = [("c1", 16, 16), ("c2", 16, 32), ("c3", 32, 64), ("c4", 64, 128),
+          ("c5", 128, 64), ("c6", 64, 32), ("c7", 32, 16), ("c8", 16, 16)]
+C0 = 16  # input feature channels
+
+
+def _lib():
+    L = capi.load()
+    if getattr(L, "_gp5_nn_sigs", False):
+        return L
+    P, I, F, S, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t, ctypes.c_longlong
+    sig = {
+        "gp5_nn_malloc": [ctypes.POINTER(ctypes.c_void_p), S], "gp5_nn_free": [P],
+        "gp5_nn_upload": [P, P, S], "gp5_nn_download": [P, P, S], "gp5_nn_zero": [P, S], "gp5_nn_sync": [],
+        "gp5_nn_copy": [P, P, S],
+        "gp5_nn_conv3d": [P, P, P, P, P, P, P, I, I, I, I, I, I],
+        "gp5_nn_conv3d_wgrad": [P, P, P, P, I, I, I, I],
+        "gp5_nn_bias_grad": [P, P, LL, I],
+        "gp5_nn_pool2_bf16": [P, P, I, I, I, F], "gp5_nn_up2_bf16": [P, P, I, I, I, F],
+        "gp5_nn_pool2_f32": [P, P, I, I, I, F], "gp5_nn_up2_f32": [P, P, I, I, I, F],
+        "gp5_nn_act_bwd": [P, P, P, LL], "gp5_nn_add_f32": [P, P, LL], "gp5_nn_cast_bf16": [P, P, LL],
+        "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
+        "gp5_nn_relayout": [P, P, P, I, I],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    L.gp5_nn_wgrad_ws.argtypes = [I, I, I, I]
+    L.gp5_nn_wgrad_ws.restype = S
+    L._gp5_nn_sigs = True
+    return L
+
+
+def _ck(rc):
+    capi.check(rc)
+
+
+class DevBuf:
+    """Device allocation of the NN kernels (freed with the object)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _ck(_lib().gp5_nn_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p
+
+    def zero(self):
+        _ck(_lib().gp5_nn_zero(self.ptr, self.nbytes))
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} B into {self.nbytes} B")
+        _ck(_lib().gp5_nn_upload(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes != self.nbytes:
+            raise ValueError("download size mismatch")
+        _ck(_lib().gp5_nn_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _lib().gp5_nn_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def to_bf16(a: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 bit patterns (uint16), round to nearest even."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+    u = u + np.uint32(0x7FFF) + ((u >> np.uint32(16)) & np.uint32(1))
+    return (u >> np.uint32(16)).astype(np.uint16)
+
+
+def from_bf16(h: np.ndarray) -> np.ndarray:
+    return (h.astype(np.uint32) << np.uint32(16)).view(np.float32)
+
+
+def init_params(seed: int = 0) -> Dict[str, np.ndarray]:
+    rng = np.random.default_rng(seed)
+    p = {}
+    for name, ci, co in SHAPES:
+        p[name + ".W"] = (rng.standard_normal((27, ci, co)) * np.sqrt(2.0 / (27 * ci))).astype(np.float32)
+        p[name + ".b"] = np.zeros(co, dtype=np.float32)
+    p["c8.W"] *= 0.1  # the network starts close to the baseline field
+    return p
+
+
+class UNet:
+    """Weights (float masters, Adam moments, bf16 copies) and the activations of
+    one batch shape (B, D)."""
+
+    def __init__(self, B: int, D: int, params: Optional[Dict[str, np.ndarray]] = None, seed: int = 0):
+        if D % 8:
+            raise ValueError("D must be divisible by 8")
+        self.B, self.D = int(B), int(D)
+        params = params if params is not None else init_params(seed)
+        self.w = {}
+        for name, ci, co in SHAPES:
+            W = np.ascontiguousarray(params[name + ".W"], dtype=np.float32)
+            b = np.ascontiguousarray(params[name + ".b"], dtype=np.float32)
+            ent = {"ci": ci, "co": co}
+            for key, arr in (("W", W), ("b", b)):
+                buf = DevBuf(arr.nbytes)
+                buf.upload(arr)
+                ent[key] = buf
+                for k2 in ("g", "m", "v"):
+                    t = DevBuf(arr.nbytes)
+                    t.zero()
+                    ent[key + k2] = t
+            ent["Wf"] = DevBuf(W.size * 2)
+            ent["Wb"] = DevBuf(W.size * 2)
+            self.w[name] = ent
+        self.step_count = 0
+        self._relayout()
+        self._alloc()
+
+    # --- memory -----------------------------------------------------------
+    def _alloc(self):
+        B, D = self.B, self.D
+        n = lambda d, c: B * d ** 3 * c  # noqa: E731
+        sizes = {  # name: (elements, bytes per element)
+            "X": (n(D, C0), 2), "e1": (n(D, 16), 2), "p1": (n(D // 2, 16), 2), "e2": (n(D // 2, 32), 2),
+            "p2": (n(D // 4, 32), 2), "e3": (n(D // 4, 64), 2), "p3": (n(D // 8, 64), 2), "e4": (n(D // 8, 128), 2),
+            "u4": (n(D // 4, 128), 2), "a5": (n(D // 4, 64), 2), "d3": (n(D // 4, 64), 2),
+            "u3": (n(D // 2, 64), 2), "a6": (n(D // 2, 32), 2), "d2": (n(D // 2, 32), 2),
+            "u2": (n(D, 32), 2), "a7": (n(D, 16), 2), "d1": (n(D, 16), 2),
+            "out": (n(D, 16), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
+            # gradients (float) and their bf16 copies
+            "g_out": (n(D, 16), 4), "h_out": (n(D, 16), 2),
+            "g16a": (n(D, 16), 4), "g16b": (n(D, 16), 4), "g16c": (n(D, 16), 4), "h16": (n(D, 16), 2),
+            "g32a": (n(D // 2, 32), 4), "g32b": (n(D // 2, 32), 4), "g32c": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
+            "g64a": (n(D // 4, 64), 4), "g64b": (n(D // 4, 64), 4), "g64c": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
+            "g128": (n(D // 8, 128), 4), "h128": (n(D // 8, 128), 2),
+            "gu32": (n(D, 32), 4), "gu64": (n(D // 2, 64), 4), "gu128": (n(D // 4, 128), 4),
+            "gp16": (n(D // 2, 16), 4), "gp32": (n(D // 4, 32), 4), "gp64": (n(D // 8, 64), 4),
+        }
+        self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
+        L = _lib()
+        ws = max(L.gp5_nn_wgrad_ws(B, d, ci, co) for d, ci, co in
+                 [(D, C0, 16), (D // 2, 16, 32), (D // 4, 32, 64), (D // 8, 64, 128), (D // 4, 128, 64),
+                  (D // 2, 64, 32), (D, 32, 16), (D, 16, 16)])
+        self.ws = DevBuf(ws)
+
+    def _relayout(self):
+        L = _lib()
+        for name, e in self.w.items():
+            _ck(L.gp5_nn_relayout(e["W"].ptr, e["Wf"].ptr, e["Wb"].ptr, e["ci"], e["co"]))
+
+    def params(self) -> Dict[str, np.ndarray]:
+        out = {}
+        for name, e in self.w.items():
+            out[name + ".W"] = e["W"].download(np.float32, (27, e["ci"], e["co"]))
+            out[name + ".b"] = e["b"].download(np.float32, (e["co"],))
+        return out
+
+    def grads(self) -> Dict[str, np.ndarray]:
+        out = {}
+        for name, e in self.w.items():
+            out[name + ".W"] = e["Wg"].download(np.float32, (27, e["ci"], e["co"]))
+            out[name + ".b"] = e["bg"].download(np.float32, (e["co"],))
+        return out
+
+    # --- passes -----------------------------------------------------------
+    def load_batch(self, Xs, bases, targets, idx, offset_bytes=(None, None, None)):
+        """Copies samples idx of device-resident arrays (bf16 features [n, D^3*16],
+        float base and target [n, D^3]) into this network's batch buffers."""
+        L = _lib()
+        P = self.D ** 3
+        for j, i in enumerate(idx):
+            _ck(L.gp5_nn_copy(ctypes.c_void_p(self.t["X"].ptr.value + j * P * C0 * 2),
+                              ctypes.c_void_p(Xs.ptr.value + int(i) * P * C0 * 2), P * C0 * 2))
+            _ck(L.gp5_nn_copy(ctypes.c_void_p(self.t["base"].ptr.value + j * P * 4),
+                              ctypes.c_void_p(bases.ptr.value + int(i) * P * 4), P * 4))
+            _ck(L.gp5_nn_copy(ctypes.c_void_p(self.t["target"].ptr.value + j * P * 4),
+                              ctypes.c_void_p(targets.ptr.value + int(i) * P * 4), P * 4))
+
+    def set_batch(self, X: np.ndarray, base: np.ndarray, target: Optional[np.ndarray] = None):
+        """X: [B, D, D, D, 16] float features; base/target: [B, D, D, D] float."""
+        self.t["X"].upload(to_bf16(X))
+        self.t["base"].upload(np.asarray(base, dtype=np.float32))
+        if target is not None:
+            self.t["target"].upload(np.asarray(target, dtype=np.float32))
+
+    def _conv(self, name, x, A=None, skip=None, Y=None, Yf=None, D=None, act=1):
+        e = self.w[name]
+        P = lambda b: b.ptr if b is not None else None  # noqa: E731
+        _ck(_lib().gp5_nn_conv3d(x.ptr, e["Wf"].ptr, e["b"].ptr, P(A), P(skip), P(Y), P(Yf), self.B, D,
+                                 e["ci"], e["co"], 0, act))
+
+    def forward(self) -> None:
+        L, t, B, D = _lib(), self.t, self.B, self.D
+        self._conv("c1", t["X"], A=t["e1"], D=D)
+        _ck(L.gp5_nn_pool2_bf16(t["e1"].ptr, t["p1"].ptr, B, D, 16, 1.0))
+        self._conv("c2", t["p1"], A=t["e2"], D=D // 2)
+        _ck(L.gp5_nn_pool2_bf16(t["e2"].ptr, t["p2"].ptr, B, D // 2, 32, 1.0))
+        self._conv("c3", t["p2"], A=t["e3"], D=D // 4)
+        _ck(L.gp5_nn_pool2_bf16(t["e3"].ptr, t["p3"].ptr, B, D // 4, 64, 1.0))
+        self._conv("c4", t["p3"], A=t["e4"], D=D // 8)
+        _ck(L.gp5_nn_up2_bf16(t["e4"].ptr, t["u4"].ptr, B, D // 8, 128, 1.0))
+        self._conv("c5", t["u4"], A=t["a5"], skip=t["e3"], Y=t["d3"], D=D // 4)
+        _ck(L.gp5_nn_up2_bf16(t["d3"].ptr, t["u3"].ptr, B, D // 4, 64, 1.0))
+        self._conv("c6", t["u3"], A=t["a6"], skip=t["e2"], Y=t["d2"], D=D // 2)
+        _ck(L.gp5_nn_up2_bf16(t["d2"].ptr, t["u2"].ptr, B, D // 2, 32, 1.0))
+        self._conv("c7", t["u2"], A=t["a7"], skip=t["e1"], Y=t["d1"], D=D)
+        self._conv("c8", t["d1"], Yf=t["out"], D=D, act=0)
+
+    def output(self) -> np.ndarray:
+        B, D = self.B, self.D
+        out = self.t["out"].download(np.float32, (B, D, D, D, 16))[..., 0]
+        return self.t["base"].download(np.float32, (B, D, D, D)) + out
+
+    def _wgrad(self, name, x, dz_bf, D):
+        e = self.w[name]
+        _ck(_lib().gp5_nn_conv3d_wgrad(x.ptr, dz_bf.ptr, e["Wg"].ptr, self.ws.ptr, self.B, D, e["ci"], e["co"]))
+
+    def _bgrad(self, name, dz, D):
+        e = self.w[name]
+        _ck(_lib().gp5_nn_bias_grad(dz.ptr, e["bg"].ptr, self.B * D ** 3, e["co"]))
+
+    def _dgrad(self, name, dz_bf, out_f, D):
+        # data gradient: forward kernel on the [27][ci][co] copy with mirrored taps
+        e = self.w[name]
+        _ck(_lib().gp5_nn_conv3d(dz_bf.ptr, e["Wb"].ptr, None, None, None, None, out_f.ptr, self.B, D, e["co"],
+                                 e["ci"], 1, 0))
+
+    def backward(self) -> float:
+        """MSE loss against the target; fills every weight and bias gradient."""
+        L, t, B, D = _lib(), self.t, self.B, self.D
+        n = lambda d, c: B * d ** 3 * c  # noqa: E731
+        loss = ctypes.c_double()
+        _ck(L.gp5_nn_mse(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, t["g_out"].ptr, B * D ** 3,
+                         ctypes.byref(loss)))
+        # c8
+        _ck(L.gp5_nn_cast_bf16(t["g_out"].ptr, t["h_out"].ptr, n(D, 16)))
+        self._wgrad("c8", t["d1"], t["h_out"], D)
+        self._bgrad("c8", t["g_out"], D)
+        self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
+        # c7: dz7 = dd1 * act'(a7)
+        _ck(L.gp5_nn_act_bwd(t["a7"].ptr, t["g16a"].ptr, t["g16b"].ptr, n(D, 16)))
+        _ck(L.gp5_nn_cast_bf16(t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        self._wgrad("c7", t["u2"], t["h16"], D)
+        self._bgrad("c7", t["g16b"], D)
+        self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
+        _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
+        # c6
+        _ck(L.gp5_nn_act_bwd(t["a6"].ptr, t["g32a"].ptr, t["g32b"].ptr, n(D // 2, 32)))
+        _ck(L.gp5_nn_cast_bf16(t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        self._wgrad("c6", t["u3"], t["h32"], D // 2)
+        self._bgrad("c6", t["g32b"], D // 2)
+        self._dgrad("c6", t["h32"], t["gu64"], D // 2)
+        _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
+        # c5
+        _ck(L.gp5_nn_act_bwd(t["a5"].ptr, t["g64a"].ptr, t["g64b"].ptr, n(D // 4, 64)))
+        _ck(L.gp5_nn_cast_bf16(t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        self._wgrad("c5", t["u4"], t["h64"], D // 4)
+        self._bgrad("c5", t["g64b"], D // 4)
+        self._dgrad("c5", t["h64"], t["gu128"], D // 4)
+        _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4
+        # c4
+        _ck(L.gp5_nn_act_bwd(t["e4"].ptr, t["g128"].ptr, t["g128"].ptr, n(D // 8, 128)))
+        _ck(L.gp5_nn_cast_bf16(t["g128"].ptr, t["h128"].ptr, n(D // 8, 128)))
+        self._wgrad("c4", t["p3"], t["h128"], D // 8)
+        self._bgrad("c4", t["g128"], D // 8)
+        self._dgrad("c4", t["h128"], t["gp64"], D // 8)
+        _ck(L.gp5_nn_up2_f32(t["gp64"].ptr, t["g64c"].ptr, B, D // 8, 64, 0.125))
+        _ck(L.gp5_nn_add_f32(t["g64a"].ptr, t["g64c"].ptr, n(D // 4, 64)))  # de3 total
+        # c3
+        _ck(L.gp5_nn_act_bwd(t["e3"].ptr, t["g64a"].ptr, t["g64b"].ptr, n(D // 4, 64)))
+        _ck(L.gp5_nn_cast_bf16(t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        self._wgrad("c3", t["p2"], t["h64"], D // 4)
+        self._bgrad("c3", t["g64b"], D // 4)
+        self._dgrad("c3", t["h64"], t["gp32"], D // 4)
+        _ck(L.gp5_nn_up2_f32(t["gp32"].ptr, t["g32c"].ptr, B, D // 4, 32, 0.125))
+        _ck(L.gp5_nn_add_f32(t["g32a"].ptr, t["g32c"].ptr, n(D // 2, 32)))  # de2 total
+        # c2
+        _ck(L.gp5_nn_act_bwd(t["e2"].ptr, t["g32a"].ptr, t["g32b"].ptr, n(D // 2, 32)))
+        _ck(L.gp5_nn_cast_bf16(t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        self._wgrad("c2", t["p1"], t["h32"], D // 2)
+        self._bgrad("c2", t["g32b"], D // 2)
+        self._dgrad("c2", t["h32"], t["gp16"], D // 2)
+        _ck(L.gp5_nn_up2_f32(t["gp16"].ptr, t["g16c"].ptr, B, D // 2, 16, 0.125))
+        _ck(L.gp5_nn_add_f32(t["g16a"].ptr, t["g16c"].ptr, n(D, 16)))  # de1 total
+        # c1
+        _ck(L.gp5_nn_act_bwd(t["e1"].ptr, t["g16a"].ptr, t["g16b"].ptr, n(D, 16)))
+        _ck(L.gp5_nn_cast_bf16(t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        self._wgrad("c1", t["X"], t["h16"], D)
+        self._bgrad("c1", t["g16b"], D)
+        return float(loss.value)
+
+    def step(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:
+        """Adam update of the float masters, then fresh bf16 copies."""
+        L = _lib()
+        self.step_count += 1
+        for name, e in self.w.items():
+            for key, n in (("W", 27 * e["ci"] * e["co"]), ("b", e["co"])):
+                _ck(L.gp5_nn_adam(e[key].ptr, e[key + "g"].ptr, e[key + "m"].ptr, e[key + "v"].ptr, n, lr, b1, b2,
+                                  eps, self.step_count))
+        self._relayout()
+
+    def sync(self):
+        _ck(_lib().gp5_nn_sync())

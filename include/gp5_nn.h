@@ -1,0 +1,73 @@
+/* gp5_nn.h -- kernels of the learned warm start (a 3D convolutional encoder-
+ * decoder that predicts droplet ground states).  No counterpart in the
+ * reference, which has no learned component: these serve the callers that
+ * seed imaginary-time ground states (the batch API of gp5.h) and the training
+ * loop that fits the network to ground states computed by that solver.
+ *
+ * Tensors are plain device pointers.  Activations are channels-last
+ * [batch][z][y][x][channel] cubes of side D, in bfloat16 (uint16 bit patterns)
+ * for the matrix-core inputs and in float for gradients and outputs.  Weights
+ * of a 3x3x3 "same" convolution are float masters W[tap][ci][co]
+ * (tap = 9 dz + 3 dy + dx over offsets -1..1) with bfloat16 copies in two
+ * layouts for the forward and data-gradient kernels.  Channel counts are
+ * multiples of 16.  All kernels run on the null stream; every call returns
+ * GP5_OK or an error code with gp5_last_error() set (include/gp5.h). */
+#ifndef GP5_NN_H
+#define GP5_NN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* device memory and transfers (synchronous) */
+int gp5_nn_malloc(void **p, size_t bytes);
+int gp5_nn_free(void *p);
+int gp5_nn_upload(void *dst, const void *host, size_t bytes);
+int gp5_nn_download(void *host, const void *src, size_t bytes);
+int gp5_nn_copy(void *dst, const void *src, size_t bytes); /* device to device */
+int gp5_nn_zero(void *p, size_t bytes); /* fill kernel, not a memset node */
+int gp5_nn_sync(void);
+
+/* Y = act(conv(X, W) + bias) (+ skip).  Implicit GEMM on bf16 MFMA tiles with
+ * float accumulation.  Wk: bf16 [27][Cout][Cin] (forward layout); with
+ * mirror = 1 the taps are read mirrored, which with the [27][ci][co] layout
+ * passed as Wk computes the data gradient (Cin/Cout then name the gradient's
+ * input/output channels).  act: 0 identity, 1 leaky ReLU (slope 0.1).
+ * Outputs (each may be NULL): A bf16 = act(...); Y bf16 = A + skip;
+ * Yf float = act(...) (for the last layer and for gradients). */
+int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
+                  int B, int D, int Cin, int Cout, int mirror, int act);
+/* dW[tap][ci][co] = sum over positions of X[p + off(tap)][ci] dY[p][co]
+ * (bf16 inputs, float result); ws: float workspace of at least
+ * gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes. */
+size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout);
+int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
+/* db[c] = sum over positions of dY[p][c] (float). */
+int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C);
+
+/* 2x2x2 average pool (D -> D/2) and nearest upsample (D -> 2D), times scale */
+int gp5_nn_pool2_bf16(const void *X, void *Y, int B, int D, int C, float scale);
+int gp5_nn_up2_bf16(const void *X, void *Y, int B, int D, int C, float scale);
+int gp5_nn_pool2_f32(const float *X, float *Y, int B, int D, int C, float scale);
+int gp5_nn_up2_f32(const float *X, float *Y, int B, int D, int C, float scale);
+/* dPre = dA * (A > 0 ? 1 : 0.1): backward of the leaky ReLU from its output */
+int gp5_nn_act_bwd(const void *A, const float *dA, float *dPre, long long n);
+int gp5_nn_add_f32(float *dst, const float *src, long long n);
+int gp5_nn_cast_bf16(const float *src, void *dst, long long n);
+/* Mean squared error of y = base + Yf[p][0] against target over P positions:
+ * dYf[p][c] = 2 (y - t) / P for c = 0 and 0 otherwise; *loss = mean. */
+int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P,
+               double *loss);
+/* Adam step on n float parameters (t: 1-based step count). */
+int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps,
+                int t);
+/* bf16 copies of W[27][Cin][Cout]: Wf [27][Cout][Cin] (forward), Wb = same layout as W. */
+int gp5_nn_relayout(const float *W, void *Wf, void *Wb, int Cin, int Cout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GP5_NN_H */

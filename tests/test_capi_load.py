@@ -78,3 +78,13 @@ def test_cpp_facade_example_runs(tmp_path, oracle_mod):
     a = O.advance(F, a, geo, stepper="RK4", dt=1e-3, imaginary=False, nsteps=10)
     assert norm == pytest.approx(O.norm(a, geo, 0), rel=1e-11)
     assert energy == pytest.approx(O.energy(F, a, geo), rel=1e-10)
+
+
+def test_library_exports_every_nn_symbol():
+    src = open(os.path.join(ROOT, "include", "gp5_nn.h")).read()
+    names = sorted(set(re.findall(r"\b(gp5_nn_[a-z_0-9]+)\s*\(", src)))
+    assert len(names) >= 18
+    from gp5_amd import capi
+
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    assert not [s for s in names if not hasattr(lib, s)]

@@ -1,0 +1,149 @@
+"""GPU kernels of the warm-start network (include/gp5_nn.h) against the CPU oracle
+oracle/nn_ref.py: convolution forward, data and weight gradients on bf16 inputs
+(exactly representable, so only the float accumulation order differs), pooling,
+Adam, and the whole network's loss and gradients (bf16 activations: agreement to
+the bf16 rounding level)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import nn_ref as R
+
+nn = pytest.importorskip("gp5_amd.nn")
+
+
+def test_bf16_round_trip_on_host():
+    x = np.array([1.0, -2.5, 3.14159, 1e-3, 65504.0], dtype=np.float32)
+    y = nn.from_bf16(nn.to_bf16(x))
+    assert np.all(np.abs(y - x) <= np.abs(x) * 2.0 ** -8)
+    assert nn.from_bf16(nn.to_bf16(np.float32([1.0 + 2.0 ** -9])))[0] == 1.0  # ties to even
+
+
+def _rb(a):
+    return nn.from_bf16(nn.to_bf16(a)).astype(np.float64)
+
+
+def _buf(a, bf16=False):
+    arr = nn.to_bf16(a) if bf16 else np.ascontiguousarray(a, dtype=np.float32)
+    b = nn.DevBuf(arr.nbytes)
+    b.upload(arr)
+    return b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(8, 16, 32, 2), (4, 128, 64, 1), (16, 32, 16, 1), (8, 16, 144, 1)])
+def test_conv_forward_dgrad_wgrad(D, Cin, Cout, B):
+    rng = np.random.default_rng(D + Cin + Cout)
+    X = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    L = nn._lib()
+    xb, gb = _buf(X, True), _buf(G, True)
+    Wf = np.ascontiguousarray(W.transpose(0, 2, 1))
+    wf, wb, bb = _buf(Wf, True), _buf(W, True), _buf(b)
+    Yf = nn.DevBuf(B * D ** 3 * Cout * 4)
+    A = nn.DevBuf(B * D ** 3 * Cout * 2)
+    nn._ck(L.gp5_nn_conv3d(xb.ptr, wf.ptr, bb.ptr, A.ptr, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+    ref = R.lrelu(R.conv(X, W, b.astype(np.float64)))
+    got = Yf.download(np.float32, ref.shape)
+    assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+    assert np.array_equal(A.download(np.uint16, ref.shape), nn.to_bf16(got))
+    # data gradient through the mirrored taps of the [27][ci][co] copy
+    dX = nn.DevBuf(B * D ** 3 * Cin * 4)
+    nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, dX.ptr, B, D, Cout, Cin, 1, 0))
+    ref = R.conv_dgrad(G, W)
+    got = dX.download(np.float32, ref.shape)
+    assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+    # weight gradient
+    ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+    dW = nn.DevBuf(27 * Cin * Cout * 4)
+    nn._ck(L.gp5_nn_conv3d_wgrad(xb.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+    ref = R.conv_wgrad(X, G)
+    got = dW.download(np.float32, ref.shape)
+    assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_pool_up_act_adam():
+    rng = np.random.default_rng(7)
+    L = nn._lib()
+    B, D, C = 2, 8, 16
+    X = _rb(rng.standard_normal((B, D, D, D, C)))
+    xb = _buf(X, True)
+    y = nn.DevBuf(B * (D // 2) ** 3 * C * 2)
+    nn._ck(L.gp5_nn_pool2_bf16(xb.ptr, y.ptr, B, D, C, 1.0))
+    assert np.array_equal(y.download(np.uint16, (B, 4, 4, 4, C)), nn.to_bf16(R.pool2(X).astype(np.float32)))
+    u = nn.DevBuf(B * (2 * D) ** 3 * C * 2)
+    nn._ck(L.gp5_nn_up2_bf16(xb.ptr, u.ptr, B, D, C, 1.0))
+    assert np.array_equal(nn.from_bf16(u.download(np.uint16, (B, 16, 16, 16, C))), R.up2(X).astype(np.float32))
+    # act backward
+    G = rng.standard_normal(X.shape).astype(np.float32)
+    gb, out = _buf(G), nn.DevBuf(G.nbytes)
+    nn._ck(L.gp5_nn_act_bwd(xb.ptr, gb.ptr, out.ptr, G.size))
+    assert np.allclose(out.download(np.float32, G.shape), G * R.lrelu_grad(X), rtol=1e-7, atol=0)
+    # Adam, two steps
+    p = rng.standard_normal(1000).astype(np.float32)
+    g = rng.standard_normal(1000).astype(np.float32)
+    pb, gb2, mb, vb = _buf(p), _buf(g), nn.DevBuf(4000), nn.DevBuf(4000)
+    mb.zero()
+    vb.zero()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in (1, 2):
+        nn._ck(L.gp5_nn_adam(pb.ptr, gb2.ptr, mb.ptr, vb.ptr, p.size, 1e-2, 0.9, 0.999, 1e-8, t))
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        p = p - 1e-2 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    assert np.allclose(pb.download(np.float32, p.shape), p, rtol=1e-5, atol=1e-6)
+
+
+def _cos(a, b):
+    a, b = a.ravel().astype(np.float64), b.ravel().astype(np.float64)
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+
+
+@pytest.mark.gpu
+def test_network_loss_and_gradients_match_oracle():
+    rng = np.random.default_rng(11)
+    B, D = 2, 16
+    params = nn.init_params(seed=3)
+    params["c8.W"] *= 10.0  # a non-trivial output for the check
+    net = nn.UNet(B, D, params)
+    X = rng.standard_normal((B, D, D, D, 16)).astype(np.float32) * 0.5
+    base = rng.standard_normal((B, D, D, D)).astype(np.float32) * 0.1
+    target = rng.standard_normal((B, D, D, D)).astype(np.float32)
+    net.set_batch(X, base, target)
+    net.forward()
+    loss = net.backward()
+    gp = net.grads()
+    # oracle on the same bf16-rounded weights and inputs, float64 activations
+    p64 = {k: (_rb(v) if k.endswith(".W") else v.astype(np.float64)) for k, v in params.items()}
+    Lr, gr, yr = R.loss_and_grads(p64, _rb(X), base.astype(np.float64), target.astype(np.float64))
+    y = net.output()
+    assert np.linalg.norm(y - yr) <= 2e-2 * np.linalg.norm(yr)
+    assert loss == pytest.approx(Lr, rel=2e-2)
+    for name in gr:
+        c = _cos(gp[name], gr[name])
+        rel = np.linalg.norm(gp[name] - gr[name]) / (np.linalg.norm(gr[name]) + 1e-30)
+        assert c > 0.995 and rel < 0.1, (name, c, rel)
+
+
+@pytest.mark.gpu
+def test_training_steps_reduce_the_loss():
+    rng = np.random.default_rng(12)
+    B, D = 2, 16
+    net = nn.UNet(B, D, seed=5)
+    X = rng.standard_normal((B, D, D, D, 16)).astype(np.float32) * 0.5
+    base = np.zeros((B, D, D, D), dtype=np.float32)
+    x = np.linspace(-1, 1, D)
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    target = np.broadcast_to(np.exp(-3 * r2), (B, D, D, D)).astype(np.float32)
+    net.set_batch(X, base, target)
+    losses = []
+    for _ in range(40):
+        net.forward()
+        losses.append(net.backward())
+        net.step(5e-4)
+    assert min(losses[-5:]) < 0.3 * losses[0], losses
