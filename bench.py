@@ -46,6 +46,11 @@ KIND_BYTES_F = {
 STEP_MODEL_F = 29.0  # ideal compulsory traffic of one RK4 step (fused, one r/w per FFT)
 
 
+def _progress(msg: str) -> None:
+    """Phase markers on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def _affinity() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -107,7 +112,11 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
     t0 = time.perf_counter()
     report = b.tune(f, dt_factor, 2)
     tune_s = time.perf_counter() - t0
+    _progress(f"cold ground states of {count} droplets")
     steps, en, ms = b.ground_state(f, dt_factor, tol, check_every, max_steps)
+    b.close()
+    _progress(f"cold: {ms / 1e3:.1f} s; surrogate-seeded run")
+    seeded = surrogate_seeded(n, L, N, f, dt_factor, tol, check_every, max_steps, ms, steps, en)
     conv = steps > 0
     inst_steps = int(np.abs(steps).sum())
     pts = np.array([k ** 3 for k in n], dtype=np.float64)
@@ -128,10 +137,40 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
         "tune_seconds": tune_s,
         "tuning": report.strip().splitlines(),
         "start": "cold (Gaussian seeds)",
-        "surrogate_seeded": None,
+        "surrogate_seeded": seeded,
     }
-    b.close()
     return out
+
+
+def surrogate_seeded(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, cold_steps, cold_en):
+    """The same batch seeded by the trained network (gp5_amd/data/surrogate.npz): seeds are
+    predicted on the GPU (bf16 MFMA U-Net) and mapped to each grid on the host; the
+    ground-state run is timed like the cold one."""
+    from gp5_amd import capi, surrogate
+
+    if not os.path.exists(surrogate.DEFAULT_WEIGHTS):
+        return None
+    t0 = time.perf_counter()
+    sur = surrogate.Surrogate()
+    seeds = [sur.seed(N[i], (0.0, 0.0, 0.0), 2.5, n[i], L[i]) for i in range(len(n))]
+    seed_s = time.perf_counter() - t0
+    b = capi.Batch(n, L, N)
+    for i, a in enumerate(seeds):
+        b.upload(i, a)
+    b.tune(f, dt_factor, 2)
+    steps, en, ms = b.ground_state(f, dt_factor, tol, check_every, max_steps)
+    b.close()
+    return {
+        "seconds": ms / 1e3,
+        "seeding_seconds": seed_s,
+        "converged": int((steps > 0).sum()),
+        "instance_steps": int(np.abs(steps).sum()),
+        "steps_median": float(np.median(np.abs(steps))),
+        "speedup_vs_cold": cold_ms / ms,
+        "speedup_vs_cold_incl_seeding": (cold_ms / 1e3) / (ms / 1e3 + seed_s),
+        "instance_steps_ratio": float(np.abs(cold_steps).sum() / np.abs(steps).sum()),
+        "max_energy_per_atom_diff_vs_cold": float(np.max(np.abs(en - cold_en) / np.array(N))),
+    }
 
 
 def main() -> int:
@@ -162,6 +201,7 @@ def main() -> int:
     f = capi.functional_desc(capi.GP5_FUNC_LHY)
     s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=5e-4, imaginary=False)
 
+    _progress(f"RK4 at {n}^3")
     # norm / energy at the start (fused reductions of stage 1)
     norms0, ens0 = capi.advance(ctx, f, s, psi, 1, log=True)
     if args.warmup > 1:
@@ -237,6 +277,7 @@ def main() -> int:
     if args.gs_count > 0:
         out["ground_state"] = ground_state_batch(args.gs_count)
     if not args.no_cpu:
+        _progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)
     print(json.dumps(out), flush=True)
     return 0
