@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgp5.so")
 ARCH = os.environ.get("GP5_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["gp5_kernels.hip", "gp5_fast.hip", "gp5_capi.hip", "gp5_nn.hip"]
+HIP_SOURCES = ["gp5_kernels.hip", "gp5_fast.hip", "gp5_capi.hip", "gp5_nn.hip", "gp5_ccl.hip", "gp5_h5.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 ops stay off (hipcc's host pass warns that it does not know
@@ -53,7 +53,7 @@ def _run(cmd, verbose):
 def build_lib(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([_hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, *srcs, "-o", tmp], verbose)
+    _run([_hipcc(), *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, *srcs, "-ldl", "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

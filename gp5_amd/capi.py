@@ -38,7 +38,7 @@ EXPORTS = [
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
     "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
-    "gp5_kernel_kind_name",
+    "gp5_kernel_kind_name", "gp5_decompose", "gp5_h5_write", "gp5_h5_read", "gp5_h5_shape",
     "gp5_batch_create", "gp5_batch_destroy", "gp5_batch_classes", "gp5_batch_class_of",
     "gp5_batch_upload", "gp5_batch_download", "gp5_batch_set_lhy", "gp5_batch_set_potential",
     "gp5_batch_advance", "gp5_batch_tune", "gp5_batch_ground_state",
@@ -111,6 +111,10 @@ def load() -> ctypes.CDLL:
         "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
                                  ctypes.POINTER(ctypes.c_int)], I),
         "gp5_kernel_kind_name": ([I], ctypes.c_char_p),
+        "gp5_decompose": ([P, P, I, D, P, ctypes.POINTER(ctypes.c_int)], I),
+        "gp5_h5_write": ([ctypes.c_char_p, P, ctypes.POINTER(ctypes.c_int), I], I),
+        "gp5_h5_read": ([ctypes.c_char_p, P, ctypes.POINTER(ctypes.c_int), I], I),
+        "gp5_h5_shape": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], I),
         "gp5_batch_create": ([PP, I, ctypes.POINTER(ctypes.c_int), DP, DP], I),
         "gp5_batch_destroy": ([P], I),
         "gp5_batch_classes": ([P], I),
@@ -349,6 +353,49 @@ def profile_advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer
     for k in range(GP5_NKINDS):
         if cnt[k]:
             out[L.gp5_kernel_kind_name(k).decode()] = (ms[k], cnt[k])
+    return out
+
+
+def decompose(ctx: Context, psi: Buffer, cutoff: float, comp: int = 0):
+    """Droplet labels (src/cluster.cpp:17-82) on the GPU: an int32 (Nx, Ny, Nz, nC)
+    Fortran-ordered array (-1 below the cut-off, droplets 1..n, other
+    components 0) and n."""
+    labels = np.zeros(ctx.shape + (ctx.nc,), dtype=np.int32, order="F")
+    n = ctypes.c_int()
+    check(load().gp5_decompose(ctx.handle, psi.handle, int(comp), float(cutoff), labels.ctypes.data,
+                               ctypes.byref(n)))
+    return labels, n.value
+
+
+def h5_shape(filename: str):
+    """((Nx, Ny, Nz), nC) of the "field" dataset of an HDF5 field file."""
+    shape, nc = (ctypes.c_int * 3)(), ctypes.c_int()
+    check(load().gp5_h5_shape(os.fsencode(filename), shape, ctypes.byref(nc)))
+    return tuple(shape), nc.value
+
+
+def h5_write(filename: str, array) -> None:
+    """Write a (Nx, Ny, Nz, nC) complex array as the reference's HDF5 "field"
+    dataset (src/io.cpp:56-152); a 3-D array is one component."""
+    a = np.asarray(array)
+    if a.ndim == 3:
+        a = a[..., None]
+    if a.ndim != 4:
+        raise ValueError("expected a (Nx, Ny, Nz, nC) array")
+    a = np.asfortranarray(a, dtype=np.complex128)
+    shape = (ctypes.c_int * 3)(*a.shape[:3])
+    check(load().gp5_h5_write(os.fsencode(filename), a.ctypes.data, shape, a.shape[3]))
+
+
+def h5_read(filename: str, shape: Optional[Sequence[int]] = None, nc: Optional[int] = None) -> np.ndarray:
+    """Read an HDF5 field file (src/io.cpp:154-254) as a (Nx, Ny, Nz, nC)
+    complex128 Fortran-ordered array; with shape/nc given, a file of other
+    dims raises."""
+    fshape, fnc = h5_shape(filename)
+    shape = tuple(fshape if shape is None else shape)
+    nc = fnc if nc is None else nc
+    out = np.zeros(tuple(shape) + (nc,), dtype=np.complex128, order="F")
+    check(load().gp5_h5_read(os.fsencode(filename), out.ctypes.data, (ctypes.c_int * 3)(*shape), nc))
     return out
 
 

@@ -14,6 +14,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <limits.h>
 
 #include <algorithm>
 #include <chrono>
@@ -79,6 +80,8 @@ struct gp5_ctx {
     size_t dlog_cap;      // steps
     hipEvent_t ev0, ev1;
     struct ProfState *prof;  // non-null while gp5_profile_advance runs
+    int *ccl;             // droplet-labelling workspace, labels and count (allocated on first use)
+    size_t ccl_cap;       // ints
 };
 
 // Event pairs around every launch while profiling.
@@ -609,7 +612,7 @@ int gp5_ctx_destroy(gp5_ctx *c)
     if (c->st) note(hipStreamSynchronize(c->st));
     for (int d = 0; d < 3; ++d) if (c->tw[d]) note(hipFree(c->tw[d]));
     void *ptrs[] = {c->T,       c->kA,      c->kB,       c->sbuf, c->eout,  c->phase_ex, c->partials,
-                    c->results, c->scales, c->blockmax, c->dlog, c->nnorm, c->lhy_ab, c->Vdev};
+                    c->results, c->scales, c->blockmax, c->dlog, c->nnorm, c->lhy_ab, c->Vdev, c->ccl};
     for (void *p : ptrs) if (p) note(hipFree(p));
     if (c->ev0) note(hipEventDestroy(c->ev0));
     if (c->ev1) note(hipEventDestroy(c->ev1));
@@ -805,6 +808,34 @@ int gp5_max_density(gp5_ctx *c, const gp5_buf *f, int comp, double *out)
     double m = 0.0;
     for (double v : bm) m = std::max(m, v);
     *out = m;
+    return GP5_OK;
+}
+
+int gp5_decompose(gp5_ctx *c, const gp5_buf *psi, int comp, double cutoff, int *labels, int *ndroplets)
+{
+    if (!c || !labels) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, psi, "psi"));
+    if (comp < 0 || comp >= c->nc) return fail(GP5_ERR_ARG, "component %d out of range", comp);
+    if (c->M >= INT_MAX) return fail(GP5_ERR_UNSUPPORTED, "labelling needs fewer than 2^31 cells");
+    const size_t ws = decompose_workspace_ints(c->M);
+    const size_t need = ws + (size_t)c->M + 1;
+    if (c->ccl_cap < need) {
+        if (c->ccl) HIPCHK(hipFree(c->ccl));
+        c->ccl = nullptr;
+        c->ccl_cap = 0;
+        HIPCHK(hipMalloc(&c->ccl, need * sizeof(int)));
+        c->ccl_cap = need;
+    }
+    int *dlab = c->ccl + ws, *dcount = c->ccl + ws + c->M;
+    HIPCHK(launch_decompose(c->st, (const double2 *)psi->d + c->M * comp, c->n[0], c->n[1], c->n[2], cutoff, c->ccl,
+                            dlab, dcount));
+    int count = 0;
+    HIPCHK(hipMemcpyAsync(labels + (size_t)c->M * comp, dlab, (size_t)c->M * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&count, dcount, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < c->nc; ++k)
+        if (k != comp) memset(labels + (size_t)c->M * k, 0, (size_t)c->M * sizeof(int));
+    if (ndroplets) *ndroplets = count;
     return GP5_OK;
 }
 

@@ -133,5 +133,9 @@ bool fast_supported(int n);
 unsigned fast_grid(int n, long long I, long long O);
 long long fast_rows_per_block(int n);  // x lines per block of the row kernels
 hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a);
+// droplet labels of one component (gp5_ccl.hip); ws holds decompose_workspace_ints(M) ints
+size_t decompose_workspace_ints(long long M);
+hipError_t launch_decompose(hipStream_t st, const double2 *f, int nx, int ny, int nz, double cutoff, int *ws,
+                            int *labels, int *ndroplets_dev);
 
 }  // namespace gp5

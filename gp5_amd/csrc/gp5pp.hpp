@@ -455,6 +455,49 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// droplet labelling (src/cluster.h:8-18, src/cluster.cpp:17-82) on the GPU:
+// labels gets the field's M * nC ints, column-major; returns the droplet count
+// ---------------------------------------------------------------------------
+namespace decompose {
+inline int decompose(const tensor_t &psi, real_t cutOff, std::vector<int> &labels, int component = 0)
+{
+    labels.assign(psi.size(), 0);
+    int n = 0;
+    check(gp5_decompose(psi.ctx(), psi.buf(), component, cutOff, labels.data(), &n));
+    return n;
+}
+}  // namespace decompose
+
+// ---------------------------------------------------------------------------
+// HDF5 field files (src/io.cpp:56-254): dataset "field", double[2] elements,
+// dims (nC, Nz, Ny, Nx); staged through host memory
+// ---------------------------------------------------------------------------
+inline std::vector<complex_t> loadHost(const std::string &filename, const discretization &discr, int nComponents)
+{
+    const auto &s = discr.getGlobalMesh()->shape();
+    const int shape[3] = {s[0], s[1], s[2]};
+    std::vector<complex_t> host((size_t)s[0] * s[1] * s[2] * nComponents);
+    check(gp5_h5_read(filename.c_str(), reinterpret_cast<double *>(host.data()), shape, nComponents));
+    return host;
+}
+
+inline void save(const tensor_t &field, const std::string &filename, const discretization &discr)
+{
+    const auto &s = discr.getGlobalMesh()->shape();
+    const int shape[3] = {s[0], s[1], s[2]};
+    auto host = field.toHost();
+    check(gp5_h5_write(filename.c_str(), reinterpret_cast<const double *>(host.data()), shape, field.nComponents()));
+}
+
+inline tensor_t load(const std::string &filename, std::shared_ptr<discretization> discr, int nComponents)
+{
+    tensor_t t(discr, nComponents);
+    auto host = loadHost(filename, *discr, nComponents);
+    t.fromHost(host.data());
+    return t;
+}
+
+// ---------------------------------------------------------------------------
 // external potentials (src/externalPotential.cpp:6-82), evaluated on the host
 // once and kept on the device as real FP64
 // ---------------------------------------------------------------------------
@@ -507,6 +550,26 @@ public:
                 for (int j = 0; j < s[1]; ++j)
                     for (int i = 0; i < s[0]; ++i, ++p) V[p] = 0.5 / (x[i] * x[i] + y[j] * y[j]);
     }
+};
+
+// potentialFromFile (src/externalPotential.cpp:76-82): V read from an HDF5
+// field file (gp::load).  The reference keeps V complex; this build adds a real
+// V on the device, so a file whose V has a nonzero imaginary part is rejected.
+class potentialFromFile : public externalPotential {
+public:
+    explicit potentialFromFile(std::string filename) : _filename(std::move(filename)) {}
+    void fill(const discretization &discr, int nc, std::vector<real_t> &V) const override
+    {
+        auto z = loadHost(_filename, discr, nc);
+        for (size_t p = 0; p < V.size(); ++p) {
+            if (z[p].imag() != 0.0)
+                throw std::runtime_error(_filename + ": complex potentials are not supported (nonzero imaginary part)");
+            V[p] = z[p].real();
+        }
+    }
+
+private:
+    std::string _filename;
 };
 
 class sumPotential : public externalPotential {

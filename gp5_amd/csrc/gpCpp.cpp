@@ -5,7 +5,7 @@
 //   * one GPU: processorGrid must be [1,1,1] (fftOrdering is accepted);
 //   * an unknown stepper name raises (the reference dereferences null);
 //   * the GIL is released while kernels run;
-//   * field.save/load (parallel HDF5) are not part of this build;
+//   * field.save/load write the HDF5 field file serially (one writer);
 //   * additions, all with reference-preserving defaults: GP and UnlockedLHY
 //     models, model.energy, field.maxDensity, timeStepper.advanceLog.
 #include <pybind11/numpy.h>
@@ -90,8 +90,13 @@ public:
         gp::check(gp5_max_density(_state->ctx(), _state->buf(), c, &out));
         return out;
     }
-    void save(const std::string &) { throw std::runtime_error("field.save: parallel HDF5 I/O is not part of this build"); }
-    void load(const std::string &) { throw std::runtime_error("field.load: parallel HDF5 I/O is not part of this build"); }
+    // src/gpCpp.cpp:105-113: the reference's HDF5 field file, written serially
+    void save(const std::string &filename) { gp::save(*_state, filename, *getDiscretization()); }
+    void load(const std::string &filename)
+    {
+        auto host = gp::loadHost(filename, *getDiscretization(), _nComponents);
+        _state->fromHost(host.data());
+    }
 
     std::shared_ptr<gp::tensor_t> getState() { return _state; }
     std::shared_ptr<gp::fourierTransform> getFFTOp() { return _fftOp; }
@@ -233,13 +238,24 @@ private:
     real_t t = 0.0;
 };
 
-// src/gpCpp.cpp:289-313 (droplet labelling; bound without a constructor, as
-// in the reference, so it cannot be instantiated from Python)
+// src/gpCpp.cpp:289-313: droplet labels of component 0 as an F-ordered int
+// array of the field's shape (labelled on the GPU)
 class decomposition {
 public:
-    py::array_t<int> decompose(field &, real_t)
+    py::array_t<int> decompose(field &psi, real_t densityCutOff)
     {
-        throw std::runtime_error("decomposition: droplet labelling is not part of this build");
+        auto state = psi.getState();
+        auto d = state->dimensions();
+        std::vector<int> labels;
+        {
+            py::gil_scoped_release nogil;
+            gp::decompose::decompose(*state, densityCutOff, labels, 0);
+        }
+        py::array_t<int> a({(py::ssize_t)d[0], (py::ssize_t)d[1], (py::ssize_t)d[2], (py::ssize_t)d[3]},
+                           {(py::ssize_t)sizeof(int), (py::ssize_t)(sizeof(int) * d[0]),
+                            (py::ssize_t)(sizeof(int) * d[0] * d[1]), (py::ssize_t)(sizeof(int) * d[0] * d[1] * d[2])});
+        std::copy(labels.begin(), labels.end(), a.mutable_data());
+        return a;
     }
 };
 
@@ -269,7 +285,8 @@ PYBIND11_MODULE(gpCpp, m)
         .def("apply", &model::apply)
         .def("energy", &model::energy)
         .def("nComponents", &model::nComponents);
-    py::class_<decomposition>(m, "decomposition").def("decompose", &decomposition::decompose);
+    // the reference binds no constructor; the default one here is an addition
+    py::class_<decomposition>(m, "decomposition").def(py::init<>()).def("decompose", &decomposition::decompose);
     py::class_<LHY, model>(m, "LHY")
         .def(py::init<field &>())
         .def(py::init<field &, real_t, real_t>(), py::arg("psi"), py::arg("a"), py::arg("b"));

@@ -9,8 +9,9 @@ of the gpCpp extension (HIP kernels on one MI355X).
 
 Differences from the reference, all on the host side:
   * mpi4py and netCDF4 are optional (one GPU, one process);
-  * field.save / field.load use numpy .npy/.npz files (the reference's
-    parallel-HDF5 and netCDF writers are not part of this build);
+  * field.save / field.load read and write the reference's HDF5 field file
+    serially (one GPU) and also .npy/.npz; the netCDF4 writer is not part of
+    this build;
   * stepper accepts "RK4" and "eulero" (also spelled "Eulero", the name the
     reference documents); an unknown name raises;
   * additions: GP / UnlockedLHY models, model.energy, field.maxDensity,
@@ -24,7 +25,7 @@ from pathlib import Path
 
 import numpy as np
 
-from . import gpCpp
+from . import capi, gpCpp
 
 
 class geometry:
@@ -106,26 +107,39 @@ class field:
         return X, Y, Z
 
     def save(self, filename, format=None):
+        """Format from the extension as python/gp/gp.py:98-117 guesses it: .hdf5 / .h5
+        write the reference's HDF5 field file (src/io.cpp:56-152); .npz / .npy are
+        additions; netCDF4 (.nc) is not part of this build."""
         if format is None:
-            format = "npz" if re.match(r".*\.npz$", filename) else ("npy" if re.match(r".*\.npy$", filename) else None)
+            for pattern, fmt in ((r".*\.nc$", "netCDF4"), (r".*\.hdf5$", "hdf5"), (r".*\.h5$", "hdf5"),
+                                 (r".*\.npz$", "npz"), (r".*\.npy$", "npy")):
+                if re.match(pattern, filename):
+                    format = fmt
+                    break
         if format is None:
-            raise RuntimeError(f"Could not guess format of {filename} (this build writes .npy / .npz)")
+            raise RuntimeError(f"Could not guess format of {filename}")
         Path(os.path.dirname(filename) or ".").mkdir(parents=True, exist_ok=True)
-        if format == "npz":
+        if format == "hdf5":
+            capi.h5_write(filename, self.array)
+        elif format == "npz":
             np.savez(filename, field=self.array)
         elif format == "npy":
             np.save(filename, self.array)
+        elif format == "netCDF4":
+            raise RuntimeError("netCDF4 output is not part of this build (use .hdf5)")
         else:
             raise RuntimeError(f"Unkown format {format}")
 
     def load(self, filename):
+        """HDF5 field file (src/io.cpp:154-254) unless the name ends in .npz / .npy."""
         if filename.endswith(".npz"):
             with np.load(filename, allow_pickle=False) as z:
                 self.array = z["field"]
         elif filename.endswith(".npy"):
             self.array = np.load(filename, allow_pickle=False)
         else:
-            raise RuntimeError(f"Unkown format {filename}")
+            shape = tuple(self.localShape)
+            self.array = capi.h5_read(filename, shape, self._nComponents)
 
 
 class _model:

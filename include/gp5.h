@@ -149,6 +149,28 @@ int gp5_profile_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_st
                         gp5_buf *psi, int nsteps, double *kind_ms, int *kind_launches);
 const char *gp5_kernel_kind_name(int kind);
 
+/* Droplet labels: the connected components of {|psi_c|^2 > cutoff} over the 26
+ * periodic neighbours of each cell, numbered 1, 2, ... in the order in which
+ * the reference's scan (k fastest, i slowest) first meets them; cells at or
+ * below the cut-off get -1.  labels: host array of M * nC ints in the field's
+ * column-major (x, y, z, c) layout; only component `comp` is labelled, the
+ * others are set to 0.  *ndroplets (may be NULL) = number of droplets.
+ * Replaces gp::decompose::decompose (src/cluster.cpp:17-82) as driven by
+ * pyInterface::decomposition::decompose (src/gpCpp.cpp:289-313). */
+int gp5_decompose(gp5_ctx *ctx, const gp5_buf *psi, int comp, double cutoff, int *labels, int *ndroplets);
+
+/* ---- field files (host side) ----------------------------------------------
+ * HDF5 files in the reference's format: dataset "field" of H5T_ARRAY double[2]
+ * elements with dims (nC, Nz, Ny, Nx), holding the column-major complex128
+ * field (2 * M * nC doubles, x fastest, components last).  Replace gp::save /
+ * gp::load (src/io.cpp:56-152, 154-254), serially written (one GPU, one
+ * writer).  Files with a plain double dataset of dims (nC, Nz, Ny, Nx, 2) are
+ * also read.  libhdf5 is loaded at run time; GP5_ERR_UNSUPPORTED if it cannot
+ * be.  No device work. */
+int gp5_h5_write(const char *filename, const double *host, const int shape[3], int nc);
+int gp5_h5_read(const char *filename, double *host, const int shape[3], int nc);
+int gp5_h5_shape(const char *filename, int shape[3], int *nc); /* grid and component count of a file */
+
 /* ---- batches of independent instances ------------------------------------
  * No counterpart in the reference, which evolves one field per process: these
  * run many single-component instances of uneven cubic grids at once, each

@@ -627,3 +627,70 @@ void or_set_threads(int n)
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* Droplet labelling: gp::decompose::decompose / traverseDepthFirst          */
+/* (src/cluster.cpp:17-82), driven as pyInterface::decomposition::decompose  */
+/* (src/gpCpp.cpp:289-313).  labels has the field's (x,y,z,c) column-major   */
+/* shape; only `component` is written, the rest stays 0.  Cells scan with k */
+/* fastest and i slowest; an unvisited cell seeds a depth-first traversal   */
+/* over its 26 periodic neighbours that gives cells with |psi|^2 > cutoff   */
+/* the current colour and marks the others -1; the colour advances when the */
+/* seed itself was coloured.  The reference recurses; this restatement uses */
+/* an explicit stack (same labels: a traversal colours exactly the seed's    */
+/* connected component, whatever the visiting order).  Returns the number of */
+/* colours used, or -1 when out of memory.                                   */
+/* ------------------------------------------------------------------------ */
+static int or_pbc(int i, int L) { return i < 0 ? i + L : (i >= L ? i - L : i); }
+
+int or_decompose(const double *psi, int nx, int ny, int nz, int nc, int component, double cutoff, int *labels)
+{
+    const size_t M = (size_t)nx * ny * nz;
+    memset(labels, 0, sizeof(int) * M * (size_t)nc);
+    int *lab = labels + (size_t)component * M;
+    const double *f = psi + 2 * (size_t)component * M;
+    size_t cap = 1024, top = 0;
+    size_t *stack = (size_t *)malloc(cap * sizeof(size_t));
+    if (!stack) return -1;
+    int color = 1;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const size_t seed = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                if (lab[seed] != 0) continue;
+                top = 0;
+                stack[top++] = seed;
+                while (top) {
+                    const size_t p = stack[--top];
+                    if (lab[p] != 0) continue;
+                    const double re = f[2 * p], im = f[2 * p + 1];
+                    if (re * re + im * im > cutoff) {
+                        lab[p] = color;
+                    } else {
+                        lab[p] = -1;
+                        continue;
+                    }
+                    const int pi = (int)(p % nx), pj = (int)((p / nx) % ny), pk = (int)(p / ((size_t)nx * ny));
+                    for (int a = -1; a <= 1; ++a)
+                        for (int b = -1; b <= 1; ++b)
+                            for (int c = -1; c <= 1; ++c) {
+                                const size_t q = (size_t)or_pbc(pi + a, nx) +
+                                                 (size_t)nx * ((size_t)or_pbc(pj + b, ny) + (size_t)ny * or_pbc(pk + c, nz));
+                                if (lab[q] != 0) continue;
+                                if (top == cap) {
+                                    size_t *s2 = (size_t *)realloc(stack, 2 * cap * sizeof(size_t));
+                                    if (!s2) {
+                                        free(stack);
+                                        return -1;
+                                    }
+                                    stack = s2;
+                                    cap *= 2;
+                                }
+                                stack[top++] = q;
+                            }
+                }
+                if (lab[seed] != -1) color += 1;
+            }
+    free(stack);
+    return color - 1;
+}

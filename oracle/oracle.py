@@ -83,6 +83,8 @@ def lib():
         L.or_num_threads.restype = I
         L.or_set_threads.argtypes = [I]
         L.or_set_threads.restype = None
+        L.or_decompose.argtypes = [P, I, I, I, I, I, D, P]
+        L.or_decompose.restype = I
         _lib = L
     return _lib
 
@@ -257,6 +259,20 @@ def advance(func: Functional, a: np.ndarray, geo: Geometry, *, stepper: str = "R
     if rc != 0:
         raise RuntimeError(f"or_advance failed ({rc})")
     return a
+
+
+def decompose(a: np.ndarray, cutoff: float, component: int = 0):
+    """Droplet labels (src/cluster.cpp:17-82): int32 array of the field's
+    (Nx, Ny, Nz, nC) shape, Fortran order, and the number of droplets."""
+    a = _f(a)
+    if a.ndim == 3:
+        a = a[..., None]
+    nx, ny, nz, nc = a.shape
+    labels = np.zeros(a.shape, dtype=np.int32, order="F")
+    n = lib().or_decompose(_ptr(a), nx, ny, nz, nc, int(component), float(cutoff), _ptr(labels))
+    if n < 0:
+        raise MemoryError("or_decompose")
+    return labels, n
 
 
 def uniform(seed: int, count: int, start: int = 0) -> np.ndarray:
