@@ -117,6 +117,8 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
     b.close()
     _progress(f"cold: {ms / 1e3:.1f} s; surrogate-seeded run")
     seeded = surrogate_seeded(n, L, N, f, dt_factor, tol, check_every, max_steps, ms, steps, en)
+    _progress("two-level runs")
+    two = two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, ms, steps, en)
     conv = steps > 0
     inst_steps = int(np.abs(steps).sum())
     pts = np.array([k ** 3 for k in n], dtype=np.float64)
@@ -138,6 +140,7 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
         "tuning": report.strip().splitlines(),
         "start": "cold (Gaussian seeds)",
         "surrogate_seeded": seeded,
+        "two_level": two,
     }
     return out
 
@@ -171,6 +174,40 @@ def surrogate_seeded(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms
         "instance_steps_ratio": float(np.abs(cold_steps).sum() / np.abs(steps).sum()),
         "max_energy_per_atom_diff_vs_cold": float(np.max(np.abs(en - cold_en) / np.array(N))),
     }
+
+
+def two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, cold_steps, cold_en):
+    """The same batch through gp5_amd.ground_state.two_level: ground states on the
+    half-resolution grid of each box, resampled on the GPU to the full grid and
+    finished there to the same tolerance.  Coarse starts: Gaussians, then the
+    learned warm start evaluated on the coarse grid.  Timed: seeding + coarse run +
+    resampling + fine run (wall clock); tuning apart, as for the cold run."""
+    from gp5_amd import ground_state, surrogate
+
+    out = {}
+    legs = [("gaussian", None)]
+    if os.path.exists(surrogate.DEFAULT_WEIGHTS):
+        sur = surrogate.Surrogate()
+        legs.append(("surrogate", lambda i, k, Lk, Nk: sur.seed(Nk, (0.0, 0.0, 0.0), 2.5, k, Lk)))
+    for name, seed in legs:
+        fb, rep = ground_state.two_level(n, L, N, f, seed=seed, dt_factor=dt_factor, tol=tol,
+                                         check_every=check_every, max_steps=max_steps)
+        fb.close()
+        steps = rep["steps"]
+        out[name] = {
+            "seconds": rep["total_s"],
+            "seeding_seconds": rep["seed_s"], "coarse_seconds": rep["coarse_s"],
+            "resample_seconds": rep["resample_s"], "fine_seconds": rep["fine_s"], "tune_seconds": rep["tune_s"],
+            "coarse_grids": sorted(set(rep["coarse_grids"])),
+            "converged": int((steps > 0).sum()),
+            "coarse_steps_median": float(np.median(np.abs(rep["coarse_steps"]))),
+            "fine_steps_median": float(np.median(np.abs(steps))),
+            "fine_instance_steps": int(np.abs(steps).sum()),
+            "speedup_vs_cold": (cold_ms / 1e3) / rep["total_s"],
+            "max_energy_per_atom_diff_vs_cold": float(np.max(np.abs(rep["energy"] - cold_en) / np.array(N))),
+        }
+        _progress(f"two-level ({name}): {rep['total_s']:.1f} s")
+    return out
 
 
 def main() -> int:

@@ -38,10 +38,10 @@ EXPORTS = [
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
     "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
-    "gp5_kernel_kind_name", "gp5_decompose", "gp5_h5_write", "gp5_h5_read", "gp5_h5_shape",
+    "gp5_kernel_kind_name", "gp5_resample", "gp5_decompose", "gp5_h5_write", "gp5_h5_read", "gp5_h5_shape",
     "gp5_batch_create", "gp5_batch_destroy", "gp5_batch_classes", "gp5_batch_class_of",
     "gp5_batch_upload", "gp5_batch_download", "gp5_batch_set_lhy", "gp5_batch_set_potential",
-    "gp5_batch_advance", "gp5_batch_tune", "gp5_batch_ground_state",
+    "gp5_batch_advance", "gp5_batch_tune", "gp5_batch_ground_state", "gp5_batch_resample",
 ]
 GP5_NKINDS = 8
 
@@ -111,6 +111,7 @@ def load() -> ctypes.CDLL:
         "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
                                  ctypes.POINTER(ctypes.c_int)], I),
         "gp5_kernel_kind_name": ([I], ctypes.c_char_p),
+        "gp5_resample": ([P, P, P, P], I),
         "gp5_decompose": ([P, P, I, D, P, ctypes.POINTER(ctypes.c_int)], I),
         "gp5_h5_write": ([ctypes.c_char_p, P, ctypes.POINTER(ctypes.c_int), I], I),
         "gp5_h5_read": ([ctypes.c_char_p, P, ctypes.POINTER(ctypes.c_int), I], I),
@@ -127,6 +128,7 @@ def load() -> ctypes.CDLL:
         "gp5_batch_tune": ([P, ctypes.POINTER(FunctionalDesc), D, I, ctypes.c_char_p, S], I),
         "gp5_batch_ground_state": ([P, ctypes.POINTER(FunctionalDesc), D, D, I, ctypes.c_longlong,
                                     ctypes.POINTER(ctypes.c_longlong), DP, DP], I),
+        "gp5_batch_resample": ([P, I, P, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -356,6 +358,11 @@ def profile_advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer
     return out
 
 
+def resample(src_ctx: Context, src: Buffer, dst_ctx: Context, dst: Buffer) -> None:
+    """dst = src carried to dst's grid by trigonometric interpolation (same box)."""
+    check(load().gp5_resample(src_ctx.handle, src.handle, dst_ctx.handle, dst.handle))
+
+
 def decompose(ctx: Context, psi: Buffer, cutoff: float, comp: int = 0):
     """Droplet labels (src/cluster.cpp:17-82) on the GPU: an int32 (Nx, Ny, Nz, nC)
     Fortran-ordered array (-1 below the cut-off, droplets 1..n, other
@@ -471,6 +478,10 @@ class Batch:
         buf = ctypes.create_string_buffer(8192)
         check(load().gp5_batch_tune(self.handle, ctypes.byref(f), float(dt_factor), int(steps), buf, len(buf)))
         return buf.value.decode()
+
+    def resample_from(self, i: int, src: "Batch", j: int) -> None:
+        """Instance i = instance j of src resampled to this instance's grid (same box)."""
+        check(load().gp5_batch_resample(self.handle, int(i), src.handle, int(j)))
 
     def ground_state(self, f: FunctionalDesc, dt_factor: float, tol: float, check_every: int = 100,
                      max_steps: int = 0):

@@ -811,6 +811,114 @@ int gp5_max_density(gp5_ctx *c, const gp5_buf *f, int comp, double *out)
     return GP5_OK;
 }
 
+// Gather table of one axis for resampling ns -> nd points over a box of length
+// L with cell-centred samples (the comment of gp5_resample.hip derives it).
+struct HostAxisTable {
+    std::vector<int> off, src;
+    std::vector<double2> w;
+};
+
+static HostAxisTable resample_table(int ns, int nd, double L)
+{
+    std::vector<std::vector<std::pair<int, double2>>> per(nd);
+    const double shift = 0.5 * (L / nd - L / ns);  // x0 of the destination minus x0 of the source
+    const int kmin = ns % 2 == 0 ? -ns / 2 : -(ns - 1) / 2;
+    const int kmax = ns % 2 == 0 ? ns / 2 : (ns - 1) / 2;
+    for (int k = kmin; k <= kmax; ++k) {
+        const double ck = (ns % 2 == 0 && (k == kmin || k == kmax)) ? 0.5 : 1.0;
+        const double ang = 2.0 * M_PI * (double)k * shift / L;
+        const double2 w = make_double2(ck * cos(ang), ck * sin(ang));
+        const int s = ((k % ns) + ns) % ns;
+        const int d = ((k % nd) + nd) % nd;
+        bool merged = false;
+        for (auto &e : per[d])
+            if (e.first == s) {
+                e.second.x += w.x;
+                e.second.y += w.y;
+                merged = true;
+            }
+        if (!merged) per[d].push_back({s, w});
+    }
+    HostAxisTable t;
+    t.off.push_back(0);
+    for (int d = 0; d < nd; ++d) {
+        for (auto &e : per[d]) {
+            t.src.push_back(e.first);
+            t.w.push_back(e.second);
+        }
+        t.off.push_back((int)t.src.size());
+    }
+    return t;
+}
+
+// dst (cd's grid, cd->nc components) = resampled src (cs's grid); both boxes equal.
+static int resample_core(gp5_ctx *cs, const double2 *src, gp5_ctx *cd, double2 *dst)
+{
+    RCHK(ensure_scratch(cs, false, false));
+    RCHK(pass_plain(cs, K_FFT, 0, src, cs->T, -1, nullptr, nullptr));
+    RCHK(pass_plain(cs, K_FFT, 1, cs->T, cs->T, -1, nullptr, nullptr));
+    RCHK(pass_plain(cs, K_FFT, 2, cs->T, cs->T, -1, nullptr, nullptr));
+    HIPCHK(hipStreamSynchronize(cs->st));
+    HostAxisTable tab[3];
+    size_t bytes = 0;
+    for (int d = 0; d < 3; ++d) {
+        tab[d] = resample_table(cs->n[d], cd->n[d], cd->right[d] - cd->left[d]);
+        bytes += (tab[d].off.size() + tab[d].src.size()) * sizeof(int) + tab[d].w.size() * sizeof(double2);
+    }
+    // nine pieces, each placed at a 16-byte boundary: up to 15 bytes of padding each
+    bytes += 9 * 16;
+    char *dev = nullptr;
+    HIPCHK(hipMalloc(&dev, bytes));
+    ResampleAxis ax[3];
+    size_t o = 0;
+    int rc = GP5_OK;
+    for (int d = 0; d < 3 && rc == GP5_OK; ++d) {
+        auto put = [&](const void *h, size_t n) -> void * {
+            o = (o + 15) & ~(size_t)15;
+            void *p = dev + o;
+            if (o + n > bytes) rc = fail(GP5_ERR_HIP, "resample table layout");
+            else if (hipMemcpy(p, h, n, hipMemcpyHostToDevice) != hipSuccess) rc = fail(GP5_ERR_HIP, "resample table upload");
+            o += n;
+            return p;
+        };
+        ax[d].w = (const double2 *)put(tab[d].w.data(), tab[d].w.size() * sizeof(double2));
+        ax[d].off = (const int *)put(tab[d].off.data(), tab[d].off.size() * sizeof(int));
+        ax[d].src = (const int *)put(tab[d].src.data(), tab[d].src.size() * sizeof(int));
+        ax[d].nd = cd->n[d];
+        ax[d].ns = cs->n[d];
+    }
+    if (rc == GP5_OK) {
+        hipError_t e = launch_resample(cd->st, cs->T, dst, ax, cs->M, cd->M, cd->nc);
+        if (e != hipSuccess) rc = fail(GP5_ERR_HIP, "resample: %s", hipGetErrorString(e));
+    }
+    for (int d = 0; d < 3 && rc == GP5_OK; ++d) rc = pass_plain(cd, K_FFT, d, dst, dst, +1, nullptr, nullptr);
+    hipError_t es = hipStreamSynchronize(cd->st);
+    (void)hipFree(dev);
+    if (rc == GP5_OK && es != hipSuccess) rc = fail(GP5_ERR_HIP, "resample: %s", hipGetErrorString(es));
+    return rc;
+}
+
+static bool same_box(const double *l0, const double *r0, const double *l1, const double *r1)
+{
+    for (int d = 0; d < 3; ++d) {
+        const double s = std::max(1.0, std::max(fabs(r0[d] - l0[d]), fabs(r1[d] - l1[d])));
+        if (fabs(l0[d] - l1[d]) > 1e-12 * s || fabs(r0[d] - r1[d]) > 1e-12 * s) return false;
+    }
+    return true;
+}
+
+int gp5_resample(gp5_ctx *cs, const gp5_buf *src, gp5_ctx *cd, gp5_buf *dst)
+{
+    if (!cs || !cd) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(cs, src, "src"));
+    RCHK(check_field(cd, dst, "dst"));
+    if (cs->nc != cd->nc) return fail(GP5_ERR_ARG, "resample: %d components into %d", cs->nc, cd->nc);
+    if (!same_box(cs->left, cs->right, cd->left, cd->right))
+        return fail(GP5_ERR_ARG, "resample: the two grids must cover the same box");
+    if (cs != cd) HIPCHK(hipStreamSynchronize(cd->st));
+    return resample_core(cs, (const double2 *)src->d, cd, (double2 *)dst->d);
+}
+
 int gp5_decompose(gp5_ctx *c, const gp5_buf *psi, int comp, double cutoff, int *labels, int *ndroplets)
 {
     if (!c || !labels) return fail(GP5_ERR_ARG, "NULL argument");
@@ -1153,6 +1261,7 @@ struct gp5_batch {
     std::vector<double *> result;            // retired instances: their field (device)
     std::vector<long long> steps;
     std::vector<double> energy;
+    std::vector<gp5_ctx *> one;              // single-component contexts of gp5_batch_resample, by (n, L)
 };
 
 static void batch_drop_graph(BatchClass &k)
@@ -1319,8 +1428,50 @@ int gp5_batch_destroy(gp5_batch *b)
     }
     for (double *p : b->result)
         if (p) (void)hipFree(p);
+    for (gp5_ctx *c : b->one) {
+        int r = gp5_ctx_destroy(c);
+        if (rc == GP5_OK) rc = r;
+    }
     delete b;
     return rc;
+}
+
+static int batch_one_ctx(gp5_batch *b, int n, double L, gp5_ctx **out)
+{
+    for (gp5_ctx *c : b->one)
+        if (c->n[0] == n && c->right[0] - c->left[0] == L) {
+            *out = c;
+            return GP5_OK;
+        }
+    const int shape[3] = {n, n, n};
+    const double left[3] = {-0.5 * L, -0.5 * L, -0.5 * L}, right[3] = {0.5 * L, 0.5 * L, 0.5 * L};
+    gp5_ctx *c = nullptr;
+    RCHK(gp5_ctx_create(&c, shape, left, right, 1));
+    b->one.push_back(c);
+    *out = c;
+    return GP5_OK;
+}
+
+int gp5_batch_resample(gp5_batch *dst, int i, const gp5_batch *src, int j)
+{
+    if (!dst || !src) return fail(GP5_ERR_ARG, "NULL batch");
+    RCHK(batch_check_inst(dst, i));
+    RCHK(batch_check_inst(src, j));
+    const int s = dst->inst_slot[i];
+    if (s < 0) return fail(GP5_ERR_ARG, "instance %d is retired (its ground state is final)", i);
+    BatchClass &kd = dst->cls[dst->inst_class[i]];
+    const BatchClass &ks = src->cls[src->inst_class[j]];
+    const double Ld = kd.L, Ls = ks.L;
+    if (fabs(Ld - Ls) > 1e-12 * std::max(Ld, Ls))
+        return fail(GP5_ERR_ARG, "resample: instance boxes differ (%g vs %g)", Ls, Ld);
+    gp5_ctx *cs = nullptr, *cd = nullptr;
+    RCHK(batch_one_ctx(dst, ks.n, ks.L, &cs));
+    RCHK(batch_one_ctx(dst, kd.n, kd.L, &cd));
+    HIPCHK(hipStreamSynchronize(ks.ctx->st));
+    HIPCHK(hipStreamSynchronize(kd.ctx->st));
+    const int ss = src->inst_slot[j];
+    const double *from = ss >= 0 ? ks.psi->d + (size_t)2 * ks.ctx->M * ss : src->result[j];
+    return resample_core(cs, (const double2 *)from, cd, (double2 *)(kd.psi->d + (size_t)2 * kd.ctx->M * s));
 }
 
 int gp5_batch_classes(const gp5_batch *b) { return b ? (int)b->cls.size() : 0; }

@@ -133,6 +133,16 @@ bool fast_supported(int n);
 unsigned fast_grid(int n, long long I, long long O);
 long long fast_rows_per_block(int n);  // x lines per block of the row kernels
 hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a);
+// spectral resampling (gp5_resample.hip): per destination index d of one axis,
+// the source indices src[off[d] .. off[d+1]) with complex weights w (device)
+struct ResampleAxis {
+    const int *off;
+    const int *src;
+    const double2 *w;
+    int nd, ns;
+};
+hipError_t launch_resample(hipStream_t st, const double2 *F, double2 *G, const ResampleAxis ax[3], long long Ms,
+                           long long Md, int nc);
 // droplet labels of one component (gp5_ccl.hip); ws holds decompose_workspace_ints(M) ints
 size_t decompose_workspace_ints(long long M);
 hipError_t launch_decompose(hipStream_t st, const double2 *f, int nx, int ny, int nz, double cutoff, int *ws,

@@ -159,6 +159,14 @@ const char *gp5_kernel_kind_name(int kind);
  * pyInterface::decomposition::decompose (src/gpCpp.cpp:289-313). */
 int gp5_decompose(gp5_ctx *ctx, const gp5_buf *psi, int comp, double cutoff, int *labels, int *ndroplets);
 
+/* dst = src resampled to dst's grid: the trigonometric interpolant of src
+ * (cell-centred samples, Nyquist modes as cosines) evaluated at dst's cell
+ * centres; zero-padding of the spectrum when dst is finer, aliases folded when
+ * it is coarser.  Both contexts cover the same box and hold the same number of
+ * components.  No counterpart in the reference: it carries a state converged
+ * on a coarse grid to a finer one (multilevel ground-state starts). */
+int gp5_resample(gp5_ctx *src_ctx, const gp5_buf *src, gp5_ctx *dst_ctx, gp5_buf *dst);
+
 /* ---- field files (host side) ----------------------------------------------
  * HDF5 files in the reference's format: dataset "field" of H5T_ARRAY double[2]
  * elements with dims (nC, Nz, Ny, Nx), holding the column-major complex128
@@ -210,6 +218,10 @@ int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor,
 int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, double tol,
                            int check_every, long long max_steps, long long *steps_out, double *energy_out,
                            double *elapsed_ms);
+/* Instance i of dst = instance j of src (its current or final field) resampled
+ * to dst's grid, as gp5_resample; the two instances share the box length.
+ * Starts a fine-grid batch from a coarse-grid one. */
+int gp5_batch_resample(gp5_batch *dst, int i, const gp5_batch *src, int j);
 
 #ifdef __cplusplus
 }
