@@ -206,8 +206,14 @@ class UNet:
                               ctypes.c_void_p(targets.ptr.value + int(i) * P * 4), P * 4))
 
     def set_batch(self, X: np.ndarray, base: np.ndarray, target: Optional[np.ndarray] = None):
-        """X: [B, D, D, D, 16] float features; base/target: [B, D, D, D] float."""
-        self.t["X"].upload(to_bf16(X))
+        """X: [B, D, D, D, 16] float features; base/target: [B, D, D, D] float.  The
+        features go up as float and are rounded to bf16 on the device (the same
+        round-to-nearest-even as to_bf16)."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if "Xf" not in self.t:
+            self.t["Xf"] = DevBuf(self.t["X"].nbytes * 2)
+        self.t["Xf"].upload(X)
+        _ck(_lib().gp5_nn_cast_bf16(self.t["Xf"].ptr, self.t["X"].ptr, X.size))
         self.t["base"].upload(np.asarray(base, dtype=np.float32))
         if target is not None:
             self.t["target"].upload(np.asarray(target, dtype=np.float32))

@@ -78,21 +78,27 @@ def _grids(n: int = GRID):
     return _GRIDS[n]
 
 
-def features(N: float, w, b: float, n: int = GRID) -> np.ndarray:
+def features(N: float, w, b: float, n: int = GRID, out: np.ndarray = None) -> np.ndarray:
     """16 input channels on the scaled grid (channels last): the baseline profile, the
-    trap shape, |u| and the parameters as constant channels."""
+    trap shape, |u| and the parameters as constant channels.  With `out` (a float32
+    array from an earlier call on the same grid) only the channels that depend on
+    (N, w, b) are rewritten."""
     u2, r = _grids(n)
-    X = np.zeros((n, n, n, nn.C0), dtype=np.float32)
+    if out is None:
+        X = np.zeros((n, n, n, nn.C0), dtype=np.float32)
+        X[..., 7] = r / U_BOX
+    else:
+        X = out
     R0 = radius(N, b)
     X[..., 0] = 0.5 * (1.0 - np.tanh((r - 1.0) * (R0 / SURFACE)))
     a = [(wi / 0.2) ** 2 / 18.0 for wi in w]
-    X[..., 1] = a[0] * u2[:, None, None] + a[1] * u2[None, :, None] + a[2] * u2[None, None, :]
+    if any(a) or out is None or X[0, 0, 0, 1] != 0.0:
+        X[..., 1] = a[0] * u2[:, None, None] + a[1] * u2[None, :, None] + a[2] * u2[None, None, :]
     X[..., 2] = np.log(N / 50.0) / np.log(100.0)
     X[..., 3] = 2.0 * (b - 2.5)
     X[..., 4] = w[0] / 0.2
     X[..., 5] = w[1] / 0.2
     X[..., 6] = w[2] / 0.2
-    X[..., 7] = r / U_BOX
     return X
 
 
@@ -192,7 +198,8 @@ class Surrogate:
 
     def predict(self, N: float, w, b: float) -> np.ndarray:
         """phi on the scaled 64^3 grid."""
-        X = features(N, w, b)[None]
+        self._X = features(N, w, b, out=getattr(self, "_X", None))
+        X = self._X[None]
         self.net.set_batch(X, X[..., 0])
         self.net.forward()
         return self.net.output()[0]
