@@ -210,6 +210,47 @@ def two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, 
     return out
 
 
+def collision_large(steps: int, n: int = 512):
+    """Two N = 4000 LHY droplets colliding head-on in a 512^3 periodic box L = 64
+    (dx = 0.125), centres x = -8 and +8, momenta +-0.49 (0.5 snapped to 2 pi m / L);
+    real-time RK4, dt = 2.5e-4.  Timed like the headline run: `steps` steps between
+    stream synchronisations, after 2 warmup steps; norm and energy from the fused
+    reductions before and after."""
+    from gp5_amd import capi, synth
+
+    _progress(f"two-droplet collision at {n}^3")
+    shape, left, right, psi0, q = synth.droplet_collision(n)
+    ctx = capi.Context(shape, left, right, 1)
+    psi = ctx.field(psi0)
+    del psi0
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=2.5e-4, imaginary=False)
+    norms0, ens0 = capi.advance(ctx, f, s, psi, 1, log=True)
+    capi.advance(ctx, f, s, psi, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    dev_ms = capi.advance_timed(ctx, f, s, psi, steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    norms1, ens1 = capi.advance(ctx, f, s, psi, 1, log=True)
+    F = 16.0 * n ** 3
+    sps = steps / wall
+    out = {
+        "workload": f"two LHY droplets N=4000 each colliding, {n}^3 periodic box L=64, x0=+-8, "
+                    f"k=+-{q:.4f}, real-time RK4 dt=2.5e-4, complex128",
+        "steps": steps, "steps_per_s": sps, "ms_per_step": 1e3 * wall / steps,
+        "device_ms_per_step": dev_ms / steps,
+        "model_bytes_per_step": STEP_MODEL_F * F,
+        "frac_of_hbm_peak": STEP_MODEL_F * F * sps / 1e9 / HBM_PEAK_GBPS,
+        "norm_start": float(norms0[0, 0]), "norm_end": float(norms1[0, 0]),
+        "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
+        "steps_between": steps + 2,
+    }
+    psi.free()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +261,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gs-count", type=int, default=64,
                     help="instances of the batched ground-state workload (0 skips it)")
+    ap.add_argument("--large-steps", type=int, default=400,
+                    help="RK4 steps of the 512^3 two-droplet collision (0 skips it)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one normalize of the field: its scale kernel reads and writes the field "
                          "once, the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -313,6 +356,8 @@ def main() -> int:
         ctx.sync()
     if args.gs_count > 0:
         out["ground_state"] = ground_state_batch(args.gs_count)
+    if args.large_steps > 0:
+        out["large"] = collision_large(args.large_steps)
     if not args.no_cpu:
         _progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)

@@ -87,6 +87,35 @@ def droplet_256(n: int = 256) -> Tuple[Tuple[int, int, int], Tuple[float, ...], 
     return shape, left, right, flat_top_droplet(shape, left, right, 6.0, 0.5, 1e-4, 2, 3000.0)
 
 
+def droplet_collision(n: int = 512, L: float = 64.0, N: float = 4000.0, x0: float = 8.0, k: float = 0.5):
+    """Two untrapped LHY droplets of N atoms each centred at x = -x0 and x = +x0 of the
+    periodic box [-L/2, L/2]^3 on an n^3 grid, moving toward each other along x:
+    psi = A [s(r-) e^{+i q x} + s(r+) e^{-i q x}], s a flat top (width 0.5) of the
+    saturation-density radius (3N / 4 pi)^{1/3}, q = k snapped to the nearest 2 pi m / L,
+    A fixed by the total norm 2N.  Built slab by slab along z (2 GiB at 512^3).
+    Returns (shape, left, right, psi (n, n, n, 1) complex128 Fortran order, q)."""
+    shape = (n, n, n)
+    left, right = (-0.5 * L,) * 3, (0.5 * L,) * 3
+    q = 2.0 * np.pi / L * round(k * L / (2.0 * np.pi))
+    R = droplet_radius(N)
+    x = axes(shape, left, right)[0]
+    psi = np.empty((n, n, n, 1), dtype=np.complex128, order="F")
+    ex_p, ex_m = np.exp(1j * q * x), np.exp(-1j * q * x)
+    yy = x[None, :] ** 2
+    for kz in range(n):
+        zz = x[kz] ** 2
+        rm = np.sqrt((x[:, None] + x0) ** 2 + yy + zz)
+        rp = np.sqrt((x[:, None] - x0) ** 2 + yy + zz)
+        sm = 0.5 * (1.0 - np.tanh((rm - R) / 0.5))
+        sp = 0.5 * (1.0 - np.tanh((rp - R) / 0.5))
+        psi[:, :, kz, 0] = sm * ex_p[:, None] + sp * ex_m[:, None]
+    norm = 0.0
+    for kz in range(n):
+        norm += float(np.sum(np.abs(psi[:, :, kz, 0]) ** 2))
+    psi *= np.sqrt(2.0 * N / (norm * dV(shape, left, right)))
+    return shape, left, right, psi, q
+
+
 def harmonic_64(n: int = 64):
     """Single-component GP, n^3 on [-10, 10]^3, Gaussian sigma = 1.5 plus seeded (seed 1)
     1e-3 complex uniform noise, normalised to N = 1; harmonic trap omega = (1,1,1)."""
