@@ -326,3 +326,58 @@ def test_field_save_load_hdf5(tmp_path):
     bad = gp.field(gp.geometry((8, 8, 12), left=[-4, -2, -3], right=[4, 2, 3]), nComponents=2)
     with pytest.raises(Exception, match="expected"):
         bad.load(fn)
+
+
+@pytest.mark.gpu
+def test_driver_unlocked_droplet_two_components(tmp_path, oracle_mod):
+    """The exampleInputFiles/dropletUnlocked.yaml pattern (UnlockedLHYDroplet,
+    eta -4, alpha 0.8838834764831843, beta 1, imaginary-time RK4, two
+    components) on a small grid, with renormalisation, against the oracle."""
+    O = oracle_mod
+    shape, box, nit, dt = (16, 12, 14), 6.0, 3, 2e-3
+    functional = {"droplet": {"name": "UnlockedLHYDroplet", "eta": -4, "alpha": 0.8838834764831843, "beta": 1}}
+    cfg = _config(tmp_path, functional, nc=2, shape=shape, box=box, dt=dt, nit=nit,
+                  max_time=nit * dt * 1.5, norms=[10.0, 100.0])
+    psi0 = seeded_field(shape, 2, seed=31, scale=0.5)
+    capi.h5_write(cfg["initialCondition"]["file"], psi0)
+    files = driver.run(_write_yaml(tmp_path, cfg))
+    assert len(files) == 2
+    geo = O.Geometry(shape, [-box] * 3, [box] * 3, 2)
+    f = O.Functional(kind=O.OR_UNLOCKED, alpha=0.8838834764831843, beta=1.0, eta=-4.0)
+    ref = O.advance(f, psi0, geo, stepper="RK4", dt=dt, imaginary=True, nsteps=nit, renormalize=[10.0, 100.0])
+    assert rel_l2(capi.h5_read(files[1], shape, 2), ref) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_driver_vortex_with_phase_constraint(tmp_path, oracle_mod):
+    """gpFunctional with harmonic + vortex potentials and a phaseConstraint read from
+    an HDF5 phase file (imprinted winding atan2(y, x)), imaginary-time eulero with
+    renormalisation, against the oracle."""
+    O = oracle_mod
+    shape, box, nit, dt = (16, 16, 8), 4.0, 4, 1e-3
+    geo = O.Geometry(shape, [-box] * 3, [box] * 3, 1)
+    X, Y, Z = geo.grid()
+    phase = np.arctan2(Y, X)[..., None]
+    capi.h5_write(str(tmp_path / "phi.hdf5"), phase.astype(complex))
+    functional = {
+        "gp": {"name": "gpFunctional", "masses": [1], "coupling": [[1.5]]},
+        "V": {"name": "externalPotential",
+              "V": {"h11": {"name": "harmonic", "omegas": [[1, 1, 2]]}, "vortex": {"name": "vortex"}}},
+    }
+    cfg = _config(tmp_path, functional, shape=shape, box=box, dt=dt, nit=nit, max_time=nit * dt * 1.5,
+                  stepper="eulero", norms=[5.0],
+                  constraint={"vortexPhase": {"file": str(tmp_path / "phi.hdf5"), "name": "phaseConstraint"}})
+    psi0 = np.asfortranarray((np.exp(-(X ** 2 + Y ** 2 + Z ** 2) / 4.0) + 0j)[..., None])
+    capi.h5_write(cfg["initialCondition"]["file"], psi0)
+    files = driver.run(_write_yaml(tmp_path, cfg))
+    assert len(files) == 2
+    V = 0.5 * (X ** 2 + Y ** 2 + 4 * Z ** 2) + 0.5 / (X ** 2 + Y ** 2)
+    f = O.Functional(kind=O.OR_GP, couplings=np.array([[1.5]]), masses=(1.0,), V=V[..., None])
+    ref = O.advance(f, psi0, geo, stepper="eulero", dt=dt, imaginary=True, nsteps=nit, renormalize=[5.0],
+                    phase=phase)
+    got = capi.h5_read(files[1], shape, 1)
+    assert rel_l2(got, ref) <= 1e-12
+    # the imprinted winding survives: the phase of the state is atan2(y, x) where it is not tiny
+    mask = np.abs(got[..., 0]) > 1e-3 * np.abs(got).max()
+    d = np.angle(got[..., 0][mask] * np.exp(-1j * phase[..., 0][mask]))
+    assert np.abs(d).max() < 1e-12
