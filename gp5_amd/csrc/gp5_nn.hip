@@ -225,6 +225,59 @@ __global__ void k_cast(const float *s, uint16_t *d, long long n)
         d[i] = f2bf(s[i]);
 }
 
+// Input channels of the droplet warm start on the scaled grid u in [-ub, ub]^3,
+// element (i, j, k) of a C-ordered [D][D][D][16] array: 0 the flat-top baseline
+// 0.5 (1 - tanh((r - 1) s)), 1 the trap shape a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
+// 2..6 constants, 7 r / ub, 8..15 zero, in float32 with the operation order of
+// the host construction, then rounded to bf16; base = channel 0 in float.
+struct FeatArgs {
+    int D;
+    double ub;
+    float s, a0, a1, a2, ub_f;
+    float c[5];
+};
+
+__device__ __forceinline__ float u_axis_f(int i, int D, double ub) { return (float)(-ub + (i + 0.5) * (2.0 * ub / D)); }
+
+__global__ __launch_bounds__(256) void k_droplet_features(uint16_t *X, float *base, FeatArgs a)
+{
+    const long long P = (long long)a.D * a.D * a.D;
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= P) return;
+    const int k = (int)(p % a.D), j = (int)((p / a.D) % a.D), i = (int)(p / ((long long)a.D * a.D));
+    const float ui = u_axis_f(i, a.D, a.ub), uj = u_axis_f(j, a.D, a.ub), uk = u_axis_f(k, a.D, a.ub);
+    const float u2i = __fmul_rn(ui, ui), u2j = __fmul_rn(uj, uj), u2k = __fmul_rn(uk, uk);
+    const float r = sqrtf(__fadd_rn(__fadd_rn(u2i, u2j), u2k));
+    const float t = tanhf(__fmul_rn(__fsub_rn(r, 1.0f), a.s));
+    const float c0 = __fmul_rn(0.5f, __fsub_rn(1.0f, t));
+    const float c1 = __fadd_rn(__fadd_rn(__fmul_rn(a.a0, u2i), __fmul_rn(a.a1, u2j)), __fmul_rn(a.a2, u2k));
+    uint16_t h[16];
+    h[0] = f2bf(c0);
+    h[1] = f2bf(c1);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) h[2 + q] = f2bf(a.c[q]);
+    h[7] = f2bf(__fdiv_rn(r, a.ub_f));
+#pragma unroll
+    for (int q = 8; q < 16; ++q) h[q] = 0;
+    uint4 *dst = reinterpret_cast<uint4 *>(X + p * 16);
+    uint4 w0, w1;
+    w0.x = h[0] | (uint32_t)h[1] << 16;
+    w0.y = h[2] | (uint32_t)h[3] << 16;
+    w0.z = h[4] | (uint32_t)h[5] << 16;
+    w0.w = h[6] | (uint32_t)h[7] << 16;
+    w1.x = w1.y = w1.z = w1.w = 0u;
+    dst[0] = w0;
+    dst[1] = w1;
+    base[p] = c0;
+}
+
+// y[p] = base[p] + Yf[p * C] (the network's prediction: baseline plus channel 0)
+__global__ __launch_bounds__(256) void k_predict_out(const float *Yf, int C, const float *base, float *y, long long P)
+{
+    const long long p = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (p < P) y[p] = base[p] + Yf[p * C];
+}
+
 __global__ void k_fill0(uint32_t *p, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -469,6 +522,32 @@ int gp5_nn_add_f32(float *dst, const float *src, long long n)
 {
     hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(256), 0, 0, dst, src, n);
     NNCHK(hipGetLastError(), "add");
+    return GP5_OK;
+}
+
+int gp5_nn_droplet_features(void *X, float *base, int D, double ub, float s, const float a[3], const float c[5])
+{
+    if (!X || !base || !a || !c || D < 1) return set_error(GP5_ERR_ARG, "gp5_nn_droplet_features: bad argument");
+    FeatArgs f;
+    f.D = D;
+    f.ub = ub;
+    f.ub_f = (float)ub;
+    f.s = s;
+    f.a0 = a[0];
+    f.a1 = a[1];
+    f.a2 = a[2];
+    for (int q = 0; q < 5; ++q) f.c[q] = c[q];
+    const long long P = (long long)D * D * D;
+    hipLaunchKernelGGL(k_droplet_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, (uint16_t *)X, base, f);
+    NNCHK(hipGetLastError(), "droplet features");
+    return GP5_OK;
+}
+
+int gp5_nn_predict_out(const float *Yf, int C, const float *base, float *y, long long P)
+{
+    if (!Yf || !base || !y || C < 1 || P < 0) return set_error(GP5_ERR_ARG, "gp5_nn_predict_out: bad argument");
+    hipLaunchKernelGGL(k_predict_out, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, Yf, C, base, y, P);
+    NNCHK(hipGetLastError(), "predict out");
     return GP5_OK;
 }
 

@@ -46,6 +46,8 @@ def _lib():
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
+        "gp5_nn_droplet_features": [P, P, I, ctypes.c_double, F, P, P],
+        "gp5_nn_predict_out": [P, I, P, P, LL],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -240,6 +242,24 @@ class UNet:
         _ck(L.gp5_nn_up2_bf16(t["d2"].ptr, t["u2"].ptr, B, D // 2, 32, 1.0))
         self._conv("c7", t["u2"], A=t["a7"], skip=t["e1"], Y=t["d1"], D=D)
         self._conv("c8", t["d1"], Yf=t["out"], D=D, act=0)
+
+    def set_droplet_features(self, s: float, a, consts, ub: float):
+        """Features of one droplet (batch of 1) built on the device (gp5_nn_droplet_features):
+        the flat-top baseline with surface factor s, trap weights a, the five constant
+        channels consts and the scaled half-box ub; the baseline also becomes `base`."""
+        if self.B != 1:
+            raise ValueError("device features are built for a batch of one")
+        af = (ctypes.c_float * 3)(*[float(v) for v in a])
+        cf = (ctypes.c_float * 5)(*[float(v) for v in consts])
+        _ck(_lib().gp5_nn_droplet_features(self.t["X"].ptr, self.t["base"].ptr, self.D, float(ub), float(s), af, cf))
+
+    def output0(self) -> np.ndarray:
+        """The prediction base + out[..., 0] as [B, D, D, D] float, combined on the device."""
+        B, D = self.B, self.D
+        if "y" not in self.t:
+            self.t["y"] = DevBuf(B * D ** 3 * 4)
+        _ck(_lib().gp5_nn_predict_out(self.t["out"].ptr, 16, self.t["base"].ptr, self.t["y"].ptr, B * D ** 3))
+        return self.t["y"].download(np.float32, (B, D, D, D))
 
     def output(self) -> np.ndarray:
         B, D = self.B, self.D

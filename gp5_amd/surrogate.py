@@ -196,8 +196,17 @@ class Surrogate:
                 params = {k: z[k] for k in z.files if "." in k}
         self.net = nn.UNet(1, GRID, params)
 
-    def predict(self, N: float, w, b: float) -> np.ndarray:
-        """phi on the scaled 64^3 grid."""
+    def predict(self, N: float, w, b: float, device_features: bool = True) -> np.ndarray:
+        """phi on the scaled 64^3 grid.  The input channels are built on the device
+        (gp5_nn_droplet_features, the float32 arithmetic of features()) unless
+        device_features is false."""
+        if device_features:
+            R0 = radius(N, b)
+            a = [(wi / 0.2) ** 2 / 18.0 for wi in w]
+            consts = [np.log(N / 50.0) / np.log(100.0), 2.0 * (b - 2.5), w[0] / 0.2, w[1] / 0.2, w[2] / 0.2]
+            self.net.set_droplet_features(np.float32(R0 / SURFACE), np.float32(a), np.float32(consts), U_BOX)
+            self.net.forward()
+            return self.net.output0()[0]
         self._X = features(N, w, b, out=getattr(self, "_X", None))
         X = self._X[None]
         self.net.set_batch(X, X[..., 0])

@@ -60,6 +60,13 @@ int gp5_nn_cast_bf16(const float *src, void *dst, long long n);
  * dYf[p][c] = 2 (y - t) / P for c = 0 and 0 otherwise; *loss = mean. */
 int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P,
                double *loss);
+/* Droplet input channels (one cube of side D on the scaled box [-ub, ub]^3, C-ordered
+ * [D][D][D][16] bf16): 0 = 0.5 (1 - tanh((r - 1) s)), 1 = a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
+ * 2..6 = c[0..4], 7 = r / ub, 8..15 = 0; base (float, D^3) = channel 0.  float32
+ * arithmetic in the host construction's order, rounded to nearest even. */
+int gp5_nn_droplet_features(void *X, float *base, int D, double ub, float s, const float a[3], const float c[5]);
+/* y[p] = base[p] + Yf[p * C]: the prediction (baseline plus output channel 0). */
+int gp5_nn_predict_out(const float *Yf, int C, const float *base, float *y, long long P);
 /* Adam step on n float parameters (t: 1-based step count). */
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps,
                 int t);

@@ -61,3 +61,39 @@ def test_seed_beats_cold_and_baseline(oracle_mod):
         for t in (100, 300):
             assert err["net"][t - 1] < err["cold"][t - 1], (i, t)
         assert np.linalg.norm(np.abs(fields["net"]) - np.abs(ref)) < 1e-3 * np.linalg.norm(ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N,w,b", [(300.0, (0.0, 0.0, 0.0), 2.5), (2345.0, (0.1, 0.05, 0.17), 2.2)])
+def test_device_features_match_host_features(N, w, b):
+    """gp5_nn_droplet_features reproduces features() (float32, rounded to bf16): the
+    trap, radius and constant channels bit for bit, the tanh baseline within one bf16
+    step; the device-built prediction matches the host-built one."""
+    from gp5_amd import nn
+
+    if not os.path.exists(S.DEFAULT_WEIGHTS):
+        pytest.skip("no trained weights")
+    sur = S.Surrogate()
+    host = S.features(N, w, b)
+    want = nn.to_bf16(host).reshape(-1, 16)
+    R0 = S.radius(N, b)
+    a = [(wi / 0.2) ** 2 / 18.0 for wi in w]
+    consts = [np.log(N / 50.0) / np.log(100.0), 2.0 * (b - 2.5), w[0] / 0.2, w[1] / 0.2, w[2] / 0.2]
+    sur.net.set_droplet_features(np.float32(R0 / S.SURFACE), np.float32(a), np.float32(consts), S.U_BOX)
+    got = sur.net.t["X"].download(np.uint16, (S.GRID ** 3 * 16,)).reshape(-1, 16)
+    assert np.array_equal(got[:, 1:], want[:, 1:])
+    # the baseline: tanhf and numpy's float32 tanh may differ in the last place, which
+    # shows as one bf16 step, or as 3e-8 against 0 far outside the droplet
+    # (1 - tanh cancels: a few float32 ulps of tanh near 1 are ~2e-7 absolute)
+    g0, w0 = nn.from_bf16(got[:, 0]), nn.from_bf16(want[:, 0])
+    excess = np.abs(g0 - w0) - 2.0 ** -7 * np.abs(w0)
+    print(f"baseline channel: max |diff| {np.abs(g0 - w0).max():.3e}, max excess {excess.max():.3e}", flush=True)
+    assert excess.max() <= 5e-7
+    # where 1 - tanh does not cancel, a last-place tanh difference stays below half a bf16 step
+    core = np.abs(w0) > 1e-2
+    assert (got[core, 0] != want[core, 0]).mean() < 1e-2
+    base = sur.net.t["base"].download(np.float32, (S.GRID ** 3,))
+    assert np.allclose(base, host[..., 0].ravel(), rtol=0, atol=2e-7)
+    dev = sur.predict(N, w, b, device_features=True)
+    hst = sur.predict(N, w, b, device_features=False)
+    assert np.linalg.norm(dev - hst) <= 1e-3 * np.linalg.norm(hst)
