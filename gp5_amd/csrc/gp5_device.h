@@ -62,6 +62,31 @@ __device__ __forceinline__ void dft4(double2 *v, double s)
     v[3] = csub(a1, b1);
 }
 
+// 12 points as 3 x 4 by the prime-factor (Good-Thomas) mapping, no twiddles:
+// input n = (4 n1 + 3 n2) mod 12, 3-point DFTs over n1, 4-point DFTs over n2,
+// output k = (4 k1 + 9 k2) mod 12 (k = k1 mod 3, k = k2 mod 4).
+__device__ __forceinline__ void dft12(double2 *v, double s)
+{
+    double2 a[3][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 3; ++n1)
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) a[n1][n2] = v[(4 * n1 + 3 * n2) % 12];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        double2 t[3] = {a[0][n2], a[1][n2], a[2][n2]};
+        dft3(t, s);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) a[k1][n2] = t[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        dft4(a[k1], s);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[(4 * k1 + 9 * k2) % 12] = a[k1][k2];
+    }
+}
+
 __device__ __forceinline__ void dft5(double2 *v, double s)
 {
     const double c1 = 0.30901699437494742410229341718282;   // cos(2 pi/5)

@@ -2,9 +2,9 @@
 // 192, 256 and 512.
 //
 // Each thread keeps E complex values of one line in registers; a length-N
-// transform runs as 1-4 radix stages (16x16 for 256, 16x8 for 128, 8x8 for 64,
-// 8x8x8 for 512, 4x4x2x3 for 96, 4x4x4x3 for 192) with one LDS exchange
-// between stages.  Global loads and stores
+// transform runs as 1-3 radix stages (16x16 for 256, 16x8 for 128, 8x8 for 64,
+// 8x8x8 for 512, 4x2x12 for 96, 4x4x12 for 192; the 12-point butterfly is a
+// prime-factor 3x4 in registers) with one LDS exchange between stages.  Global loads and stores
 // go straight between HBM and registers: element (t + TL*i) + (N/R)*r of a line
 // belongs to thread t, so for fixed (i, r) consecutive threads touch consecutive
 // elements (x lines) or consecutive columns (y/z axes, columns interleaved).
@@ -29,6 +29,7 @@ __device__ __forceinline__ void dftR(double2 *v, double s)
     else if constexpr (R == 4) dft4(v, s);
     else if constexpr (R == 8) dft8(v, s);
     else if constexpr (R == 16) dft16(v, s);
+    else if constexpr (R == 12) dft12(v, s);
 }
 
 // Radix plans per layout (ROWS: x lines; else interleaved y/z columns): R0
@@ -41,16 +42,16 @@ template <int N, bool ROWS> struct FPlan;
 GP5_PLAN(16, false, 16, 0, 0, 0, 16);
 GP5_PLAN(32, false, 8, 4, 0, 0, 8);
 GP5_PLAN(64, false, 8, 8, 0, 0, 8);
-GP5_PLAN(96, false, 4, 4, 2, 3, 12);
+GP5_PLAN(96, false, 4, 2, 12, 0, 12);
 GP5_PLAN(128, false, 16, 8, 0, 0, 16);
-GP5_PLAN(192, false, 4, 4, 4, 3, 12);
+GP5_PLAN(192, false, 4, 4, 12, 0, 12);
 GP5_PLAN(256, false, 16, 16, 0, 0, 16);
 GP5_PLAN(512, false, 8, 8, 8, 0, 8);
 GP5_PLAN(16, true, 4, 4, 0, 0, 4);
 GP5_PLAN(32, true, 8, 4, 0, 0, 8);
 GP5_PLAN(64, true, 8, 8, 0, 0, 8);
-GP5_PLAN(96, true, 4, 4, 2, 3, 12);
-GP5_PLAN(192, true, 4, 4, 4, 3, 12);
+GP5_PLAN(96, true, 4, 2, 12, 0, 12);
+GP5_PLAN(192, true, 4, 4, 12, 0, 12);
 #ifdef GP5_ROWS_E16
 GP5_PLAN(128, true, 16, 8, 0, 0, 16);
 GP5_PLAN(256, true, 16, 16, 0, 0, 16);
