@@ -142,6 +142,12 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
         "surrogate_seeded": seeded,
         "two_level": two,
     }
+    # speed-up of every start over the cold run, same convergence rule and energies
+    out["speedup_vs_cold"] = {
+        "network_seed_only": seeded["speedup_vs_cold"] if seeded else None,
+        "two_level_gaussian_coarse": two.get("gaussian", {}).get("speedup_vs_cold"),
+        "two_level_network_coarse": two.get("surrogate", {}).get("speedup_vs_cold"),
+    }
     return out
 
 
