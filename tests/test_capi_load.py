@@ -41,6 +41,28 @@ def test_library_reports_version_and_error_text():
     assert b"NULL" in L.gp5_last_error()
 
 
+def test_entry_points_reject_null_arguments_on_the_host():
+    """Argument checks of the labelling, resampling, batch-resampling and HDF5 entry
+    points run before any device work (no GPU needed)."""
+    import numpy as np
+
+    from gp5_amd import capi
+
+    L = capi.load()
+    n = ctypes.c_int()
+    assert L.gp5_decompose(None, None, 0, 0.5, None, ctypes.byref(n)) == capi.GP5_ERR_ARG
+    assert L.gp5_resample(None, None, None, None) == capi.GP5_ERR_ARG
+    assert L.gp5_batch_resample(None, 0, None, 0) == capi.GP5_ERR_ARG
+    shape = (ctypes.c_int * 3)(2, 2, 2)
+    buf = np.zeros(16)
+    assert L.gp5_h5_write(None, buf.ctypes.data, shape, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_h5_read(b"x.h5", None, shape, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_h5_shape(None, shape, ctypes.byref(n)) == capi.GP5_ERR_ARG
+    bad = (ctypes.c_int * 3)(2, 0, 2)
+    assert L.gp5_h5_write(b"x.h5", buf.ctypes.data, bad, 1) == capi.GP5_ERR_ARG
+    assert b"bad shape" in L.gp5_last_error()
+
+
 def _build_facade_example(out_dir):
     gxx = shutil.which("g++")
     if gxx is None:
