@@ -232,13 +232,7 @@ template <int N, bool ROWS> struct FirstLast {
 // MODE 2: rows only: inverse, epilogue, optional forward of the epilogue output.
 // MODE 3: MODE 2 with the component-local epilogue (no coupling, no constraint).
 template <int N, bool ROWS, int MODE, int DIR>
-#ifdef GP5_WAVES_PER_EU
-#define GP5_KATTR __attribute__((amdgpu_waves_per_eu(GP5_WAVES_PER_EU)))
-#else
-#define GP5_KATTR
-#endif
-__global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<N, ROWS>::COL_THREADS))
-    GP5_KATTR k_fast(FastArgs a)
+__device__ __forceinline__ void fast_body(const FastArgs &a)
 {
     using S = FShape<N, ROWS>;
     using FL = FirstLast<N, ROWS>;
@@ -414,6 +408,29 @@ __global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<
     }
 }
 
+#ifdef GP5_WAVES_PER_EU
+#define GP5_KATTR __attribute__((amdgpu_waves_per_eu(GP5_WAVES_PER_EU)))
+#else
+#define GP5_KATTR
+#endif
+template <int N, bool ROWS, int MODE, int DIR>
+__global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<N, ROWS>::COL_THREADS))
+    GP5_KATTR k_fast(FastArgs a)
+{
+    fast_body<N, ROWS, MODE, DIR>(a);
+}
+
+// The same pass held to 3 waves per SIMD (at most 168 VGPRs).  Used for the
+// 256-point z pass, which needs 170 registers otherwise and then runs 2 waves
+// per SIMD: without spills at 3 it measured 96.3 -> 88.0 us at 256^3.  The
+// epilogue kernels would spill under this limit and stay on k_fast.
+template <int N, bool ROWS, int MODE, int DIR>
+__global__ void __launch_bounds__((ROWS ? FShape<N, ROWS>::ROW_THREADS : FShape<N, ROWS>::COL_THREADS))
+    __attribute__((amdgpu_waves_per_eu(3))) k_fast_w3(FastArgs a)
+{
+    fast_body<N, ROWS, MODE, DIR>(a);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -459,8 +476,10 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
     } else {
         if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(false, 0, -1);
         else if (mode == 0) GP5_FAST_LAUNCH(false, 0, 1);
-        else if (mode == 1) GP5_FAST_LAUNCH(false, 1, -1);
-        else return hipErrorInvalidValue;  // the epilogue pass needs x lines
+        else if (mode == 1) {
+            if constexpr (N == 256) hipLaunchKernelGGL((k_fast_w3<N, false, 1, -1>), g, b, lds, st, a);
+            else GP5_FAST_LAUNCH(false, 1, -1);
+        } else return hipErrorInvalidValue;  // the epilogue pass needs x lines
     }
 #undef GP5_FAST_LAUNCH
     return hipGetLastError();
