@@ -7,6 +7,9 @@ Layers:
   gp5_amd/csrc/gpCpp.cpp              pybind11 module gpCpp (reference module surface)
   gp5_amd/capi.py                     ctypes view of the C ABI
   gp5_amd/gp.py                       Python API with the reference's names (gp.geometry, ...)
+  gp5_amd/driver.py                   YAML-configured runs (the reference's gp executable)
+  gp5_amd/ground_state.py             batched ground states with coarse-grid starts
+  gp5_amd/nn.py, gp5_amd/surrogate.py learned warm start (bf16 MFMA U-Net)
 """
 from . import capi  # noqa: F401
 

@@ -1,7 +1,6 @@
 """The large workload: two LHY droplets colliding in a 512^3 box (L = 64, real-time
 RK4, dt = 2.5e-4), on the HIP path, against the oracle after one step (relative L2
 <= 1e-12), with norm and energy drift over a short run."""
-import numpy as np
 import pytest
 
 from conftest import rel_l2

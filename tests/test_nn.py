@@ -3,7 +3,6 @@ oracle/nn_ref.py: convolution forward, data and weight gradients on bf16 inputs
 (exactly representable, so only the float accumulation order differs), pooling,
 Adam, and the whole network's loss and gradients (bf16 activations: agreement to
 the bf16 rounding level)."""
-import ctypes
 
 import numpy as np
 import pytest
