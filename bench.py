@@ -216,6 +216,54 @@ def two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, 
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
+
+
+def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
+    """MFMA throughput of the warm-start network (3D U-Net, bf16 MFMA with float
+    accumulation) at D^3 with `batch` samples: forward passes and training steps
+    (forward, backward, Adam) on synthetic inputs.  Flops: the sum over the eight
+    3x3x3 convolutions of 2 * 27 * Cin * Cout * D_layer^3 * batch for a forward pass,
+    three times that for a training step (data and weight gradients); pooling,
+    upsampling and the elementwise kernels count 0.  Wall time between device
+    synchronisations, so launch costs are included."""
+    from gp5_amd import nn
+
+    div = {"c1": 1, "c2": 2, "c3": 4, "c4": 8, "c5": 4, "c6": 2, "c7": 1, "c8": 1}
+    fwd_flops = sum(2.0 * 27 * ci * co * (D // div[name]) ** 3 * batch for name, ci, co in nn.SHAPES)
+    net = nn.UNet(batch, D, seed=0)
+    rng = np.random.default_rng(0)
+    X = (0.5 * rng.standard_normal((batch, D, D, D, nn.C0))).astype(np.float32)
+    base = np.ascontiguousarray(X[..., 0])
+    target = (base + 0.01 * rng.standard_normal(base.shape)).astype(np.float32)
+    net.set_batch(X, base, target)
+    net.forward()
+    net.backward()
+    net.step(1e-4)
+    net.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward()
+    net.sync()
+    fwd_s = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward()
+        net.backward()
+        net.step(1e-4)
+    net.sync()
+    train_s = (time.perf_counter() - t0) / steps
+    fwd_tf = fwd_flops / fwd_s / 1e12
+    train_tf = 3 * fwd_flops / train_s / 1e12
+    return {
+        "workload": f"3D U-Net 16-32-64-128 channels, {D}^3, batch {batch}, bf16 MFMA, float accumulation",
+        "forward_flops": fwd_flops, "forward_ms": 1e3 * fwd_s, "forward_TFLOPs": fwd_tf,
+        "train_step_flops": 3 * fwd_flops, "train_step_ms": 1e3 * train_s, "train_TFLOPs": train_tf,
+        "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS},
+    }
+
+
 def collision_large(steps: int, n: int = 512):
     """Two N = 4000 LHY droplets colliding head-on in a 512^3 periodic box L = 64
     (dx = 0.125), centres x = -8 and +8, momenta +-0.49 (0.5 snapped to 2 pi m / L);
@@ -267,6 +315,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gs-count", type=int, default=64,
                     help="instances of the batched ground-state workload (0 skips it)")
+    ap.add_argument("--nn-batch", type=int, default=8,
+                    help="batch of the warm-start network's MFMA measurement (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=400,
                     help="RK4 steps of the 512^3 two-droplet collision (0 skips it)")
     ap.add_argument("--calibrate", action="store_true",
@@ -364,6 +414,9 @@ def main() -> int:
         out["ground_state"] = ground_state_batch(args.gs_count)
     if args.large_steps > 0:
         out["large"] = collision_large(args.large_steps)
+    if args.nn_batch > 0:
+        _progress("warm-start network MFMA throughput")
+        out["network"] = network_mfma(args.nn_batch)
     if not args.no_cpu:
         _progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(psi0, shape, left, right)

@@ -50,118 +50,332 @@ __device__ __forceinline__ long long neighbour(long long p, int D, int tap)
 // channel offsets are multiples of 4)
 __device__ __forceinline__ bf16x4 ld4(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
 
-// Forward / data-gradient convolution.  A workgroup of 4 waves covers 64
-// consecutive positions (16 per wave) and NB 16-wide blocks of output
-// channels; the K loop runs over 27 taps x Cin/16 channel chunks.
-template <int NB>
+// Forward / data-gradient convolution.  A workgroup of 4 waves covers 64 * MT
+// consecutive positions: each wave holds MT tiles of 16 positions that share
+// every weight fragment it loads, and NB 16-wide blocks of output channels.  The
+// K loop runs over 27 taps x Cin/16 channel chunks; positions are decoded once
+// and the taps only add offsets.  Each accumulator sees its MFMAs in the same
+// (tap, channel chunk) order whatever MT is.
+template <int NB, int MT>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                               const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                               uint16_t *Y, float *Yf, long long P, int D, int Cin, int Cout,
-                                              int nb0, int mirror, int act)
+                                              int mirror, int act)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const long long p = (long long)blockIdx.x * 64 + wave * 16 + r;  // A row of this lane
-    f32x4 acc[NB];
+    const int nb0 = blockIdx.y * NB;  // first 16-wide output block of this group
+    const long long base = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
+    long long pp[MT];
+    int px[MT], py[MT], pz[MT];
+    bool pin[MT];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < MT; ++t) {
+        pp[t] = base + 16 * t + r;  // A row of this lane in tile t
+        pin[t] = pp[t] < P;
+        px[t] = (int)(pp[t] % D);
+        py[t] = (int)((pp[t] / D) % D);
+        pz[t] = (int)((pp[t] / ((long long)D * D)) % D);
+    }
+    f32x4 acc[MT][NB];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
     for (int tap = 0; tap < 27; ++tap) {
-        const long long q = p < P ? neighbour(p, D, tap) : -1;
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const long long off = ((long long)dz * D + dy) * D + dx;
+        long long q[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const int xx = px[t] + dx, yy = py[t] + dy, zz = pz[t] + dz;
+            const bool ok = pin[t] && xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D;
+            q[t] = ok ? pp[t] + off : -1;
+        }
         const int wt = mirror ? 26 - tap : tap;
         const uint16_t *wrow = Wk + ((size_t)wt * Cout + (size_t)nb0 * 16 + r) * Cin + 4 * g;
         for (int c0 = 0; c0 < Cin; c0 += 16) {
-            const bf16x4 a = q >= 0 ? ld4(X + (size_t)q * Cin + c0 + 4 * g) : zero4;
+            bf16x4 a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) a[t] = q[t] >= 0 ? ld4(X + (size_t)q[t] * Cin + c0 + 4 * g) : zero4;
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 const bf16x4 bv = ld4(wrow + (size_t)j * 16 * Cin + c0);
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[j], 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+                    acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bv, acc[t][j], 0, 0, 0);
             }
         }
     }
     // epilogue: lane holds D[4g + i][r] of each block: position row 4g + i, channel 16 (nb0 + j) + r
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const long long po = (long long)blockIdx.x * 64 + wave * 16 + 4 * g + i;
-        if (po >= P) continue;
+    for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const int co = 16 * (nb0 + j) + r;
-            float v = acc[j][i] + (bias ? bias[co] : 0.f);
-            if (act == 1) v = v > 0.f ? v : kLeak * v;
-            const size_t o = (size_t)po * Cout + co;
-            if (Yf) Yf[o] = v;
-            const uint16_t h = f2bf(v);
-            if (A) A[o] = h;
-            if (Y) Y[o] = skip ? f2bf(v + bf2f(skip[o])) : h;
+        for (int i = 0; i < 4; ++i) {
+            const long long po = base + 16 * t + 4 * g + i;
+            if (po >= P) continue;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int co = 16 * (nb0 + j) + r;
+                float v = acc[t][j][i] + (bias ? bias[co] : 0.f);
+                if (act == 1) v = v > 0.f ? v : kLeak * v;
+                const size_t o = (size_t)po * Cout + co;
+                if (Yf) Yf[o] = v;
+                const uint16_t h = f2bf(v);
+                if (A) A[o] = h;
+                if (Y) Y[o] = skip ? f2bf(v + bf2f(skip[o])) : h;
+            }
         }
+}
+
+// Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
+// positions) accumulates all 27 taps at once, so each X and dY element of the
+// chunk is fetched by one wave instead of 27; partial sums go to
+// ws[chunk][tap][ci][co] and are summed over chunks in a fixed order.  X and dY
+// are first copied channel-major (XT[c][p], dYT[c][p]), so a lane's four
+// consecutive positions of one channel are one aligned 8-byte load; the x-shifted
+// taps come from the previous / current / next 4-position chunks of the same row.
+__host__ __device__ inline long long wgrad_chunk(long long P, int Cin, int Cout)
+{
+    // about 2048 waves over the layer, at least 256 positions each, a multiple of 16
+    const long long tiles = (long long)(Cin / 16) * (Cout / 16);
+    long long c = (P * tiles + 2047) / 2048;
+    c = c < 256 ? 256 : c;
+    return (c + 15) / 16 * 16;
+}
+
+// out[c * P + p] = in[p * C + c] (bf16), 64 positions per block through LDS, 128
+// channels at a time
+__global__ __launch_bounds__(256) void k_transpose_pc(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                                                      long long P, int C)
+{
+    __shared__ uint16_t t[64][129];
+    const long long p0 = (long long)blockIdx.x * 64;
+    for (int c0 = 0; c0 < C; c0 += 128) {
+        const int cc = min(128, C - c0);
+        for (int e = threadIdx.x; e < 64 * cc; e += 256) {
+            const int pl = e / cc, c = e % cc;
+            t[pl][c] = p0 + pl < P ? in[(size_t)(p0 + pl) * C + c0 + c] : (uint16_t)0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * cc; e += 256) {
+            const int c = e / 64, pl = e % 64;
+            if (p0 + pl < P) out[(size_t)(c0 + c) * P + p0 + pl] = t[pl][c];
+        }
+        __syncthreads();
     }
 }
 
-// Weight gradient: one wave per (tap, 16 input channels, 16 output channels,
-// position chunk); partial sums go to ws[chunk][tap][ci][co].
-constexpr int kWChunk = 4096;  // positions per wave
-__global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
-                                              float *ws, long long P, int D, int Cin, int Cout)
+// The same for cube sides that are not a multiple of 4 (small test cubes):
+// scalar gathers from the position-major X and dY.
+__global__ __launch_bounds__(64) void k_wgrad_gather(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                     float *ws, long long P, int D, int Cin, int Cout, long long chunk)
 {
     const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
-    const int nci = Cin / 16, nco = Cout / 16;
-    int blk = blockIdx.x;
-    const int cob = blk % nco;
-    blk /= nco;
-    const int cib = blk % nci;
-    const int tap = blk / nci;
-    const long long chunk = blockIdx.y;
-    const long long p0 = chunk * kWChunk, p1 = min(P, p0 + kWChunk);
+    const int nco = Cout / 16;
+    const int cob = blockIdx.x % nco, cib = blockIdx.x / nco;
+    const long long ck = blockIdx.y;
+    const long long p0 = ck * chunk, p1 = min(P, p0 + chunk);
     const int ci = cib * 16 + r, co = cob * 16 + r;
-    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (long long pb = p0; pb < p1; pb += 16) {
-        // A[m = ci][k = position]: this lane's row ci, positions pb + 4g .. 4g+3
-        // B[k = position][n = co]: this lane's column co, same positions
-        bf16x4 a, bv;
+        long long pq[4];
+        int px[4], py[4], pz[4];
+        bool in[4];
+        bf16x4 bv;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const long long p = pb + 4 * g + i;
-            short xa = 0, yb = 0;
-            if (p < p1) {
-                const long long q = neighbour(p, D, tap);
-                if (q >= 0) xa = (short)X[(size_t)q * Cin + ci];
-                yb = (short)dY[(size_t)p * Cout + co];
-            }
-            a[i] = xa;
-            bv[i] = yb;
+            in[i] = p < p1;
+            pq[i] = p;
+            px[i] = (int)(p % D);
+            py[i] = (int)((p / D) % D);
+            pz[i] = (int)((p / ((long long)D * D)) % D);
+            bv[i] = in[i] ? (short)dY[(size_t)p * Cout + co] : (short)0;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
-    }
-    // D[m = ci row 4g+i][n = co column r]
-    float *o = ws + (((size_t)chunk * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[i];
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const long long off = ((long long)dz * D + dy) * D + dx;
+            bf16x4 a;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int xx = px[i] + dx, yy = py[i] + dy, zz = pz[i] + dz;
+                const bool ok = in[i] && xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D;
+                a[i] = ok ? (short)X[(size_t)(pq[i] + off) * Cin + ci] : (short)0;
+            }
+            acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[tap], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+        float *o = ws + (((size_t)ck * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[tap][i];
+    }
 }
 
-__global__ void k_wgrad_reduce(const float *ws, float *dW, long long n, int chunks)
+__global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ XT, const uint16_t *__restrict__ dYT,
+                                              float *ws, long long P, int D, int Cin, int Cout, long long chunk)
+{
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int nco = Cout / 16;
+    const int cob = blockIdx.x % nco, cib = blockIdx.x / nco;
+    const long long ck = blockIdx.y;
+    const long long p0 = ck * chunk, p1 = min(P, p0 + chunk);
+    const int ci = cib * 16 + r, co = cob * 16 + r;
+    const uint16_t *xrow = XT + (size_t)ci * P;
+    const uint16_t *yrow = dYT + (size_t)co * P;
+    const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long long pb = p0; pb < p1; pb += 16) {
+        // A[m = ci][k = position]: positions pg .. pg+3 (one x-row, x a multiple of 4);
+        // B[k = position][n = co]: the same positions of channel co
+        const long long pg = pb + 4 * g;
+        if (pg >= p1) continue;  // p1 is a multiple of 16: groups are wholly in or out
+        const int x = (int)(pg % D), y = (int)((pg / D) % D), z = (int)((pg / ((long long)D * D)) % D);
+        const bf16x4 bv = ld4(yrow + pg);
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int t0 = (dz + 1) * 9 + (dy + 1) * 3;  // tap of dx = -1
+                const bool row_ok = z + dz >= 0 && z + dz < D && y + dy >= 0 && y + dy < D;
+                bf16x4 cur = zero4, prv = zero4, nxt = zero4;
+                if (row_ok) {
+                    const long long q = pg + ((long long)dz * D + dy) * D;
+                    cur = ld4(xrow + q);
+                    if (x > 0) prv = ld4(xrow + q - 4);
+                    if (x + 4 < D) nxt = ld4(xrow + q + 4);
+                }
+                const bf16x4 am = bf16x4{prv[3], cur[0], cur[1], cur[2]};
+                const bf16x4 ap = bf16x4{cur[1], cur[2], cur[3], nxt[0]};
+                acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bv, acc[t0], 0, 0, 0);
+                acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cur, bv, acc[t0 + 1], 0, 0, 0);
+                acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ap, bv, acc[t0 + 2], 0, 0, 0);
+            }
+    }
+    // D[m = ci row 4g+i][n = co column r]
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+        float *o = ws + (((size_t)ck * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[tap][i];
+    }
+}
+
+// Sum of the chunk partials in two fixed-order levels: kRedGroups groups of
+// consecutive chunks per output first (one thread each), then the group sums.
+constexpr int kRedGroups = 32;
+__global__ void k_wgrad_reduce_groups(const float *ws, float *part, long long n, int chunks)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.y;
+    if (i >= n) return;
+    const int per = (chunks + kRedGroups - 1) / kRedGroups;
+    const int c0 = grp * per, c1 = min(chunks, c0 + per);
+    float s = 0.f;
+    for (int c = c0; c < c1; ++c) s += ws[(size_t)c * n + i];
+    part[(size_t)grp * n + i] = s;
+}
+
+__global__ void k_wgrad_reduce(const float *part, float *dW, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         float s = 0.f;
-        for (int c = 0; c < chunks; ++c) s += ws[(size_t)c * n + i];
+        for (int g = 0; g < kRedGroups; ++g) s += part[(size_t)g * n + i];
         dW[i] = s;
     }
 }
 
-// db[c] = sum_p dY[p][c]: one block per channel, fixed-order reduction
-__global__ __launch_bounds__(256) void k_bias_grad(const float *dY, float *db, long long P, int C)
+// db[c] = sum_p dY[p][c] in two fixed-order passes: kBiasBlocks blocks each sum a
+// contiguous range of positions for every channel (consecutive threads read
+// consecutive channels), then one thread per channel adds the block sums in order.
+constexpr int kBiasBlocks = 512;
+__global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ dY, float *part, long long P, int C)
+{
+    __shared__ float s[256];
+    const int per_pos = 256 / C;  // C divides 256 (a multiple of 16, at most 256)
+    const int c = threadIdx.x % C, sub = threadIdx.x / C;
+    const long long per = (P + gridDim.x - 1) / gridDim.x;
+    const long long p0 = (long long)blockIdx.x * per, p1 = min(P, p0 + per);
+    float a = 0.f;
+    if (sub < per_pos)
+        for (long long p = p0 + sub; p < p1; p += per_pos) a += dY[(size_t)p * C + c];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.f;
+        for (int k = 0; k < per_pos; ++k) t += s[threadIdx.x + k * C];
+        part[(size_t)blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+// one block per channel: each thread sums a strided subset of the block partials,
+// then a fixed tree over the threads
+__global__ __launch_bounds__(256) void k_bias_final(const float *part, float *db, int nblk, int C)
 {
     __shared__ float s[256];
     const int c = blockIdx.x;
-    float a = 0.f;
-    for (long long p = threadIdx.x; p < P; p += 256) a += dY[(size_t)p * C + c];
-    s[threadIdx.x] = a;
+    float t = 0.f;
+    for (int b = threadIdx.x; b < nblk; b += 256) t += part[(size_t)b * C + c];
+    s[threadIdx.x] = t;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) db[c] = s[0];
+}
+
+// bf16 pooling and upsampling, 4 channels (8 bytes) per thread; same arithmetic
+// as the scalar templates below (the 8 children summed in order, then scaled)
+__global__ void k_pool2_bf16x4(const uint16_t *__restrict__ X, uint16_t *__restrict__ Y, int B, int D, int C,
+                               float scale)
+{
+    const int Dh = D / 2, C4 = C / 4;
+    const long long n = (long long)B * Dh * Dh * Dh * C4;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c4 = (int)(i % C4);
+        const long long p = i / C4;
+        const int x = (int)(p % Dh), y = (int)((p / Dh) % Dh), z = (int)((p / ((long long)Dh * Dh)) % Dh);
+        const long long b = p / ((long long)Dh * Dh * Dh);
+        float s[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < 8; ++k) {
+            const long long q = ((b * D + 2 * z + (k >> 2)) * D + 2 * y + ((k >> 1) & 1)) * D + 2 * x + (k & 1);
+            const bf16x4 v = ld4(X + (size_t)q * C + 4 * c4);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
+        }
+        bf16x4 o;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[e] = (short)f2bf(s[e] * (0.125f * scale));
+        *reinterpret_cast<bf16x4 *>(Y + (size_t)p * C + 4 * c4) = o;
+    }
+}
+
+__global__ void k_up2_bf16x4(const uint16_t *__restrict__ X, uint16_t *__restrict__ Y, int B, int D, int C, float scale)
+{
+    const int D2 = 2 * D, C4 = C / 4;
+    const long long n = (long long)B * D2 * D2 * D2 * C4;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c4 = (int)(i % C4);
+        const long long p = i / C4;
+        const int x = (int)(p % D2), y = (int)((p / D2) % D2), z = (int)((p / ((long long)D2 * D2)) % D2);
+        const long long b = p / ((long long)D2 * D2 * D2);
+        const long long q = ((b * D + z / 2) * D + y / 2) * D + x / 2;
+        bf16x4 v = ld4(X + (size_t)q * C + 4 * c4);
+        if (scale != 1.0f) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = (short)f2bf(scale * bf2f((uint16_t)v[e]));
+        }
+        *reinterpret_cast<bf16x4 *>(Y + (size_t)p * C + 4 * c4) = v;
+    }
 }
 
 template <typename T> __device__ __forceinline__ float ldv(const T *p);
@@ -423,21 +637,38 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
     const long long P = (long long)B * D * D * D;
-    const dim3 grid((unsigned)((P + 63) / 64)), block(256);
-    // output channels in groups of up to 8 16-wide blocks per launch
-    for (int nb0 = 0; nb0 < Cout / 16; nb0 += 8) {
-        const int nb = std::min(8, Cout / 16 - nb0);
+    // Output channels in groups of NB 16-wide blocks (a divisor of Cout / 16, at
+    // most 8; blockIdx.y is the group) and MT position tiles per wave: the widest
+    // group and the most tiles that still give about 2048 waves, so that small
+    // cubes fill the machine.
+    const int nblk = Cout / 16;
+    int NB = 1, MT = 1;
+    bool found = false;
+    for (int nb = 8; nb >= 1 && !found; --nb) {
+        if (nblk % nb) continue;
+        const int mts[2] = {nb == 1 ? 4 : (nb <= 4 ? 2 : 1), 1};
+        for (int k = 0; k < 2 && !found; ++k) {
+            const long long waves = (P + 16LL * mts[k] - 1) / (16LL * mts[k]) * (nblk / nb);
+            if (waves >= 2048) {
+                NB = nb;
+                MT = mts[k];
+                found = true;
+            }
+        }
+    }
+    {
         const uint16_t *x = (const uint16_t *)X, *w = (const uint16_t *)Wk, *sk = (const uint16_t *)skip;
         uint16_t *a = (uint16_t *)A, *y = (uint16_t *)Y;
-        switch (nb) {
-#define GP5_CONV_CASE(K)                                                                                   \
-    case K:                                                                                                \
-        hipLaunchKernelGGL(k_conv<K>, grid, block, 0, 0, x, w, bias, a, sk, y, Yf, P, D, Cin, Cout, nb0, mirror, act); \
-        break;
-            GP5_CONV_CASE(1) GP5_CONV_CASE(2) GP5_CONV_CASE(3) GP5_CONV_CASE(4)
-            GP5_CONV_CASE(5) GP5_CONV_CASE(6) GP5_CONV_CASE(7) GP5_CONV_CASE(8)
+        const dim3 block(256);
+        const unsigned gy = (unsigned)(nblk / NB);
+#define GP5_CONV_CASE(K, T)                                                                                  \
+    if (NB == K && MT == T)                                                                                  \
+        hipLaunchKernelGGL((k_conv<K, T>), dim3((unsigned)((P + 64 * T - 1) / (64 * T)), gy), block, 0, 0, x, w, bias, a, \
+                           sk, y, Yf, P, D, Cin, Cout, mirror, act);
+        GP5_CONV_CASE(1, 4) GP5_CONV_CASE(1, 1) GP5_CONV_CASE(2, 2) GP5_CONV_CASE(2, 1) GP5_CONV_CASE(3, 2)
+        GP5_CONV_CASE(3, 1) GP5_CONV_CASE(4, 2) GP5_CONV_CASE(4, 1) GP5_CONV_CASE(5, 1) GP5_CONV_CASE(6, 1)
+        GP5_CONV_CASE(7, 1) GP5_CONV_CASE(8, 1)
 #undef GP5_CONV_CASE
-        }
         NNCHK(hipGetLastError(), "conv3d launch");
     }
     return GP5_OK;
@@ -446,8 +677,10 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
 {
     const long long P = (long long)B * D * D * D;
-    const long long chunks = (P + kWChunk - 1) / kWChunk;
-    return (size_t)chunks * 27 * Cin * Cout * sizeof(float);
+    const long long chunk = wgrad_chunk(P, Cin, Cout);
+    const long long chunks = (P + chunk - 1) / chunk;
+    // channel-major copies of X and dY, then the partial sums
+    return (size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * 27 * Cin * Cout * sizeof(float);
 }
 
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
@@ -457,29 +690,57 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
     const long long P = (long long)B * D * D * D;
-    const long long chunks = (P + kWChunk - 1) / kWChunk;
-    const dim3 grid((unsigned)(27 * (Cin / 16) * (Cout / 16)), (unsigned)chunks);
-    hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, P, D, Cin, Cout);
+    const long long chunk = wgrad_chunk(P, Cin, Cout);
+    const long long chunks = (P + chunk - 1) / chunk;
+    uint16_t *XT = reinterpret_cast<uint16_t *>(ws);
+    uint16_t *dYT = XT + (size_t)P * Cin;
+    float *part0 = reinterpret_cast<float *>(dYT + (size_t)P * Cout);
+    const dim3 grid((unsigned)((Cin / 16) * (Cout / 16)), (unsigned)chunks);
+    if (D % 4 == 0) {
+        const unsigned tb = (unsigned)((P + 63) / 64);
+        hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
+        NNCHK(hipGetLastError(), "wgrad transpose");
+        hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
+        NNCHK(hipGetLastError(), "wgrad transpose");
+        hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+    } else {
+        hipLaunchKernelGGL(k_wgrad_gather, grid, dim3(64), 0, 0, (const uint16_t *)X, (const uint16_t *)dY, part0, P, D,
+                           Cin, Cout, chunk);
+    }
     NNCHK(hipGetLastError(), "wgrad launch");
     const long long n = 27LL * Cin * Cout;
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, ws, dW, n, (int)chunks);
+    float *part = part0 + (size_t)chunks * n;  // after the chunk partials
+    hipLaunchKernelGGL(k_wgrad_reduce_groups, dim3((unsigned)((n + 255) / 256), kRedGroups), dim3(256), 0, 0, part0,
+                       part, n, (int)chunks);
+    NNCHK(hipGetLastError(), "wgrad reduce launch");
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, part, dW, n);
     NNCHK(hipGetLastError(), "wgrad reduce launch");
     return GP5_OK;
 }
 
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)
 {
-    if (!dY || !db || C < 1) return set_error(GP5_ERR_ARG, "bad argument");
-    hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, 0, dY, db, P, C);
-    NNCHK(hipGetLastError(), "bias grad launch");
+    if (!dY || !db || C < 1 || C > 256 || 256 % C) return set_error(GP5_ERR_ARG, "bias grad: C must divide 256");
+    float *part = nullptr;
+    NNCHK(hipMalloc(&part, (size_t)kBiasBlocks * C * sizeof(float)), "bias grad scratch");
+    hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, 0, dY, part, P, C);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
+        e = hipGetLastError();
+    }
+    hipError_t ef = hipFree(part);
+    NNCHK(e, "bias grad launch");
+    NNCHK(ef, "bias grad scratch free");
     return GP5_OK;
 }
 
 int gp5_nn_pool2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
 {
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
-    const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * C;
-    hipLaunchKernelGGL(k_pool2<uint16_t>, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
+    if (C % 4) return set_error(GP5_ERR_ARG, "pool2: C must be a multiple of 4");
+    const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * (C / 4);
+    hipLaunchKernelGGL(k_pool2_bf16x4, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
                        C, scale);
     NNCHK(hipGetLastError(), "pool2");
     return GP5_OK;
@@ -487,8 +748,9 @@ int gp5_nn_pool2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
 
 int gp5_nn_up2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
 {
-    const long long n = (long long)B * 8 * D * D * D * C;
-    hipLaunchKernelGGL(k_up2<uint16_t>, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
+    if (C % 4) return set_error(GP5_ERR_ARG, "up2: C must be a multiple of 4");
+    const long long n = (long long)B * 8 * D * D * D * (C / 4);
+    hipLaunchKernelGGL(k_up2_bf16x4, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
                        C, scale);
     NNCHK(hipGetLastError(), "up2");
     return GP5_OK;
