@@ -13,6 +13,7 @@
 
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <vector>
@@ -50,70 +51,62 @@ __device__ __forceinline__ long long neighbour(long long p, int D, int tap)
 // channel offsets are multiples of 4)
 __device__ __forceinline__ bf16x4 ld4(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
 
-// Forward / data-gradient convolution.  A workgroup of 4 waves covers 64 * MT
-// consecutive positions: each wave holds MT tiles of 16 positions that share
-// every weight fragment it loads, and NB 16-wide blocks of output channels.  The
-// K loop runs over 27 taps x Cin/16 channel chunks; positions are decoded once
-// and the taps only add offsets.  Each accumulator sees its MFMAs in the same
-// (tap, channel chunk) order whatever MT is.
-template <int NB, int MT>
-__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
-                                              const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
-                                              uint16_t *Y, float *Yf, long long P, int D, int Cin, int Cout,
-                                              int mirror, int act)
+// Raw buffer resource over [base, base + bytes).  A load whose lane offset is
+// at or beyond `bytes` returns zero (the buffer range check), which is how the
+// convolution reads its zero padding without branches; kOffNone is such an
+// offset for every resource here (all are below 2 GiB).
+constexpr int kOffNone = (int)0x80000000u;
+
+#ifndef GP5_CONV_PD
+#define GP5_CONV_PD 2
+#endif
+constexpr int kConvPD = GP5_CONV_PD;  // taps of loads in flight ahead of the MFMAs
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int bytes)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, bytes, 0x00020000);
+}
+
+// LV consecutive bf16 (LV = 4, 8 or 16) at byte offset voff + soff, as LV/4
+// MFMA fragments; voff + soff is a multiple of 2 LV bytes
+template <int LV>
+struct Frag {
+    bf16x4 v[LV / 4];
+};
+
+template <int LV>
+__device__ __forceinline__ Frag<LV> bld(__amdgpu_buffer_rsrc_t rs, int voff, int soff)
+{
+    Frag<LV> f;
+    if constexpr (LV == 4) {
+        f.v[0] = __builtin_bit_cast(bf16x4, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+    } else {
+#pragma unroll
+        for (int h = 0; h < LV / 8; ++h) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff + 16 * h, 0);
+            f.v[2 * h] = __builtin_bit_cast(bf16x4, __builtin_shufflevector(q, q, 0, 1));
+            f.v[2 * h + 1] = __builtin_bit_cast(bf16x4, __builtin_shufflevector(q, q, 2, 3));
+        }
+    }
+    return f;
+}
+
+// Convolution epilogue: lane holds D[4g + i][r] of each block, i.e. position
+// base + 16 t + 4g + i and output channel 16 (nb0 + j) + r: bias, leaky ReLU
+// (act = 1), then the float output Yf, the bf16 output A and the bf16 output Y
+// (plus the skip input when given), each only when its pointer is set.
+template <int NB, int MT>
+__device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
+                                              const float *__restrict__ bias, int act, uint16_t *A,
+                                              const uint16_t *skip, uint16_t *Y, float *Yf)
+{
+    const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
-    const int nb0 = blockIdx.y * NB;  // first 16-wide output block of this group
-    const long long base = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
-    long long pp[MT];
-    int px[MT], py[MT], pz[MT];
-    bool pin[MT];
-#pragma unroll
-    for (int t = 0; t < MT; ++t) {
-        pp[t] = base + 16 * t + r;  // A row of this lane in tile t
-        pin[t] = pp[t] < P;
-        px[t] = (int)(pp[t] % D);
-        py[t] = (int)((pp[t] / D) % D);
-        pz[t] = (int)((pp[t] / ((long long)D * D)) % D);
-    }
-    f32x4 acc[MT][NB];
-#pragma unroll
-    for (int t = 0; t < MT; ++t)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
-    for (int tap = 0; tap < 27; ++tap) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const long long off = ((long long)dz * D + dy) * D + dx;
-        long long q[MT];
-#pragma unroll
-        for (int t = 0; t < MT; ++t) {
-            const int xx = px[t] + dx, yy = py[t] + dy, zz = pz[t] + dz;
-            const bool ok = pin[t] && xx >= 0 && xx < D && yy >= 0 && yy < D && zz >= 0 && zz < D;
-            q[t] = ok ? pp[t] + off : -1;
-        }
-        const int wt = mirror ? 26 - tap : tap;
-        const uint16_t *wrow = Wk + ((size_t)wt * Cout + (size_t)nb0 * 16 + r) * Cin + 4 * g;
-        for (int c0 = 0; c0 < Cin; c0 += 16) {
-            bf16x4 a[MT];
-#pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = q[t] >= 0 ? ld4(X + (size_t)q[t] * Cin + c0 + 4 * g) : zero4;
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const bf16x4 bv = ld4(wrow + (size_t)j * 16 * Cin + c0);
-#pragma unroll
-                for (int t = 0; t < MT; ++t)
-                    acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bv, acc[t][j], 0, 0, 0);
-            }
-        }
-    }
-    // epilogue: lane holds D[4g + i][r] of each block: position row 4g + i, channel 16 (nb0 + j) + r
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long long po = base + 16 * t + 4 * g + i;
+            const int po = base + 16 * t + 4 * g + i;
             if (po >= P) continue;
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
@@ -129,6 +122,207 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, co
         }
 }
 
+// Forward / data-gradient convolution.  A workgroup of 4 waves covers 64 * MT
+// consecutive positions: each wave holds MT tiles of 16 positions that share
+// every weight fragment it loads, and NB 16-wide blocks of output channels.
+//
+// The reduction over K = 27 taps x Cin channels runs in chunks of KC channels
+// (KC = 16, 32 or 64 divides Cin).  Within a chunk, lane (r, g) loads the KC/4
+// consecutive channels [g KC/4, (g+1) KC/4) of its position (A) and of its output
+// channel (B) in one contiguous piece, and MFMA step s takes channels
+// g KC/4 + 4s .. +3 from both: a permutation of K that A and B share, so that
+// the 4 lanes of a position read its KC channels as one run and a load
+// instruction covers whole rows of 16 positions.
+//
+// Positions are decoded once; a tap is a scalar byte offset on a resource that
+// starts (D^2 + D + 1) Cin elements before X (the most negative tap), so every
+// tap offset is non-negative, and a neighbour outside the cube gets the lane
+// offset kOffNone (reads zero).  Offsets are 32-bit: the host keeps every tensor
+// below 2 GiB.  The loads run PD taps ahead of the MFMAs.  Each accumulator sees
+// its MFMAs in the same (chunk, tap, step) order whatever NB, MT and PD are.
+template <int KC, int NB, int MT, int PD>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
+                                              const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
+                                              uint16_t *Y, float *Yf, int P, int D, int Cin, int Cout, int mirror,
+                                              int act)
+{
+    constexpr int LV = KC / 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nb0 = blockIdx.y * NB;  // first 16-wide output block of this group
+    const int base = ((int)blockIdx.x * 4 + wave) * (16 * MT);
+    const int halo = (D * D + D + 1) * Cin * 2;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc((const char *)X - halo, halo + P * Cin * 2);
+    const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * Cin * 2);
+    int vo[MT];
+    bool xm[MT], xp[MT], ym[MT], yp[MT], zm[MT], zp[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int p = base + 16 * t + r;  // A row of this lane in tile t
+        const unsigned q1 = (unsigned)p / (unsigned)D, q2 = q1 / (unsigned)D;
+        const int x = p - (int)q1 * D, y = (int)(q1 - q2 * D), z = (int)(q2 % (unsigned)D);
+        vo[t] = p < P ? (p * Cin + g * LV) * 2 : kOffNone;
+        xm[t] = x > 0;
+        xp[t] = x < D - 1;
+        ym[t] = y > 0;
+        yp[t] = y < D - 1;
+        zm[t] = z > 0;
+        zp[t] = z < D - 1;
+    }
+    const int wvo = ((nb0 * 16 + r) * Cin + g * LV) * 2;
+    f32x4 acc[MT][NB];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // fragments of tap `tap` into ring slot `slot`
+    Frag<LV> a[PD + 1][MT], b[PD + 1][NB];
+    auto fetch = [&](int c0, int tap, int slot) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int soff = halo + ((dz * D + dy) * D + dx) * Cin * 2 + 2 * c0;
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const bool ok = (dx < 0 ? xm[t] : dx > 0 ? xp[t] : true) && (dy < 0 ? ym[t] : dy > 0 ? yp[t] : true) &&
+                            (dz < 0 ? zm[t] : dz > 0 ? zp[t] : true);
+            a[slot][t] = bld<LV>(xr, ok ? vo[t] : kOffNone, soff);
+        }
+        const int wso = ((mirror ? 26 - tap : tap) * Cout * Cin + c0) * 2;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) b[slot][j] = bld<LV>(wr, wvo, wso + j * 16 * Cin * 2);
+    };
+    for (int c0 = 0; c0 < Cin; c0 += KC) {
+        // keeps the 27 MT per-tap offsets from being hoisted out of this loop
+        // into registers (they cost one select each where they are used)
+#pragma unroll
+        for (int t = 0; t < MT; ++t) asm volatile("" : "+v"(vo[t]));
+        // software pipeline: the loads of tap + PD are in flight during the MFMAs of tap
+#pragma unroll
+        for (int tap = 0; tap < PD; ++tap) fetch(c0, tap, tap);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            if (tap + PD < 27) fetch(c0, tap + PD, (tap + PD) % (PD + 1));
+            const int sl = tap % (PD + 1);
+#pragma unroll
+            for (int st = 0; st < LV / 4; ++st)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[sl][t].v[st], b[sl][j].v[st], acc[t][j],
+                                                                              0, 0, 0);
+        }
+    }
+    conv_epilogue<NB, MT>(acc, base, nb0, P, Cout, bias, act, A, skip, Y, Yf);
+}
+
+// n / d for 0 <= n < 2^22 and 1 <= d < 2^11, with inv = 1 / d: the float
+// quotient is within one of the true one, and the two corrections make it exact
+__device__ __forceinline__ int udiv_small(int n, int d, float inv)
+{
+    int q = (int)((float)n * inv);
+    q -= q * d > n;
+    q += (q + 1) * d <= n;
+    return q;
+}
+
+constexpr int kStageBatch = 8;  // staging loads in flight per thread
+
+// LDS-staged convolution for cubes of side D = 8 .. 128, a power of two.  A
+// workgroup of 4 waves owns 256 consecutive positions: Rb = min(D, 256 / D)
+// whole rows of PZ = 256 / (D Rb) planes of one sample.  Per chunk of KC input
+// channels it stages the input region those positions read, (PZ + 2) planes x
+// (Rb + 2) rows x (D + 2) columns x KC channels with zeros outside the cube,
+// into LDS with 16-byte loads (each input element is fetched about 4.5 times per
+// output instead of 27), then every wave runs 4 tiles of 16 positions x NB
+// 16-wide output blocks over the 27 taps, A fragments from LDS and weight
+// fragments from global memory PD taps ahead.  Same K permutation and
+// accumulation order as k_conv (chunk, tap, step), so for the same KC the two
+// kernels give bit-identical results.
+template <int KC, int NB, int PD>
+__global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
+                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
+                                                  uint16_t *Y, float *Yf, int P, int D, int Cin, int Cout, int mirror,
+                                                  int act, int Rb, int PZ)
+{
+    constexpr int MT = 4, LV = KC / 4, U8 = KC / 8;  // U8: 16-byte units per staged position
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nb0 = blockIdx.y * NB;
+    const int RX = D + 2, RY = Rb + 2, RZ = PZ + 2;
+    const int UR = RX * U8, U = RZ * RY * UR;  // 16-byte units per region row, in the region
+    const float invUR = 1.f / (float)UR, invRY = 1.f / (float)RY;
+    const int p0 = (int)blockIdx.x * 256;
+    const int y0 = (p0 / D) % D, z0 = (p0 / (D * D)) % D;
+    const int s0 = p0 - (z0 * D + y0) * D;  // position of (z, y, x) = (0, 0, 0) in this sample
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, P * Cin * 2);
+    const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * Cin * 2);
+    int lo[MT];  // LDS byte offset of this lane's tap-(0,0,0) fragment in tile t
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int q = 64 * wave + 16 * t + r;
+        const int lx = q & (D - 1), ly = (q / D) % Rb, lz = q / (D * Rb);
+        lo[t] = (((lz + 1) * RY + ly + 1) * RX + lx + 1) * KC * 2 + g * LV * 2;
+    }
+    const int wvo = ((nb0 * 16 + r) * Cin + g * LV) * 2;
+    f32x4 acc[MT][NB];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    Frag<LV> b[PD + 1][NB];
+    auto fetch_w = [&](int c0, int tap, int slot) {
+        const int wso = ((mirror ? 26 - tap : tap) * Cout * Cin + c0) * 2;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) b[slot][j] = bld<LV>(wr, wvo, wso + j * 16 * Cin * 2);
+    };
+    for (int c0 = 0; c0 < Cin; c0 += KC) {
+        if (c0) __syncthreads();  // every wave is done with the previous chunk
+        // stage the region, unit u = 16 bytes at LDS byte 16 u, row-major over
+        // (plane, row, column, 8-channel group); kStageBatch loads in flight per
+        // thread before their LDS writes
+        for (int u0 = tid; u0 < U; u0 += 256 * kStageBatch) {
+            uint4 v[kStageBatch];
+#pragma unroll
+            for (int k = 0; k < kStageBatch; ++k) {
+                const int u = u0 + 256 * k;
+                const int row = udiv_small(u, UR, invUR), e = u - row * UR;
+                const int iz = udiv_small(row, RY, invRY), iy = row - iz * RY;
+                const int ix = e / U8, c8 = e - ix * U8;
+                const int y = y0 - 1 + iy, z = z0 - 1 + iz;
+                const bool in = u < U && y >= 0 && y < D && z >= 0 && z < D && ix >= 1 && ix <= D;
+                const int voff = in ? ((s0 + (z * D + y) * D + ix - 1) * Cin + c0 + 8 * c8) * 2 : kOffNone;
+                v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, 0, 0));
+            }
+#pragma unroll
+            for (int k = 0; k < kStageBatch; ++k)
+                if (u0 + 256 * k < U) *reinterpret_cast<uint4 *>(sx + 16 * (u0 + 256 * k)) = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < PD; ++tap) fetch_w(c0, tap, tap);
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            if (tap + PD < 27) fetch_w(c0, tap + PD, (tap + PD) % (PD + 1));
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int toff = ((dz * RY + dy) * RX + dx) * KC * 2;
+            Frag<LV> a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) a[t] = *reinterpret_cast<const Frag<LV> *>(sx + lo[t] + toff);
+            const int sl = tap % (PD + 1);
+#pragma unroll
+            for (int st = 0; st < LV / 4; ++st)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t].v[st], b[sl][j].v[st], acc[t][j],
+                                                                              0, 0, 0);
+        }
+    }
+    conv_epilogue<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf);
+}
+
 // Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
 // positions) accumulates all 27 taps at once, so each X and dY element of the
 // chunk is fetched by one wave instead of 27; partial sums go to
@@ -138,11 +332,11 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, co
 // taps come from the previous / current / next 4-position chunks of the same row.
 __host__ __device__ inline long long wgrad_chunk(long long P, int Cin, int Cout)
 {
-    // about 2048 waves over the layer, at least 256 positions each, a multiple of 16
+    // about 2048 waves over the layer, at least 256 positions each, a multiple of 32
     const long long tiles = (long long)(Cin / 16) * (Cout / 16);
     long long c = (P * tiles + 2047) / 2048;
     c = c < 256 ? 256 : c;
-    return (c + 15) / 16 * 16;
+    return (c + 31) / 32 * 32;
 }
 
 // out[c * P + p] = in[p * C + c] (bf16), 64 positions per block through LDS, 128
@@ -261,6 +455,63 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ XT, c
             }
     }
     // D[m = ci row 4g+i][n = co column r]
+#pragma unroll
+    for (int tap = 0; tap < 27; ++tap) {
+        float *o = ws + (((size_t)ck * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[tap][i];
+    }
+}
+
+// The same for cube sides that are a multiple of 8, 32 positions per step: a
+// lane loads 8 consecutive positions of its channel (16 bytes) and feeds two
+// MFMA k-steps with them, positions 8g .. 8g+3 and 8g+4 .. 8g+7 (the same split
+// for dY), so one aligned load serves 6 MFMAs of a row (three x-shifted taps x
+// two steps) with the 4 positions on either side from two 8-byte loads.
+__global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, const uint16_t *__restrict__ dYT,
+                                               float *ws, long long P, int D, int Cin, int Cout, long long chunk)
+{
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int nco = Cout / 16;
+    const int cob = blockIdx.x % nco, cib = blockIdx.x / nco;
+    const long long ck = blockIdx.y;
+    const long long p0 = ck * chunk, p1 = min(P, p0 + chunk);
+    const int ci = cib * 16 + r, co = cob * 16 + r;
+    const uint16_t *xrow = XT + (size_t)ci * P;
+    const uint16_t *yrow = dYT + (size_t)co * P;
+    const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
+    typedef short bf16x8 __attribute__((ext_vector_type(8)));
+    f32x4 acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (long long pb = p0; pb < p1; pb += 32) {
+        const long long pg = pb + 8 * g;
+        if (pg >= p1) continue;  // p1 is a multiple of 32 or P: groups of 8 are wholly in or out
+        const int x = (int)(pg % D), y = (int)((pg / D) % D), z = (int)((pg / ((long long)D * D)) % D);
+        const bf16x8 b8 = *reinterpret_cast<const bf16x8 *>(yrow + pg);
+        const bf16x4 b1 = bf16x4{b8[0], b8[1], b8[2], b8[3]}, b2 = bf16x4{b8[4], b8[5], b8[6], b8[7]};
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int t0 = (dz + 1) * 9 + (dy + 1) * 3;  // tap of dx = -1
+                const bool row_ok = z + dz >= 0 && z + dz < D && y + dy >= 0 && y + dy < D;
+                bf16x8 c = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                bf16x4 pv = zero4, nx = zero4;
+                if (row_ok) {
+                    const long long q = pg + ((long long)dz * D + dy) * D;
+                    c = *reinterpret_cast<const bf16x8 *>(xrow + q);
+                    if (x > 0) pv = ld4(xrow + q - 4);
+                    if (x + 8 < D) nx = ld4(xrow + q + 8);
+                }
+                acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{pv[3], c[0], c[1], c[2]}, b1, acc[t0], 0, 0, 0);
+                acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[3], c[4], c[5], c[6]}, b2, acc[t0], 0, 0, 0);
+                acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[0], c[1], c[2], c[3]}, b1, acc[t0 + 1], 0, 0, 0);
+                acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[4], c[5], c[6], c[7]}, b2, acc[t0 + 1], 0, 0, 0);
+                acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[1], c[2], c[3], c[4]}, b1, acc[t0 + 2], 0, 0, 0);
+                acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[5], c[6], c[7], nx[0]}, b2, acc[t0 + 2], 0, 0, 0);
+            }
+    }
 #pragma unroll
     for (int tap = 0; tap < 27; ++tap) {
         float *o = ws + (((size_t)ck * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
@@ -637,6 +888,10 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
     const long long P = (long long)B * D * D * D;
+    // 32-bit byte offsets in the kernel (halo included) and element offsets of the outputs
+    const long long halo = ((long long)D * D + D + 1) * Cin * 2;
+    if (halo + P * Cin * 2 > 0x7fffffffLL || P * Cout * 4 > 0x7fffffffLL || 27LL * Cin * Cout * 2 > 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "conv3d: tensors must stay below 2 GiB (B*D^3*C elements)");
     // Output channels in groups of NB 16-wide blocks (a divisor of Cout / 16, at
     // most 8; blockIdx.y is the group) and MT position tiles per wave: the widest
     // group and the most tiles that still give about 2048 waves, so that small
@@ -656,18 +911,52 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
             }
         }
     }
+    const uint16_t *x = (const uint16_t *)X, *w = (const uint16_t *)Wk, *sk = (const uint16_t *)skip;
+    uint16_t *a = (uint16_t *)A, *y = (uint16_t *)Y;
+    const dim3 block(256);
+    // LDS-staged kernel: D a power of two in [8, 128] (workgroups of 256 positions
+    // are whole rows of one sample), channel chunk KC = 32 or 16 with a staged
+    // region of at most 64 KB, output blocks per wave NB = 4, 2 or 1, the widest
+    // that still gives 1024 workgroups.  GP5_CONV_LDS=0 selects the direct kernel.
+    const char *lds_env = getenv("GP5_CONV_LDS");
+    if (D >= 8 && D <= 128 && (D & (D - 1)) == 0 && !(lds_env && lds_env[0] == '0')) {
+        const int Rb = std::min(D, 256 / D), PZ = 256 / (D * Rb);
+        const long long region = (long long)(PZ + 2) * (Rb + 2) * (D + 2);
+        int KCl = 0;
+        for (int kc : {32, 16})
+            if (!KCl && Cin % kc == 0 && region * kc * 2 <= 65536) KCl = kc;
+        if (KCl) {
+            const long long wgs = P / 256;
+            int NBl = 1;
+            for (int nb : {4, 2})
+                if (NBl == 1 && nblk % nb == 0 && wgs * (nblk / nb) >= 1024) NBl = nb;
+            const dim3 grid((unsigned)wgs, (unsigned)(nblk / NBl));
+            const size_t shm = (size_t)region * KCl * 2;
+#define GP5_CONV_LDS_CASE(C, K)                                                                                   \
+    if (KCl == C && NBl == K)                                                                                     \
+        hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cin, \
+                           Cout, mirror, act, Rb, PZ);
+            GP5_CONV_LDS_CASE(16, 1) GP5_CONV_LDS_CASE(16, 2) GP5_CONV_LDS_CASE(16, 4)
+            GP5_CONV_LDS_CASE(32, 1) GP5_CONV_LDS_CASE(32, 2) GP5_CONV_LDS_CASE(32, 4)
+#undef GP5_CONV_LDS_CASE
+            NNCHK(hipGetLastError(), "conv3d launch");
+            return GP5_OK;
+        }
+    }
     {
-        const uint16_t *x = (const uint16_t *)X, *w = (const uint16_t *)Wk, *sk = (const uint16_t *)skip;
-        uint16_t *a = (uint16_t *)A, *y = (uint16_t *)Y;
-        const dim3 block(256);
         const unsigned gy = (unsigned)(nblk / NB);
-#define GP5_CONV_CASE(K, T)                                                                                  \
-    if (NB == K && MT == T)                                                                                  \
-        hipLaunchKernelGGL((k_conv<K, T>), dim3((unsigned)((P + 64 * T - 1) / (64 * T)), gy), block, 0, 0, x, w, bias, a, \
-                           sk, y, Yf, P, D, Cin, Cout, mirror, act);
-        GP5_CONV_CASE(1, 4) GP5_CONV_CASE(1, 1) GP5_CONV_CASE(2, 2) GP5_CONV_CASE(2, 1) GP5_CONV_CASE(3, 2)
-        GP5_CONV_CASE(3, 1) GP5_CONV_CASE(4, 2) GP5_CONV_CASE(4, 1) GP5_CONV_CASE(5, 1) GP5_CONV_CASE(6, 1)
-        GP5_CONV_CASE(7, 1) GP5_CONV_CASE(8, 1)
+        // channels per chunk: the widest of 64, 32, 16 that divides Cin
+        const int KC = Cin % 64 == 0 ? 64 : (Cin % 32 == 0 ? 32 : 16);
+#define GP5_CONV_CASE(C, K, T)                                                                                 \
+    if (KC == C && NB == K && MT == T)                                                                         \
+        hipLaunchKernelGGL((k_conv<C, K, T, kConvPD>), dim3((unsigned)((P + 64 * T - 1) / (64 * T)), gy), block, 0, 0, x, w, \
+                           bias, a, sk, y, Yf, (int)P, D, Cin, Cout, mirror, act);
+#define GP5_CONV_KC(C)                                                                                         \
+    GP5_CONV_CASE(C, 1, 4) GP5_CONV_CASE(C, 1, 1) GP5_CONV_CASE(C, 2, 2) GP5_CONV_CASE(C, 2, 1)                \
+    GP5_CONV_CASE(C, 3, 2) GP5_CONV_CASE(C, 3, 1) GP5_CONV_CASE(C, 4, 2) GP5_CONV_CASE(C, 4, 1)                \
+    GP5_CONV_CASE(C, 5, 1) GP5_CONV_CASE(C, 6, 1) GP5_CONV_CASE(C, 7, 1) GP5_CONV_CASE(C, 8, 1)
+        GP5_CONV_KC(16) GP5_CONV_KC(32) GP5_CONV_KC(64)
+#undef GP5_CONV_KC
 #undef GP5_CONV_CASE
         NNCHK(hipGetLastError(), "conv3d launch");
     }
@@ -702,7 +991,10 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
         NNCHK(hipGetLastError(), "wgrad transpose");
         hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
         NNCHK(hipGetLastError(), "wgrad transpose");
-        hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+        if (D % 8 == 0)
+            hipLaunchKernelGGL(k_wgrad8, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+        else
+            hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
     } else {
         hipLaunchKernelGGL(k_wgrad_gather, grid, dim3(64), 0, 0, (const uint16_t *)X, (const uint16_t *)dY, part0, P, D,
                            Cin, Cout, chunk);

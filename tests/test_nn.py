@@ -146,3 +146,49 @@ def test_training_steps_reduce_the_loss():
         losses.append(net.backward())
         net.step(5e-4)
     assert min(losses[-5:]) < 0.3 * losses[0], losses
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B,exact", [(64, 16, 16, 1, True), (32, 32, 32, 1, True), (16, 32, 16, 2, True),
+                                                (16, 128, 64, 2, False), (8, 64, 128, 1, False)])
+def test_conv_lds_staged_path(D, Cin, Cout, B, exact, monkeypatch):
+    """Cubes of side 8..128 (a power of two) run the kernel that stages each
+    workgroup's input region in LDS.  Where both kernels take the same channel chunk
+    (Cin = 16, or Cin = 32 on a cube of side 32 or less) they sum in the same order
+    and agree bit for bit with the direct kernel (GP5_CONV_LDS=0), forward and
+    mirrored (data gradient); every case agrees with the oracle."""
+    rng = np.random.default_rng(D * Cin + Cout)
+    X = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    L = nn._lib()
+    xb = _buf(X, True)
+    wf, bb = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(b)
+    shape = (B, D, D, D, Cout)
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GP5_CONV_LDS", flag)
+        Yf = nn.DevBuf(B * D ** 3 * Cout * 4)
+        A = nn.DevBuf(B * D ** 3 * Cout * 2)
+        nn._ck(L.gp5_nn_conv3d(xb.ptr, wf.ptr, bb.ptr, A.ptr, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        outs.append((Yf.download(np.float32, shape), A.download(np.uint16, shape)))
+    ref = R.lrelu(R.conv(X, W, b.astype(np.float64)))
+    for got, Ab in outs:
+        assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+        assert np.array_equal(Ab, nn.to_bf16(got))
+    if exact:
+        assert np.array_equal(outs[0][0], outs[1][0])
+    # mirrored taps (data gradient through the [27][ci][co] copy), Cout -> Cin
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    gb, wb = _buf(G, True), _buf(W, True)
+    dref = R.conv_dgrad(G, W)
+    douts = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GP5_CONV_LDS", flag)
+        dX = nn.DevBuf(B * D ** 3 * Cin * 4)
+        nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, dX.ptr, B, D, Cout, Cin, 1, 0))
+        douts.append(dX.download(np.float32, dref.shape))
+    for got in douts:
+        assert np.max(np.abs(got - dref)) <= 1e-4 * np.max(np.abs(dref))
+    if exact and Cout == Cin:
+        assert np.array_equal(douts[0], douts[1])
