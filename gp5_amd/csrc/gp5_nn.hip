@@ -122,6 +122,80 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
         }
 }
 
+// The same epilogue for the LDS kernels, through a per-wave LDS scratch of
+// kEpiScratch bytes: two tiles (32 positions x 16 channels of one output block)
+// at a time go to the scratch in float, in the MFMA layout, and come back one
+// row piece per lane, so that the outputs are written, and the skip input read,
+// with 16-byte accesses that cover 1 KB of consecutive memory per instruction.
+// Same values as conv_epilogue.  Every wave of the workgroup must call it (it
+// synchronises the workgroup).
+constexpr int kEpiStride = 20;                       // floats per scratch row (16 channels, conflict-free)
+constexpr int kEpiScratch = 32 * kEpiStride * 4;     // bytes per wave
+
+template <int NB, int MT>
+__device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
+                                                  const float *__restrict__ bias, int act, uint16_t *A,
+                                                  const uint16_t *skip, uint16_t *Y, float *Yf, unsigned char *scr)
+{
+    static_assert(MT % 2 == 0, "tiles go through the scratch in pairs");
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    float *T = reinterpret_cast<float *>(scr);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int cb = 16 * (nb0 + j);
+        const float bco = bias ? bias[cb + r] : 0.f;
+#pragma unroll
+        for (int h = 0; h < MT / 2; ++h) {
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float v = acc[2 * h + tt][j][i] + bco;
+                    if (act == 1) v = v > 0.f ? v : kLeak * v;
+                    T[(16 * tt + 4 * g + i) * kEpiStride + r] = v;
+                }
+            __syncthreads();
+            const int pb = base + 32 * h;  // first position of the pair
+            if (Yf) {
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int pos = lane / 4 + 16 * k, c4 = 4 * (lane & 3);
+                    if (pb + pos < P)
+                        *reinterpret_cast<float4 *>(Yf + (size_t)(pb + pos) * Cout + cb + c4) =
+                            *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c4);
+                }
+            }
+            if (A || Y) {
+                const int pos = lane / 2, c8 = 8 * (lane & 1);
+                if (pb + pos < P) {
+                    const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
+                    const float4 hi = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8 + 4);
+                    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    const size_t o = (size_t)(pb + pos) * Cout + cb + c8;
+                    uint16_t hb[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) hb[e] = f2bf(v[e]);
+                    if (A) *reinterpret_cast<uint4 *>(A + o) = __builtin_bit_cast(uint4, hb);
+                    if (Y) {
+                        if (skip) {
+                            const uint4 sk4 = *reinterpret_cast<const uint4 *>(skip + o);
+                            const uint16_t *sk = reinterpret_cast<const uint16_t *>(&sk4);
+                            uint16_t yb[8];
+#pragma unroll
+                            for (int e = 0; e < 8; ++e) yb[e] = f2bf(v[e] + bf2f(sk[e]));
+                            *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, yb);
+                        } else {
+                            *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // the scratch is rewritten by the next pair
+        }
+    }
+}
+
 // Forward / data-gradient convolution.  A workgroup of 4 waves covers 64 * MT
 // consecutive positions: each wave holds MT tiles of 16 positions that share
 // every weight fragment it loads, and NB 16-wide blocks of output channels.
@@ -320,7 +394,123 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
                                                                               0, 0, 0);
         }
     }
-    conv_epilogue<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf);
+    __syncthreads();  // the staged region becomes the epilogue scratch
+    conv_epilogue_lds<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf, sx + wave * kEpiScratch);
+}
+
+// slab units (16 bytes) per thread of k_conv_zs: a slab has at most
+// 4 x 130 x CIN/8 units (D = 128)
+__host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 16 ? 5 : 9; }
+
+// z-sliding convolution for inputs of CIN = 16 or 32 channels on cubes of side
+// D = 16 .. 128 (a power of two).  A workgroup of 4 waves owns Rb = 256 / D
+// rows of ZL consecutive planes of one sample and walks them in z.  LDS holds a
+// ring of three slabs, one per plane z-1, z, z+1, each (Rb + 2) rows x (D + 2)
+// columns x CIN channels with zeros outside the cube.  While the waves run the
+// 27 taps of plane z, the global loads of the slab of plane z+2 are in flight
+// in registers; they go to the ring slot of plane z-1 once every wave is done
+// with it.  Each input slab is thus fetched once per plane instead of three
+// times, and its latency overlaps the MFMAs.  All 27 weight fragments of the
+// wave's NB output blocks stay in registers for the whole walk.  Same K
+// permutation and accumulation order (tap, step) as k_conv with KC = CIN, so
+// the two agree bit for bit.
+template <int CIN, int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
+                                                 const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
+                                                 uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
+                                                 int ZL)
+{
+    constexpr int MT = 4, LV = CIN / 4, U8 = CIN / 8;
+    constexpr int NU = zs_units_per_thread(CIN);
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nb0 = blockIdx.y * NB;
+    const int Rb = 256 / D, RY = Rb + 2, RX = D + 2;
+    const int UR = RX * U8, SU = RY * UR;  // 16-byte units per slab row, per slab
+    const float invUR = 1.f / (float)UR;
+    const int nyb = D / Rb, nzs = D / ZL;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * ZL;
+    const int s0 = smp * D * D * D;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, P * CIN * 2);
+    const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * CIN * 2);
+    int lo[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int q = 64 * wave + 16 * t + r;
+        lo[t] = (((q / D) + 1) * RX + (q & (D - 1)) + 1) * CIN * 2 + g * LV * 2;
+    }
+    // weights of all taps
+    Frag<LV> w[27][NB];
+    {
+        const int wvo = ((nb0 * 16 + r) * CIN + g * LV) * 2;
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                w[tap][j] = bld<LV>(wr, wvo, ((mirror ? 26 - tap : tap) * Cout * CIN + j * 16 * CIN) * 2);
+    }
+    uint4 v[NU];
+    auto load_slab = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+            const int u = tid + 256 * k;
+            const int row = udiv_small(u, UR, invUR), e = u - row * UR;
+            const int ix = e / U8, c8 = e - ix * U8;
+            const int y = y0 - 1 + row;
+            const bool in = u < SU && z >= 0 && z < D && y >= 0 && y < D && ix >= 1 && ix <= D;
+            const int voff = in ? ((s0 + (z * D + y) * D + ix - 1) * CIN + 8 * c8) * 2 : kOffNone;
+            v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, 0, 0));
+        }
+    };
+    auto store_slab = [&](int slot) {
+        unsigned char *dst = sx + (size_t)slot * SU * 16;
+#pragma unroll
+        for (int k = 0; k < NU; ++k)
+            if (tid + 256 * k < SU) *reinterpret_cast<uint4 *>(dst + 16 * (tid + 256 * k)) = v[k];
+    };
+    load_slab(zb - 1);
+    store_slab(0);
+    load_slab(zb);
+    store_slab(1);
+    load_slab(zb + 1);
+    store_slab(2);
+    __syncthreads();
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;
+        if (zi + 1 < ZL) load_slab(z + 2);  // in flight during this plane's MFMAs
+        f32x4 acc[MT][NB];
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // slots of planes z-1, z, z+1
+        const int sl[3] = {zi % 3, (zi + 1) % 3, (zi + 2) % 3};
+#pragma unroll
+        for (int tap = 0; tap < 27; ++tap) {
+            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+            const int toff = sl[dz + 1] * SU * 16 + (dy * RX + dx) * CIN * 2;
+            Frag<LV> a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) a[t] = *reinterpret_cast<const Frag<LV> *>(sx + lo[t] + toff);
+#pragma unroll
+            for (int st = 0; st < LV / 4; ++st)
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+                        acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t].v[st], w[tap][j].v[st], acc[t][j],
+                                                                              0, 0, 0);
+        }
+        __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
+        conv_epilogue_lds<NB, MT>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf,
+                                  sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
+        if (zi + 1 < ZL) {
+            store_slab(sl[0]);
+            __syncthreads();
+        }
+    }
 }
 
 // Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
@@ -914,17 +1104,52 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
     const uint16_t *x = (const uint16_t *)X, *w = (const uint16_t *)Wk, *sk = (const uint16_t *)skip;
     uint16_t *a = (uint16_t *)A, *y = (uint16_t *)Y;
     const dim3 block(256);
+    const bool pow2 = D >= 8 && D <= 128 && (D & (D - 1)) == 0;
+    // z-sliding kernel: 16 or 32 input channels, D a power of two in [16, 128], the
+    // longest plane run ZL (a power of two, at least 2) that keeps 1024 workgroups,
+    // as long as there are at least 512.  GP5_CONV_ZS=0 turns it off.
+    const char *zs_env = getenv("GP5_CONV_ZS");
+    if (pow2 && D >= 16 && (Cin == 16 || Cin == 32) && !(zs_env && zs_env[0] == '0')) {
+        const int NBz = (Cin == 16 && nblk % 2 == 0) ? 2 : 1;
+        const int Rb = 256 / D;
+        const long long per_z = (long long)B * (D / Rb) * (nblk / NBz);  // workgroups per plane run
+        int ZL = 0;
+        for (int zl = D; zl >= 2; zl /= 2)
+            if (!ZL && per_z * (D / zl) >= 1024) ZL = zl;
+        if (!ZL && per_z * (D / 2) >= 512) ZL = 2;
+        const int SU = (Rb + 2) * (D + 2) * (Cin / 8);
+        if (ZL && SU <= 256 * zs_units_per_thread(Cin) && SU * 16 >= 4 * kEpiScratch) {
+            const dim3 grid((unsigned)(per_z / (nblk / NBz) * (D / ZL)), (unsigned)(nblk / NBz));
+            const size_t shm = (size_t)3 * SU * 16;
+#define GP5_CONV_ZS_CASE(C, K)                                                                                   \
+    if (Cin == C && NBz == K) {                                                                                  \
+        static size_t shm_set = 65536; /* the default limit of dynamic LDS */                                    \
+        if (shm > shm_set) {                                                                                     \
+            NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                                      (int)shm),                                                                 \
+                  "conv3d LDS size");                                                                            \
+            shm_set = shm;                                                                                       \
+        }                                                                                                        \
+        hipLaunchKernelGGL((k_conv_zs<C, K>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
+                           act, ZL);                                                                             \
+    }
+            GP5_CONV_ZS_CASE(16, 1) GP5_CONV_ZS_CASE(16, 2) GP5_CONV_ZS_CASE(32, 1)
+#undef GP5_CONV_ZS_CASE
+            NNCHK(hipGetLastError(), "conv3d launch");
+            return GP5_OK;
+        }
+    }
     // LDS-staged kernel: D a power of two in [8, 128] (workgroups of 256 positions
     // are whole rows of one sample), channel chunk KC = 32 or 16 with a staged
     // region of at most 64 KB, output blocks per wave NB = 4, 2 or 1, the widest
     // that still gives 1024 workgroups.  GP5_CONV_LDS=0 selects the direct kernel.
     const char *lds_env = getenv("GP5_CONV_LDS");
-    if (D >= 8 && D <= 128 && (D & (D - 1)) == 0 && !(lds_env && lds_env[0] == '0')) {
+    if (pow2 && !(lds_env && lds_env[0] == '0')) {
         const int Rb = std::min(D, 256 / D), PZ = 256 / (D * Rb);
         const long long region = (long long)(PZ + 2) * (Rb + 2) * (D + 2);
         int KCl = 0;
         for (int kc : {32, 16})
-            if (!KCl && Cin % kc == 0 && region * kc * 2 <= 65536) KCl = kc;
+            if (!KCl && Cin % kc == 0 && region * kc * 2 <= 65536 && region * kc * 2 >= 4 * kEpiScratch) KCl = kc;
         if (KCl) {
             const long long wgs = P / 256;
             int NBl = 1;

@@ -165,6 +165,7 @@ def test_conv_lds_staged_path(D, Cin, Cout, B, exact, monkeypatch):
     xb = _buf(X, True)
     wf, bb = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(b)
     shape = (B, D, D, D, Cout)
+    monkeypatch.setenv("GP5_CONV_ZS", "0")
     outs = []
     for flag in ("1", "0"):
         monkeypatch.setenv("GP5_CONV_LDS", flag)
@@ -192,3 +193,42 @@ def test_conv_lds_staged_path(D, Cin, Cout, B, exact, monkeypatch):
         assert np.max(np.abs(got - dref)) <= 1e-4 * np.max(np.abs(dref))
     if exact and Cout == Cin:
         assert np.array_equal(douts[0], douts[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 16, 1), (64, 32, 16, 1), (64, 16, 32, 1), (32, 32, 16, 8),
+                                          (128, 16, 16, 1)])
+def test_conv_z_sliding_path(D, Cin, Cout, B, monkeypatch):
+    """16 or 32 input channels on cubes of side 16..128 with enough workgroups run
+    the kernel that walks plane runs in z with a three-slab LDS ring and weights in
+    registers.  It sums in the same order as the direct kernel with a chunk of Cin
+    channels (GP5_CONV_ZS=0, GP5_CONV_LDS=0), so the two agree bit for bit, forward
+    and mirrored; both agree with the oracle."""
+    rng = np.random.default_rng(D + Cin * Cout + B)
+    X = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    L = nn._lib()
+    xb, gb = _buf(X, True), _buf(G, True)
+    wf, wb, bb = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(W, True), _buf(b)
+    shape = (B, D, D, D, Cout)
+    fwd, bwd = [], []
+    for zs in ("1", "0"):
+        monkeypatch.setenv("GP5_CONV_ZS", zs)
+        monkeypatch.setenv("GP5_CONV_LDS", zs)
+        Yf = nn.DevBuf(B * D ** 3 * Cout * 4)
+        A = nn.DevBuf(B * D ** 3 * Cout * 2)
+        nn._ck(L.gp5_nn_conv3d(xb.ptr, wf.ptr, bb.ptr, A.ptr, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        fwd.append((Yf.download(np.float32, shape), A.download(np.uint16, shape)))
+        dX = nn.DevBuf(B * D ** 3 * Cin * 4)
+        nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, dX.ptr, B, D, Cout, Cin, 1, 0))
+        bwd.append(dX.download(np.float32, (B, D, D, D, Cin)))
+    assert np.array_equal(fwd[0][0], fwd[1][0]) and np.array_equal(fwd[0][1], fwd[1][1])
+    ref = R.lrelu(R.conv(X, W, b.astype(np.float64)))
+    assert np.max(np.abs(fwd[0][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+    assert np.array_equal(fwd[0][1], nn.to_bf16(fwd[0][0]))
+    dref = R.conv_dgrad(G, W)
+    assert np.max(np.abs(bwd[0] - dref)) <= 1e-4 * np.max(np.abs(dref))
+    if Cout in (16, 32):  # the mirrored call has Cout input channels: the same kernels and order
+        assert np.array_equal(bwd[0], bwd[1])
