@@ -16,6 +16,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace gp5 {
@@ -551,6 +552,33 @@ __global__ __launch_bounds__(256) void k_transpose_pc(const uint16_t *__restrict
     }
 }
 
+// The same for P a multiple of 128 and C a multiple of 16, with 16-byte accesses:
+// a workgroup moves 128 positions x 16 channels, each thread loading 8 channels
+// of one position and, after the LDS transpose, storing 8 positions of one
+// channel (16 threads cover a 256-byte piece of a channel row).
+__global__ __launch_bounds__(256) void k_transpose_pc16(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                                                        long long P, int C)
+{
+    constexpr int kRow = 128 + 8;  // LDS row of one channel, padded (16-byte aligned rows)
+    __shared__ __align__(16) uint16_t t[16 * kRow];
+    const int tid = threadIdx.x;
+    const long long p0 = (long long)blockIdx.x * 128;
+    const int c0 = 16 * (int)blockIdx.y;
+    {
+        const int pos = tid >> 1, h = tid & 1;
+        const uint4 v = *reinterpret_cast<const uint4 *>(in + (size_t)(p0 + pos) * C + c0 + 8 * h);
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[(8 * h + k) * kRow + pos] = e[k];
+    }
+    __syncthreads();
+    {
+        const int c = tid >> 4, seg = tid & 15;
+        *reinterpret_cast<uint4 *>(out + (size_t)(c0 + c) * P + p0 + 8 * seg) =
+            *reinterpret_cast<const uint4 *>(t + c * kRow + 8 * seg);
+    }
+}
+
 // The same for cube sides that are not a multiple of 4 (small test cubes):
 // scalar gathers from the position-major X and dY.
 __global__ __launch_bounds__(64) void k_wgrad_gather(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
@@ -737,7 +765,7 @@ __global__ void k_wgrad_reduce(const float *part, float *dW, long long n)
 // db[c] = sum_p dY[p][c] in two fixed-order passes: kBiasBlocks blocks each sum a
 // contiguous range of positions for every channel (consecutive threads read
 // consecutive channels), then one thread per channel adds the block sums in order.
-constexpr int kBiasBlocks = 512;
+constexpr int kBiasBlocks = 2048;
 __global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ dY, float *part, long long P, int C)
 {
     __shared__ float s[256];
@@ -949,8 +977,14 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float
         const float y = base[p] + Yf[(size_t)p * C];
         const float e = y - target[p];
         a += (double)e * e;
-        dYf[(size_t)p * C] = inv * e;
-        for (int c = 1; c < C; ++c) dYf[(size_t)p * C + c] = 0.f;
+        if (C % 4 == 0) {  // the row as 16-byte stores (rows are 16-byte aligned)
+            float4 *row = reinterpret_cast<float4 *>(dYf + (size_t)p * C);
+            row[0] = make_float4(inv * e, 0.f, 0.f, 0.f);
+            for (int c = 1; c < C / 4; ++c) row[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            dYf[(size_t)p * C] = inv * e;
+            for (int c = 1; c < C; ++c) dYf[(size_t)p * C + c] = 0.f;
+        }
     }
     s[threadIdx.x] = a;
     __syncthreads();
@@ -1211,10 +1245,18 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     float *part0 = reinterpret_cast<float *>(dYT + (size_t)P * Cout);
     const dim3 grid((unsigned)((Cin / 16) * (Cout / 16)), (unsigned)chunks);
     if (D % 4 == 0) {
-        const unsigned tb = (unsigned)((P + 63) / 64);
-        hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
-        NNCHK(hipGetLastError(), "wgrad transpose");
-        hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
+        if (P % 128 == 0) {
+            hipLaunchKernelGGL(k_transpose_pc16, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
+                               (const uint16_t *)X, XT, P, Cin);
+            NNCHK(hipGetLastError(), "wgrad transpose");
+            hipLaunchKernelGGL(k_transpose_pc16, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
+                               (const uint16_t *)dY, dYT, P, Cout);
+        } else {
+            const unsigned tb = (unsigned)((P + 63) / 64);
+            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
+            NNCHK(hipGetLastError(), "wgrad transpose");
+            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
+        }
         NNCHK(hipGetLastError(), "wgrad transpose");
         if (D % 8 == 0)
             hipLaunchKernelGGL(k_wgrad8, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
@@ -1238,17 +1280,26 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)
 {
     if (!dY || !db || C < 1 || C > 256 || 256 % C) return set_error(GP5_ERR_ARG, "bias grad: C must divide 256");
-    float *part = nullptr;
-    NNCHK(hipMalloc(&part, (size_t)kBiasBlocks * C * sizeof(float)), "bias grad scratch");
-    hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, 0, dY, part, P, C);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
-        e = hipGetLastError();
+    // block partials in a process-wide scratch that only grows (the kernels of
+    // successive calls run in order on the null stream, so they can share it)
+    static std::mutex mu;
+    static float *part = nullptr;
+    static size_t cap = 0;
+    const size_t need = (size_t)kBiasBlocks * C;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (cap < need) {
+            if (part) NNCHK(hipFree(part), "bias grad scratch free");
+            part = nullptr;
+            cap = 0;
+            NNCHK(hipMalloc(&part, need * sizeof(float)), "bias grad scratch");
+            cap = need;
+        }
     }
-    hipError_t ef = hipFree(part);
-    NNCHK(e, "bias grad launch");
-    NNCHK(ef, "bias grad scratch free");
+    hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, 0, dY, part, P, C);
+    NNCHK(hipGetLastError(), "bias grad launch");
+    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
+    NNCHK(hipGetLastError(), "bias grad launch");
     return GP5_OK;
 }
 
