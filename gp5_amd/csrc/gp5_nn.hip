@@ -552,12 +552,14 @@ __global__ __launch_bounds__(256) void k_transpose_pc(const uint16_t *__restrict
     }
 }
 
-// The same for P a multiple of 128 and C a multiple of 16, with 16-byte accesses:
-// a workgroup moves 128 positions x 16 channels, each thread loading 8 channels
-// of one position and, after the LDS transpose, storing 8 positions of one
-// channel (16 threads cover a 256-byte piece of a channel row).
-__global__ __launch_bounds__(256) void k_transpose_pc16(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
-                                                        long long P, int C)
+// Blocked channel-major copy for P a multiple of 128 and C a multiple of 16:
+// element (p, c) goes to out[((p / 8) C + c) 8 + p % 8], i.e. runs of 8
+// positions of one channel, the channels of one 8-position block side by side.
+// A workgroup moves 128 positions x 16 channels: each thread loads 8 channels of
+// one position (16 bytes), and after the LDS transpose stores 8 positions of one
+// channel (16 bytes; 16 threads cover 256 consecutive bytes).
+__global__ __launch_bounds__(256) void k_transpose_b8(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
+                                                      long long P, int C)
 {
     constexpr int kRow = 128 + 8;  // LDS row of one channel, padded (16-byte aligned rows)
     __shared__ __align__(16) uint16_t t[16 * kRow];
@@ -573,9 +575,9 @@ __global__ __launch_bounds__(256) void k_transpose_pc16(const uint16_t *__restri
     }
     __syncthreads();
     {
-        const int c = tid >> 4, seg = tid & 15;
-        *reinterpret_cast<uint4 *>(out + (size_t)(c0 + c) * P + p0 + 8 * seg) =
-            *reinterpret_cast<const uint4 *>(t + c * kRow + 8 * seg);
+        const int blk = tid >> 4, c = tid & 15;  // 8-position block of the 16, channel
+        *reinterpret_cast<uint4 *>(out + ((size_t)(p0 / 8 + blk) * C + c0 + c) * 8) =
+            *reinterpret_cast<const uint4 *>(t + c * kRow + 8 * blk);
     }
 }
 
@@ -681,11 +683,16 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ XT, c
     }
 }
 
-// The same for cube sides that are a multiple of 8, 32 positions per step: a
-// lane loads 8 consecutive positions of its channel (16 bytes) and feeds two
+// The same for cube sides that are a multiple of 8, on the blocked copies of X
+// and dY (k_transpose_b8), 32 positions per step: a lane loads 8 consecutive
+// positions of its channel (16 bytes; the 16 lanes of a k-group read 256
+// consecutive bytes) and feeds two
 // MFMA k-steps with them, positions 8g .. 8g+3 and 8g+4 .. 8g+7 (the same split
 // for dY), so one aligned load serves 6 MFMAs of a row (three x-shifted taps x
-// two steps) with the 4 positions on either side from two 8-byte loads.
+// two steps) with the 4 positions on either side from two 8-byte loads.  The
+// taps are split by dz over blockIdx.z: each wave holds the 9 taps of one dz
+// and reads only the 3 input rows of that plane, which triples the waves per
+// chunk and cuts the accumulators per wave to 9.
 __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, const uint16_t *__restrict__ dYT,
                                                float *ws, long long P, int D, int Cin, int Cout, long long chunk)
 {
@@ -693,48 +700,51 @@ __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, 
     const int nco = Cout / 16;
     const int cob = blockIdx.x % nco, cib = blockIdx.x / nco;
     const long long ck = blockIdx.y;
+    const int dz = (int)blockIdx.z - 1;
     const long long p0 = ck * chunk, p1 = min(P, p0 + chunk);
     const int ci = cib * 16 + r, co = cob * 16 + r;
-    const uint16_t *xrow = XT + (size_t)ci * P;
-    const uint16_t *yrow = dYT + (size_t)co * P;
+    // blocked layout: positions q .. q+7 (q a multiple of 8) of channel c at ((q / 8) C + c) 8
+    const uint16_t *xb = XT + (size_t)ci * 8;
+    const uint16_t *yb = dYT + (size_t)co * 8;
+    const size_t xs = (size_t)Cin * 8;  // elements per 8-position block
     const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
     typedef short bf16x8 __attribute__((ext_vector_type(8)));
-    f32x4 acc[27];
+    f32x4 acc[9];
 #pragma unroll
-    for (int t = 0; t < 27; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (long long pb = p0; pb < p1; pb += 32) {
         const long long pg = pb + 8 * g;
         if (pg >= p1) continue;  // p1 is a multiple of 32 or P: groups of 8 are wholly in or out
         const int x = (int)(pg % D), y = (int)((pg / D) % D), z = (int)((pg / ((long long)D * D)) % D);
-        const bf16x8 b8 = *reinterpret_cast<const bf16x8 *>(yrow + pg);
+        const bf16x8 b8 = *reinterpret_cast<const bf16x8 *>(yb + (size_t)(pg / 8) * Cout * 8);
         const bf16x4 b1 = bf16x4{b8[0], b8[1], b8[2], b8[3]}, b2 = bf16x4{b8[4], b8[5], b8[6], b8[7]};
+        const bool plane_ok = z + dz >= 0 && z + dz < D;
 #pragma unroll
-        for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int t0 = (dz + 1) * 9 + (dy + 1) * 3;  // tap of dx = -1
-                const bool row_ok = z + dz >= 0 && z + dz < D && y + dy >= 0 && y + dy < D;
-                bf16x8 c = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                bf16x4 pv = zero4, nx = zero4;
-                if (row_ok) {
-                    const long long q = pg + ((long long)dz * D + dy) * D;
-                    c = *reinterpret_cast<const bf16x8 *>(xrow + q);
-                    if (x > 0) pv = ld4(xrow + q - 4);
-                    if (x + 8 < D) nx = ld4(xrow + q + 8);
-                }
-                acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{pv[3], c[0], c[1], c[2]}, b1, acc[t0], 0, 0, 0);
-                acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[3], c[4], c[5], c[6]}, b2, acc[t0], 0, 0, 0);
-                acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[0], c[1], c[2], c[3]}, b1, acc[t0 + 1], 0, 0, 0);
-                acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[4], c[5], c[6], c[7]}, b2, acc[t0 + 1], 0, 0, 0);
-                acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[1], c[2], c[3], c[4]}, b1, acc[t0 + 2], 0, 0, 0);
-                acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[5], c[6], c[7], nx[0]}, b2, acc[t0 + 2], 0, 0, 0);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int t0 = (dy + 1) * 3;  // tap of dx = -1 within this dz
+            const bool row_ok = plane_ok && y + dy >= 0 && y + dy < D;
+            bf16x8 c = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            bf16x4 pv = zero4, nx = zero4;
+            if (row_ok) {
+                const uint16_t *qb = xb + (size_t)((pg + ((long long)dz * D + dy) * D) / 8) * xs;
+                c = *reinterpret_cast<const bf16x8 *>(qb);
+                if (x > 0) pv = ld4(qb - xs + 4);
+                if (x + 8 < D) nx = ld4(qb + xs);
             }
+            acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{pv[3], c[0], c[1], c[2]}, b1, acc[t0], 0, 0, 0);
+            acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[3], c[4], c[5], c[6]}, b2, acc[t0], 0, 0, 0);
+            acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[0], c[1], c[2], c[3]}, b1, acc[t0 + 1], 0, 0, 0);
+            acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[4], c[5], c[6], c[7]}, b2, acc[t0 + 1], 0, 0, 0);
+            acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[1], c[2], c[3], c[4]}, b1, acc[t0 + 2], 0, 0, 0);
+            acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[5], c[6], c[7], nx[0]}, b2, acc[t0 + 2], 0, 0, 0);
+        }
     }
 #pragma unroll
-    for (int tap = 0; tap < 27; ++tap) {
+    for (int t = 0; t < 9; ++t) {
+        const int tap = (dz + 1) * 9 + t;
         float *o = ws + (((size_t)ck * 27 + tap) * Cin + cib * 16) * Cout + cob * 16 + r;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[tap][i];
+        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[t][i];
     }
 }
 
@@ -1227,7 +1237,7 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
     const long long P = (long long)B * D * D * D;
     const long long chunk = wgrad_chunk(P, Cin, Cout);
     const long long chunks = (P + chunk - 1) / chunk;
-    // channel-major copies of X and dY, then the partial sums
+    // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums
     return (size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * 27 * Cin * Cout * sizeof(float);
 }
 
@@ -1245,11 +1255,11 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     float *part0 = reinterpret_cast<float *>(dYT + (size_t)P * Cout);
     const dim3 grid((unsigned)((Cin / 16) * (Cout / 16)), (unsigned)chunks);
     if (D % 4 == 0) {
-        if (P % 128 == 0) {
-            hipLaunchKernelGGL(k_transpose_pc16, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
+        if (D % 8 == 0) {  // P = B D^3 is a multiple of 512
+            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
                                (const uint16_t *)X, XT, P, Cin);
             NNCHK(hipGetLastError(), "wgrad transpose");
-            hipLaunchKernelGGL(k_transpose_pc16, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
+            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
                                (const uint16_t *)dY, dYT, P, Cout);
         } else {
             const unsigned tb = (unsigned)((P + 63) / 64);
@@ -1259,7 +1269,7 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
         }
         NNCHK(hipGetLastError(), "wgrad transpose");
         if (D % 8 == 0)
-            hipLaunchKernelGGL(k_wgrad8, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+            hipLaunchKernelGGL(k_wgrad8, dim3(grid.x, grid.y, 3), dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
         else
             hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
     } else {

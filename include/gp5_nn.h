@@ -41,8 +41,9 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
                   int B, int D, int Cin, int Cout, int mirror, int act);
 /* dW[tap][ci][co] = sum over positions of X[p + off(tap)][ci] dY[p][co]
  * (bf16 inputs, float result); ws: device workspace of at least
- * gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes (channel-major copies of X and dY and
- * the partial sums). */
+ * gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes (channel-major copies of X and dY, in
+ * runs of 8 positions per channel when D is a multiple of 8, and the partial
+ * sums). */
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout);
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
 /* db[c] = sum over positions of dY[p][c] (float). */
