@@ -906,6 +906,40 @@ __global__ void k_act_bwd(const uint16_t *A, const float *dA, float *dP, long lo
         dP[i] = dA[i] * (bf2f(A[i]) > 0.f ? 1.f : kLeak);
 }
 
+// dP = dA act'(A) and its bf16 copy H in one pass, 4 elements per thread (n4 = n / 4);
+// dP may be dA
+__global__ void k_act_bwd_cast4(const uint16_t *__restrict__ A, const float *dA, float *dP,
+                                uint16_t *__restrict__ H, long long n4)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (long long)gridDim.x * blockDim.x) {
+        const uint2 a = reinterpret_cast<const uint2 *>(A)[i];
+        const float4 d = reinterpret_cast<const float4 *>(dA)[i];
+        const uint16_t *ah = reinterpret_cast<const uint16_t *>(&a);
+        const float o[4] = {d.x * (bf2f(ah[0]) > 0.f ? 1.f : kLeak), d.y * (bf2f(ah[1]) > 0.f ? 1.f : kLeak),
+                            d.z * (bf2f(ah[2]) > 0.f ? 1.f : kLeak), d.w * (bf2f(ah[3]) > 0.f ? 1.f : kLeak)};
+        reinterpret_cast<float4 *>(dP)[i] = make_float4(o[0], o[1], o[2], o[3]);
+        uint16_t h[4] = {f2bf(o[0]), f2bf(o[1]), f2bf(o[2]), f2bf(o[3])};
+        reinterpret_cast<uint2 *>(H)[i] = __builtin_bit_cast(uint2, h);
+    }
+}
+
+// Y (side 2D) += scale * X at the parent, 4 channels per thread (C % 4 == 0)
+__global__ void k_up2_add4(const float *__restrict__ X, float *Y, int B, int D, int C, float scale)
+{
+    const int D2 = 2 * D, C4 = C / 4;
+    const long long n = (long long)B * D2 * D2 * D2 * C4;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c4 = (int)(i % C4);
+        const long long p = i / C4;
+        const int x = (int)(p % D2), y = (int)((p / D2) % D2), z = (int)((p / ((long long)D2 * D2)) % D2);
+        const long long b = p / ((long long)D2 * D2 * D2);
+        const long long q = ((b * D + z / 2) * D + y / 2) * D + x / 2;
+        const float4 v = reinterpret_cast<const float4 *>(X)[q * C4 + c4];
+        float4 &o = reinterpret_cast<float4 *>(Y)[i];
+        o = make_float4(o.x + scale * v.x, o.y + scale * v.y, o.z + scale * v.z, o.w + scale * v.w);
+    }
+}
+
 __global__ void k_add(float *d, const float *s, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -1355,6 +1389,26 @@ int gp5_nn_act_bwd(const void *A, const float *dA, float *dPre, long long n)
 {
     hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)A, dA, dPre, n);
     NNCHK(hipGetLastError(), "act_bwd");
+    return GP5_OK;
+}
+
+int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, long long n)
+{
+    if (!A || !dA || !dPre || !H) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (n % 4) return set_error(GP5_ERR_ARG, "act_bwd_cast: n must be a multiple of 4");
+    hipLaunchKernelGGL(k_act_bwd_cast4, dim3(grid_for(n / 4)), dim3(256), 0, 0, (const uint16_t *)A, dA, dPre,
+                       (uint16_t *)H, n / 4);
+    NNCHK(hipGetLastError(), "act_bwd_cast");
+    return GP5_OK;
+}
+
+int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale)
+{
+    if (!X || !Y) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (C % 4) return set_error(GP5_ERR_ARG, "up2_add: C must be a multiple of 4");
+    const long long n = (long long)B * 8 * D * D * D * (C / 4);
+    hipLaunchKernelGGL(k_up2_add4, dim3(grid_for(n)), dim3(256), 0, 0, X, Y, B, D, C, scale);
+    NNCHK(hipGetLastError(), "up2_add");
     return GP5_OK;
 }
 

@@ -43,6 +43,7 @@ def _lib():
         "gp5_nn_pool2_bf16": [P, P, I, I, I, F], "gp5_nn_up2_bf16": [P, P, I, I, I, F],
         "gp5_nn_pool2_f32": [P, P, I, I, I, F], "gp5_nn_up2_f32": [P, P, I, I, I, F],
         "gp5_nn_act_bwd": [P, P, P, LL], "gp5_nn_add_f32": [P, P, LL], "gp5_nn_cast_bf16": [P, P, LL],
+        "gp5_nn_act_bwd_cast": [P, P, P, P, LL], "gp5_nn_up2_add_f32": [P, P, I, I, I, F],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -160,9 +161,9 @@ class UNet:
             "out": (n(D, 16), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
             "g_out": (n(D, 16), 4), "h_out": (n(D, 16), 2),
-            "g16a": (n(D, 16), 4), "g16b": (n(D, 16), 4), "g16c": (n(D, 16), 4), "h16": (n(D, 16), 2),
-            "g32a": (n(D // 2, 32), 4), "g32b": (n(D // 2, 32), 4), "g32c": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
-            "g64a": (n(D // 4, 64), 4), "g64b": (n(D // 4, 64), 4), "g64c": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
+            "g16a": (n(D, 16), 4), "g16b": (n(D, 16), 4), "h16": (n(D, 16), 2),
+            "g32a": (n(D // 2, 32), 4), "g32b": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
+            "g64a": (n(D // 4, 64), 4), "g64b": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
             "g128": (n(D // 8, 128), 4), "h128": (n(D // 8, 128), 2),
             "gu32": (n(D, 32), 4), "gu64": (n(D // 2, 64), 4), "gu128": (n(D // 4, 128), 4),
             "gp16": (n(D // 2, 16), 4), "gp32": (n(D // 4, 32), 4), "gp64": (n(D // 8, 64), 4),
@@ -293,53 +294,43 @@ class UNet:
         self._bgrad("c8", t["g_out"], D)
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
         # c7: dz7 = dd1 * act'(a7)
-        _ck(L.gp5_nn_act_bwd(t["a7"].ptr, t["g16a"].ptr, t["g16b"].ptr, n(D, 16)))
-        _ck(L.gp5_nn_cast_bf16(t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        _ck(L.gp5_nn_act_bwd_cast(t["a7"].ptr, t["g16a"].ptr, t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
         self._wgrad("c7", t["u2"], t["h16"], D)
         self._bgrad("c7", t["g16b"], D)
         self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
         _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
         # c6
-        _ck(L.gp5_nn_act_bwd(t["a6"].ptr, t["g32a"].ptr, t["g32b"].ptr, n(D // 2, 32)))
-        _ck(L.gp5_nn_cast_bf16(t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        _ck(L.gp5_nn_act_bwd_cast(t["a6"].ptr, t["g32a"].ptr, t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
         self._wgrad("c6", t["u3"], t["h32"], D // 2)
         self._bgrad("c6", t["g32b"], D // 2)
         self._dgrad("c6", t["h32"], t["gu64"], D // 2)
         _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
         # c5
-        _ck(L.gp5_nn_act_bwd(t["a5"].ptr, t["g64a"].ptr, t["g64b"].ptr, n(D // 4, 64)))
-        _ck(L.gp5_nn_cast_bf16(t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        _ck(L.gp5_nn_act_bwd_cast(t["a5"].ptr, t["g64a"].ptr, t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
         self._wgrad("c5", t["u4"], t["h64"], D // 4)
         self._bgrad("c5", t["g64b"], D // 4)
         self._dgrad("c5", t["h64"], t["gu128"], D // 4)
         _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4
         # c4
-        _ck(L.gp5_nn_act_bwd(t["e4"].ptr, t["g128"].ptr, t["g128"].ptr, n(D // 8, 128)))
-        _ck(L.gp5_nn_cast_bf16(t["g128"].ptr, t["h128"].ptr, n(D // 8, 128)))
+        _ck(L.gp5_nn_act_bwd_cast(t["e4"].ptr, t["g128"].ptr, t["g128"].ptr, t["h128"].ptr, n(D // 8, 128)))
         self._wgrad("c4", t["p3"], t["h128"], D // 8)
         self._bgrad("c4", t["g128"], D // 8)
         self._dgrad("c4", t["h128"], t["gp64"], D // 8)
-        _ck(L.gp5_nn_up2_f32(t["gp64"].ptr, t["g64c"].ptr, B, D // 8, 64, 0.125))
-        _ck(L.gp5_nn_add_f32(t["g64a"].ptr, t["g64c"].ptr, n(D // 4, 64)))  # de3 total
+        _ck(L.gp5_nn_up2_add_f32(t["gp64"].ptr, t["g64a"].ptr, B, D // 8, 64, 0.125))  # de3 total
         # c3
-        _ck(L.gp5_nn_act_bwd(t["e3"].ptr, t["g64a"].ptr, t["g64b"].ptr, n(D // 4, 64)))
-        _ck(L.gp5_nn_cast_bf16(t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        _ck(L.gp5_nn_act_bwd_cast(t["e3"].ptr, t["g64a"].ptr, t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
         self._wgrad("c3", t["p2"], t["h64"], D // 4)
         self._bgrad("c3", t["g64b"], D // 4)
         self._dgrad("c3", t["h64"], t["gp32"], D // 4)
-        _ck(L.gp5_nn_up2_f32(t["gp32"].ptr, t["g32c"].ptr, B, D // 4, 32, 0.125))
-        _ck(L.gp5_nn_add_f32(t["g32a"].ptr, t["g32c"].ptr, n(D // 2, 32)))  # de2 total
+        _ck(L.gp5_nn_up2_add_f32(t["gp32"].ptr, t["g32a"].ptr, B, D // 4, 32, 0.125))  # de2 total
         # c2
-        _ck(L.gp5_nn_act_bwd(t["e2"].ptr, t["g32a"].ptr, t["g32b"].ptr, n(D // 2, 32)))
-        _ck(L.gp5_nn_cast_bf16(t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        _ck(L.gp5_nn_act_bwd_cast(t["e2"].ptr, t["g32a"].ptr, t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
         self._wgrad("c2", t["p1"], t["h32"], D // 2)
         self._bgrad("c2", t["g32b"], D // 2)
         self._dgrad("c2", t["h32"], t["gp16"], D // 2)
-        _ck(L.gp5_nn_up2_f32(t["gp16"].ptr, t["g16c"].ptr, B, D // 2, 16, 0.125))
-        _ck(L.gp5_nn_add_f32(t["g16a"].ptr, t["g16c"].ptr, n(D, 16)))  # de1 total
+        _ck(L.gp5_nn_up2_add_f32(t["gp16"].ptr, t["g16a"].ptr, B, D // 2, 16, 0.125))  # de1 total
         # c1
-        _ck(L.gp5_nn_act_bwd(t["e1"].ptr, t["g16a"].ptr, t["g16b"].ptr, n(D, 16)))
-        _ck(L.gp5_nn_cast_bf16(t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        _ck(L.gp5_nn_act_bwd_cast(t["e1"].ptr, t["g16a"].ptr, t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
         self._wgrad("c1", t["X"], t["h16"], D)
         self._bgrad("c1", t["g16b"], D)
         return float(loss.value)

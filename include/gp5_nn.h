@@ -57,6 +57,12 @@ int gp5_nn_up2_f32(const float *X, float *Y, int B, int D, int C, float scale);
 /* dPre = dA * (A > 0 ? 1 : 0.1): backward of the leaky ReLU from its output */
 int gp5_nn_act_bwd(const void *A, const float *dA, float *dPre, long long n);
 int gp5_nn_add_f32(float *dst, const float *src, long long n);
+/* dPre = dA act'(A) (float) and H = bf16(dPre) in one pass; n a multiple of 4;
+ * dPre may be dA. */
+int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, long long n);
+/* Y (side 2D) += scale * X at the parent cell (the adjoint of pooling added into
+ * Y); C a multiple of 4. */
+int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale);
 int gp5_nn_cast_bf16(const float *src, void *dst, long long n);
 /* Mean squared error of y = base + Yf[p][0] against target over P positions:
  * dYf[p][c] = 2 (y - t) / P for c = 0 and 0 otherwise; *loss = mean. */
