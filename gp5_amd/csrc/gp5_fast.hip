@@ -81,6 +81,20 @@ template <> struct FPlanZ<512> {
 };
 #endif
 
+// compile-time overrides of other z plans: R0, R1, R2, R3, E, COLS
+#define GP5_ZPLAN_OVERRIDE(NN, PLAN)                                                                  \
+    template <> struct FPlanZ<NN> {                                                                  \
+        static constexpr int p[6] = {PLAN};                                                          \
+        static constexpr int NP = NN, R0 = p[0], R1 = p[1], R2 = p[2], R3 = p[3], E = p[4], COLS = p[5]; \
+    };
+#ifdef GP5_Z256_PLAN
+GP5_ZPLAN_OVERRIDE(256, GP5_Z256_PLAN)
+#endif
+#ifdef GP5_Z192_PLAN
+GP5_ZPLAN_OVERRIDE(192, GP5_Z192_PLAN)
+#endif
+#undef GP5_ZPLAN_OVERRIDE
+
 template <int N, bool ROWS, int MODE>
 using PlanOf = std::conditional_t<ROWS, FPlan<N, true>, std::conditional_t<MODE == 1, FPlanZ<N>, FPlan<N, false>>>;
 
