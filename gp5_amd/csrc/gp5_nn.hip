@@ -37,16 +37,7 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 constexpr float kLeak = 0.1f;
 
-// Positions are linear indices p = ((b*D + z)*D + y)*D + x.  Neighbour of p at
-// tap (dz, dy, dx) in {-1, 0, 1}^3, or -1 outside the cube (zero padding).
-__device__ __forceinline__ long long neighbour(long long p, int D, int tap)
-{
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const int x = (int)(p % D), y = (int)((p / D) % D), z = (int)((p / ((long long)D * D)) % D);
-    const int xx = x + dx, yy = y + dy, zz = z + dz;
-    if (xx < 0 || xx >= D || yy < 0 || yy >= D || zz < 0 || zz >= D) return -1;
-    return p + ((long long)dz * D + dy) * D + dx;
-}
+// Positions are linear indices p = ((b*D + z)*D + y)*D + x.
 
 // 8-byte load of 4 consecutive bf16 (the caller guarantees 8-byte alignment:
 // channel offsets are multiples of 4)
@@ -559,7 +550,7 @@ __global__ __launch_bounds__(256) void k_transpose_pc(const uint16_t *__restrict
 // one position (16 bytes), and after the LDS transpose stores 8 positions of one
 // channel (16 bytes; 16 threads cover 256 consecutive bytes).
 __global__ __launch_bounds__(256) void k_transpose_b8(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
-                                                      long long P, int C)
+                                                      int C)
 {
     constexpr int kRow = 128 + 8;  // LDS row of one channel, padded (16-byte aligned rows)
     __shared__ __align__(16) uint16_t t[16 * kRow];
@@ -788,7 +779,7 @@ __global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ 
         for (long long p = p0 + sub; p < p1; p += per_pos) a += dY[(size_t)p * C + c];
     s[threadIdx.x] = a;
     __syncthreads();
-    if (threadIdx.x < C) {
+    if ((int)threadIdx.x < C) {
         float t = 0.f;
         for (int k = 0; k < per_pos; ++k) t += s[threadIdx.x + k * C];
         part[(size_t)blockIdx.x * C + threadIdx.x] = t;
@@ -806,7 +797,7 @@ __global__ __launch_bounds__(256) void k_bias_final(const float *part, float *db
     s[threadIdx.x] = t;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) db[c] = s[0];
@@ -857,11 +848,11 @@ __global__ void k_up2_bf16x4(const uint16_t *__restrict__ X, uint16_t *__restric
     }
 }
 
+// element access of the float pool / upsample kernels (the bf16 ones move 4
+// channels per thread)
 template <typename T> __device__ __forceinline__ float ldv(const T *p);
-template <> __device__ __forceinline__ float ldv<uint16_t>(const uint16_t *p) { return bf2f(*p); }
 template <> __device__ __forceinline__ float ldv<float>(const float *p) { return *p; }
 template <typename T> __device__ __forceinline__ void stv(T *p, float v);
-template <> __device__ __forceinline__ void stv<uint16_t>(uint16_t *p, float v) { *p = f2bf(v); }
 template <> __device__ __forceinline__ void stv<float>(float *p, float v) { *p = v; }
 
 // Y (side D/2) = scale * mean of the 8 children
@@ -1033,7 +1024,7 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float
     s[threadIdx.x] = a;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
@@ -1291,10 +1282,10 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     if (D % 4 == 0) {
         if (D % 8 == 0) {  // P = B D^3 is a multiple of 512
             hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
-                               (const uint16_t *)X, XT, P, Cin);
+                               (const uint16_t *)X, XT, Cin);
             NNCHK(hipGetLastError(), "wgrad transpose");
             hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
-                               (const uint16_t *)dY, dYT, P, Cout);
+                               (const uint16_t *)dY, dYT, Cout);
         } else {
             const unsigned tb = (unsigned)((P + 63) / 64);
             hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
