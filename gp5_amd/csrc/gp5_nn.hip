@@ -308,7 +308,7 @@ template <int KC, int NB, int PD>
 __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                   const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                   uint16_t *Y, float *Yf, int P, int D, int Cin, int Cout, int mirror,
-                                                  int act, int Rb, int PZ)
+                                                  int act, int Rb, int PZ, int up)
 {
     constexpr int MT = 4, LV = KC / 4, U8 = KC / 8;  // U8: 16-byte units per staged position
     extern __shared__ __align__(16) unsigned char sx[];
@@ -320,8 +320,10 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
     const float invUR = 1.f / (float)UR, invRY = 1.f / (float)RY;
     const int p0 = (int)blockIdx.x * 256;
     const int y0 = (p0 / D) % D, z0 = (p0 / (D * D)) % D;
-    const int s0 = p0 - (z0 * D + y0) * D;  // position of (z, y, x) = (0, 0, 0) in this sample
-    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, P * Cin * 2);
+    // up = 1: X is stored at side Ds = D/2 and read at the parent cell (nearest upsampling)
+    const int Ds = D >> up;
+    const int sb = p0 / (D * D * D) * Ds * Ds * Ds;  // stored position of this sample's origin
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> (3 * up)) * Cin * 2);
     const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * Cin * 2);
     int lo[MT];  // LDS byte offset of this lane's tap-(0,0,0) fragment in tile t
 #pragma unroll
@@ -357,7 +359,8 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
                 const int ix = e / U8, c8 = e - ix * U8;
                 const int y = y0 - 1 + iy, z = z0 - 1 + iz;
                 const bool in = u < U && y >= 0 && y < D && z >= 0 && z < D && ix >= 1 && ix <= D;
-                const int voff = in ? ((s0 + (z * D + y) * D + ix - 1) * Cin + c0 + 8 * c8) * 2 : kOffNone;
+                const int voff =
+                    in ? ((sb + ((z >> up) * Ds + (y >> up)) * Ds + ((ix - 1) >> up)) * Cin + c0 + 8 * c8) * 2 : kOffNone;
                 v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, 0, 0));
             }
 #pragma unroll
@@ -410,7 +413,7 @@ template <int CIN, int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                  uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
-                                                 int ZL)
+                                                 int ZL, int up)
 {
     constexpr int MT = 4, LV = CIN / 4, U8 = CIN / 8;
     constexpr int NU = zs_units_per_thread(CIN);
@@ -425,7 +428,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
     const int y0 = yb * Rb, zb = zsg * ZL;
     const int s0 = smp * D * D * D;
-    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, P * CIN * 2);
+    // up = 1: X is stored at side Ds = D/2 and read at the parent cell (nearest upsampling)
+    const int Ds = D >> up;
+    const int sb = smp * Ds * Ds * Ds;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> (3 * up)) * CIN * 2);
     const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * CIN * 2);
     int lo[MT];
 #pragma unroll
@@ -452,7 +458,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int ix = e / U8, c8 = e - ix * U8;
             const int y = y0 - 1 + row;
             const bool in = u < SU && z >= 0 && z < D && y >= 0 && y < D && ix >= 1 && ix <= D;
-            const int voff = in ? ((s0 + (z * D + y) * D + ix - 1) * CIN + 8 * c8) * 2 : kOffNone;
+            const int voff =
+                in ? ((sb + ((z >> up) * Ds + (y >> up)) * Ds + ((ix - 1) >> up)) * CIN + 8 * c8) * 2 : kOffNone;
             v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, 0, 0));
         }
     };
@@ -549,8 +556,10 @@ __global__ __launch_bounds__(256) void k_transpose_pc(const uint16_t *__restrict
 // A workgroup moves 128 positions x 16 channels: each thread loads 8 channels of
 // one position (16 bytes), and after the LDS transpose stores 8 positions of one
 // channel (16 bytes; 16 threads cover 256 consecutive bytes).
+// With up = 1 the input has side D/2 and position p (side D) reads its parent
+// cell: the copy of the nearest upsampling.
 __global__ __launch_bounds__(256) void k_transpose_b8(const uint16_t *__restrict__ in, uint16_t *__restrict__ out,
-                                                      int C)
+                                                      int C, int D, int up)
 {
     constexpr int kRow = 128 + 8;  // LDS row of one channel, padded (16-byte aligned rows)
     __shared__ __align__(16) uint16_t t[16 * kRow];
@@ -559,7 +568,13 @@ __global__ __launch_bounds__(256) void k_transpose_b8(const uint16_t *__restrict
     const int c0 = 16 * (int)blockIdx.y;
     {
         const int pos = tid >> 1, h = tid & 1;
-        const uint4 v = *reinterpret_cast<const uint4 *>(in + (size_t)(p0 + pos) * C + c0 + 8 * h);
+        long long q = p0 + pos;
+        if (up) {
+            const int x = (int)(q % D), y = (int)((q / D) % D), z = (int)((q / ((long long)D * D)) % D);
+            const long long smp = q / ((long long)D * D * D), Ds = D / 2;
+            q = ((smp * Ds + z / 2) * Ds + y / 2) * Ds + x / 2;
+        }
+        const uint4 v = *reinterpret_cast<const uint4 *>(in + (size_t)q * C + c0 + 8 * h);
         const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
 #pragma unroll
         for (int k = 0; k < 8; ++k) t[(8 * h + k) * kRow + pos] = e[k];
@@ -1137,8 +1152,10 @@ int gp5_nn_sync(void)
     return GP5_OK;
 }
 
-int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
-                  int B, int D, int Cin, int Cout, int mirror, int act)
+// up = 1: X has side D/2 and the convolution reads its nearest upsampling (only
+// the LDS-staged and z-sliding kernels do that)
+static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y,
+                      float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up)
 {
     if (!X || !Wk) return set_error(GP5_ERR_ARG, "NULL input or weights");
     if (skip && !Y) return set_error(GP5_ERR_ARG, "a skip input needs the Y output");
@@ -1200,7 +1217,7 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
             shm_set = shm;                                                                                       \
         }                                                                                                        \
         hipLaunchKernelGGL((k_conv_zs<C, K>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
-                           act, ZL);                                                                             \
+                           act, ZL, up);                                                                         \
     }
             GP5_CONV_ZS_CASE(16, 1) GP5_CONV_ZS_CASE(16, 2) GP5_CONV_ZS_CASE(32, 1)
 #undef GP5_CONV_ZS_CASE
@@ -1229,7 +1246,7 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
 #define GP5_CONV_LDS_CASE(C, K)                                                                                   \
     if (KCl == C && NBl == K)                                                                                     \
         hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cin, \
-                           Cout, mirror, act, Rb, PZ);
+                           Cout, mirror, act, Rb, PZ, up);
             GP5_CONV_LDS_CASE(16, 1) GP5_CONV_LDS_CASE(16, 2) GP5_CONV_LDS_CASE(16, 4)
             GP5_CONV_LDS_CASE(32, 1) GP5_CONV_LDS_CASE(32, 2) GP5_CONV_LDS_CASE(32, 4)
 #undef GP5_CONV_LDS_CASE
@@ -1237,6 +1254,7 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
             return GP5_OK;
         }
     }
+    if (up) return set_error(GP5_ERR_ARG, "conv3d_up: the cube side must be a power of two in [8, 128]");
     {
         const unsigned gy = (unsigned)(nblk / NB);
         // channels per chunk: the widest of 64, 32, 16 that divides Cin
@@ -1257,6 +1275,18 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
     return GP5_OK;
 }
 
+int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
+                  int B, int D, int Cin, int Cout, int mirror, int act)
+{
+    return conv3d_run(X, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 0);
+}
+
+int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
+                     int B, int D, int Cin, int Cout, int mirror, int act)
+{
+    return conv3d_run(Xh, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 1);
+}
+
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
 {
     const long long P = (long long)B * D * D * D;
@@ -1266,9 +1296,12 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
     return (size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * 27 * Cin * Cout * sizeof(float);
 }
 
-int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+// up = 1: X has side D/2 and the gradient is that of a convolution of its
+// nearest upsampling (cube sides that are a multiple of 8)
+static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout, int up)
 {
     if (!X || !dY || !dW || !ws) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (up && D % 8) return set_error(GP5_ERR_ARG, "conv3d_wgrad_up: the cube side must be a multiple of 8");
     int rc = check_channels(Cin, "Cin");
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
@@ -1282,10 +1315,10 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     if (D % 4 == 0) {
         if (D % 8 == 0) {  // P = B D^3 is a multiple of 512
             hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
-                               (const uint16_t *)X, XT, Cin);
+                               (const uint16_t *)X, XT, Cin, D, up);
             NNCHK(hipGetLastError(), "wgrad transpose");
             hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
-                               (const uint16_t *)dY, dYT, Cout);
+                               (const uint16_t *)dY, dYT, Cout, D, 0);
         } else {
             const unsigned tb = (unsigned)((P + 63) / 64);
             hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
@@ -1310,6 +1343,16 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, part, dW, n);
     NNCHK(hipGetLastError(), "wgrad reduce launch");
     return GP5_OK;
+}
+
+int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+{
+    return wgrad_run(X, dY, dW, ws, B, D, Cin, Cout, 0);
+}
+
+int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+{
+    return wgrad_run(Xh, dY, dW, ws, B, D, Cin, Cout, 1);
 }
 
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)

@@ -28,6 +28,12 @@ SHAPES = [("c1", 16, 16), ("c2", 16, 32), ("c3", 32, 64), ("c4", 64, 128),
 C0 = 16  # input feature channels
 
 
+def _up_fused(d: int) -> bool:
+    """Whether the convolutions of side d read their upsampled input in place
+    (gp5_nn_conv3d_up / gp5_nn_conv3d_wgrad_up: a power of two in [8, 128])."""
+    return 8 <= d <= 128 and d & (d - 1) == 0
+
+
 def _lib():
     L = capi.load()
     if getattr(L, "_gp5_nn_sigs", False):
@@ -44,6 +50,8 @@ def _lib():
         "gp5_nn_pool2_f32": [P, P, I, I, I, F], "gp5_nn_up2_f32": [P, P, I, I, I, F],
         "gp5_nn_act_bwd": [P, P, P, LL], "gp5_nn_add_f32": [P, P, LL], "gp5_nn_cast_bf16": [P, P, LL],
         "gp5_nn_act_bwd_cast": [P, P, P, P, LL], "gp5_nn_up2_add_f32": [P, P, I, I, I, F],
+        "gp5_nn_conv3d_up": [P, P, P, P, P, P, P, I, I, I, I, I, I],
+        "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -168,6 +176,9 @@ class UNet:
             "gu32": (n(D, 32), 4), "gu64": (n(D // 2, 64), 4), "gu128": (n(D // 4, 128), 4),
             "gp16": (n(D // 2, 16), 4), "gp32": (n(D // 4, 32), 4), "gp64": (n(D // 8, 64), 4),
         }
+        for name, d in (("u4", D // 4), ("u3", D // 2), ("u2", D)):
+            if _up_fused(d):  # read in place from the half-side tensor
+                del sizes[name]
         self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
         L = _lib()
         ws = max(L.gp5_nn_wgrad_ws(B, d, ci, co) for d, ci, co in
@@ -221,11 +232,21 @@ class UNet:
         if target is not None:
             self.t["target"].upload(np.asarray(target, dtype=np.float32))
 
-    def _conv(self, name, x, A=None, skip=None, Y=None, Yf=None, D=None, act=1):
+    def _conv(self, name, x, A=None, skip=None, Y=None, Yf=None, D=None, act=1, up=False):
         e = self.w[name]
         P = lambda b: b.ptr if b is not None else None  # noqa: E731
-        _ck(_lib().gp5_nn_conv3d(x.ptr, e["Wf"].ptr, e["b"].ptr, P(A), P(skip), P(Y), P(Yf), self.B, D,
-                                 e["ci"], e["co"], 0, act))
+        f = _lib().gp5_nn_conv3d_up if up else _lib().gp5_nn_conv3d
+        _ck(f(x.ptr, e["Wf"].ptr, e["b"].ptr, P(A), P(skip), P(Y), P(Yf), self.B, D, e["ci"], e["co"], 0, act))
+
+    def _conv_up(self, name, src, tmp, D, **kw):
+        """The convolution of the nearest upsampling of src (side D/2) at side D:
+        read in place where the kernels support the side, else formed in tmp."""
+        if _up_fused(D):
+            self._conv(name, self.t[src], D=D, up=True, **kw)
+        else:
+            e = self.w[name]
+            _ck(_lib().gp5_nn_up2_bf16(self.t[src].ptr, self.t[tmp].ptr, self.B, D // 2, e["ci"], 1.0))
+            self._conv(name, self.t[tmp], D=D, **kw)
 
     def forward(self) -> None:
         L, t, B, D = _lib(), self.t, self.B, self.D
@@ -236,12 +257,9 @@ class UNet:
         self._conv("c3", t["p2"], A=t["e3"], D=D // 4)
         _ck(L.gp5_nn_pool2_bf16(t["e3"].ptr, t["p3"].ptr, B, D // 4, 64, 1.0))
         self._conv("c4", t["p3"], A=t["e4"], D=D // 8)
-        _ck(L.gp5_nn_up2_bf16(t["e4"].ptr, t["u4"].ptr, B, D // 8, 128, 1.0))
-        self._conv("c5", t["u4"], A=t["a5"], skip=t["e3"], Y=t["d3"], D=D // 4)
-        _ck(L.gp5_nn_up2_bf16(t["d3"].ptr, t["u3"].ptr, B, D // 4, 64, 1.0))
-        self._conv("c6", t["u3"], A=t["a6"], skip=t["e2"], Y=t["d2"], D=D // 2)
-        _ck(L.gp5_nn_up2_bf16(t["d2"].ptr, t["u2"].ptr, B, D // 2, 32, 1.0))
-        self._conv("c7", t["u2"], A=t["a7"], skip=t["e1"], Y=t["d1"], D=D)
+        self._conv_up("c5", "e4", "u4", D // 4, A=t["a5"], skip=t["e3"], Y=t["d3"])
+        self._conv_up("c6", "d3", "u3", D // 2, A=t["a6"], skip=t["e2"], Y=t["d2"])
+        self._conv_up("c7", "d2", "u2", D, A=t["a7"], skip=t["e1"], Y=t["d1"])
         self._conv("c8", t["d1"], Yf=t["out"], D=D, act=0)
 
     def set_droplet_features(self, s: float, a, consts, ub: float):
@@ -271,6 +289,15 @@ class UNet:
         e = self.w[name]
         _ck(_lib().gp5_nn_conv3d_wgrad(x.ptr, dz_bf.ptr, e["Wg"].ptr, self.ws.ptr, self.B, D, e["ci"], e["co"]))
 
+    def _wgrad_up(self, name, src, tmp, dz_bf, D):
+        """Weight gradient of _conv_up: the input is the upsampling of src."""
+        if _up_fused(D):
+            e = self.w[name]
+            _ck(_lib().gp5_nn_conv3d_wgrad_up(self.t[src].ptr, dz_bf.ptr, e["Wg"].ptr, self.ws.ptr, self.B, D,
+                                              e["ci"], e["co"]))
+        else:
+            self._wgrad(name, self.t[tmp], dz_bf, D)
+
     def _bgrad(self, name, dz, D):
         e = self.w[name]
         _ck(_lib().gp5_nn_bias_grad(dz.ptr, e["bg"].ptr, self.B * D ** 3, e["co"]))
@@ -295,19 +322,19 @@ class UNet:
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
         # c7: dz7 = dd1 * act'(a7)
         _ck(L.gp5_nn_act_bwd_cast(t["a7"].ptr, t["g16a"].ptr, t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
-        self._wgrad("c7", t["u2"], t["h16"], D)
+        self._wgrad_up("c7", "d2", "u2", t["h16"], D)
         self._bgrad("c7", t["g16b"], D)
         self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
         _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
         # c6
         _ck(L.gp5_nn_act_bwd_cast(t["a6"].ptr, t["g32a"].ptr, t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
-        self._wgrad("c6", t["u3"], t["h32"], D // 2)
+        self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2)
         self._bgrad("c6", t["g32b"], D // 2)
         self._dgrad("c6", t["h32"], t["gu64"], D // 2)
         _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
         # c5
         _ck(L.gp5_nn_act_bwd_cast(t["a5"].ptr, t["g64a"].ptr, t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
-        self._wgrad("c5", t["u4"], t["h64"], D // 4)
+        self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4)
         self._bgrad("c5", t["g64b"], D // 4)
         self._dgrad("c5", t["h64"], t["gu128"], D // 4)
         _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4

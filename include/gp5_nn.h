@@ -39,6 +39,10 @@ int gp5_nn_sync(void);
  * Yf float = act(...) (for the last layer and for gradients). */
 int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                   int B, int D, int Cin, int Cout, int mirror, int act);
+/* The same convolution of the nearest upsampling of Xh (side D/2, Cin channels)
+ * without forming it: output side D, a power of two in [8, 128]. */
+int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
+                     int B, int D, int Cin, int Cout, int mirror, int act);
 /* dW[tap][ci][co] = sum over positions of X[p + off(tap)][ci] dY[p][co]
  * (bf16 inputs, float result); ws: device workspace of at least
  * gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes (channel-major copies of X and dY, in
@@ -46,6 +50,9 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
  * sums). */
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout);
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
+/* The weight gradient of gp5_nn_conv3d_up: X is Xh (side D/2), read upsampled;
+ * D a multiple of 8; the same workspace size. */
+int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
 /* db[c] = sum over positions of dY[p][c] (float). */
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C);
 

@@ -232,3 +232,37 @@ def test_conv_z_sliding_path(D, Cin, Cout, B, monkeypatch):
     assert np.max(np.abs(bwd[0] - dref)) <= 1e-4 * np.max(np.abs(dref))
     if Cout in (16, 32):  # the mirrored call has Cout input channels: the same kernels and order
         assert np.array_equal(bwd[0], bwd[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 32, 16, 1), (32, 64, 32, 1), (16, 128, 64, 2), (8, 64, 32, 2)])
+def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B):
+    """gp5_nn_conv3d_up / gp5_nn_conv3d_wgrad_up on a cube of side D/2 give bit for
+    bit what gp5_nn_conv3d / gp5_nn_conv3d_wgrad give on its nearest upsampling."""
+    rng = np.random.default_rng(D * 7 + Cin)
+    Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
+    U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    L = nn._lib()
+    xh, u, gb = _buf(Xh, True), _buf(U, True), _buf(G, True)
+    wf, bb = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(b)
+    shape = (B, D, D, D, Cout)
+    outs = []
+    for f, x in ((L.gp5_nn_conv3d_up, xh), (L.gp5_nn_conv3d, u)):
+        Yf = nn.DevBuf(B * D ** 3 * Cout * 4)
+        nn._ck(f(x.ptr, wf.ptr, bb.ptr, None, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        outs.append(Yf.download(np.float32, shape))
+    assert np.array_equal(outs[0], outs[1])
+    ref = R.lrelu(R.conv(U, W, b.astype(np.float64)))
+    assert np.max(np.abs(outs[0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+    ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+    grads = []
+    for f, x in ((L.gp5_nn_conv3d_wgrad_up, xh), (L.gp5_nn_conv3d_wgrad, u)):
+        dW = nn.DevBuf(27 * Cin * Cout * 4)
+        nn._ck(f(x.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+        grads.append(dW.download(np.float32, (27, Cin, Cout)))
+    assert np.array_equal(grads[0], grads[1])
+    gref = R.conv_wgrad(U, G)
+    assert np.max(np.abs(grads[0] - gref)) <= 1e-4 * np.max(np.abs(gref))
