@@ -700,43 +700,60 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ XT, c
 // and reads only the 3 input rows of that plane, which triples the waves per
 // chunk and cuts the accumulators per wave to 9.
 __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, const uint16_t *__restrict__ dYT,
-                                               float *ws, long long P, int D, int Cin, int Cout, long long chunk)
+                                               float *ws, int P, int D, int Cin, int Cout, int chunk)
 {
     const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
     const int nco = Cout / 16;
     const int cob = blockIdx.x % nco, cib = blockIdx.x / nco;
-    const long long ck = blockIdx.y;
+    const int ck = blockIdx.y;
     const int dz = (int)blockIdx.z - 1;
-    const long long p0 = ck * chunk, p1 = min(P, p0 + chunk);
+    const int p0 = ck * chunk, p1 = min(P, p0 + chunk);
     const int ci = cib * 16 + r, co = cob * 16 + r;
-    // blocked layout: positions q .. q+7 (q a multiple of 8) of channel c at ((q / 8) C + c) 8
-    const uint16_t *xb = XT + (size_t)ci * 8;
-    const uint16_t *yb = dYT + (size_t)co * 8;
-    const size_t xs = (size_t)Cin * 8;  // elements per 8-position block
-    const bf16x4 zero4 = bf16x4{0, 0, 0, 0};
+    // blocked layout: positions q .. q+7 (q a multiple of 8) of channel c at bytes ((q / 8) C + c) 16;
+    // loads out of the cube get kOffNone and read zero
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(XT, P * Cin * 2);
+    const __amdgpu_buffer_rsrc_t yr = make_rsrc(dYT, P * Cout * 2);
     typedef short bf16x8 __attribute__((ext_vector_type(8)));
+    struct Step {
+        bf16x8 c[3];
+        bf16x4 pv[3], nx[3];
+        bf16x8 b;
+    };
+    auto fetch = [&](int pb, Step &st) {
+        const int pg = pb + 8 * g;
+        const bool in = pg < p1;  // p1 is a multiple of 32 or P: groups of 8 are wholly in or out
+        const unsigned q1 = (unsigned)pg / (unsigned)D, q2 = q1 / (unsigned)D;
+        const int x = pg - (int)q1 * D, y = (int)(q1 - q2 * D), z = (int)(q2 % (unsigned)D);
+        st.b = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                              yr, in ? ((pg / 8) * Cout + co) * 16 : kOffNone, 0, 0));
+        const bool plane_ok = in && z + dz >= 0 && z + dz < D;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int dy = k - 1;
+            const bool row_ok = plane_ok && y + dy >= 0 && y + dy < D;
+            const int qb = (pg + (dz * D + dy) * D) / 8;  // 8-position block of the row piece
+            const int co0 = (qb * Cin + ci) * 16;
+            st.c[k] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, row_ok ? co0 : kOffNone, 0, 0));
+            st.pv[k] = __builtin_bit_cast(
+                bf16x4, __builtin_amdgcn_raw_buffer_load_b64(xr, row_ok && x > 0 ? co0 - Cin * 16 + 8 : kOffNone, 0, 0));
+            st.nx[k] = __builtin_bit_cast(
+                bf16x4, __builtin_amdgcn_raw_buffer_load_b64(xr, row_ok && x + 8 < D ? co0 + Cin * 16 : kOffNone, 0, 0));
+        }
+    };
     f32x4 acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (long long pb = p0; pb < p1; pb += 32) {
-        const long long pg = pb + 8 * g;
-        if (pg >= p1) continue;  // p1 is a multiple of 32 or P: groups of 8 are wholly in or out
-        const int x = (int)(pg % D), y = (int)((pg / D) % D), z = (int)((pg / ((long long)D * D)) % D);
-        const bf16x8 b8 = *reinterpret_cast<const bf16x8 *>(yb + (size_t)(pg / 8) * Cout * 8);
-        const bf16x4 b1 = bf16x4{b8[0], b8[1], b8[2], b8[3]}, b2 = bf16x4{b8[4], b8[5], b8[6], b8[7]};
-        const bool plane_ok = z + dz >= 0 && z + dz < D;
+    Step cur, nxt;
+    fetch(p0, cur);
+    for (int pb = p0; pb < p1; pb += 32) {
+        fetch(pb + 32, nxt);  // in flight during this step's MFMAs (past p1: out of range, zeros)
+        const bf16x4 b1 = bf16x4{cur.b[0], cur.b[1], cur.b[2], cur.b[3]};
+        const bf16x4 b2 = bf16x4{cur.b[4], cur.b[5], cur.b[6], cur.b[7]};
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int t0 = (dy + 1) * 3;  // tap of dx = -1 within this dz
-            const bool row_ok = plane_ok && y + dy >= 0 && y + dy < D;
-            bf16x8 c = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-            bf16x4 pv = zero4, nx = zero4;
-            if (row_ok) {
-                const uint16_t *qb = xb + (size_t)((pg + ((long long)dz * D + dy) * D) / 8) * xs;
-                c = *reinterpret_cast<const bf16x8 *>(qb);
-                if (x > 0) pv = ld4(qb - xs + 4);
-                if (x + 8 < D) nx = ld4(qb + xs);
-            }
+        for (int k = 0; k < 3; ++k) {
+            const int t0 = 3 * k;  // tap of dx = -1 within this dz, dy = k - 1
+            const bf16x8 c = cur.c[k];
+            const bf16x4 pv = cur.pv[k], nx = cur.nx[k];
             acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{pv[3], c[0], c[1], c[2]}, b1, acc[t0], 0, 0, 0);
             acc[t0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[3], c[4], c[5], c[6]}, b2, acc[t0], 0, 0, 0);
             acc[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[0], c[1], c[2], c[3]}, b1, acc[t0 + 1], 0, 0, 0);
@@ -744,6 +761,7 @@ __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, 
             acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[1], c[2], c[3], c[4]}, b1, acc[t0 + 2], 0, 0, 0);
             acc[t0 + 2] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bf16x4{c[5], c[6], c[7], nx[0]}, b2, acc[t0 + 2], 0, 0, 0);
         }
+        cur = nxt;
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -1302,6 +1320,8 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
 {
     if (!X || !dY || !dW || !ws) return set_error(GP5_ERR_ARG, "NULL argument");
     if (up && D % 8) return set_error(GP5_ERR_ARG, "conv3d_wgrad_up: the cube side must be a multiple of 8");
+    if ((long long)B * D * D * D * std::max(Cin, Cout) * 2 > 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "conv3d_wgrad: tensors must stay below 2 GiB (B*D^3*C elements)");
     int rc = check_channels(Cin, "Cin");
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
@@ -1326,8 +1346,9 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
             hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
         }
         NNCHK(hipGetLastError(), "wgrad transpose");
-        if (D % 8 == 0)
-            hipLaunchKernelGGL(k_wgrad8, dim3(grid.x, grid.y, 3), dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+        if (D % 8 == 0)  // 32-bit offsets: P * C * 2 < 2^31 (checked above)
+            hipLaunchKernelGGL(k_wgrad8, dim3(grid.x, grid.y, 3), dim3(64), 0, 0, XT, dYT, part0, (int)P, D, Cin, Cout,
+                               (int)chunk);
         else
             hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
     } else {
