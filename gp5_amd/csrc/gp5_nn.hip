@@ -698,7 +698,10 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t *__restrict__ XT, c
 // two steps) with the 4 positions on either side from two 8-byte loads.  The
 // taps are split by dz over blockIdx.z: each wave holds the 9 taps of one dz
 // and reads only the 3 input rows of that plane, which triples the waves per
-// chunk and cuts the accumulators per wave to 9.
+// chunk and cuts the accumulators per wave to 9.  The loads are buffer loads
+// without branches (a piece outside the cube reads zero through the range
+// check), and the 10 loads of the next 32-position step are in flight while
+// the 18 MFMAs of the current one run.
 __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, const uint16_t *__restrict__ dYT,
                                                float *ws, int P, int D, int Cin, int Cout, int chunk)
 {
