@@ -1038,8 +1038,9 @@ __global__ void k_fill0(uint32_t *p, long long n)
         p[i] = 0u;
 }
 
+// H (optional) receives the bf16 copy of the dYf rows
 __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float *base, const float *target, float *dYf,
-                                             long long P, double *partial)
+                                             uint16_t *H, long long P, double *partial)
 {
     __shared__ double s[256];
     double a = 0.0;
@@ -1055,6 +1056,11 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float
         } else {
             dYf[(size_t)p * C] = inv * e;
             for (int c = 1; c < C; ++c) dYf[(size_t)p * C + c] = 0.f;
+        }
+        if (H) {  // C % 8 == 0 (checked on the host): 16-byte stores
+            uint4 *hrow = reinterpret_cast<uint4 *>(H + (size_t)p * C);
+            hrow[0] = make_uint4((uint32_t)f2bf(inv * e), 0u, 0u, 0u);
+            for (int c = 1; c < C / 8; ++c) hrow[c] = make_uint4(0u, 0u, 0u, 0u);
         }
     }
     s[threadIdx.x] = a;
@@ -1510,13 +1516,15 @@ int gp5_nn_cast_bf16(const float *src, void *dst, long long n)
     return GP5_OK;
 }
 
-int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P, double *loss)
+static int mse_run(const float *Yf, int C, const float *base, const float *target, float *dYf, uint16_t *H, long long P,
+                   double *loss)
 {
     if (!Yf || !base || !target || !dYf || !loss) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (H && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
     const unsigned blocks = (unsigned)std::min<long long>((P + 255) / 256, 1024);
     double *part = nullptr;
     NNCHK(hipMalloc(&part, blocks * sizeof(double)), "mse scratch");
-    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, C, base, target, dYf, P, part);
+    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, C, base, target, dYf, H, P, part);
     hipError_t e = hipGetLastError();
     std::vector<double> h(blocks);
     if (e == hipSuccess) e = hipMemcpy(h.data(), part, blocks * sizeof(double), hipMemcpyDeviceToHost);
@@ -1526,6 +1534,18 @@ int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, f
     for (double v : h) s += v;
     *loss = s / (double)P;
     return GP5_OK;
+}
+
+int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P, double *loss)
+{
+    return mse_run(Yf, C, base, target, dYf, nullptr, P, loss);
+}
+
+int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, long long P,
+                    double *loss)
+{
+    if (!H) return set_error(GP5_ERR_ARG, "NULL argument");
+    return mse_run(Yf, C, base, target, dYf, (uint16_t *)H, P, loss);
 }
 
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps, int t)

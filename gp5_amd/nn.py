@@ -52,6 +52,7 @@ def _lib():
         "gp5_nn_act_bwd_cast": [P, P, P, P, LL], "gp5_nn_up2_add_f32": [P, P, I, I, I, F],
         "gp5_nn_conv3d_up": [P, P, P, P, P, P, P, I, I, I, I, I, I],
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
+        "gp5_nn_mse_bf16": [P, I, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -313,10 +314,9 @@ class UNet:
         L, t, B, D = _lib(), self.t, self.B, self.D
         n = lambda d, c: B * d ** 3 * c  # noqa: E731
         loss = ctypes.c_double()
-        _ck(L.gp5_nn_mse(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, t["g_out"].ptr, B * D ** 3,
-                         ctypes.byref(loss)))
+        _ck(L.gp5_nn_mse_bf16(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, t["g_out"].ptr, t["h_out"].ptr,
+                              B * D ** 3, ctypes.byref(loss)))
         # c8
-        _ck(L.gp5_nn_cast_bf16(t["g_out"].ptr, t["h_out"].ptr, n(D, 16)))
         self._wgrad("c8", t["d1"], t["h_out"], D)
         self._bgrad("c8", t["g_out"], D)
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
