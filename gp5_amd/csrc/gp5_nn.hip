@@ -822,6 +822,48 @@ __global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ 
     }
 }
 
+// Backward of the leaky ReLU with the bias gradient folded in: dP = dA act'(A)
+// goes out as its bf16 copy H, and each of the kBiasBlocks blocks sums dP over a
+// contiguous range of positions per channel (threads take 4 channels of one
+// position; C / 4 divides 256), for k_bias_final.
+__global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__restrict__ A, const float *__restrict__ dA,
+                                                           uint16_t *__restrict__ H, float *part, long long P, int C)
+{
+    __shared__ float4 s[256];
+    const int C4 = C / 4, per_pos = 256 / C4;
+    const int c4 = threadIdx.x % C4, sub = threadIdx.x / C4;
+    const long long per = (P + gridDim.x - 1) / gridDim.x;
+    const long long p0 = (long long)blockIdx.x * per, p1 = min(P, p0 + per);
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (long long p = p0 + sub; p < p1; p += per_pos) {
+        const long long i = p * C4 + c4;
+        const uint2 av = reinterpret_cast<const uint2 *>(A)[i];
+        const float4 d = reinterpret_cast<const float4 *>(dA)[i];
+        const uint16_t *ah = reinterpret_cast<const uint16_t *>(&av);
+        const float o[4] = {d.x * (bf2f(ah[0]) > 0.f ? 1.f : kLeak), d.y * (bf2f(ah[1]) > 0.f ? 1.f : kLeak),
+                            d.z * (bf2f(ah[2]) > 0.f ? 1.f : kLeak), d.w * (bf2f(ah[3]) > 0.f ? 1.f : kLeak)};
+        uint16_t h[4] = {f2bf(o[0]), f2bf(o[1]), f2bf(o[2]), f2bf(o[3])};
+        reinterpret_cast<uint2 *>(H)[i] = __builtin_bit_cast(uint2, h);
+        a.x += o[0];
+        a.y += o[1];
+        a.z += o[2];
+        a.w += o[3];
+    }
+    s[threadIdx.x] = a;
+    __syncthreads();
+    if ((int)threadIdx.x < C4) {
+        float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < per_pos; ++k) {
+            const float4 v = s[threadIdx.x + k * C4];
+            t.x += v.x;
+            t.y += v.y;
+            t.z += v.z;
+            t.w += v.w;
+        }
+        *reinterpret_cast<float4 *>(part + (size_t)blockIdx.x * C + 4 * threadIdx.x) = t;
+    }
+}
+
 // one block per channel: each thread sums a strided subset of the block partials,
 // then a fixed tree over the threads
 __global__ __launch_bounds__(256) void k_bias_final(const float *part, float *db, int nblk, int C)
@@ -1385,25 +1427,32 @@ int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws,
     return wgrad_run(Xh, dY, dW, ws, B, D, Cin, Cout, 1);
 }
 
-int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)
+// Block partials of the bias gradients in a process-wide scratch that only grows
+// (the kernels of successive calls run in order on the null stream, so they can
+// share it).
+static int bias_scratch(size_t need, float **out)
 {
-    if (!dY || !db || C < 1 || C > 256 || 256 % C) return set_error(GP5_ERR_ARG, "bias grad: C must divide 256");
-    // block partials in a process-wide scratch that only grows (the kernels of
-    // successive calls run in order on the null stream, so they can share it)
     static std::mutex mu;
     static float *part = nullptr;
     static size_t cap = 0;
-    const size_t need = (size_t)kBiasBlocks * C;
-    {
-        std::lock_guard<std::mutex> lock(mu);
-        if (cap < need) {
-            if (part) NNCHK(hipFree(part), "bias grad scratch free");
-            part = nullptr;
-            cap = 0;
-            NNCHK(hipMalloc(&part, need * sizeof(float)), "bias grad scratch");
-            cap = need;
-        }
+    std::lock_guard<std::mutex> lock(mu);
+    if (cap < need) {
+        if (part) NNCHK(hipFree(part), "bias grad scratch free");
+        part = nullptr;
+        cap = 0;
+        NNCHK(hipMalloc(&part, need * sizeof(float)), "bias grad scratch");
+        cap = need;
     }
+    *out = part;
+    return GP5_OK;
+}
+
+int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)
+{
+    if (!dY || !db || C < 1 || C > 256 || 256 % C) return set_error(GP5_ERR_ARG, "bias grad: C must divide 256");
+    float *part = nullptr;
+    const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
+    if (rc != GP5_OK) return rc;
     hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, 0, dY, part, P, C);
     NNCHK(hipGetLastError(), "bias grad launch");
     hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
@@ -1463,6 +1512,21 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
     hipLaunchKernelGGL(k_act_bwd_cast4, dim3(grid_for(n / 4)), dim3(256), 0, 0, (const uint16_t *)A, dA, dPre,
                        (uint16_t *)H, n / 4);
     NNCHK(hipGetLastError(), "act_bwd_cast");
+    return GP5_OK;
+}
+
+int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db, long long P, int C)
+{
+    if (!A || !dA || !H || !db) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (C < 4 || C % 4 || 256 % (C / 4)) return set_error(GP5_ERR_ARG, "act_bwd_cast_bias: C / 4 must divide 256");
+    float *part = nullptr;
+    const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
+    if (rc != GP5_OK) return rc;
+    hipLaunchKernelGGL(k_act_bwd_cast_bias, dim3(kBiasBlocks), dim3(256), 0, 0, (const uint16_t *)A, dA, (uint16_t *)H,
+                       part, P, C);
+    NNCHK(hipGetLastError(), "act_bwd_cast_bias");
+    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
+    NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     return GP5_OK;
 }
 

@@ -53,6 +53,7 @@ def _lib():
         "gp5_nn_conv3d_up": [P, P, P, P, P, P, P, I, I, I, I, I, I],
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
         "gp5_nn_mse_bf16": [P, I, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -170,9 +171,9 @@ class UNet:
             "out": (n(D, 16), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
             "g_out": (n(D, 16), 4), "h_out": (n(D, 16), 2),
-            "g16a": (n(D, 16), 4), "g16b": (n(D, 16), 4), "h16": (n(D, 16), 2),
-            "g32a": (n(D // 2, 32), 4), "g32b": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
-            "g64a": (n(D // 4, 64), 4), "g64b": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
+            "g16a": (n(D, 16), 4), "h16": (n(D, 16), 2),
+            "g32a": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
+            "g64a": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
             "g128": (n(D // 8, 128), 4), "h128": (n(D // 8, 128), 2),
             "gu32": (n(D, 32), 4), "gu64": (n(D // 2, 64), 4), "gu128": (n(D // 4, 128), 4),
             "gp16": (n(D // 2, 16), 4), "gp32": (n(D // 4, 32), 4), "gp64": (n(D // 8, 64), 4),
@@ -299,6 +300,14 @@ class UNet:
         else:
             self._wgrad(name, self.t[tmp], dz_bf, D)
 
+    def _act_bwd(self, name, a, g, h, D):
+        """Backward of layer name's leaky ReLU from its output t[a] and the incoming
+        gradient t[g] (float): t[h] = bf16 gradient of the pre-activation, and the
+        layer's bias gradient, in one pass."""
+        e = self.w[name]
+        _ck(_lib().gp5_nn_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[h].ptr, e["bg"].ptr,
+                                            self.B * D ** 3, e["co"]))
+
     def _bgrad(self, name, dz, D):
         e = self.w[name]
         _ck(_lib().gp5_nn_bias_grad(dz.ptr, e["bg"].ptr, self.B * D ** 3, e["co"]))
@@ -321,45 +330,38 @@ class UNet:
         self._bgrad("c8", t["g_out"], D)
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
         # c7: dz7 = dd1 * act'(a7)
-        _ck(L.gp5_nn_act_bwd_cast(t["a7"].ptr, t["g16a"].ptr, t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        self._act_bwd("c7", "a7", "g16a", "h16", D)
         self._wgrad_up("c7", "d2", "u2", t["h16"], D)
-        self._bgrad("c7", t["g16b"], D)
         self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
         _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
         # c6
-        _ck(L.gp5_nn_act_bwd_cast(t["a6"].ptr, t["g32a"].ptr, t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        self._act_bwd("c6", "a6", "g32a", "h32", D // 2)
         self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2)
-        self._bgrad("c6", t["g32b"], D // 2)
         self._dgrad("c6", t["h32"], t["gu64"], D // 2)
         _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
         # c5
-        _ck(L.gp5_nn_act_bwd_cast(t["a5"].ptr, t["g64a"].ptr, t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        self._act_bwd("c5", "a5", "g64a", "h64", D // 4)
         self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4)
-        self._bgrad("c5", t["g64b"], D // 4)
         self._dgrad("c5", t["h64"], t["gu128"], D // 4)
         _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4
         # c4
-        _ck(L.gp5_nn_act_bwd_cast(t["e4"].ptr, t["g128"].ptr, t["g128"].ptr, t["h128"].ptr, n(D // 8, 128)))
+        self._act_bwd("c4", "e4", "g128", "h128", D // 8)
         self._wgrad("c4", t["p3"], t["h128"], D // 8)
-        self._bgrad("c4", t["g128"], D // 8)
         self._dgrad("c4", t["h128"], t["gp64"], D // 8)
         _ck(L.gp5_nn_up2_add_f32(t["gp64"].ptr, t["g64a"].ptr, B, D // 8, 64, 0.125))  # de3 total
         # c3
-        _ck(L.gp5_nn_act_bwd_cast(t["e3"].ptr, t["g64a"].ptr, t["g64b"].ptr, t["h64"].ptr, n(D // 4, 64)))
+        self._act_bwd("c3", "e3", "g64a", "h64", D // 4)
         self._wgrad("c3", t["p2"], t["h64"], D // 4)
-        self._bgrad("c3", t["g64b"], D // 4)
         self._dgrad("c3", t["h64"], t["gp32"], D // 4)
         _ck(L.gp5_nn_up2_add_f32(t["gp32"].ptr, t["g32a"].ptr, B, D // 4, 32, 0.125))  # de2 total
         # c2
-        _ck(L.gp5_nn_act_bwd_cast(t["e2"].ptr, t["g32a"].ptr, t["g32b"].ptr, t["h32"].ptr, n(D // 2, 32)))
+        self._act_bwd("c2", "e2", "g32a", "h32", D // 2)
         self._wgrad("c2", t["p1"], t["h32"], D // 2)
-        self._bgrad("c2", t["g32b"], D // 2)
         self._dgrad("c2", t["h32"], t["gp16"], D // 2)
         _ck(L.gp5_nn_up2_add_f32(t["gp16"].ptr, t["g16a"].ptr, B, D // 2, 16, 0.125))  # de1 total
         # c1
-        _ck(L.gp5_nn_act_bwd_cast(t["e1"].ptr, t["g16a"].ptr, t["g16b"].ptr, t["h16"].ptr, n(D, 16)))
+        self._act_bwd("c1", "e1", "g16a", "h16", D)
         self._wgrad("c1", t["X"], t["h16"], D)
-        self._bgrad("c1", t["g16b"], D)
         return float(loss.value)
 
     def step(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:

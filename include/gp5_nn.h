@@ -67,6 +67,9 @@ int gp5_nn_add_f32(float *dst, const float *src, long long n);
 /* dPre = dA act'(A) (float) and H = bf16(dPre) in one pass; n a multiple of 4;
  * dPre may be dA. */
 int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, long long n);
+/* H = bf16(dA act'(A)) for P positions of C channels (C / 4 divides 256) and the
+ * bias gradient db[c] = sum over positions of dA act'(A) (float), in one pass. */
+int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db, long long P, int C);
 /* Y (side 2D) += scale * X at the parent cell (the adjoint of pooling added into
  * Y); C a multiple of 4. */
 int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale);
