@@ -519,11 +519,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // are first copied channel-major (XT[c][p], dYT[c][p]), so a lane's four
 // consecutive positions of one channel are one aligned 8-byte load; the x-shifted
 // taps come from the previous / current / next 4-position chunks of the same row.
+#ifndef GP5_WGRAD_TILES
+#define GP5_WGRAD_TILES 1024
+#endif
+constexpr long long kWgradTiles = GP5_WGRAD_TILES;
 __host__ __device__ inline long long wgrad_chunk(long long P, int Cin, int Cout)
 {
-    // about 2048 waves over the layer, at least 256 positions each, a multiple of 32
+    // about kWgradTiles (chunk, channel tile) pairs over the layer, at least 256
+    // positions each, a multiple of 32
     const long long tiles = (long long)(Cin / 16) * (Cout / 16);
-    long long c = (P * tiles + 2047) / 2048;
+    long long c = (P * tiles + kWgradTiles - 1) / kWgradTiles;
     c = c < 256 ? 256 : c;
     return (c + 31) / 32 * 32;
 }
