@@ -1085,22 +1085,25 @@ __global__ void k_fill0(uint32_t *p, long long n)
         p[i] = 0u;
 }
 
-// H (optional) receives the bf16 copy of the dYf rows
+// dYf and H (each optional) receive the gradient rows in float and bf16;
+// partial[b] and partial[gridDim.x + b] the block sums of e^2 and of the
+// channel-0 gradient 2 e / P (in double)
 __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float *base, const float *target, float *dYf,
                                              uint16_t *H, long long P, double *partial)
 {
-    __shared__ double s[256];
-    double a = 0.0;
+    __shared__ double s[256], sg[256];
+    double a = 0.0, ag = 0.0;
     const float inv = 2.0f / (float)P;
     for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < P; p += (long long)gridDim.x * 256) {
         const float y = base[p] + Yf[(size_t)p * C];
         const float e = y - target[p];
         a += (double)e * e;
-        if (C % 4 == 0) {  // the row as 16-byte stores (rows are 16-byte aligned)
+        ag += (double)(inv * e);
+        if (dYf && C % 4 == 0) {  // the row as 16-byte stores (rows are 16-byte aligned)
             float4 *row = reinterpret_cast<float4 *>(dYf + (size_t)p * C);
             row[0] = make_float4(inv * e, 0.f, 0.f, 0.f);
             for (int c = 1; c < C / 4; ++c) row[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
+        } else if (dYf) {
             dYf[(size_t)p * C] = inv * e;
             for (int c = 1; c < C; ++c) dYf[(size_t)p * C + c] = 0.f;
         }
@@ -1111,12 +1114,19 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float
         }
     }
     s[threadIdx.x] = a;
+    sg[threadIdx.x] = ag;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            s[threadIdx.x] += s[threadIdx.x + w];
+            sg[threadIdx.x] += sg[threadIdx.x + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s[0];
+        partial[gridDim.x + blockIdx.x] = sg[0];
+    }
 }
 
 __global__ void k_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps,
@@ -1585,36 +1595,49 @@ int gp5_nn_cast_bf16(const float *src, void *dst, long long n)
     return GP5_OK;
 }
 
-static int mse_run(const float *Yf, int C, const float *base, const float *target, float *dYf, uint16_t *H, long long P,
-                   double *loss)
+// db (optional, device, C floats): the bias gradient of the output layer,
+// sum over positions of the gradient rows: channel 0 from the double block sums,
+// 0 elsewhere
+static int mse_run(const float *Yf, int C, const float *base, const float *target, float *dYf, uint16_t *H, float *db,
+                   long long P, double *loss)
 {
-    if (!Yf || !base || !target || !dYf || !loss) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (!Yf || !base || !target || !loss) return set_error(GP5_ERR_ARG, "NULL argument");
     if (H && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
     const unsigned blocks = (unsigned)std::min<long long>((P + 255) / 256, 1024);
     double *part = nullptr;
-    NNCHK(hipMalloc(&part, blocks * sizeof(double)), "mse scratch");
+    NNCHK(hipMalloc(&part, 2 * blocks * sizeof(double)), "mse scratch");
     hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, C, base, target, dYf, H, P, part);
     hipError_t e = hipGetLastError();
-    std::vector<double> h(blocks);
-    if (e == hipSuccess) e = hipMemcpy(h.data(), part, blocks * sizeof(double), hipMemcpyDeviceToHost);
+    std::vector<double> h(2 * blocks);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), part, 2 * blocks * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(part);
     if (e != hipSuccess) return hip_fail(e, "mse");
-    double s = 0.0;
-    for (double v : h) s += v;
+    double s = 0.0, sg = 0.0;
+    for (unsigned b = 0; b < blocks; ++b) {
+        s += h[b];
+        sg += h[blocks + b];
+    }
     *loss = s / (double)P;
+    if (db) {
+        std::vector<float> g(C, 0.f);
+        g[0] = (float)sg;
+        e = hipMemcpy(db, g.data(), C * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "mse bias gradient");
+    }
     return GP5_OK;
 }
 
 int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P, double *loss)
 {
-    return mse_run(Yf, C, base, target, dYf, nullptr, P, loss);
+    if (!dYf) return set_error(GP5_ERR_ARG, "NULL argument");
+    return mse_run(Yf, C, base, target, dYf, nullptr, nullptr, P, loss);
 }
 
-int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, long long P,
-                    double *loss)
+int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, float *db,
+                    long long P, double *loss)
 {
     if (!H) return set_error(GP5_ERR_ARG, "NULL argument");
-    return mse_run(Yf, C, base, target, dYf, (uint16_t *)H, P, loss);
+    return mse_run(Yf, C, base, target, dYf, (uint16_t *)H, db, P, loss);
 }
 
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps, int t)

@@ -52,7 +52,7 @@ def _lib():
         "gp5_nn_act_bwd_cast": [P, P, P, P, LL], "gp5_nn_up2_add_f32": [P, P, I, I, I, F],
         "gp5_nn_conv3d_up": [P, P, P, P, P, P, P, I, I, I, I, I, I],
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
-        "gp5_nn_mse_bf16": [P, I, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_mse_bf16": [P, I, P, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
@@ -170,7 +170,7 @@ class UNet:
             "u2": (n(D, 32), 2), "a7": (n(D, 16), 2), "d1": (n(D, 16), 2),
             "out": (n(D, 16), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
-            "g_out": (n(D, 16), 4), "h_out": (n(D, 16), 2),
+            "h_out": (n(D, 16), 2),
             "g16a": (n(D, 16), 4), "h16": (n(D, 16), 2),
             "g32a": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
             "g64a": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
@@ -308,10 +308,6 @@ class UNet:
         _ck(_lib().gp5_nn_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[h].ptr, e["bg"].ptr,
                                             self.B * D ** 3, e["co"]))
 
-    def _bgrad(self, name, dz, D):
-        e = self.w[name]
-        _ck(_lib().gp5_nn_bias_grad(dz.ptr, e["bg"].ptr, self.B * D ** 3, e["co"]))
-
     def _dgrad(self, name, dz_bf, out_f, D):
         # data gradient: forward kernel on the [27][ci][co] copy with mirrored taps
         e = self.w[name]
@@ -323,11 +319,11 @@ class UNet:
         L, t, B, D = _lib(), self.t, self.B, self.D
         n = lambda d, c: B * d ** 3 * c  # noqa: E731
         loss = ctypes.c_double()
-        _ck(L.gp5_nn_mse_bf16(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, t["g_out"].ptr, t["h_out"].ptr,
-                              B * D ** 3, ctypes.byref(loss)))
+        # loss, the bf16 output gradient and c8's bias gradient in one pass
+        _ck(L.gp5_nn_mse_bf16(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, None, t["h_out"].ptr,
+                              self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss)))
         # c8
         self._wgrad("c8", t["d1"], t["h_out"], D)
-        self._bgrad("c8", t["g_out"], D)
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
         # c7: dz7 = dd1 * act'(a7)
         self._act_bwd("c7", "a7", "g16a", "h16", D)

@@ -78,9 +78,11 @@ int gp5_nn_cast_bf16(const float *src, void *dst, long long n);
  * dYf[p][c] = 2 (y - t) / P for c = 0 and 0 otherwise; *loss = mean. */
 int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P,
                double *loss);
-/* The same, also writing H = bf16(dYf) (C a multiple of 8). */
-int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, long long P,
-                    double *loss);
+/* The same, also writing H = bf16(dYf) (C a multiple of 8); dYf may be NULL (not
+ * written); db, if not NULL, receives the output layer's bias gradient, the sum
+ * over positions of the gradient rows (channel 0; the others are 0). */
+int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, float *db,
+                    long long P, double *loss);
 /* Droplet input channels (one cube of side D on the scaled box [-ub, ub]^3, C-ordered
  * [D][D][D][16] bf16): 0 = 0.5 (1 - tanh((r - 1) s)), 1 = a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
  * 2..6 = c[0..4], 7 = r / ub, 8..15 = 0; base (float, D^3) = channel 0.  float32

@@ -266,3 +266,26 @@ def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B):
     assert np.array_equal(grads[0], grads[1])
     gref = R.conv_wgrad(U, G)
     assert np.max(np.abs(grads[0] - gref)) <= 1e-4 * np.max(np.abs(gref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("C", [16, 64])
+def test_bias_gradient_entries(C):
+    """gp5_nn_bias_grad, and gp5_nn_act_bwd_cast_bias (the leaky-ReLU backward with
+    its bf16 copy and the bias gradient in one pass), against numpy."""
+    rng = np.random.default_rng(C)
+    P = 5000
+    dA = rng.standard_normal((P, C)).astype(np.float32)
+    A = nn.to_bf16(rng.standard_normal((P, C)).astype(np.float32))
+    L = nn._lib()
+    db = nn.DevBuf(C * 4)
+    nn._ck(L.gp5_nn_bias_grad(_buf(dA).ptr, db.ptr, P, C))
+    ref = dA.astype(np.float64).sum(axis=0)
+    assert np.allclose(db.download(np.float32, (C,)), ref, rtol=1e-5, atol=1e-4)
+    ab, gb = nn.DevBuf(A.nbytes), _buf(dA)
+    ab.upload(A)
+    H = nn.DevBuf(P * C * 2)
+    nn._ck(L.gp5_nn_act_bwd_cast_bias(ab.ptr, gb.ptr, H.ptr, db.ptr, P, C))
+    dP = dA * np.where(nn.from_bf16(A) > 0, np.float32(1.0), np.float32(0.1))
+    assert np.array_equal(H.download(np.uint16, (P, C)), nn.to_bf16(dP))
+    assert np.allclose(db.download(np.float32, (C,)), dP.astype(np.float64).sum(axis=0), rtol=1e-5, atol=1e-4)
