@@ -243,18 +243,31 @@ def _taper(r):
     return np.where(r <= 2.0, 1.0, np.where(r >= 2.6, 0.0, np.cos(0.5 * np.pi * (r - 2.0) / 0.6) ** 2))
 
 
+_SHELLS = {}
+
+
+def _shells(n: int, dr: float):
+    """Shell index of every cell of the scaled n^3 grid, the cells per shell, the
+    occupied shells and their mean radii (they depend on the grid only)."""
+    key = (n, dr)
+    if key not in _SHELLS:
+        _, r = _grids(n)
+        idx = np.floor(r / dr).astype(int).ravel()
+        cnt = np.bincount(idx)
+        rs = np.bincount(idx, weights=r.ravel())
+        keep = cnt > 0
+        _SHELLS[key] = (idx, cnt, keep, rs[keep] / cnt[keep])
+    return _SHELLS[key]
+
+
 def radial_profile(phi: np.ndarray, dr: float = None):
     """Shell averages of a field on the scaled grid: (shell radii, mean values); the
     first shell is the centre."""
-    _, r = _grids(phi.shape[0])
     h = 2 * U_BOX / phi.shape[0]
     dr = dr or 0.5 * h
-    idx = np.floor(r / dr).astype(int).ravel()
-    sums = np.bincount(idx, weights=phi.ravel())
-    cnt = np.bincount(idx)
-    rs = np.bincount(idx, weights=r.ravel())
-    keep = cnt > 0
-    return rs[keep] / cnt[keep], sums[keep] / cnt[keep]
+    idx, cnt, keep, rmean = _shells(phi.shape[0], dr)
+    sums = np.bincount(idx, weights=phi.ravel(), minlength=cnt.size)
+    return rmean, sums[keep] / cnt[keep]
 
 
 def symmetrize(phi: np.ndarray, w) -> np.ndarray:
