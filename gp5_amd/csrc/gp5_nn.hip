@@ -303,7 +303,8 @@ constexpr int kStageBatch = 8;  // staging loads in flight per thread
 // 16-wide output blocks over the 27 taps, A fragments from LDS and weight
 // fragments from global memory PD taps ahead.  Same K permutation and
 // accumulation order as k_conv (chunk, tap, step), so for the same KC the two
-// kernels give bit-identical results.
+// kernels give bit-identical results.  With up = 1, X has side D/2 and each
+// staged cell is read from its parent cell (the nearest upsampling of X).
 template <int KC, int NB, int PD>
 __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                   const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
@@ -408,7 +409,8 @@ __host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 1
 // times, and its latency overlaps the MFMAs.  All 27 weight fragments of the
 // wave's NB output blocks stay in registers for the whole walk.  Same K
 // permutation and accumulation order (tap, step) as k_conv with KC = CIN, so
-// the two agree bit for bit.
+// the two agree bit for bit.  With up = 1, X has side D/2 and each slab cell is
+// read from its parent cell (the nearest upsampling of X).
 template <int CIN, int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
