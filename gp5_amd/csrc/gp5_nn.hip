@@ -293,6 +293,31 @@ __device__ __forceinline__ int udiv_small(int n, int d, float inv)
 
 constexpr int kStageBatch = 8;  // staging loads in flight per thread
 
+// A fragment from LDS at byte offset off.  The offset goes through an empty asm
+// so that the compiler cannot see the common base of neighbouring taps: it would
+// pair the reads of x-neighbours (32 or 64 bytes apart) into ds_read2 pairs, in
+// which each lane's second read hits the banks of the next lane's first (2-way
+// conflicts, measured 40 % of the kernel time); single reads of 16 consecutive
+// positions cover 512 or 1024 consecutive bytes without conflicts.
+template <int LV>
+__device__ __forceinline__ Frag<LV> lds_frag(const unsigned char *sx, int off)
+{
+    asm volatile("" : "+v"(off));
+    if constexpr (LV == 4) {
+        return *reinterpret_cast<const Frag<LV> *>(sx + off);
+    } else {  // 16-byte pieces, 16-byte aligned (off is a multiple of 2 LV)
+        const uint4 *q = reinterpret_cast<const uint4 *>(__builtin_assume_aligned(sx + off, 16));
+        Frag<LV> f;
+#pragma unroll
+        for (int h = 0; h < LV / 8; ++h) {
+            const uint4 v = q[h];
+            f.v[2 * h] = __builtin_bit_cast(bf16x4, make_uint2(v.x, v.y));
+            f.v[2 * h + 1] = __builtin_bit_cast(bf16x4, make_uint2(v.z, v.w));
+        }
+        return f;
+    }
+}
+
 // LDS-staged convolution for cubes of side D = 8 .. 128, a power of two.  A
 // workgroup of 4 waves owns 256 consecutive positions: Rb = min(D, 256 / D)
 // whole rows of PZ = 256 / (D Rb) planes of one sample.  Per chunk of KC input
@@ -378,7 +403,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
             const int toff = ((dz * RY + dy) * RX + dx) * KC * 2;
             Frag<LV> a[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = *reinterpret_cast<const Frag<LV> *>(sx + lo[t] + toff);
+            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t] + toff);
             const int sl = tap % (PD + 1);
 #pragma unroll
             for (int st = 0; st < LV / 4; ++st)
@@ -494,7 +519,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int toff = sl[dz + 1] * SU * 16 + (dy * RX + dx) * CIN * 2;
             Frag<LV> a[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = *reinterpret_cast<const Frag<LV> *>(sx + lo[t] + toff);
+            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t] + toff);
 #pragma unroll
             for (int st = 0; st < LV / 4; ++st)
 #pragma unroll
