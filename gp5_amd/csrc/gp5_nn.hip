@@ -293,6 +293,25 @@ __device__ __forceinline__ int udiv_small(int n, int d, float inv)
 
 constexpr int kStageBatch = 8;  // staging loads in flight per thread
 
+// Bank swizzle of staged positions of KC = 16 or 32 channels (2 or 4 units of 16
+// bytes): unit c of the position in column ix sits in slot c ^ swz_unit(ix).  A
+// wave reads 16 consecutive columns; with the plain layout the columns 8 apart
+// (KC = 16, 8-byte reads) or 4 apart (KC = 32, 16-byte reads) hit the same banks,
+// and the swizzle gives each of them a different slot.
+template <int KC>
+__device__ __forceinline__ int swz_unit(int ix)
+{
+    return KC == 16 ? (ix >> 3) & 1 : (ix >> 2) & 3;
+}
+
+// byte offset, within its position, of lane group g's fragment (LV = KC / 4 channels)
+template <int KC>
+__device__ __forceinline__ int swz_frag(int g, int ix)
+{
+    constexpr int FB = KC / 2;  // fragment bytes
+    return 16 * (((g * FB) >> 4) ^ swz_unit<KC>(ix)) + ((g * FB) & 15);
+}
+
 // A fragment from LDS at byte offset off.  The offset goes through an empty asm
 // so that the compiler cannot see the common base of neighbouring taps: it would
 // pair the reads of x-neighbours (32 or 64 bytes apart) into ds_read2 pairs, in
@@ -351,12 +370,16 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
     const int sb = p0 / (D * D * D) * Ds * Ds * Ds;  // stored position of this sample's origin
     const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> (3 * up)) * Cin * 2);
     const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * Cin * 2);
-    int lo[MT];  // LDS byte offset of this lane's tap-(0,0,0) fragment in tile t
+    int lo[MT][3];  // LDS byte offset of this lane's fragment in tile t at taps (0, 0, dx = d - 1)
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
         const int q = 64 * wave + 16 * t + r;
         const int lx = q & (D - 1), ly = (q / D) % Rb, lz = q / (D * Rb);
-        lo[t] = (((lz + 1) * RY + ly + 1) * RX + lx + 1) * KC * 2 + g * LV * 2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ix = lx + d;  // column of the neighbour dx = d - 1 (column 0 is x = -1)
+            lo[t][d] = (((lz + 1) * RY + ly + 1) * RX + ix) * KC * 2 + swz_frag<KC>(g, ix);
+        }
     }
     const int wvo = ((nb0 * 16 + r) * Cin + g * LV) * 2;
     f32x4 acc[MT][NB];
@@ -372,17 +395,24 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
     };
     for (int c0 = 0; c0 < Cin; c0 += KC) {
         if (c0) __syncthreads();  // every wave is done with the previous chunk
+        // keeps the 27 MT tap offsets from being hoisted out of this loop into registers
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) asm volatile("" : "+v"(lo[t][d]));
         // stage the region, unit u = 16 bytes at LDS byte 16 u, row-major over
         // (plane, row, column, 8-channel group); kStageBatch loads in flight per
         // thread before their LDS writes
         for (int u0 = tid; u0 < U; u0 += 256 * kStageBatch) {
             uint4 v[kStageBatch];
+            int us[kStageBatch];  // swizzled unit of the LDS write
 #pragma unroll
             for (int k = 0; k < kStageBatch; ++k) {
                 const int u = u0 + 256 * k;
                 const int row = udiv_small(u, UR, invUR), e = u - row * UR;
                 const int iz = udiv_small(row, RY, invRY), iy = row - iz * RY;
                 const int ix = e / U8, c8 = e - ix * U8;
+                us[k] = u - c8 + (c8 ^ swz_unit<KC>(ix));
                 const int y = y0 - 1 + iy, z = z0 - 1 + iz;
                 const bool in = u < U && y >= 0 && y < D && z >= 0 && z < D && ix >= 1 && ix <= D;
                 const int voff =
@@ -391,7 +421,7 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
             }
 #pragma unroll
             for (int k = 0; k < kStageBatch; ++k)
-                if (u0 + 256 * k < U) *reinterpret_cast<uint4 *>(sx + 16 * (u0 + 256 * k)) = v[k];
+                if (u0 + 256 * k < U) *reinterpret_cast<uint4 *>(sx + 16 * us[k]) = v[k];
         }
         __syncthreads();
 #pragma unroll
@@ -400,10 +430,10 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
         for (int tap = 0; tap < 27; ++tap) {
             if (tap + PD < 27) fetch_w(c0, tap + PD, (tap + PD) % (PD + 1));
             const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            const int toff = ((dz * RY + dy) * RX + dx) * KC * 2;
+            const int toff = (dz * RY + dy) * RX * KC * 2;
             Frag<LV> a[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t] + toff);
+            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t][dx + 1] + toff);
             const int sl = tap % (PD + 1);
 #pragma unroll
             for (int st = 0; st < LV / 4; ++st)
@@ -460,11 +490,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int sb = smp * Ds * Ds * Ds;
     const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> (3 * up)) * CIN * 2);
     const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * CIN * 2);
-    int lo[MT];
+    int lo[MT][3];  // LDS byte offset of this lane's fragment in tile t at taps (0, 0, dx = d - 1), slot 0
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
         const int q = 64 * wave + 16 * t + r;
-        lo[t] = (((q / D) + 1) * RX + (q & (D - 1)) + 1) * CIN * 2 + g * LV * 2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ix = (q & (D - 1)) + d;
+            lo[t][d] = (((q / D) + 1) * RX + ix) * CIN * 2 + swz_frag<CIN>(g, ix);
+        }
     }
     // weights of all taps
     Frag<LV> w[27][NB];
@@ -477,12 +511,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 w[tap][j] = bld<LV>(wr, wvo, ((mirror ? 26 - tap : tap) * Cout * CIN + j * 16 * CIN) * 2);
     }
     uint4 v[NU];
+    int us[NU];  // swizzled unit of the LDS write
     auto load_slab = [&](int z) {
 #pragma unroll
         for (int k = 0; k < NU; ++k) {
             const int u = tid + 256 * k;
             const int row = udiv_small(u, UR, invUR), e = u - row * UR;
             const int ix = e / U8, c8 = e - ix * U8;
+            us[k] = u - c8 + (c8 ^ swz_unit<CIN>(ix));
             const int y = y0 - 1 + row;
             const bool in = u < SU && z >= 0 && z < D && y >= 0 && y < D && ix >= 1 && ix <= D;
             const int voff =
@@ -494,7 +530,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         unsigned char *dst = sx + (size_t)slot * SU * 16;
 #pragma unroll
         for (int k = 0; k < NU; ++k)
-            if (tid + 256 * k < SU) *reinterpret_cast<uint4 *>(dst + 16 * (tid + 256 * k)) = v[k];
+            if (tid + 256 * k < SU) *reinterpret_cast<uint4 *>(dst + 16 * us[k]) = v[k];
     };
     load_slab(zb - 1);
     store_slab(0);
@@ -505,6 +541,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     __syncthreads();
     for (int zi = 0; zi < ZL; ++zi) {
         const int z = zb + zi;
+        // keeps the 27 MT tap offsets from being hoisted out of this loop into registers
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) asm volatile("" : "+v"(lo[t][d]));
         if (zi + 1 < ZL) load_slab(z + 2);  // in flight during this plane's MFMAs
         f32x4 acc[MT][NB];
 #pragma unroll
@@ -516,10 +557,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
         for (int tap = 0; tap < 27; ++tap) {
             const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            const int toff = sl[dz + 1] * SU * 16 + (dy * RX + dx) * CIN * 2;
+            const int toff = sl[dz + 1] * SU * 16 + dy * RX * CIN * 2;
             Frag<LV> a[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t] + toff);
+            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t][dx + 1] + toff);
 #pragma unroll
             for (int st = 0; st < LV / 4; ++st)
 #pragma unroll
