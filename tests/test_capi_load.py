@@ -110,3 +110,26 @@ def test_library_exports_every_nn_symbol():
 
     lib = ctypes.CDLL(capi.LIB_PATH)
     assert not [s for s in names if not hasattr(lib, s)]
+
+
+def test_network_entry_points_reject_bad_shapes_on_the_host():
+    """The in-place-upsampling convolution, its weight gradient, the fused
+    activation-backward and the fused loss check their shapes before any device work
+    (no GPU needed; the pointers are never dereferenced)."""
+    from gp5_amd import capi, nn
+
+    L = nn._lib()
+    fake = ctypes.c_void_p(0x1000)
+    loss = ctypes.c_double()
+    # conv3d_up: the cube side must be a power of two in [8, 128]
+    assert L.gp5_nn_conv3d_up(fake, fake, None, None, None, None, fake, 1, 6, 16, 16, 0, 1) == capi.GP5_ERR_ARG
+    assert b"power of two" in L.gp5_last_error()
+    # wgrad_up: the cube side must be a multiple of 8
+    assert L.gp5_nn_conv3d_wgrad_up(fake, fake, fake, fake, 1, 12, 16, 16) == capi.GP5_ERR_ARG
+    assert b"multiple of 8" in L.gp5_last_error()
+    # act_bwd_cast_bias: C / 4 must divide 256
+    assert L.gp5_nn_act_bwd_cast_bias(fake, fake, fake, fake, 64, 48) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_act_bwd_cast_bias(None, fake, fake, fake, 64, 16) == capi.GP5_ERR_ARG
+    # mse_bf16: C a multiple of 8, H required
+    assert L.gp5_nn_mse_bf16(fake, 12, fake, fake, None, fake, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_mse_bf16(fake, 16, fake, fake, None, None, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
