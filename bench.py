@@ -191,13 +191,20 @@ def two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, 
     from gp5_amd import ground_state, surrogate
 
     out = {}
-    legs = [("gaussian", None)]
+    legs = [("gaussian", {})]
+    warm_s = None
     if os.path.exists(surrogate.DEFAULT_WEIGHTS):
-        sur = surrogate.Surrogate()
-        legs.append(("surrogate", lambda i, k, Lk, Nk: sur.seed(Nk, (0.0, 0.0, 0.0), 2.5, k, Lk)))
-    for name, seed in legs:
-        fb, rep = ground_state.two_level(n, L, N, f, seed=seed, dt_factor=dt_factor, tol=tol,
-                                         check_every=check_every, max_steps=max_steps)
+        sur = surrogate.Surrogate(batch=16)  # 16 droplets per forward pass
+        # one forward pass before the timed legs: loads the network's kernels (a
+        # one-time cost of the process, reported apart like the tuning)
+        t0 = time.perf_counter()
+        sur.predict_many([(100.0, (0.0, 0.0, 0.0), 2.5)])
+        warm_s = time.perf_counter() - t0
+        legs.append(("surrogate", {"seed_many": lambda items: sur.seed_many(
+            [(Nk, (0.0, 0.0, 0.0), 2.5, k, Lk) for _, k, Lk, Nk in items])}))
+    for name, kw in legs:
+        fb, rep = ground_state.two_level(n, L, N, f, dt_factor=dt_factor, tol=tol, check_every=check_every,
+                                         max_steps=max_steps, **kw)
         fb.close()
         steps = rep["steps"]
         out[name] = {
@@ -212,6 +219,8 @@ def two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, cold_ms, 
             "speedup_vs_cold": (cold_ms / 1e3) / rep["total_s"],
             "max_energy_per_atom_diff_vs_cold": float(np.max(np.abs(rep["energy"] - cold_en) / np.array(N))),
         }
+        if name == "surrogate":
+            out[name]["network_warmup_seconds"] = warm_s
         _progress(f"two-level ({name}): {rep['total_s']:.1f} s")
     return out
 

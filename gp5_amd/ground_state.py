@@ -36,7 +36,9 @@ def gaussian_seed(n: int, L: float, N: float) -> np.ndarray:
     x = -0.5 * L + (np.arange(n) + 0.5) * (L / n)
     r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
     s = 0.6 * (3.0 * N / (4.0 * np.pi)) ** (1.0 / 3.0)
-    return np.asfortranarray(np.exp(-r2 / (2.0 * s * s)).astype(np.complex128))
+    # spherical: the transpose is Fortran-ordered with the same values up to the
+    # summation order of r2, which saves the upload a transposing copy
+    return np.exp(-r2 / (2.0 * s * s)).astype(np.complex128).T
 
 
 def level_sizes(n: int, levels: int, minimum: int = 16, L: Optional[float] = None,
@@ -57,7 +59,8 @@ def multilevel(n: Sequence[int], L: Sequence[float], N: Sequence[float], f: capi
                levels: int = 2, seed: Optional[Callable[[int, int, float, float], np.ndarray]] = None,
                dt_factor: float = 0.1, tol: float = 1e-10, coarse_tol: float = 1e-9, check_every: int = 100,
                max_steps: int = 200000, tune: bool = True, max_spacing: Optional[float] = None,
-               potentials: Optional[Callable[[int, int, float], np.ndarray]] = None):
+               potentials: Optional[Callable[[int, int, float], np.ndarray]] = None,
+               seed_many: Optional[Callable[[list], list]] = None):
     """Ground states of instances (n[i]^3 grid over [-L[i]/2, L[i]/2]^3, norm N[i]),
     converged level by level on grids of the same box, each level halving the
     resolution of the next (level_sizes); every level but the last stops at
@@ -70,7 +73,9 @@ def multilevel(n: Sequence[int], L: Sequence[float], N: Sequence[float], f: capi
     that the energy-change rule took for convergence.
 
     seed(i, g, L, N) -> complex (g, g, g) start on the coarsest grid g (default:
-    Gaussian); potentials(i, g, L) -> real (g, g, g) potential or None.  Returns
+    Gaussian), or seed_many([(i, g, L, N), ...]) -> the starts of all instances in
+    order (any iterable; timed the same way); potentials(i, g, L) -> real (g, g, g)
+    potential or None.  Returns
     (finest batch, report); the caller downloads the states from the batch and closes it."""
     count = len(n)
     sizes = [level_sizes(int(k), levels, L=float(Lk), max_spacing=max_spacing) for k, Lk in zip(n, L)]
@@ -89,9 +94,13 @@ def multilevel(n: Sequence[int], L: Sequence[float], N: Sequence[float], f: capi
 
     t0 = time.perf_counter()
     b = capi.Batch([g[0] for g in grids], L, N)
-    for i in range(count):
-        g = grids[i][0]
-        b.upload(i, seed(i, g, L[i], N[i]) if seed else gaussian_seed(g, L[i], N[i]))
+    if seed_many:
+        for i, psi in enumerate(seed_many([(i, grids[i][0], L[i], N[i]) for i in range(count)])):
+            b.upload(i, psi)
+    else:
+        for i in range(count):
+            g = grids[i][0]
+            b.upload(i, seed(i, g, L[i], N[i]) if seed else gaussian_seed(g, L[i], N[i]))
     load_potentials(b, 0)
     report["seed_s"] = time.perf_counter() - t0
     for lev in range(depth):

@@ -264,15 +264,19 @@ class UNet:
         self._conv_up("c7", "d2", "u2", D, A=t["a7"], skip=t["e1"], Y=t["d1"])
         self._conv("c8", t["d1"], Yf=t["out"], D=D, act=0)
 
-    def set_droplet_features(self, s: float, a, consts, ub: float):
-        """Features of one droplet (batch of 1) built on the device (gp5_nn_droplet_features):
-        the flat-top baseline with surface factor s, trap weights a, the five constant
-        channels consts and the scaled half-box ub; the baseline also becomes `base`."""
-        if self.B != 1:
-            raise ValueError("device features are built for a batch of one")
+    def set_droplet_features(self, s: float, a, consts, ub: float, index: int = 0):
+        """Features of one droplet built on the device (gp5_nn_droplet_features) as
+        sample `index` of the batch: the flat-top baseline with surface factor s, trap
+        weights a, the five constant channels consts and the scaled half-box ub; the
+        baseline also becomes that sample's `base`."""
+        if not 0 <= index < self.B:
+            raise ValueError(f"sample {index} of a batch of {self.B}")
+        cube = self.D ** 3
+        X = ctypes.c_void_p(self.t["X"].ptr.value + index * cube * C0 * 2)
+        base = ctypes.c_void_p(self.t["base"].ptr.value + index * cube * 4)
         af = (ctypes.c_float * 3)(*[float(v) for v in a])
         cf = (ctypes.c_float * 5)(*[float(v) for v in consts])
-        _ck(_lib().gp5_nn_droplet_features(self.t["X"].ptr, self.t["base"].ptr, self.D, float(ub), float(s), af, cf))
+        _ck(_lib().gp5_nn_droplet_features(X, base, self.D, float(ub), float(s), af, cf))
 
     def output0(self) -> np.ndarray:
         """The prediction base + out[..., 0] as [B, D, D, D] float, combined on the device."""

@@ -190,21 +190,28 @@ def train(N, w, b, phi, epochs: int = 12, batch: int = 4, lr: float = 1e-3, seed
 class Surrogate:
     """Inference with trained weights (float masters from train() or an .npz file)."""
 
-    def __init__(self, params=None, path: str = DEFAULT_WEIGHTS):
+    def __init__(self, params=None, path: str = DEFAULT_WEIGHTS, batch: int = 1):
+        """batch > 1 also sets up a network of that batch size for predict_many /
+        seed_many (one forward pass per `batch` droplets)."""
         if params is None:
             with np.load(path, allow_pickle=False) as z:
                 params = {k: z[k] for k in z.files if "." in k}
         self.net = nn.UNet(1, GRID, params)
+        self.batch_net = nn.UNet(int(batch), GRID, params) if batch > 1 else None
+
+    @staticmethod
+    def _set_features(net, N: float, w, b: float, index: int = 0):
+        R0 = radius(N, b)
+        a = [(wi / 0.2) ** 2 / 18.0 for wi in w]
+        consts = [np.log(N / 50.0) / np.log(100.0), 2.0 * (b - 2.5), w[0] / 0.2, w[1] / 0.2, w[2] / 0.2]
+        net.set_droplet_features(np.float32(R0 / SURFACE), np.float32(a), np.float32(consts), U_BOX, index)
 
     def predict(self, N: float, w, b: float, device_features: bool = True) -> np.ndarray:
         """phi on the scaled 64^3 grid.  The input channels are built on the device
         (gp5_nn_droplet_features, the float32 arithmetic of features()) unless
         device_features is false."""
         if device_features:
-            R0 = radius(N, b)
-            a = [(wi / 0.2) ** 2 / 18.0 for wi in w]
-            consts = [np.log(N / 50.0) / np.log(100.0), 2.0 * (b - 2.5), w[0] / 0.2, w[1] / 0.2, w[2] / 0.2]
-            self.net.set_droplet_features(np.float32(R0 / SURFACE), np.float32(a), np.float32(consts), U_BOX)
+            self._set_features(self.net, N, w, b)
             self.net.forward()
             return self.net.output0()[0]
         self._X = features(N, w, b, out=getattr(self, "_X", None))
@@ -212,6 +219,29 @@ class Surrogate:
         self.net.set_batch(X, X[..., 0])
         self.net.forward()
         return self.net.output()[0]
+
+    def predict_many(self, items):
+        """predict() for a list of (N, w, b), `batch` droplets per forward pass when
+        the Surrogate was made with batch > 1 (the kernels may then take another
+        summation order than at batch 1: results agree to the bf16 level)."""
+        if self.batch_net is None:
+            return [self.predict(N, w, b) for N, w, b in items]
+        out = []
+        B = self.batch_net.B
+        for c0 in range(0, len(items), B):
+            chunk = items[c0:c0 + B]
+            for j, (N, w, b) in enumerate(chunk):
+                self._set_features(self.batch_net, N, w, b, j)
+            self.batch_net.forward()  # samples past the chunk hold stale inputs; their outputs are unused
+            y = self.batch_net.output0()
+            out += [y[j] for j in range(len(chunk))]
+        return out
+
+    def seed_many(self, items):
+        """seed() for a list of (N, w, b, n, L), the predictions batched (predict_many);
+        the seeds come one at a time (a generator), so that only one is held."""
+        phis = self.predict_many([(N, w, b) for N, w, b, _, _ in items])
+        return (self._seed_from(phi, N, w, b, n, L) for phi, (N, w, b, n, L) in zip(phis, items))
 
     def seed(self, N: float, w, b: float, n: int, L: float) -> np.ndarray:
         """psi on the cubic grid n^3 over [-L/2, L/2]^3 (complex128, column-major).
@@ -226,14 +256,30 @@ class Surrogate:
         The profile is clipped at zero and tapered to zero between |u| = 2 and 2.6,
         since stray amplitude far from the droplet is a dilute gas that imaginary
         time removes only slowly."""
-        phi = self.predict(N, w, b).astype(np.float64)
+        return self._seed_from(self.predict(N, w, b), N, w, b, n, L)
+
+    def _seed_from(self, phi: np.ndarray, N: float, w, b: float, n: int, L: float) -> np.ndarray:
+        """The seed of seed() from a prediction phi."""
+        phi = phi.astype(np.float64)
         R0 = radius(N, b)
         x = -0.5 * L + (np.arange(n) + 0.5) * (L / n)
         if max(abs(float(v)) for v in w) == 0.0:
             ru, prof = radial_profile(phi)
-            rx = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2) / R0
-            out = np.interp(rx, ru, np.maximum(prof, 0.0) * _taper(ru), right=0.0)
-            return (np.sqrt(rho0(b)) * out).astype(np.complex128)
+            fp = np.maximum(prof, 0.0) * _taper(ru)
+            if n % 2 == 0:
+                # the cell centres are mirror-symmetric about 0: one octant, then flips
+                xa = x[n // 2:]
+                rx = np.sqrt(xa[:, None, None] ** 2 + xa[None, :, None] ** 2 + xa[None, None, :] ** 2) / R0
+                out = np.interp(rx, ru, fp, right=0.0)
+                for ax in range(3):
+                    out = np.concatenate([np.flip(out, axis=ax), out], axis=ax)
+            else:
+                rx = np.sqrt(x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2) / R0
+                out = np.interp(rx, ru, fp, right=0.0)
+            # the field is radial, so its transpose (Fortran-ordered, no copy) holds the
+            # same values up to the summation order of x^2 + y^2 + z^2; the solver takes
+            # column-major fields, so the upload then needs no transposing copy
+            return (np.sqrt(rho0(b)) * out).astype(np.complex128).T
         phi = symmetrize(phi, w)
         _, r = _grids(GRID)
         return sqrt_rho_interp(np.maximum(phi, 0.0) * _taper(r), N, b, n, L)

@@ -97,3 +97,22 @@ def test_device_features_match_host_features(N, w, b):
     dev = sur.predict(N, w, b, device_features=True)
     hst = sur.predict(N, w, b, device_features=False)
     assert np.linalg.norm(dev - hst) <= 1e-3 * np.linalg.norm(hst)
+
+
+@pytest.mark.gpu
+def test_batched_predictions_match_single():
+    """predict_many / seed_many with a batch-4 network (6 droplets: one full and one
+    partial forward pass) give the batch-1 predictions to the bf16 level (the kernels
+    may sum in another order at another batch size), and the seeds agree likewise."""
+    sur = S.Surrogate(batch=4)
+    items = [(60.0, (0.0, 0.0, 0.0), 2.5), (300.0, (0.0, 0.0, 0.0), 2.5), (1500.0, (0.0, 0.0, 0.0), 2.5),
+             (4000.0, (0.0, 0.0, 0.0), 2.5), (800.0, (0.1, 0.1, 0.05), 2.2), (120.0, (0.0, 0.0, 0.0), 2.8)]
+    many = sur.predict_many(items)
+    for (N, w, b), got in zip(items, many):
+        one = sur.predict(N, w, b)
+        assert got.shape == one.shape
+        assert np.max(np.abs(got - one)) <= 2e-2 * np.max(np.abs(one)), (N, w, b)
+    seeds = sur.seed_many([(N, w, b, 32, 24.0) for N, w, b in items])
+    for (N, w, b), psi in zip(items, seeds):
+        ref = sur.seed(N, w, b, 32, 24.0)
+        assert np.max(np.abs(psi - ref)) <= 2e-2 * np.max(np.abs(ref)), (N, w, b)
