@@ -69,7 +69,8 @@ GP5_PLAN(512, true, 8, 8, 8, 0, 8);
 // inverse): the column plan unless specialised.  At 512 the z columns lie 4 MiB
 // apart; 16 values per thread over 8 columns (256-thread blocks, plan 16.8.4)
 // measured 11 % faster there than the 8.8.8 column plan, which stays the better
-// one for the y pass.
+// one for the y pass.  At 192 the radix-12 stage first (12.4.4 over 16 columns)
+// measured 3.5 % faster than the 4.4.12 column plan.
 template <int N> struct FPlanZ : FPlan<N, false> {};
 #ifndef GP5_Z512_DEFAULT
 #ifndef GP5_Z512_PLAN
@@ -90,7 +91,13 @@ template <> struct FPlanZ<512> {
 #ifdef GP5_Z256_PLAN
 GP5_ZPLAN_OVERRIDE(256, GP5_Z256_PLAN)
 #endif
-#ifdef GP5_Z192_PLAN
+#ifdef GP5_Z96_PLAN
+GP5_ZPLAN_OVERRIDE(96, GP5_Z96_PLAN)
+#endif
+#ifndef GP5_Z192_DEFAULT
+#ifndef GP5_Z192_PLAN
+#define GP5_Z192_PLAN 12, 4, 4, 0, 12, 16
+#endif
 GP5_ZPLAN_OVERRIDE(192, GP5_Z192_PLAN)
 #endif
 #undef GP5_ZPLAN_OVERRIDE
