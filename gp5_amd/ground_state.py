@@ -94,33 +94,44 @@ def multilevel(n: Sequence[int], L: Sequence[float], N: Sequence[float], f: capi
 
     t0 = time.perf_counter()
     b = capi.Batch([g[0] for g in grids], L, N)
-    if seed_many:
-        for i, psi in enumerate(seed_many([(i, grids[i][0], L[i], N[i]) for i in range(count)])):
-            b.upload(i, psi)
-    else:
-        for i in range(count):
-            g = grids[i][0]
-            b.upload(i, seed(i, g, L[i], N[i]) if seed else gaussian_seed(g, L[i], N[i]))
-    load_potentials(b, 0)
-    report["seed_s"] = time.perf_counter() - t0
-    for lev in range(depth):
-        if lev > 0:
-            t = time.perf_counter()
-            nb = capi.Batch([g[lev] for g in grids], L, N)
+    nb = None
+    try:
+        if seed_many:
+            for i, psi in enumerate(seed_many([(i, grids[i][0], L[i], N[i]) for i in range(count)])):
+                b.upload(i, psi)
+        else:
             for i in range(count):
-                nb.resample_from(i, b, i)
-            load_potentials(nb, lev)
-            b.close()
-            b = nb
-            report["resample_s"] += time.perf_counter() - t
-        if tune:
-            t = time.perf_counter()
-            b.tune(f, dt_factor, 2)
-            report["tune_s"] += time.perf_counter() - t
-        last = lev == depth - 1
-        steps, en, ms = b.ground_state(f, dt_factor, tol if last else coarse_tol, check_every, max_steps)
-        report["levels"].append({"grids": sorted(set(g[lev] for g in grids)), "seconds": ms / 1e3,
-                                 "steps": steps, "energy": en})
+                g = grids[i][0]
+                b.upload(i, seed(i, g, L[i], N[i]) if seed else gaussian_seed(g, L[i], N[i]))
+        load_potentials(b, 0)
+        report["seed_s"] = time.perf_counter() - t0
+        for lev in range(depth):
+            if lev > 0:
+                t = time.perf_counter()
+                nb = capi.Batch([g[lev] for g in grids], L, N)
+                for i in range(count):
+                    nb.resample_from(i, b, i)
+                load_potentials(nb, lev)
+                b.close()
+                b, nb = nb, None
+                report["resample_s"] += time.perf_counter() - t
+            if tune:
+                t = time.perf_counter()
+                b.tune(f, dt_factor, 2)
+                report["tune_s"] += time.perf_counter() - t
+            last = lev == depth - 1
+            steps, en, ms = b.ground_state(f, dt_factor, tol if last else coarse_tol, check_every, max_steps)
+            report["levels"].append({"grids": sorted(set(g[lev] for g in grids)), "seconds": ms / 1e3,
+                                     "steps": steps, "energy": en})
+    except BaseException:
+        # release the device batches now rather than when the traceback is dropped
+        for x in (nb, b):
+            if x is not None:
+                try:
+                    x.close()
+                except Exception:
+                    pass
+        raise
     fin = report["levels"][-1]
     report["coarse_s"] = sum(lv["seconds"] for lv in report["levels"][:-1])
     report["coarse_steps"] = report["levels"][-2]["steps"] if depth > 1 else fin["steps"]
