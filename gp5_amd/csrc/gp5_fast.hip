@@ -491,7 +491,9 @@ __global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::
 // The same pass held to 3 waves per SIMD (at most 168 VGPRs).  Used for the
 // 256-point z pass, which needs 170 registers otherwise and then runs 2 waves
 // per SIMD: without spills at 3 it measured 96.3 -> 88.0 us at 256^3.  The
-// epilogue kernels would spill under this limit and stay on k_fast.
+// epilogue kernels would spill under this limit and stay on k_fast (at 192
+// the x epilogue needs 175 registers; held to 168 it spills 32 bytes per lane
+// and measured 106.2 -> 109.6 us).
 template <int N, bool ROWS, int MODE, int DIR>
 __global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::ROW_THREADS
                                           : FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::COL_THREADS))
