@@ -493,7 +493,9 @@ __global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::
 // per SIMD: without spills at 3 it measured 96.3 -> 88.0 us at 256^3.  The
 // epilogue kernels would spill under this limit and stay on k_fast (at 192
 // the x epilogue needs 175 registers; held to 168 it spills 32 bytes per lane
-// and measured 106.2 -> 109.6 us).
+// and measured 106.2 -> 109.6 us).  More waves do not help the 96 and 192 z
+// passes either: held to 5 waves per SIMD (no spills at 96) they measured
+// 17.7 -> 18.1 us and 47.3 -> 50.0 us.
 template <int N, bool ROWS, int MODE, int DIR>
 __global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::ROW_THREADS
                                           : FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::COL_THREADS))
