@@ -67,7 +67,13 @@ GP5_PLAN(128, false, 16, 8, 0, 0, 16);
 GP5_PLAN_EXPAND(96, false, GP5_COL96_PLAN);
 GP5_PLAN_EXPAND(192, false, GP5_COL192_PLAN);
 GP5_PLAN(256, false, 16, 16, 0, 0, 16);
+#ifdef GP5_COL512_COLS
+template <> struct FPlan<512, false> {
+    static constexpr int NP = 512, R0 = 8, R1 = 8, R2 = 8, R3 = 0, E = 8, COLS = GP5_COL512_COLS;
+};
+#else
 GP5_PLAN(512, false, 8, 8, 8, 0, 8);
+#endif
 GP5_PLAN(16, true, 4, 4, 0, 0, 4);
 GP5_PLAN(32, true, 8, 4, 0, 0, 8);
 GP5_PLAN(64, true, 8, 8, 0, 0, 8);
