@@ -202,6 +202,30 @@ def test_advance_matches_oracle(C, oracle_mod, name, imag, renorm, case, shape):
     assert rel_l2(psi.download_complex(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("shape", [(256, 8, 12), (8, 256, 12), (12, 8, 256)], ids=["x256", "y256", "z256"])
+@pytest.mark.parametrize("name,imag,renorm,case", STEP_CASES)
+def test_advance_256_axes_match_oracle(C, oracle_mod, name, imag, renorm, case, shape):
+    """The 256-point register passes of the 256^3 workload in every stage mode: the fused
+    x epilogue (component-local and coupled variants, all four RK stages, Euler, with
+    per-stage renormalisation), the 256-point y pass and the fused z multiplier pass.
+    dt = 2e-4 keeps dt k_max^2 / 2 = 0.65 inside the explicit limit (256 points over 10)."""
+    print(f"advance256 {name} imag={imag} renorm={renorm} {case} {shape}", flush=True)
+    left, right = (-5.0, -5.0, -4.0), (5.0, 5.0, 4.0)
+    nc, gspec, ospec = _functional_cases(oracle_mod, shape, left, right)[case]
+    a = _smooth(shape, left, right, nc, seed=37)
+    dt, nsteps = 2e-4, 12
+    geo = oracle_mod.Geometry(shape, left, right, nc)
+    ref = oracle_mod.advance(oracle_mod.Functional(**ospec), a, geo, stepper=name, dt=dt, imaginary=imag,
+                             nsteps=nsteps, renormalize=renorm)
+    ctx = _ctx(C, shape, left, right, nc)
+    f, vb = _desc(C, ctx, gspec)
+    psi = ctx.field(a)
+    kind = C.GP5_STEP_RK4 if name == "RK4" else C.GP5_STEP_EULERO
+    s = C.stepper_desc(kind, dt=dt, imaginary=imag, renormalize=renorm)
+    C.advance(ctx, f, s, psi, nsteps)
+    assert rel_l2(psi.download_complex(), ref) <= 1e-12
+
+
 @pytest.mark.parametrize("shape", [(16, 12, 192), (192, 8, 12), (12, 96, 96)])
 def test_radix3_register_passes_rk4(C, oracle_mod, shape):
     """RK4 real time through the register passes of lengths 96 and 192 on every axis role

@@ -9,6 +9,11 @@ its own formula:
   * closed forms: plane waves are Laplacian eigenfunctions, Gaussian norms,
     the harmonic-oscillator ground-state energy, Euler/RK4 amplification
     polynomials on a plane wave;
+  * the nonlinear branches (contact g|psi|^2 psi, the two-component couplings
+    with masses, the unlocked LHY form, the LHY coefficients, uniform
+    potentials) on plane waves, where every stage is a multiple of the wave and
+    a 3-D run reduces to a scalar recurrence on the amplitudes, and energies of
+    plane waves;
   * an independent numpy restatement of the steppers (tests/np_ref.py) for
     every functional, component count, renormalisation and constraint.
 """
@@ -197,6 +202,162 @@ def test_rk4_stage_renormalisation_plane_wave(oracle_mod):
             a = nrm(a - (tau / 6) * (((k1 + 2 * k2) + 2 * k3) + k4))
         got = oracle_mod.advance(F, f, geo, stepper="RK4", dt=dt, imaginary=imag, nsteps=3, renormalize=[N])
         assert rel_l2(got, a / 0.7 * f) < 1e-13
+
+
+def _wave(shape, left, right, m, amp):
+    from gp5_amd import synth
+
+    x, y, z = synth.axes(shape, left, right)
+    L = [right[d] - left[d] for d in range(3)]
+    k = [2 * np.pi * m[d] / L[d] for d in range(3)]
+    f = amp * np.exp(1j * (k[0] * x[:, None, None] + k[1] * y[None, :, None] + k[2] * z[None, None, :]))
+    return f, 0.5 * sum(kk * kk for kk in k)
+
+
+# Nonlinear branches on plane waves.  With V uniform, |psi_c| is uniform at every
+# stage of every step, so H acts on component c as multiplication by a scalar
+# h_c(a_0, a_1) and the 3-D run reduces to a recurrence on the amplitudes a_c.
+# The scalar h_c comes from the reference's formulas with psi_c = a_c e^{i k_c.x}:
+#   gpFunctional, 1 comp. (functional.cpp:13-68):  lam/m + V + g |a|^2
+#   gpFunctional, 2 comp. (functional.cpp:71-89):  lam_c/m_c + V_c + g_cc |a_c|^2 + g_01 |a_other|^2
+#   unlocked LHY (functional.cpp:125-150):         lam_c + V_c + (r1 + eta r2 + alpha s, eta beta r1 + beta r2
+#                                                   + alpha beta^2 s), s = (r1 + beta r2)^{3/2}
+# and the energy density is uniform too: e = sum_c (lam_c/m_c + V_c) r_c + e_int(r).
+PW_SHAPE, PW_LEFT, PW_RIGHT = (8, 6, 10), (-2.0, -1.5, -2.5), (2.0, 1.5, 2.5)
+PW_VOL = 4.0 * 3.0 * 5.0
+
+
+def _pw_cases():
+    # asymmetric coupling input: couplings(0,1) = 0.45 must be used for both components,
+    # couplings(1,0) = -0.8 never (functional.cpp:76)
+    G2 = np.array([[1.1, 0.45], [-0.8, 0.6]])
+    return {
+        "gp_g": dict(nc=1, m=[(1, -2, 1)], amp=[0.9], masses=(1.0, 1.0), V=[0.3],
+                     ofunc=dict(kind=0, couplings=np.array([[1.7]])),
+                     h=lambda lam, r, V: [lam[0] + V[0] + 1.7 * r[0]],
+                     e=lambda r: 0.5 * 1.7 * r[0] ** 2),
+        "gp_g_mass": dict(nc=1, m=[(2, 1, -1)], amp=[0.6], masses=(2.5, 1.0), V=[0.0],
+                          ofunc=dict(kind=0, couplings=np.array([[-0.9]]), masses=(2.5, 1.0)),
+                          h=lambda lam, r, V: [lam[0] / 2.5 + V[0] - 0.9 * r[0]],
+                          e=lambda r: 0.5 * -0.9 * r[0] ** 2),
+        "gp_2c": dict(nc=2, m=[(1, 0, 2), (-1, 1, 0)], amp=[0.8, 0.5], masses=(1.0, 1.7), V=[0.2, -0.1],
+                      ofunc=dict(kind=0, couplings=G2, masses=(1.0, 1.7)),
+                      h=lambda lam, r, V: [lam[0] + V[0] + 1.1 * r[0] + 0.45 * r[1],
+                                           lam[1] / 1.7 + V[1] + 0.6 * r[1] + 0.45 * r[0]],
+                      e=lambda r: 0.5 * 1.1 * r[0] ** 2 + 0.5 * 0.6 * r[1] ** 2 + 0.45 * r[0] * r[1]),
+        "unlocked": dict(nc=2, m=[(0, 1, -1), (2, 0, 1)], amp=[0.7, 0.9], masses=(1.0, 1.0), V=[0.1, 0.25],
+                         ofunc=dict(kind=2, alpha=0.6, beta=0.8, eta=-0.9),
+                         h=lambda lam, r, V: [
+                             lam[0] + V[0] + r[0] - 0.9 * r[1] + 0.6 * (r[0] + 0.8 * r[1]) ** 1.5,
+                             lam[1] + V[1] - 0.9 * 0.8 * r[0] + 0.8 * r[1] + 0.6 * 0.64 * (r[0] + 0.8 * r[1]) ** 1.5],
+                         e=None),
+        "lhy": dict(nc=1, m=[(1, 1, 1)], amp=[1.2], masses=(1.0, 1.0), V=[0.0],
+                    ofunc=dict(kind=1, lhy_a=-2.7, lhy_b=2.2),
+                    h=lambda lam, r, V: [lam[0] + V[0] - 2.7 * r[0] + 2.2 * r[0] ** 1.5],
+                    e=lambda r: 0.5 * -2.7 * r[0] ** 2 + 0.4 * 2.2 * r[0] ** 2.5),
+    }
+
+
+def _pw_field(case):
+    nc = case["nc"]
+    f = np.empty(PW_SHAPE + (nc,), dtype=np.complex128, order="F")
+    lam = []
+    for c in range(nc):
+        w, l_c = _wave(PW_SHAPE, PW_LEFT, PW_RIGHT, case["m"][c], 1.0)
+        f[..., c] = w
+        lam.append(l_c)
+    V = np.empty(PW_SHAPE + (nc,), order="F")
+    for c in range(nc):
+        V[..., c] = case["V"][c]
+    return f, lam, V
+
+
+def _pw_recurrence(case, lam, stepper, tau, nsteps, N=None):
+    """Scalar restatement of euleroStepper / RK4Stepper (stepper.cpp:30-70) on the amplitudes."""
+    nc = case["nc"]
+    hfun = case["h"]
+    V = case["V"]
+
+    def H(a):
+        h = hfun(lam, [abs(x) ** 2 for x in a], V)
+        return [h[c] * a[c] for c in range(nc)]
+
+    def nrm(a):
+        if N is None:
+            return a
+        return [a[c] * np.sqrt(N[c] / (abs(a[c]) ** 2 * PW_VOL)) for c in range(nc)]
+
+    def axpy(a, cf, k):
+        return [a[c] - cf * k[c] for c in range(nc)]
+
+    a = [complex(x) for x in case["amp"]]
+    for _ in range(nsteps):
+        if stepper == "eulero":
+            a = nrm(axpy(a, tau, H(a)))
+            continue
+        k1 = H(a)
+        k2 = H(nrm(axpy(a, tau * 0.5, k1)))
+        k3 = H(nrm(axpy(a, tau * 0.5, k2)))
+        k4 = H(nrm(axpy(a, tau, k3)))
+        s = [((k1[c] + 2 * k2[c]) + 2 * k3[c]) + k4[c] for c in range(nc)]
+        a = nrm(axpy(a, tau / 6, s))
+    return a
+
+
+@pytest.mark.parametrize("case", ["gp_g", "gp_g_mass", "gp_2c", "unlocked", "lhy"])
+def test_nonlinear_apply_plane_wave(oracle_mod, case):
+    """H of a plane wave (or of one plane wave per component) is h_c times the wave."""
+    cs = _pw_cases()[case]
+    f, lam, V = _pw_field(cs)
+    a = np.asfortranarray(f * np.array(cs["amp"]))
+    geo = oracle_mod.Geometry(PW_SHAPE, PW_LEFT, PW_RIGHT, cs["nc"])
+    got = oracle_mod.apply(oracle_mod.Functional(V=V, **cs["ofunc"]), a, geo)
+    h = cs["h"](lam, [x * x for x in cs["amp"]], cs["V"])
+    for c in range(cs["nc"]):
+        assert rel_l2(got[..., c], h[c] * a[..., c]) < 1e-13, c
+
+
+@pytest.mark.parametrize("case", ["gp_g", "gp_g_mass", "gp_2c", "unlocked", "lhy"])
+@pytest.mark.parametrize("stepper,imag,renorm", [("RK4", True, None), ("RK4", False, None), ("RK4", False, [2.0, 0.7]),
+                                                  ("eulero", True, None), ("eulero", False, [1.5, 3.0])])
+def test_nonlinear_steppers_plane_wave(oracle_mod, case, stepper, imag, renorm):
+    """The 3-D oracle run equals the scalar amplitude recurrence on every nonlinear branch.
+    Renormalised runs are taken in real time: in imaginary time h_c is real, so a
+    renormalised plane wave keeps its amplitude whatever h_c is, and the check would pin
+    nothing."""
+    cs = _pw_cases()[case]
+    nc = cs["nc"]
+    f, lam, V = _pw_field(cs)
+    a0 = np.asfortranarray(f * np.array(cs["amp"]))
+    geo = oracle_mod.Geometry(PW_SHAPE, PW_LEFT, PW_RIGHT, nc)
+    dt, nsteps = 0.01, 5
+    N = None if renorm is None else renorm[:nc]
+    got = oracle_mod.advance(oracle_mod.Functional(V=V, **cs["ofunc"]), a0, geo, stepper=stepper, dt=dt,
+                             imaginary=imag, nsteps=nsteps, renormalize=N)
+    tau = dt if imag else 1j * dt
+    amp = _pw_recurrence(cs, lam, stepper, tau, nsteps, N)
+    for c in range(nc):
+        assert rel_l2(got[..., c], amp[c] * f[..., c]) < 1e-13, c
+    if case == "gp_2c":
+        # the transposed coupling entry would give another trajectory: the check can tell them apart
+        wrong = dict(cs, h=lambda lam, r, V: [lam[0] + V[0] + 1.1 * r[0] - 0.8 * r[1],
+                                               lam[1] / 1.7 + V[1] + 0.6 * r[1] - 0.8 * r[0]])
+        amp_w = _pw_recurrence(wrong, lam, stepper, tau, nsteps, N)
+        assert rel_l2(got[..., 0], amp_w[0] * f[..., 0]) > 1e-4
+
+
+@pytest.mark.parametrize("case", ["gp_g", "gp_g_mass", "gp_2c", "lhy"])
+def test_energy_plane_wave(oracle_mod, case):
+    """E of plane waves = volume * (sum_c (lam_c/m_c + V_c) r_c + e_int(r))."""
+    cs = _pw_cases()[case]
+    f, lam, V = _pw_field(cs)
+    a = np.asfortranarray(f * np.array(cs["amp"]))
+    geo = oracle_mod.Geometry(PW_SHAPE, PW_LEFT, PW_RIGHT, cs["nc"])
+    r = [x * x for x in cs["amp"]]
+    kin = sum((lam[c] / cs["masses"][c] + cs["V"][c]) * r[c] for c in range(cs["nc"]))
+    expect = PW_VOL * (kin + cs["e"](r))
+    got = oracle_mod.energy(oracle_mod.Functional(V=V, **cs["ofunc"]), a, geo)
+    assert got == pytest.approx(expect, rel=1e-13, abs=1e-13)
 
 
 CASES = {

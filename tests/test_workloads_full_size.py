@@ -1,0 +1,151 @@
+"""The benchmark workloads at their own sizes, on the HIP path, against the oracle.
+
+* The 256^3 LHY droplet (box L = 40, flat top r = 6 scaled to N = 3000, seeded
+  1e-4 phase noise): 100 real-time RK4 steps at dt = 5e-4, relative L2 <= 1e-12
+  against the oracle, which follows src/stepper.cpp:46-70 and
+  src/functional.cpp:105-122; norm and energy logged every step (N(t), E(t)).
+  These are the kernels the headline steps/s times: the fused 256-point x
+  epilogue in its four RK stage modes, the 256-point y passes and the 256-point
+  fused z multiplier pass.
+* The batched droplet ground states: two instances of each grid class
+  (64^3, 96^3, 128^3, 192^3) of the 64-droplet batch (N log-uniform in
+  [50, 5000], seed 3, spacing 0.35), advanced 20 steps in imaginary time with
+  per-stage renormalisation (set up as src/main.cpp:131-145) in one batch,
+  each instance against the oracle run of that instance alone; then ground
+  states of the 64^3 and 96^3 members to 1e-10, whose energies and convergence
+  the oracle confirms from the returned fields.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return 8
+
+
+@pytest.mark.timeout(900)
+def test_droplet_256_rk4_100_steps_matches_oracle(oracle_mod):
+    from gp5_amd import capi, synth
+
+    O = oracle_mod
+    O.set_threads(_threads())
+    shape, left, right, psi0 = synth.droplet_256(256)
+    print(f"droplet 256^3: GPU 100 RK4 steps, oracle on {O.num_threads()} threads", flush=True)
+    ctx = capi.Context(shape, left, right, 1)
+    psi = ctx.field(psi0)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=5e-4, imaginary=False)
+    norms, ens = capi.advance(ctx, f, s, psi, 100, log=True)
+    got = psi.download_complex()
+    psi.free()
+    ctx.close()
+    geo = O.Geometry(shape, left, right, 1)
+    # N(t), E(t) at the start of steps 0, 10, ..., 90 (the log holds every step)
+    print("N(t):", [f"{v:.12f}" for v in norms[::10, 0]], flush=True)
+    print("E(t):", [f"{v:.10f}" for v in ens[::10]], flush=True)
+    assert norms[0, 0] == pytest.approx(O.norm(psi0, geo), rel=1e-12)
+    assert ens[0] == pytest.approx(O.energy(O.Functional(kind=O.OR_LHY), psi0, geo), rel=1e-11)
+    ref = O.advance(O.Functional(kind=O.OR_LHY), psi0, geo, stepper="RK4", dt=5e-4, imaginary=False, nsteps=100)
+    err = rel_l2(got, ref)
+    print(f"relative L2 after 100 steps: {err:.3e}", flush=True)
+    assert err <= 1e-12
+    # conserved in real time: |R4(i theta)| < 1 costs ~1e-13 of norm per step here
+    assert np.max(np.abs(norms[:, 0] - 3000.0)) / 3000.0 < 1e-10
+    assert np.max(np.abs(ens - ens[0])) / abs(ens[0]) < 1e-9
+
+
+def _class_members(count_per_class=2):
+    from gp5_amd import synth
+
+    n, L, N, psi0 = synth.droplet_batch(64, 3)
+    pick = []
+    for g in synth.BATCH_GRIDS:
+        idx = [i for i in range(len(n)) if n[i] == g]
+        assert len(idx) >= count_per_class, (g, len(idx))
+        pick += idx[:count_per_class]
+    return [n[i] for i in pick], [L[i] for i in pick], [N[i] for i in pick], [psi0[i] for i in pick], pick
+
+
+@pytest.mark.timeout(900)
+def test_batch_classes_64_to_192_match_oracle(oracle_mod):
+    from gp5_amd import capi
+
+    O = oracle_mod
+    O.set_threads(_threads())
+    n, L, N, psi0, pick = _class_members(2)
+    print(f"batch of droplets {pick} on grids {n}", flush=True)
+    b = capi.Batch(n, L, N)
+    assert b.classes == 4
+    for i, a in enumerate(psi0):
+        b.upload(i, a)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    dtf, nsteps = 0.1, 20
+    b.advance(f, dtf, nsteps)
+    out = [b.download(i) for i in range(len(n))]
+    b.close()
+    for i in range(len(n)):
+        dx = L[i] / n[i]
+        geo = O.Geometry((n[i],) * 3, (-L[i] / 2,) * 3, (L[i] / 2,) * 3, 1)
+        ref = O.advance(O.Functional(kind=O.OR_LHY), psi0[i][..., None], geo, stepper="RK4", dt=dtf * dx * dx,
+                        imaginary=True, nsteps=nsteps, renormalize=[N[i]])
+        err = rel_l2(out[i][..., None], ref)
+        print(f"instance {pick[i]} n={n[i]}: relative L2 {err:.2e}", flush=True)
+        assert err <= 1e-12, (pick[i], n[i])
+
+
+@pytest.mark.timeout(900)
+def test_batch_ground_states_64_96_confirmed_by_oracle(oracle_mod):
+    """Ground states (1e-10 relative energy change per 100 steps) of the 64^3 and 96^3
+    members; the oracle evaluates each returned field's energy and evolves it 100 more
+    steps, which must leave the energy within the same tolerance.  One 64^3 instance is
+    recomputed end to end on the oracle."""
+    from gp5_amd import capi
+
+    O = oracle_mod
+    O.set_threads(_threads())
+    n, L, N, psi0, pick = _class_members(2)
+    keep = [i for i in range(len(n)) if n[i] in (64, 96)]
+    n, L, N, psi0 = [n[i] for i in keep], [L[i] for i in keep], [N[i] for i in keep], [psi0[i] for i in keep]
+    b = capi.Batch(n, L, N)
+    for i, a in enumerate(psi0):
+        b.upload(i, a)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    dtf, tol, K = 0.1, 1e-10, 100
+    steps, en, ms = b.ground_state(f, dtf, tol=tol, check_every=K, max_steps=20000)
+    out = [b.download(i) for i in range(len(n))]
+    b.close()
+    print("steps", steps.tolist(), "E/N", [e / v for e, v in zip(en, N)], f"{ms:.0f} ms", flush=True)
+    assert np.all(steps > 0)
+    F = O.Functional(kind=O.OR_LHY)
+    for i in range(len(n)):
+        dx = L[i] / n[i]
+        geo = O.Geometry((n[i],) * 3, (-L[i] / 2,) * 3, (L[i] / 2,) * 3, 1)
+        a = out[i][..., None]
+        assert O.norm(a, geo) == pytest.approx(N[i], rel=1e-12)
+        e0 = O.energy(F, a, geo)
+        # en[i] is the energy at the start of the last step; the field is one step later
+        assert e0 == pytest.approx(en[i], rel=2 * tol)
+        a1 = O.advance(F, a, geo, stepper="RK4", dt=dtf * dx * dx, imaginary=True, nsteps=K, renormalize=[N[i]])
+        e1 = O.energy(F, a1, geo)
+        print(f"n={n[i]} N={N[i]:.1f}: E/N {e0 / N[i]:.12f}, oracle change over {K} steps {abs(e1 - e0) / abs(e0):.2e}",
+              flush=True)
+        assert abs(e1 - e0) <= tol * abs(e0)
+    # end to end on the oracle for the first 64^3 member
+    i = 0
+    assert n[i] == 64
+    dx = L[i] / n[i]
+    geo = O.Geometry((64,) * 3, (-L[i] / 2,) * 3, (L[i] / 2,) * 3, 1)
+    ref = O.advance(F, psi0[i][..., None], geo, stepper="RK4", dt=dtf * dx * dx, imaginary=True, nsteps=int(steps[i]),
+                    renormalize=[N[i]])
+    err = rel_l2(out[i][..., None], ref)
+    print(f"64^3 member after {int(steps[i])} steps: relative L2 vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-11
