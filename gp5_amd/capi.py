@@ -37,7 +37,7 @@ EXPORTS = [
     "gp5_ctx_points", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
-    "gp5_phase_constraint", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
+    "gp5_phase_constraint", "gp5_axpy", "gp5_rk4_combine", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
     "gp5_kernel_kind_name", "gp5_resample", "gp5_decompose", "gp5_h5_write", "gp5_h5_read", "gp5_h5_shape",
     "gp5_batch_create", "gp5_batch_destroy", "gp5_batch_classes", "gp5_batch_class_of",
     "gp5_batch_upload", "gp5_batch_download", "gp5_batch_set_lhy", "gp5_batch_set_potential",
@@ -106,6 +106,8 @@ def load() -> ctypes.CDLL:
         "gp5_max_density": ([P, P, I, DP], I),
         "gp5_energy": ([P, ctypes.POINTER(FunctionalDesc), P, DP], I),
         "gp5_phase_constraint": ([P, P, P], I),
+        "gp5_axpy": ([P, P, P, D, D, P], I),
+        "gp5_rk4_combine": ([P, P, P, D, D, P, P, P, P], I),
         "gp5_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, P, P], I),
         "gp5_advance_timed": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP], I),
         "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
@@ -320,6 +322,18 @@ def energy(ctx: Context, f: FunctionalDesc, psi: Buffer) -> float:
 
 def phase_constraint(ctx: Context, psi: Buffer, phase: Buffer) -> None:
     check(load().gp5_phase_constraint(ctx.handle, psi.handle, phase.handle))
+
+
+def axpy(ctx: Context, out: Buffer, a: Buffer, c: complex, k: Buffer) -> None:
+    """out = a - c k (one stage update of src/stepper.cpp:30-70 as its own pass)."""
+    check(load().gp5_axpy(ctx.handle, out.handle, a.handle, float(np.real(c)), float(np.imag(c)), k.handle))
+
+
+def rk4_combine(ctx: Context, out: Buffer, a: Buffer, c: complex, k1: Buffer, k2: Buffer, k3: Buffer,
+                k4: Buffer) -> None:
+    """out = a - c (((k1 + 2 k2) + 2 k3) + k4) (src/stepper.cpp:64)."""
+    check(load().gp5_rk4_combine(ctx.handle, out.handle, a.handle, float(np.real(c)), float(np.imag(c)), k1.handle,
+                                 k2.handle, k3.handle, k4.handle))
 
 
 def advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int, *,

@@ -32,6 +32,10 @@ hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int com
 hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax);
 hipError_t launch_phase(hipStream_t st, double2 *f, const double *phase, long long len);
 hipError_t launch_phase_exp(hipStream_t st, double2 *ex, const double *phase, long long len);
+hipError_t launch_axpy(hipStream_t st, double2 *out, const double2 *a, double cr, double ci, const double2 *k,
+                       long long len);
+hipError_t launch_rk4_combine(hipStream_t st, double2 *out, const double2 *a, double cr, double ci, const double2 *k1,
+                              const double2 *k2, const double2 *k3, const double2 *k4, long long len);
 hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks, long long lpb, long long lpc, int ncomp);
 }  // namespace gp5
 
@@ -967,6 +971,33 @@ int gp5_energy(gp5_ctx *c, const gp5_functional_desc *f, const gp5_buf *psi, dou
     double e = 0.0;
     for (int k = 0; k < c->nc; ++k) e += r[(size_t)c->nc + k];
     *out = e;
+    return GP5_OK;
+}
+
+int gp5_axpy(gp5_ctx *c, gp5_buf *out, const gp5_buf *a, double cr, double ci, const gp5_buf *k)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, out, "out"));
+    RCHK(check_field(c, a, "a"));
+    RCHK(check_field(c, k, "k"));
+    HIPCHK(launch_axpy(c->st, (double2 *)out->d, (const double2 *)a->d, cr, ci, (const double2 *)k->d, c->M * c->nc));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return GP5_OK;
+}
+
+int gp5_rk4_combine(gp5_ctx *c, gp5_buf *out, const gp5_buf *a, double cr, double ci, const gp5_buf *k1,
+                    const gp5_buf *k2, const gp5_buf *k3, const gp5_buf *k4)
+{
+    if (!c) return fail(GP5_ERR_ARG, "NULL context");
+    RCHK(check_field(c, out, "out"));
+    RCHK(check_field(c, a, "a"));
+    RCHK(check_field(c, k1, "k1"));
+    RCHK(check_field(c, k2, "k2"));
+    RCHK(check_field(c, k3, "k3"));
+    RCHK(check_field(c, k4, "k4"));
+    HIPCHK(launch_rk4_combine(c->st, (double2 *)out->d, (const double2 *)a->d, cr, ci, (const double2 *)k1->d,
+                              (const double2 *)k2->d, (const double2 *)k3->d, (const double2 *)k4->d, c->M * c->nc));
+    HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
 

@@ -314,6 +314,30 @@ __global__ __launch_bounds__(kThreads) void k_phase(double2 *f, const double *__
     }
 }
 
+// out = a - c k, c = cr + i ci (the stage update fieldDataOld - timeStep() * f * k of
+// src/stepper.cpp:35,50,55,60 as a separate pass; out may alias a or k)
+__global__ __launch_bounds__(kThreads) void k_axpy(double2 *out, const double2 *a, double cr, double ci,
+                                                   const double2 *k, long long len)
+{
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < len; i += (long long)gridDim.x * kThreads) {
+        const double2 kv = k[i], av = a[i];
+        out[i] = make_double2(av.x - (cr * kv.x - ci * kv.y), av.y - (cr * kv.y + ci * kv.x));
+    }
+}
+
+// out = a - c (((k1 + 2 k2) + 2 k3) + k4) (src/stepper.cpp:64, in the reference's sum order)
+__global__ __launch_bounds__(kThreads) void k_rk4_combine(double2 *out, const double2 *a, double cr, double ci,
+                                                          const double2 *k1, const double2 *k2, const double2 *k3,
+                                                          const double2 *k4, long long len)
+{
+    for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < len; i += (long long)gridDim.x * kThreads) {
+        const double2 v1 = k1[i], v2 = k2[i], v3 = k3[i], v4 = k4[i], av = a[i];
+        const double sr = ((v1.x + 2 * v2.x) + 2 * v3.x) + v4.x;
+        const double si = ((v1.y + 2 * v2.y) + 2 * v3.y) + v4.y;
+        out[i] = make_double2(av.x - (cr * sr - ci * si), av.y - (cr * si + ci * sr));
+    }
+}
+
 // ex = e^{i phase}: the phaseConstraint factor, precomputed once per advance as
 // the reference does in its constructor (src/constraint.cpp:12-17)
 __global__ __launch_bounds__(kThreads) void k_phase_exp(double2 *ex, const double *__restrict__ phase, long long len)
@@ -380,6 +404,22 @@ hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int com
 hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax)
 {
     hipLaunchKernelGGL(k_maxdens, dim3(blocks), dim3(kThreads), 0, st, f, len, blockmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(hipStream_t st, double2 *out, const double2 *a, double cr, double ci, const double2 *k,
+                       long long len)
+{
+    unsigned blocks = (unsigned)std::min<long long>((len + kThreads - 1) / kThreads, 8192);
+    hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(kThreads), 0, st, out, a, cr, ci, k, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_rk4_combine(hipStream_t st, double2 *out, const double2 *a, double cr, double ci, const double2 *k1,
+                              const double2 *k2, const double2 *k3, const double2 *k4, long long len)
+{
+    unsigned blocks = (unsigned)std::min<long long>((len + kThreads - 1) / kThreads, 8192);
+    hipLaunchKernelGGL(k_rk4_combine, dim3(blocks), dim3(kThreads), 0, st, out, a, cr, ci, k1, k2, k3, k4, len);
     return hipGetLastError();
 }
 

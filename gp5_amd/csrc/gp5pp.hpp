@@ -30,6 +30,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <typeinfo>
 #include <vector>
 
 #include "../../include/gp5.h"
@@ -201,6 +202,13 @@ public:
     gp5_buf *buf() const { return _buf->get(); }
     gp5_ctx *ctx() const { return _buf->context()->get(); }
     const std::shared_ptr<discretization> &getDiscretization() const { return _discr; }
+    // Exchanges the device storage of two fields of one shape (the reference's
+    // timeStepper re-points its field at the stepper's buffer, src/gpCpp.cpp:255-269).
+    void swap(tensor_t &o)
+    {
+        if (o._dims != _dims) throw std::runtime_error("swap of fields of different shapes");
+        std::swap(_buf, o._buf);
+    }
 
 private:
     std::shared_ptr<discretization> _discr;
@@ -208,6 +216,19 @@ private:
     std::array<int, 4> _dims;
     std::shared_ptr<device_buffer> _buf;
 };
+
+// out = a - c k and out = a - c (((k1 + 2 k2) + 2 k3) + k4): the Eigen stage
+// expressions of src/stepper.cpp:35,50-64 as device passes.
+inline void axpy(tensor_t &out, const tensor_t &a, complex_t c, const tensor_t &k)
+{
+    check(gp5_axpy(out.ctx(), out.buf(), a.buf(), c.real(), c.imag(), k.buf()));
+}
+
+inline void rk4Combine(tensor_t &out, const tensor_t &a, complex_t c, const tensor_t &k1, const tensor_t &k2,
+                       const tensor_t &k3, const tensor_t &k4)
+{
+    check(gp5_rk4_combine(out.ctx(), out.buf(), a.buf(), c.real(), c.imag(), k1.buf(), k2.buf(), k3.buf(), k4.buf()));
+}
 
 // Device-resident real field (potential, phase): M * nC doubles.
 class realTensor_t {
@@ -364,8 +385,13 @@ public:
         check(gp5_energy(psi.ctx(), &d, psi.buf(), &e));
         return e;
     }
-    // The parameters as the C ABI descriptor.
-    virtual gp5_functional_desc desc() const = 0;
+    // The parameters as the C ABI descriptor.  A functional defined outside this
+    // header has none: steppers then call its apply() per stage (as the reference's
+    // steppers do through the virtual call, src/stepper.cpp:32,47-59).
+    virtual gp5_functional_desc desc() const
+    {
+        throw std::runtime_error("this functional has no device descriptor (it is applied through apply())");
+    }
 
 protected:
     gp5_functional_desc base_desc(int kind) const
@@ -655,6 +681,13 @@ inline void initGaussian(real_t sigma, std::shared_ptr<discretization> discr, te
 
 // ---------------------------------------------------------------------------
 // steppers (src/stepper.h:8-97, src/stepper.cpp:7-81)
+//
+// With a functional and a constraint of the kinds above, a step runs on the fused
+// device path (gp5_advance: every stage is five FFT passes with the functional and
+// the stage algebra in the epilogue).  Any other functional or constraint (a class
+// derived by the user, as the reference's virtual interfaces allow) is honoured
+// through its own apply(): the step then runs stage by stage as the reference
+// writes it, with the stage algebra and renormalisation as device passes.
 // ---------------------------------------------------------------------------
 class stepper {
 public:
@@ -663,16 +696,53 @@ public:
     // One step old -> new (the reference's signature).
     virtual void advance(tensor_t &fieldOld, tensor_t &fieldNew, real_t time)
     {
-        (void)time;
-        fieldNew.copyFrom(fieldOld);
-        advance(fieldNew, 1);
+        if (fused()) {
+            fieldNew.copyFrom(fieldOld);
+            advance(fieldNew, 1);
+        } else {
+            stageStep(fieldOld, fieldNew, time);
+        }
     }
     // nSteps in place on the device: the hot loop (one call, no host round trip).
-    void advance(tensor_t &field, int nSteps, double *normLog = nullptr, double *energyLog = nullptr)
+    // normLog[s*nC + c] / energyLog[s]: norm and energy at the start of step s
+    // (the energy is NaN for a functional without a device descriptor).  time: the
+    // time of the first step, handed to a user functional's apply (the fused
+    // functionals do not depend on it, as in the reference).
+    void advance(tensor_t &field, int nSteps, double *normLog = nullptr, double *energyLog = nullptr,
+                 real_t time = 0)
     {
-        gp5_functional_desc f = _func->desc();
-        gp5_stepper_desc s = desc();
-        check(gp5_advance(field.ctx(), &f, &s, field.buf(), nSteps, normLog, energyLog));
+        if (!_func) throw std::runtime_error("stepper has no functional");
+        if (fused()) {
+            gp5_functional_desc f = _func->desc();
+            gp5_stepper_desc s = desc();
+            check(gp5_advance(field.ctx(), &f, &s, field.buf(), nSteps, normLog, energyLog));
+            return;
+        }
+        if (!_next || _next->dimensions() != field.dimensions())
+            _next = std::make_shared<tensor_t>(field.getDiscretization(), field.nComponents());
+        const int nc = field.nComponents();
+        real_t t = time;
+        for (int st = 0; st < nSteps; ++st) {
+            for (int c = 0; c < nc && normLog; ++c) normLog[(size_t)st * nc + c] = norm(field, c);
+            if (energyLog) energyLog[st] = std::nan("");
+            stageStep(field, *_next, t);
+            field.swap(*_next);
+            t += std::abs(_timeStep);
+        }
+    }
+    // True when the functional and the constraint are exactly the kinds the
+    // fused device path implements.
+    bool fused() const
+    {
+        if (!_func) return false;
+        const functional &f = *_func;
+        const std::type_info &tf = typeid(f);
+        const bool fk = tf == typeid(gpFunctional) || tf == typeid(LHYDropletFunctional) ||
+                        tf == typeid(LHYDropletUnlockedFunctional);
+        if (!_constraint) return fk;
+        const constraint &c = *_constraint;
+        const std::type_info &tc = typeid(c);
+        return fk && (tc == typeid(nullConstraint) || tc == typeid(phaseConstraint));
     }
     void setTimeStep(complex_t timeStep) { _timeStep = timeStep; }
     complex_t timeStep() const { return _timeStep; }
@@ -692,6 +762,27 @@ public:
 
 protected:
     virtual int kind() const = 0;
+    // One step old -> new through the functional's and the constraint's own apply.
+    virtual void stageStep(tensor_t &fieldOld, tensor_t &fieldNew, real_t time) = 0;
+    // stepper::normalize (src/stepper.cpp:7-22)
+    void normalizeField(tensor_t &f)
+    {
+        if (!_reNormalize) return;
+        for (int c = 0; c < f.nComponents(); ++c) {
+            if ((size_t)c >= _normalizations.size()) throw std::runtime_error("missing normalization");
+            gp::normalize(_normalizations[c], f, c);
+        }
+    }
+    void applyConstraint(tensor_t &f)
+    {
+        if (_constraint) _constraint->apply(f);
+    }
+    tensor_t &scratch(std::shared_ptr<tensor_t> &p, const tensor_t &like)
+    {
+        if (!p || p->dimensions() != like.dimensions())
+            p = std::make_shared<tensor_t>(like.getDiscretization(), like.nComponents());
+        return *p;
+    }
     gp5_stepper_desc desc() const
     {
         if (!_func) throw std::runtime_error("stepper has no functional");
@@ -714,16 +805,50 @@ private:
     std::shared_ptr<discretization> _discr;
     bool _reNormalize = false;
     std::shared_ptr<constraint> _constraint;
+    std::shared_ptr<tensor_t> _next;  // the other state buffer of the stage-by-stage path
 };
 
 class euleroStepper : public stepper {
 protected:
     int kind() const override { return GP5_STEP_EULERO; }
+    // src/stepper.cpp:30-43
+    void stageStep(tensor_t &fieldOld, tensor_t &fieldNew, real_t time) override
+    {
+        getFunctional()->apply(fieldOld, fieldNew, time);
+        axpy(fieldNew, fieldOld, timeStep(), fieldNew);
+        applyConstraint(fieldNew);
+        normalizeField(fieldNew);
+    }
 };
 
 class RK4Stepper : public stepper {
 protected:
     int kind() const override { return GP5_STEP_RK4; }
+    // src/stepper.cpp:46-70 (k1..k4 are allocated on first use, not in init())
+    void stageStep(tensor_t &fieldOld, tensor_t &fieldNew, real_t time) override
+    {
+        tensor_t &k1 = scratch(_k[0], fieldOld), &k2 = scratch(_k[1], fieldOld);
+        tensor_t &k3 = scratch(_k[2], fieldOld), &k4 = scratch(_k[3], fieldOld);
+        auto f = getFunctional();
+        const complex_t tau = timeStep();
+        const real_t h = std::abs(tau);
+        f->apply(fieldOld, k1, time);
+        axpy(fieldNew, fieldOld, tau * 0.5, k1);
+        normalizeField(fieldNew);
+        f->apply(fieldNew, k2, time + 0.5 * h);
+        axpy(fieldNew, fieldOld, tau * 0.5, k2);
+        normalizeField(fieldNew);
+        f->apply(fieldNew, k3, time + 0.5 * h);
+        axpy(fieldNew, fieldOld, tau, k3);
+        normalizeField(fieldNew);
+        f->apply(fieldNew, k4, time + h);
+        rk4Combine(fieldNew, fieldOld, (1. / 6) * tau, k1, k2, k3, k4);
+        applyConstraint(fieldNew);
+        normalizeField(fieldNew);
+    }
+
+private:
+    std::shared_ptr<tensor_t> _k[4];
 };
 
 }  // namespace gp

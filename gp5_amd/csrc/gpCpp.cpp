@@ -211,9 +211,9 @@ public:
     {
         {
             py::gil_scoped_release nogil;
-            _stepper->advance(*f.getState(), nSteps);
+            _stepper->advance(*f.getState(), nSteps, nullptr, nullptr, t);
         }
-        t += _timeStep * nSteps;
+        tick(nSteps);
     }
     // nSteps, returning per-step norms [nSteps][nC] and energies [nSteps] of the
     // state at the start of each step (fused reductions; addition)
@@ -225,14 +225,21 @@ public:
         double *pn = norms.mutable_data(), *pe = ens.mutable_data();
         {
             py::gil_scoped_release nogil;
-            _stepper->advance(*f.getState(), nSteps, pn, pe);
+            _stepper->advance(*f.getState(), nSteps, pn, pe, t);
         }
-        t += _timeStep * nSteps;
+        tick(nSteps);
         return py::make_tuple(norms, ens);
     }
     real_t getTime() { return t; }
 
 private:
+    // t += dt once per step, as the reference's loop does (src/gpCpp.cpp:260-265),
+    // so getTime() carries the same rounding as nSteps single additions
+    void tick(int nSteps)
+    {
+        for (int i = 0; i < nSteps; ++i) t += _timeStep;
+    }
+
     std::shared_ptr<gp::stepper> _stepper;
     real_t _timeStep;
     real_t t = 0.0;

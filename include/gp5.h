@@ -124,6 +124,17 @@ int gp5_energy(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_buf *psi, d
 /* psi <- |psi| e^{i phase}.  Replaces phaseConstraint::apply (src/constraint.cpp:6-9). */
 int gp5_phase_constraint(gp5_ctx *ctx, gp5_buf *psi, const gp5_buf *phase);
 
+/* Stage algebra as separate passes, for steppers that call a functional's own
+ * apply per stage (a functional or constraint the fused path does not know):
+ * out = a - (cr + i ci) * k replaces the Eigen expression
+ * fieldDataOld - timeStep()*f*k (src/stepper.cpp:35,50,55,60), and
+ * out = a - (cr + i ci) * (((k1 + 2 k2) + 2 k3) + k4) replaces
+ * fieldDataOld - (1./6)*timeStep()*(k1 + 2*k2 + 2*k3 + k4) (src/stepper.cpp:64).
+ * out may alias any input. */
+int gp5_axpy(gp5_ctx *ctx, gp5_buf *out, const gp5_buf *a, double cr, double ci, const gp5_buf *k);
+int gp5_rk4_combine(gp5_ctx *ctx, gp5_buf *out, const gp5_buf *a, double cr, double ci, const gp5_buf *k1,
+                    const gp5_buf *k2, const gp5_buf *k3, const gp5_buf *k4);
+
 /* Advance psi in place by nsteps of euleroStepper / RK4Stepper
  * (src/stepper.cpp:30-81) as driven by pyInterface::timeStepper::advance
  * (src/gpCpp.cpp:255-269).  Optional per-step logs (host arrays, may be NULL):
