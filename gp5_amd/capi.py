@@ -37,7 +37,7 @@ EXPORTS = [
     "gp5_ctx_points", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
-    "gp5_phase_constraint", "gp5_axpy", "gp5_rk4_combine", "gp5_advance", "gp5_advance_timed", "gp5_profile_advance",
+    "gp5_phase_constraint", "gp5_axpy", "gp5_rk4_combine", "gp5_advance", "gp5_advance_noalloc", "gp5_advance_timed", "gp5_profile_advance",
     "gp5_kernel_kind_name", "gp5_resample", "gp5_decompose", "gp5_h5_write", "gp5_h5_read", "gp5_h5_shape",
     "gp5_batch_create", "gp5_batch_destroy", "gp5_batch_classes", "gp5_batch_class_of",
     "gp5_batch_upload", "gp5_batch_download", "gp5_batch_set_lhy", "gp5_batch_set_potential",
@@ -109,6 +109,7 @@ def load() -> ctypes.CDLL:
         "gp5_axpy": ([P, P, P, D, D, P], I),
         "gp5_rk4_combine": ([P, P, P, D, D, P, P, P, P], I),
         "gp5_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, P, P], I),
+        "gp5_advance_noalloc": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I], I),
         "gp5_advance_timed": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP], I),
         "gp5_profile_advance": ([P, ctypes.POINTER(FunctionalDesc), ctypes.POINTER(StepperDesc), P, I, DP,
                                  ctypes.POINTER(ctypes.c_int)], I),
@@ -348,6 +349,11 @@ def advance(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps
         return norms, ens
     check(L.gp5_advance(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps), None, None))
     return None
+
+
+def advance_noalloc(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int) -> None:
+    """gp5_advance_noalloc: advance on the library's static device scratch (no allocation)."""
+    check(load().gp5_advance_noalloc(ctx.handle, ctypes.byref(f), ctypes.byref(s), psi.handle, int(nsteps)))
 
 
 def advance_timed(ctx: Context, f: FunctionalDesc, s: StepperDesc, psi: Buffer, nsteps: int) -> float:

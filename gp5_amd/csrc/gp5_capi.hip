@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -176,6 +177,36 @@ int set_error(int code, const char *msg)
         int r_ = (expr);         \
         if (r_ != GP5_OK) return r_; \
     } while (0)
+
+// ---------------------------------------------------------------------------
+// static scratch of gp5_advance_noalloc
+//
+// One file-scope device array, held by every GPU the library runs on until the
+// process ends.  In host code its name is only a stand-in for the symbol: its
+// device address (one per GPU) comes from hipGetSymbolAddress, taken once per
+// device at context creation with that device current, and only that address is
+// ever handed to kernels and copies.  Calls of gp5_advance_noalloc that overlap
+// on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)200 << 20;  // 200 MiB
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+static std::mutex g_scratch_mtx;
+static double2 *g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device, or null
+
+// Called by gp5_ctx_create with `device` current.
+static int scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(GP5_ERR_UNSUPPORTED, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mtx);
+    if (g_scratch_ptr[device]) return GP5_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(GP5_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (double2 *)ptr;
+    return GP5_OK;
+}
 
 // ---------------------------------------------------------------------------
 // plans
@@ -566,6 +597,8 @@ int gp5_ctx_create(gp5_ctx **out, const int shape[3], const double left[3], cons
     int rc = GP5_OK;
     auto bail = [&](int code) { gp5_ctx_destroy(c); return code; };
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(GP5_ERR_HIP, "no HIP device"));
+    rc = scratch_setup(c->device);
+    if (rc != GP5_OK) return bail(rc);
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(GP5_ERR_HIP, "hipStreamCreate failed"));
     for (int d = 0; d < 3; ++d) {
@@ -1217,6 +1250,90 @@ int gp5_advance_timed(gp5_ctx *c, const gp5_functional_desc *f, const gp5_steppe
 {
     if (!elapsed_ms) return fail(GP5_ERR_ARG, "NULL elapsed_ms");
     return advance_impl(c, f, s, psi, nsteps, nullptr, nullptr, elapsed_ms);
+}
+
+// gp5_advance on the static scratch: the stage fields (T, K twice, S, the output
+// field, e^{i phase}) and the epilogue's reduction partials are carved out of
+// `scratch` instead of the context's lazily allocated buffers, which are left as
+// they are.  Checks and enqueue only: nothing here allocates device or host memory.
+int gp5_advance_noalloc(gp5_ctx *c, const gp5_functional_desc *f, const gp5_stepper_desc *s, gp5_buf *psi, int nsteps)
+{
+    if (!c || !f || !s) return fail(GP5_ERR_ARG, "NULL argument");
+    RCHK(check_field(c, psi, "psi"));
+    if (s->kind != GP5_STEP_EULERO && s->kind != GP5_STEP_RK4) return fail(GP5_ERR_ARG, "unknown stepper kind %d", s->kind);
+    if (nsteps < 0) return fail(GP5_ERR_ARG, "nsteps < 0");
+    if (s->renormalize) {
+        if (c->nc > 2 && !c->nnorm)
+            return fail(GP5_ERR_UNSUPPORTED, "renormalisation of more than two components needs per-component targets (batch API)");
+        for (int k = 0; k < std::min(c->nc, 2); ++k)
+            if (!c->nnorm && !(s->normalizations[k] > 0.0)) return fail(GP5_ERR_ARG, "normalization %d must be > 0", k);
+    }
+    if (s->phase) RCHK(check_real(c, s->phase, "phase"));
+    if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+        return fail(GP5_ERR_HIP, "static scratch of device %d is not set up", c->device);
+    if (nsteps == 0) return GP5_OK;
+    // carve-out: fields of M * nc_cap complex values at 256-byte offsets, then partials
+    const bool rk = s->kind == GP5_STEP_RK4;
+    const int nfields = (rk ? 5 : 2) + (s->phase ? 1 : 0);
+    const size_t field_bytes = ((size_t)c->M * c->nc_cap * sizeof(double2) + 255) & ~(size_t)255;
+    size_t blocks = axis_grid(c->geom[0]);
+    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O));
+    const size_t partial_bytes = blocks * kRedQ * sizeof(double);
+    const size_t need = (size_t)nfields * field_bytes + partial_bytes;
+    if (need > kScratchBytes)
+        return fail(GP5_ERR_UNSUPPORTED,
+                    "gp5_advance_noalloc: this field needs %zu B of scratch (%d fields of %zu B + %zu B of partials); "
+                    "the static scratch holds %zu B",
+                    need, nfields, field_bytes, partial_bytes, kScratchBytes);
+    char *base = (char *)g_scratch_ptr[c->device];
+    double2 *region[6];
+    for (int k = 0; k < nfields; ++k) region[k] = (double2 *)(base + (size_t)k * field_bytes);
+    // the context's own scratch is set aside for the call and restored afterwards
+    double2 *keep_T = c->T, *keep_kA = c->kA, *keep_kB = c->kB, *keep_s = c->sbuf, *keep_e = c->eout;
+    double2 *keep_ph = c->phase_ex;
+    double *keep_part = c->partials;
+    const size_t keep_cap = c->partials_cap;
+    int r = 0;
+    c->T = region[r++];
+    if (rk) {
+        c->kA = region[r++];
+        c->kB = region[r++];
+        c->sbuf = region[r++];
+    }
+    c->eout = region[r++];
+    if (s->phase) c->phase_ex = region[r++];
+    c->partials = (double *)(base + (size_t)nfields * field_bytes);
+    c->partials_cap = blocks;
+    double *const home = psi->d;
+    int rc = GP5_OK;
+    if (s->phase) {
+        hipError_t e = launch_phase_exp(c->st, c->phase_ex, s->phase->d, c->M * c->nc);
+        if (e != hipSuccess) rc = fail(GP5_ERR_HIP, "launch_phase_exp: %s", hipGetErrorString(e));
+    }
+    if (rc == GP5_OK) rc = advance_enqueue(c, f, s, psi, nsteps, false);
+    // after an odd number of steps the state sits in the scratch output field
+    if (psi->d != home) {
+        hipError_t e = hipMemcpyAsync(home, psi->d, psi->n * sizeof(double), hipMemcpyDeviceToDevice, c->st);
+        if (rc == GP5_OK && e != hipSuccess) rc = fail(GP5_ERR_HIP, "copy of the state home: %s", hipGetErrorString(e));
+        psi->d = home;
+    }
+    double last[kMaxComp];
+    hipError_t e = hipMemcpyAsync(last, c->results, (size_t)c->nc * sizeof(double), hipMemcpyDeviceToHost, c->st);
+    hipError_t es = hipStreamSynchronize(c->st);
+    c->T = keep_T;
+    c->kA = keep_kA;
+    c->kB = keep_kB;
+    c->sbuf = keep_s;
+    c->eout = keep_e;
+    c->phase_ex = keep_ph;
+    c->partials = keep_part;
+    c->partials_cap = keep_cap;
+    RCHK(rc);
+    if (e != hipSuccess || es != hipSuccess)
+        return fail(GP5_ERR_HIP, "gp5_advance_noalloc: %s", hipGetErrorString(e != hipSuccess ? e : es));
+    for (int k = 0; k < c->nc; ++k)
+        if (!std::isfinite(last[k])) return fail(GP5_ERR_NONFINITE, "norm of component %d became %g during the step", k, last[k]);
+    return GP5_OK;
 }
 
 const char *gp5_kernel_kind_name(int kind)

@@ -143,6 +143,20 @@ int gp5_rk4_combine(gp5_ctx *ctx, gp5_buf *out, const gp5_buf *a, double cr, dou
 int gp5_advance(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
                 gp5_buf *psi, int nsteps, double *norm_log, double *energy_log);
 
+/* gp5_advance (no logs) that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call, on the device or the host.  No counterpart in the reference;
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  The stage fields and the
+ * reduction partials live in one static device array of 200 MiB per GPU, held
+ * until the process ends: RK4 needs 5 fields of 16*M*nC bytes (6 with a phase
+ * constraint), Euler 2 (3), plus 48 B per x-pass block, e.g. 160 MiB for one
+ * 128^3 component; larger inputs return GP5_ERR_UNSUPPORTED.  The context must
+ * exist (gp5_ctx_create allocates, once).  Results equal gp5_advance's.  Calls
+ * that overlap on one GPU share the array and race; calls in sequence are safe
+ * (nothing in the array needs to start at zero). */
+int gp5_advance_noalloc(gp5_ctx *ctx, const gp5_functional_desc *f, const gp5_stepper_desc *s,
+                        gp5_buf *psi, int nsteps);
+
 /* Time the hot loop: run nsteps of gp5_advance with no logs and report the
  * elapsed device time between HIP events recorded on the context's stream,
  * in milliseconds. */
