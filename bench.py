@@ -276,13 +276,16 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
 def collision_large(steps: int, n: int = 512):
     """Two N = 4000 LHY droplets colliding head-on in a 512^3 periodic box L = 64
     (dx = 0.125), centres x = -8 and +8, momenta +-0.49 (0.5 snapped to 2 pi m / L);
-    real-time RK4, dt = 2.5e-4.  Timed like the headline run: `steps` steps between
-    stream synchronisations, after 2 warmup steps; norm and energy from the fused
-    reductions before and after."""
-    from gp5_amd import capi, synth
+    real-time RK4, dt = 2.5e-4.  The droplets are one ground state of the box
+    (learned warm start on the 256^3 coarse grid, FP64 imaginary time to 1e-10 on
+    both levels, gp5_amd.ground_state.droplet_pair), moved to +-8.  Timed like the
+    headline run: `steps` steps between stream synchronisations, after 2 warmup
+    steps; norm and energy from the fused reductions before and after."""
+    from gp5_amd import capi, ground_state
 
-    _progress(f"two-droplet collision at {n}^3")
-    shape, left, right, psi0, q = synth.droplet_collision(n)
+    _progress(f"two-droplet collision at {n}^3: ground state of one droplet")
+    shape, left, right, psi0, q, init = ground_state.droplet_pair(n)
+    _progress(f"ground state in {init['seconds']:.1f} s; collision")
     ctx = capi.Context(shape, left, right, 1)
     psi = ctx.field(psi0)
     del psi0
@@ -308,8 +311,80 @@ def collision_large(steps: int, n: int = 512):
         "norm_start": float(norms0[0, 0]), "norm_end": float(norms1[0, 0]),
         "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
         "steps_between": steps + 2,
+        "initial_state": init,
     }
     psi.free()
+    ctx.close()
+    return out
+
+
+def config1_harmonic(gpu_steps: int = 2000, cpu: bool = True, cpu_budget_s: float = 20.0):
+    """The reference's CPU-runnable case: single-component GP (g = 1, mass 1), 64^3 on
+    [-10, 10]^3, harmonic trap omega = (1, 1, 1), N = 1, Gaussian sigma = 1.5 plus
+    seeded 1e-3 complex noise, imaginary-time Euler dt = 1e-3 renormalised every step,
+    2000 steps.  GPU steps/s over the 2000 steps (field resident in HBM); GPU vs the
+    CPU oracle after 100 steps (relative L2); the oracle's steps/s on all host cores
+    (up to 16, the 2000 steps in blocks of 100 within cpu_budget_s) and on one core
+    (the reference as shipped: one rank, FFTW without threads; 100 steps); mu and E
+    of the final state."""
+    from gp5_amd import capi, synth
+
+    shape, left, right, psi0, V = synth.harmonic_64()
+    ctx = capi.Context(shape, left, right, 1)
+    vb = ctx.real(V)
+    f = capi.functional_desc(capi.GP5_FUNC_GP, couplings=[[1.0]], potential=vb)
+    s = capi.stepper_desc(capi.GP5_STEP_EULERO, dt=1e-3, imaginary=True, renormalize=[1.0])
+    psi = ctx.field(psi0)
+    capi.advance(ctx, f, s, psi, 100)
+    got100 = psi.download_complex()
+    psi.upload_complex(psi0)
+    capi.advance(ctx, f, s, psi, 20)  # warm-up
+    psi.upload_complex(psi0)
+    ctx.sync()
+    t0 = time.perf_counter()
+    capi.advance(ctx, f, s, psi, gpu_steps)
+    ctx.sync()
+    gpu_s = time.perf_counter() - t0
+    # mu = <psi|H|psi> / N of the final state, E from the energy functional
+    H = ctx.field()
+    capi.functional_apply(ctx, f, psi, H)
+    a, h = psi.download_complex(), H.download_complex()
+    dV = synth.dV(shape, left, right)
+    out = {"workload": "GP g=1, 64^3 on [-10,10]^3, harmonic omega=(1,1,1), N=1, Gaussian sigma=1.5 + 1e-3 noise "
+                       "(seed 1), imaginary-time Euler dt=1e-3, renormalised every step, 2000 steps",
+           "gpu_steps_per_s": gpu_steps / gpu_s, "gpu_steps": gpu_steps,
+           "mu": float(np.real(np.vdot(a, h)) * dV), "energy": capi.energy(ctx, f, psi)}
+    if cpu:
+        from oracle import oracle as O
+
+        O.build()
+        cores = min(_affinity(), 16)
+        O.set_threads(cores)
+        geo = O.Geometry(shape, left, right, 1)
+        F = O.Functional(kind=O.OR_GP, couplings=np.array([[1.0]]), V=V)
+        t0 = time.perf_counter()
+        ref = O.advance(F, psi0, geo, stepper="eulero", dt=1e-3, imaginary=True, nsteps=100, renormalize=[1.0])
+        done, el = 100, time.perf_counter() - t0
+        out["rel_l2_vs_cpu_at_step_100"] = float(np.linalg.norm((got100 - ref).ravel()) / np.linalg.norm(ref.ravel()))
+        cur = ref
+        while done < gpu_steps and el * (done + 100) / done < cpu_budget_s:
+            cur = O.advance(F, cur, geo, stepper="eulero", dt=1e-3, imaginary=True, nsteps=100, renormalize=[1.0])
+            done += 100
+            el = time.perf_counter() - t0
+        out["cpu_all_cores"] = {"value": done / el, "unit": "steps/s", "cores": cores, "steps": done,
+                                "sample": f"{done} of the 2000 Euler steps, CPU oracle with OpenMP over {cores} threads"}
+        if done == gpu_steps:
+            out["rel_l2_vs_cpu_at_step_2000"] = float(np.linalg.norm((a - cur).ravel()) / np.linalg.norm(cur.ravel()))
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        O.advance(F, psi0, geo, stepper="eulero", dt=1e-3, imaginary=True, nsteps=100, renormalize=[1.0])
+        one = time.perf_counter() - t0
+        O.set_threads(cores)
+        out["cpu_one_core"] = {"value": 100 / one, "unit": "steps/s", "cores": 1, "steps": 100,
+                               "sample": "100 Euler steps, CPU oracle on one thread"}
+    H.free()
+    psi.free()
+    vb.free()
     ctx.close()
     return out
 
@@ -328,6 +403,8 @@ def main() -> int:
                     help="batch of the warm-start network's MFMA measurement (0 skips it)")
     ap.add_argument("--large-steps", type=int, default=400,
                     help="RK4 steps of the 512^3 two-droplet collision (0 skips it)")
+    ap.add_argument("--no-config1", dest="config1", action="store_false",
+                    help="skip the 64^3 harmonic-trap Euler case (GPU and CPU steps/s)")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with one normalize of the field: its scale kernel reads and writes the field "
                          "once, the reference for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -363,6 +440,11 @@ def main() -> int:
 
     # per-kernel timing (HIP events around each launch) for the roofline object
     prof = capi.profile_advance(ctx, f, s, psi, args.profile_steps)
+    # N(t) and E(t) of the same workload from its start, every 10 steps of args.steps
+    # (fused stage-1 reductions; outside the timed region)
+    logf = ctx.field(psi0)
+    nlog, elog = capi.advance(ctx, f, s, logf, args.steps, log=True)
+    logf.free()
     norms1, ens1 = capi.advance(ctx, f, s, psi, 1, log=True)
     F = 16.0 * shape[0] * shape[1] * shape[2]
     dom = max((k for k in prof if k in KIND_BYTES_F), key=lambda k: prof[k][0])
@@ -404,6 +486,10 @@ def main() -> int:
         "checks": {"norm_start": float(norms0[0, 0]), "norm_end": float(norms1[0, 0]),
                    "energy_start": float(ens0[0]), "energy_end": float(ens1[0]),
                    "steps_between": args.warmup + args.steps + args.profile_steps},
+        "conservation": {"every_steps": 10, "steps": args.steps,
+                         "norm": [float(v) for v in nlog[::10, 0]], "energy": [float(v) for v in elog[::10]],
+                         "norm_rel_drift_max": float(np.max(np.abs(nlog[:, 0] - nlog[0, 0])) / nlog[0, 0]),
+                         "energy_rel_drift_max": float(np.max(np.abs(elog - elog[0])) / abs(elog[0]))},
     }
     # host <-> HBM copies of the field through the boundary (gp5_buf_upload / download of
     # a pageable numpy array); outside the timed region, reported for the PCIe-inclusive rate
@@ -419,6 +505,9 @@ def main() -> int:
     if args.calibrate:
         capi.normalize(ctx, psi, 0, 3000.0)
         ctx.sync()
+    if args.config1:
+        _progress("config 1: 64^3 harmonic trap, Euler imaginary time")
+        out["config1"] = config1_harmonic(cpu=not args.no_cpu)
     if args.gs_count > 0:
         out["ground_state"] = ground_state_batch(args.gs_count)
     if args.large_steps > 0:

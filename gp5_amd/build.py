@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgp5.so")
 ARCH = os.environ.get("GP5_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["gp5_kernels.hip", "gp5_fast.hip", "gp5_capi.hip", "gp5_nn.hip", "gp5_ccl.hip", "gp5_resample.hip", "gp5_h5.cpp"]
+HIP_SOURCES = ["gp5_kernels.hip", "gp5_fast.hip", "gp5_capi.hip", "gp5_batch.hip", "gp5_nn.hip", "gp5_ccl.hip", "gp5_resample.hip", "gp5_h5.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 ops stay off (hipcc's host pass warns that it does not know

@@ -34,7 +34,7 @@ GP5_STEP_EULERO, GP5_STEP_RK4 = 0, 1
 # every symbol include/gp5.h declares
 EXPORTS = [
     "gp5_last_error", "gp5_version", "gp5_ctx_create", "gp5_ctx_destroy", "gp5_ctx_sync",
-    "gp5_ctx_points", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
+    "gp5_ctx_points", "gp5_ctx_fft_plans", "gp5_ctx_set_fft_plans", "gp5_ctx_fft_plan_name", "gp5_buf_alloc", "gp5_buf_free", "gp5_buf_size", "gp5_buf_upload",
     "gp5_buf_download", "gp5_buf_copy", "gp5_buf_zero", "gp5_fft", "gp5_laplacian",
     "gp5_functional_apply", "gp5_norm", "gp5_normalize", "gp5_max_density", "gp5_energy",
     "gp5_phase_constraint", "gp5_axpy", "gp5_rk4_combine", "gp5_advance", "gp5_advance_noalloc", "gp5_advance_timed", "gp5_profile_advance",
@@ -91,6 +91,9 @@ def load() -> ctypes.CDLL:
         "gp5_ctx_destroy": ([P], I),
         "gp5_ctx_sync": ([P], I),
         "gp5_ctx_points": ([P], S),
+        "gp5_ctx_fft_plans": ([P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], I),
+        "gp5_ctx_set_fft_plans": ([P, ctypes.POINTER(ctypes.c_int)], I),
+        "gp5_ctx_fft_plan_name": ([P, I], ctypes.c_char_p),
         "gp5_buf_alloc": ([P, S, PP], I),
         "gp5_buf_free": ([P], I),
         "gp5_buf_size": ([P], S),
@@ -195,6 +198,18 @@ class Context:
         if data is not None:
             b.upload_real(data)
         return b
+
+    def fft_plans(self):
+        """(variants, counts) of the register plans per role (x lines, y columns, z pass)."""
+        v, c = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+        check(load().gp5_ctx_fft_plans(self.handle, v, c))
+        return list(v), list(c)
+
+    def set_fft_plans(self, variants) -> None:
+        check(load().gp5_ctx_set_fft_plans(self.handle, (ctypes.c_int * 3)(*[int(x) for x in variants])))
+
+    def fft_plan_names(self):
+        return [load().gp5_ctx_fft_plan_name(self.handle, r).decode() for r in range(3)]
 
     def sync(self) -> None:
         check(load().gp5_ctx_sync(self._h))

@@ -15,10 +15,12 @@
 #include <stdio.h>
 #include <string.h>
 #include <limits.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <thread>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -29,7 +31,8 @@ hipError_t launch_axis(hipStream_t st, int mode, const double2 *src, double2 *ds
                        const MidArgs &mid, const EpiArgs &epi);
 unsigned axis_grid(const AxisGeom &G);
 hipError_t launch_norm(hipStream_t st, const double2 *f, long long len, unsigned blocks, const EpiArgs &E);
-hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp);
+hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp,
+                        const int *active);
 hipError_t launch_maxdens(hipStream_t st, const double2 *f, long long len, unsigned blocks, double *blockmax);
 hipError_t launch_phase(hipStream_t st, double2 *f, const double *phase, long long len);
 hipError_t launch_phase_exp(hipStream_t st, double2 *ex, const double *phase, long long len);
@@ -87,6 +90,12 @@ struct gp5_ctx {
     struct ProfState *prof;  // non-null while gp5_profile_advance runs
     int *ccl;             // droplet-labelling workspace, labels and count (allocated on first use)
     size_t ccl_cap;       // ints
+    // batch classes: live components [0, active) on the device (the kernels gate on
+    // it; owned by the batch) and its host copy, valid between ground-state calls
+    const int *active_dev;
+    int active;
+    // register-plan variants of the fast passes (0: default): x lines, y columns, z pass
+    int pvar[3];
 };
 
 // Event pairs around every launch while profiling.
@@ -288,6 +297,7 @@ static void set_geometry(gp5_ctx *c)
             B = (int)std::min<long long>(std::min(B, 64), I);
         }
         G.B = std::max(1, B);
+        G.active = c->active_dev;
         I *= n;
     }
 }
@@ -355,6 +365,7 @@ static EpiArgs base_epi(gp5_ctx *c)
     E.dV = c->dV;
     E.partials = c->partials;
     E.results = c->results;
+    E.active = c->active_dev;
     return E;
 }
 
@@ -423,6 +434,7 @@ static FastArgs fast_args(gp5_ctx *c, int axis, const double2 *src, double2 *dst
     a.O = c->geom[axis].O;
     a.Opc = c->Opc[axis];
     a.nx = c->n[0];
+    a.active = c->active_dev;
     return a;
 }
 
@@ -446,13 +458,13 @@ static int middle_passes(gp5_ctx *c, const MidArgs &mid)
             FastArgs ay = fast_args(c, 1, c->T, c->T);
             ay.x0 = x0;
             ay.xcount = xc;
-            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, -1, ay));
+            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, -1, ay, c->pvar[1]));
             FastArgs az = fast_args(c, 2, c->T, c->T);
             az.x0 = x0;
             az.xcount = xc;
             az.mid = mid;
-            PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, az));
-            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, +1, ay));
+            PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, az, c->pvar[2]));
+            PLAUNCH(c, K_Y, launch_fast(c->st, c->n[1], 0, +1, ay, c->pvar[1]));
         }
         return GP5_OK;
     }
@@ -460,7 +472,7 @@ static int middle_passes(gp5_ctx *c, const MidArgs &mid)
     if (c->fast[2]) {
         FastArgs a = fast_args(c, 2, c->T, c->T);
         a.mid = mid;
-        PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, a));
+        PLAUNCH(c, K_Z_MID, launch_fast(c->st, c->n[2], 1, -1, a, c->pvar[2]));
     } else {
         EpiArgs none;
         memset(&none, 0, sizeof none);
@@ -484,7 +496,7 @@ static int pass_plain(gp5_ctx *c, int kind, int axis, const double2 *src, double
         a.src2 = pre->src2;
         a.cfr = pre->cfr;
         a.cfi = pre->cfi;
-        PLAUNCH(c, kind, launch_fast(c->st, c->n[axis], 0, sign, a));
+        PLAUNCH(c, kind, launch_fast(c->st, c->n[axis], 0, sign, a, axis == 0 ? c->pvar[0] : c->pvar[1]));
     } else {
         MidArgs mid;
         memset(&mid, 0, sizeof mid);
@@ -530,8 +542,8 @@ static int stage_passes(gp5_ctx *c, const StageIn &in, const double coef[2], con
     long long lpb;  // lines per block of the epilogue pass (component slots of its partials)
     EpiArgs e = E;
     if (c->fast[0]) {
-        nblocks = fast_grid(c->n[0], c->geom[0].I, c->geom[0].O);
-        lpb = fast_rows_per_block(c->n[0]);
+        nblocks = fast_grid(c->n[0], c->geom[0].I, c->geom[0].O, c->pvar[0]);
+        lpb = fast_rows_per_block(c->n[0], c->pvar[0]);
         RCHK(ensure_partials(c, nblocks));
         FastArgs a = fast_args(c, 0, c->T, nullptr);
         e.partials = c->partials;
@@ -541,7 +553,7 @@ static int stage_passes(gp5_ctx *c, const StageIn &in, const double coef[2], con
         const bool coupled = e.fkind == GP5_FUNC_LHY_UNLOCKED ||
                              (e.fkind == GP5_FUNC_GP && e.has_couplings && e.nc == 2 && !e.uncoupled);
         const int mode = (coupled || e.phase_exp) ? 2 : 3;
-        PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], mode, +1, a));
+        PLAUNCH(c, fused ? K_X_EPI_FWD : K_X_EPI, launch_fast(c->st, c->n[0], mode, +1, a, c->pvar[0]));
     } else {
         if (fused) return fail(GP5_ERR_UNSUPPORTED, "fused stages need a register-kernel x length");
         nblocks = axis_grid(c->geom[0]);
@@ -667,6 +679,42 @@ int gp5_ctx_sync(gp5_ctx *c)
 }
 
 size_t gp5_ctx_points(const gp5_ctx *c) { return c ? (size_t)c->M : 0; }
+
+int gp5_ctx_fft_plans(const gp5_ctx *c, int variant[3], int count[3])
+{
+    if (!c || !variant || !count) return fail(GP5_ERR_ARG, "NULL argument");
+    for (int r = 0; r < 3; ++r) {
+        const int axis = r;  // role 0 runs on x, 1 on y, 2 on z
+        variant[r] = c->pvar[r];
+        count[r] = c->fast[axis] ? fast_variants(c->n[axis]) : 0;
+    }
+    return GP5_OK;
+}
+
+int gp5_ctx_set_fft_plans(gp5_ctx *c, const int variant[3])
+{
+    if (!c || !variant) return fail(GP5_ERR_ARG, "NULL argument");
+    for (int r = 0; r < 3; ++r) {
+        const int cnt = c->fast[r] ? fast_variants(c->n[r]) : 1;
+        if (variant[r] < 0 || variant[r] >= std::max(1, cnt))
+            return fail(GP5_ERR_ARG, "plan variant %d of role %d: %d available", variant[r], r, std::max(1, cnt));
+    }
+    // x-line blocks of a many-component context must not span two component boundaries
+    if (c->fast[0] && c->nc_cap > 2 &&
+        fast_rows_per_block(c->n[0], variant[0]) > (long long)c->n[1] * c->n[2])
+        return fail(GP5_ERR_UNSUPPORTED, "x-line plan variant %d: blocks would span components", variant[0]);
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int r = 0; r < 3; ++r) c->pvar[r] = variant[r];
+    return GP5_OK;
+}
+
+const char *gp5_ctx_fft_plan_name(const gp5_ctx *c, int role)
+{
+    static thread_local std::string name;
+    if (!c || role < 0 || role > 2) return "";
+    name = c->fast[role] ? fast_plan_name(c->n[role], role, c->pvar[role]) : "generic";
+    return name.c_str();
+}
 
 int gp5_buf_alloc(gp5_ctx *c, size_t n, gp5_buf **out)
 {
@@ -827,7 +875,7 @@ int gp5_normalize(gp5_ctx *c, gp5_buf *f, int comp, double N)
     double nrm = 0.0;
     RCHK(norm_impl(c, (const double2 *)f->d, comp, N, &nrm, scale_slot(c, 7)));
     if (!std::isfinite(nrm) || nrm <= 0.0) return fail(GP5_ERR_NONFINITE, "norm of component %d is %g", comp, nrm);
-    HIPCHK(launch_scale(c->st, (double2 *)f->d, c->M, c->nc, comp, 0.0, scale_slot(c, 7)));
+    HIPCHK(launch_scale(c->st, (double2 *)f->d, c->M, c->nc, comp, 0.0, scale_slot(c, 7), nullptr));
     HIPCHK(hipStreamSynchronize(c->st));
     return GP5_OK;
 }
@@ -1164,7 +1212,7 @@ static int advance_prepare(gp5_ctx *c, const gp5_functional_desc *f, const gp5_s
         HIPCHK(launch_phase_exp(c->st, c->phase_ex, s->phase->d, c->M * c->nc));
     }
     size_t blocks = axis_grid(c->geom[0]);
-    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O));
+    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O, c->pvar[0]));
     RCHK(ensure_partials(c, blocks));
     if (logging && (size_t)nsteps > c->dlog_cap) {
         if (c->dlog) HIPCHK(hipFree(c->dlog));
@@ -1186,7 +1234,8 @@ static int advance_enqueue(gp5_ctx *c, const gp5_functional_desc *f, const gp5_s
     for (int st = 0; st < nsteps; ++st)
         RCHK(step_once(c, f, s, psi, psi_slot, st, logging ? c->dlog : nullptr, fused, st == nsteps - 1));
     if (psi_slot >= 0)
-        PLAUNCH(c, K_SCALE, launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, scale_slot(c, psi_slot)));
+        PLAUNCH(c, K_SCALE,
+                launch_scale(c->st, (double2 *)psi->d, c->M, c->nc, -1, 0.0, scale_slot(c, psi_slot), c->active_dev));
     return GP5_OK;
 }
 
@@ -1216,7 +1265,8 @@ static int advance_collect(gp5_ctx *c, const gp5_functional_desc *f, int nsteps,
         }
         if (energy_log) energy_log[st] = no_energy ? NAN : e;
     }
-    for (int k = 0; k < nc; ++k)
+    const int live = c->active_dev ? c->active : nc;  // batch classes: slots beyond the live ones are dead
+    for (int k = 0; k < live; ++k)
         if (!std::isfinite(r[k])) return fail(GP5_ERR_NONFINITE, "norm of component %d became %g during the step", k, r[k]);
     return GP5_OK;
 }
@@ -1277,7 +1327,7 @@ int gp5_advance_noalloc(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
     const int nfields = (rk ? 5 : 2) + (s->phase ? 1 : 0);
     const size_t field_bytes = ((size_t)c->M * c->nc_cap * sizeof(double2) + 255) & ~(size_t)255;
     size_t blocks = axis_grid(c->geom[0]);
-    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O));
+    if (c->fast[0]) blocks = std::max<size_t>(blocks, fast_grid(c->n[0], c->geom[0].I, c->geom[0].O, c->pvar[0]));
     const size_t partial_bytes = blocks * kRedQ * sizeof(double);
     const size_t need = (size_t)nfields * field_bytes + partial_bytes;
     if (need > kScratchBytes)
@@ -1375,30 +1425,41 @@ int gp5_profile_advance(gp5_ctx *c, const gp5_functional_desc *f, const gp5_step
 //
 // The instances that share a grid and box form one class: they are the
 // components of one context, so every pass is one launch over all of them.
+// Live instances occupy the slots [0, active) of their class; `active` lives
+// in device memory and every pass of the class lets the blocks of later slots
+// exit, so launches keep their shape while the class shrinks.
+//
 // Ground states run the reference's imaginary-time RK4 with per-stage
-// renormalisation (src/stepper.cpp:46-70 as src/main.cpp:131-145 sets it up)
-// in hipGraph replays of an even number of steps (the state buffer pointers
-// return to their places after an even number of steps).  After each replay
-// the per-instance energies are read; an instance whose energy moved by at
-// most tol relative since the previous replay is retired: its field is copied
-// out and the class's last active instance takes its slot, so the class's
-// launches shrink to the active instances and the graph is captured again.
+// renormalisation (src/stepper.cpp:46-70 as src/main.cpp:131-145 sets it up).
+// One graph holds a replay of check_every (even) steps of every class, each
+// class on its own stream (fork/join), followed per class by the on-device
+// convergence check and compaction (gp5_batch.hip): an instance whose energy
+// moved by at most tol relative since the previous check retires, its field
+// goes to its result (preallocated at creation) and the class's last live
+// slot takes its place.  The graph is captured once per call and replayed
+// until no class has a live slot; the host reads the classes' states into
+// pinned memory one replay late, so the device always has the next replay
+// queued, and nothing is allocated between replays.
 // ---------------------------------------------------------------------------
 struct BatchClass {
     int n = 0;
     double L = 0.0;
     gp5_ctx *ctx = nullptr;
-    gp5_buf *psi = nullptr;      // 2*M*cap doubles; ->n tracks the active slots
+    gp5_buf *psi = nullptr;      // 2*M*cap doubles: every slot (slots >= active are dead)
+    int cap = 0;
+    // host copies of the device bookkeeping, valid between ground-state calls
     std::vector<int> slot_inst;  // slot -> instance; slots < active are live
     int active = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_nc = -1;
-    int graph_steps = 0;
     long long steps_done = 0;
     std::vector<double> prevE;   // per slot: energy at the previous check
     std::vector<int> checks;     // per slot: checks done
-    std::vector<double> host_log;
+    // device bookkeeping (one allocation)
+    char *dmem = nullptr;
+    BatchClassDev *dev = nullptr;  // entry of gp5_batch::status_dev
+    int *d_slot_inst = nullptr, *d_checks = nullptr;
+    int *d_copy_slot = nullptr, *d_copy_inst = nullptr, *d_move_src = nullptr, *d_move_dst = nullptr;
+    double *d_prevE = nullptr;
+    hipEvent_t join = nullptr;
 };
 
 struct gp5_batch {
@@ -1406,68 +1467,25 @@ struct gp5_batch {
     std::vector<int> inst_class, inst_slot;  // instance -> class, slot (-1: retired)
     std::vector<double> N;
     std::vector<BatchClass> cls;
-    std::vector<double *> result;            // retired instances: their field (device)
+    std::vector<double *> result;            // per instance: its result field (in result_store)
+    double *result_store = nullptr;
+    double2 **res_ptr_dev = nullptr;         // device copy of the result pointers
+    long long *res_steps_dev = nullptr;
+    double *res_energy_dev = nullptr;
     std::vector<long long> steps;
     std::vector<double> energy;
+    BatchClassDev *status_dev = nullptr;     // [classes]
+    BatchClassDev *status_host = nullptr;    // pinned [2][classes]
+    hipStream_t st = nullptr;                // the stream the joined graph is launched on
+    hipEvent_t fork = nullptr;
     std::vector<gp5_ctx *> one;              // single-component contexts of gp5_batch_resample, by (n, L)
 };
-
-static void batch_drop_graph(BatchClass &k)
-{
-    if (k.exec) (void)hipGraphExecDestroy(k.exec);
-    if (k.graph) (void)hipGraphDestroy(k.graph);
-    k.exec = nullptr;
-    k.graph = nullptr;
-    k.graph_nc = -1;
-}
 
 static int batch_check_inst(const gp5_batch *b, int i)
 {
     if (!b) return fail(GP5_ERR_ARG, "NULL batch");
     if (i < 0 || i >= b->count) return fail(GP5_ERR_ARG, "instance %d out of range (batch of %d)", i, b->count);
     return GP5_OK;
-}
-
-// Retire slot s of class k: copy its field out, move the last active slot in.
-static int batch_retire(gp5_batch *b, BatchClass &k, int s, long long steps, double energy)
-{
-    gp5_ctx *c = k.ctx;
-    const long long M = c->M;
-    const int inst = k.slot_inst[s];
-    double *dst = nullptr;
-    HIPCHK(hipMalloc(&dst, (size_t)M * sizeof(double2)));
-    HIPCHK(hipMemcpyAsync(dst, k.psi->d + (size_t)2 * M * s, (size_t)M * sizeof(double2), hipMemcpyDeviceToDevice, c->st));
-    b->result[inst] = dst;
-    b->steps[inst] = steps;
-    b->energy[inst] = energy;
-    b->inst_slot[inst] = -1;
-    const int a = k.active - 1;
-    if (s != a) {
-        HIPCHK(hipMemcpyAsync(k.psi->d + (size_t)2 * M * s, k.psi->d + (size_t)2 * M * a, (size_t)M * sizeof(double2),
-                              hipMemcpyDeviceToDevice, c->st));
-        HIPCHK(hipMemcpyAsync(c->nnorm + s, c->nnorm + a, sizeof(double), hipMemcpyDeviceToDevice, c->st));
-        if (c->lhy_ab)
-            HIPCHK(hipMemcpyAsync(c->lhy_ab + 2 * s, c->lhy_ab + 2 * a, 2 * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-        if (c->Vdev)
-            HIPCHK(hipMemcpyAsync(c->Vdev + (size_t)M * s, c->Vdev + (size_t)M * a, (size_t)M * sizeof(double),
-                                  hipMemcpyDeviceToDevice, c->st));
-        k.slot_inst[s] = k.slot_inst[a];
-        b->inst_slot[k.slot_inst[s]] = s;
-        k.prevE[s] = k.prevE[a];
-        k.checks[s] = k.checks[a];
-    }
-    k.active = a;
-    HIPCHK(hipStreamSynchronize(c->st));
-    return GP5_OK;
-}
-
-// The class's context now runs its active slots only.
-static void batch_shrink(BatchClass &k)
-{
-    gp5_ctx *c = k.ctx;
-    c->nc = std::max(1, k.active);
-    set_geometry(c);
-    k.psi->n = (size_t)2 * c->M * c->nc;
 }
 
 static gp5_stepper_desc batch_stepper(const BatchClass &k, double dt_factor)
@@ -1494,6 +1512,44 @@ static int batch_time(BatchClass &k, const gp5_functional_desc *f, const gp5_ste
     HIPCHK(hipEventRecord(c->ev1, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    return GP5_OK;
+}
+
+// Host bookkeeping -> device (before a ground-state call; synchronous).
+static int batch_push_state(gp5_batch *b, BatchClass &k)
+{
+    BatchClassDev h;
+    memset(&h, 0, sizeof h);
+    h.active = k.active;
+    h.err_inst = -1;
+    h.steps_done = k.steps_done;
+    HIPCHK(hipMemcpy(k.dev, &h, sizeof h, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(k.d_slot_inst, k.slot_inst.data(), k.cap * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(k.d_prevE, k.prevE.data(), k.cap * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(k.d_checks, k.checks.data(), k.cap * sizeof(int), hipMemcpyHostToDevice));
+    k.ctx->active = k.active;
+    (void)b;
+    return GP5_OK;
+}
+
+// Device bookkeeping -> host (after a ground-state call; the streams are idle).
+static int batch_pull_state(gp5_batch *b, BatchClass &k, const BatchClassDev &h, const std::vector<long long> &rs,
+                            const std::vector<double> &re)
+{
+    std::vector<int> prev(k.slot_inst.begin(), k.slot_inst.begin() + k.active);
+    HIPCHK(hipMemcpy(k.slot_inst.data(), k.d_slot_inst, k.cap * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(k.prevE.data(), k.d_prevE, k.cap * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(k.checks.data(), k.d_checks, k.cap * sizeof(int), hipMemcpyDeviceToHost));
+    k.active = h.active;
+    k.steps_done = h.steps_done;
+    k.ctx->active = k.active;
+    for (int inst : prev) b->inst_slot[inst] = -1;
+    for (int sl = 0; sl < k.active; ++sl) b->inst_slot[k.slot_inst[sl]] = sl;
+    for (int inst : prev)
+        if (b->inst_slot[inst] < 0) {  // retired during this call
+            b->steps[inst] = rs[inst];
+            b->energy[inst] = re[inst];
+        }
     return GP5_OK;
 }
 
@@ -1532,28 +1588,91 @@ int gp5_batch_create(gp5_batch **out, int count, const int *n, const double *L, 
         b->inst_slot[i] = (int)b->cls[k].slot_inst.size();
         b->cls[k].slot_inst.push_back(i);
     }
-    for (BatchClass &k : b->cls) {
+    auto bail = [&](int rc) {
+        gp5_batch_destroy(b);
+        return rc;
+    };
+    const size_t ncls = b->cls.size();
+    if (hipMalloc(&b->status_dev, ncls * sizeof(BatchClassDev)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "batch state"));
+    if (hipHostMalloc((void **)&b->status_host, 2 * ncls * sizeof(BatchClassDev), hipHostMallocDefault) != hipSuccess) {
+        b->status_host = nullptr;
+        return bail(fail(GP5_ERR_ALLOC, "pinned batch state"));
+    }
+    if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+        b->st = nullptr;
+        return bail(fail(GP5_ERR_HIP, "batch stream"));
+    }
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        b->fork = nullptr;
+        return bail(fail(GP5_ERR_HIP, "batch events"));
+    }
+    // the results of every instance, allocated once (retirement copies into them)
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) total += (size_t)2 * n[i] * n[i] * n[i];
+    if (hipMalloc(&b->result_store, total * sizeof(double)) != hipSuccess) {
+        b->result_store = nullptr;
+        return bail(fail(GP5_ERR_ALLOC, "cannot allocate %zu B for the batch results", total * sizeof(double)));
+    }
+    {
+        size_t o = 0;
+        for (int i = 0; i < count; ++i) {
+            b->result[i] = b->result_store + o;
+            o += (size_t)2 * n[i] * n[i] * n[i];
+        }
+        if (hipMalloc(&b->res_ptr_dev, count * sizeof(double2 *)) != hipSuccess ||
+            hipMalloc(&b->res_steps_dev, count * sizeof(long long)) != hipSuccess ||
+            hipMalloc(&b->res_energy_dev, count * sizeof(double)) != hipSuccess)
+            return bail(fail(GP5_ERR_ALLOC, "batch result tables"));
+        if (hipMemcpy(b->res_ptr_dev, b->result.data(), count * sizeof(double2 *), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(GP5_ERR_HIP, "batch result tables"));
+        std::vector<long long> zs(count, 0);
+        std::vector<double> ze(count, NAN);
+        if (hipMemcpy(b->res_steps_dev, zs.data(), count * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->res_energy_dev, ze.data(), count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(GP5_ERR_HIP, "batch result tables"));
+    }
+    for (size_t ki = 0; ki < ncls; ++ki) {
+        BatchClass &k = b->cls[ki];
         const int nc = (int)k.slot_inst.size();
+        k.cap = nc;
         const int shape[3] = {k.n, k.n, k.n};
         const double left[3] = {-0.5 * k.L, -0.5 * k.L, -0.5 * k.L}, right[3] = {0.5 * k.L, 0.5 * k.L, 0.5 * k.L};
         int rc = gp5_ctx_create(&k.ctx, shape, left, right, nc);
-        if (rc == GP5_OK) k.ctx->uncoupled = true;
-        if (rc == GP5_OK) rc = gp5_buf_alloc(k.ctx, (size_t)2 * k.ctx->M * nc, &k.psi);
-        if (rc == GP5_OK && hipMalloc(&k.ctx->nnorm, nc * sizeof(double)) != hipSuccess)
-            rc = fail(GP5_ERR_ALLOC, "batch normalisations");
-        if (rc == GP5_OK) {
-            std::vector<double> h(nc);
-            for (int sl = 0; sl < nc; ++sl) h[sl] = N[k.slot_inst[sl]];
-            if (hipMemcpy(k.ctx->nnorm, h.data(), nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-                rc = fail(GP5_ERR_HIP, "batch normalisations upload");
+        if (rc != GP5_OK) return bail(rc);
+        k.ctx->uncoupled = true;
+        k.dev = b->status_dev + ki;
+        k.ctx->active_dev = &k.dev->active;
+        set_geometry(k.ctx);  // the generic passes carry the gate too
+        rc = gp5_buf_alloc(k.ctx, (size_t)2 * k.ctx->M * nc, &k.psi);
+        if (rc != GP5_OK) return bail(rc);
+        if (hipMalloc(&k.ctx->nnorm, nc * sizeof(double)) != hipSuccess) return bail(fail(GP5_ERR_ALLOC, "batch normalisations"));
+        std::vector<double> h(nc);
+        for (int sl = 0; sl < nc; ++sl) h[sl] = N[k.slot_inst[sl]];
+        if (hipMemcpy(k.ctx->nnorm, h.data(), nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(GP5_ERR_HIP, "batch normalisations upload"));
+        // slot_inst, checks, copy/move lists (ints), prevE (doubles)
+        const size_t ints = 6 * (size_t)nc, bytes = ((ints * sizeof(int) + 15) & ~(size_t)15) + nc * sizeof(double);
+        if (hipMalloc(&k.dmem, bytes) != hipSuccess) {
+            k.dmem = nullptr;
+            return bail(fail(GP5_ERR_ALLOC, "batch bookkeeping"));
         }
-        if (rc != GP5_OK) {
-            gp5_batch_destroy(b);
-            return rc;
+        int *ip = (int *)k.dmem;
+        k.d_slot_inst = ip;
+        k.d_checks = ip + nc;
+        k.d_copy_slot = ip + 2 * nc;
+        k.d_copy_inst = ip + 3 * nc;
+        k.d_move_src = ip + 4 * nc;
+        k.d_move_dst = ip + 5 * nc;
+        k.d_prevE = (double *)(k.dmem + ((ints * sizeof(int) + 15) & ~(size_t)15));
+        if (hipEventCreateWithFlags(&k.join, hipEventDisableTiming) != hipSuccess) {
+            k.join = nullptr;
+            return bail(fail(GP5_ERR_HIP, "batch events"));
         }
         k.active = nc;
         k.prevE.assign(nc, NAN);
         k.checks.assign(nc, 0);
+        rc = batch_push_state(b, k);
+        if (rc != GP5_OK) return bail(rc);
     }
     *out = b;
     return GP5_OK;
@@ -1563,23 +1682,26 @@ int gp5_batch_destroy(gp5_batch *b)
 {
     if (!b) return GP5_OK;
     int rc = GP5_OK;
-    for (BatchClass &k : b->cls) {
-        batch_drop_graph(k);
-        if (k.psi) {
-            int r = gp5_buf_free(k.psi);
-            if (rc == GP5_OK) rc = r;
-        }
-        if (k.ctx) {
-            int r = gp5_ctx_destroy(k.ctx);
-            if (rc == GP5_OK) rc = r;
-        }
-    }
-    for (double *p : b->result)
-        if (p) (void)hipFree(p);
-    for (gp5_ctx *c : b->one) {
-        int r = gp5_ctx_destroy(c);
+    auto note = [&](int r) {
         if (rc == GP5_OK) rc = r;
+    };
+    if (b->st) (void)hipStreamSynchronize(b->st);
+    for (BatchClass &k : b->cls) {
+        if (k.psi) note(gp5_buf_free(k.psi));
+        if (k.ctx) {
+            k.ctx->active_dev = nullptr;
+            note(gp5_ctx_destroy(k.ctx));
+        }
+        if (k.dmem) (void)hipFree(k.dmem);
+        if (k.join) (void)hipEventDestroy(k.join);
     }
+    for (gp5_ctx *c : b->one) note(gp5_ctx_destroy(c));
+    void *ptrs[] = {b->result_store, b->res_ptr_dev, b->res_steps_dev, b->res_energy_dev, b->status_dev};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->status_host) (void)hipHostFree(b->status_host);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->st) (void)hipStreamDestroy(b->st);
     delete b;
     return rc;
 }
@@ -1672,7 +1794,6 @@ int gp5_batch_set_lhy(gp5_batch *b, int i, double a, double bcoef)
     }
     const double ab[2] = {a, bcoef};
     HIPCHK(hipMemcpy(c->lhy_ab + 2 * s, ab, sizeof ab, hipMemcpyHostToDevice));
-    batch_drop_graph(k);
     return GP5_OK;
 }
 
@@ -1689,7 +1810,6 @@ int gp5_batch_set_potential(gp5_batch *b, int i, const double *V)
         HIPCHK(hipMemset(c->Vdev, 0, (size_t)c->M * c->nc_cap * sizeof(double)));
     }
     HIPCHK(hipMemcpy(c->Vdev + (size_t)c->M * s, V, (size_t)c->M * sizeof(double), hipMemcpyHostToDevice));
-    batch_drop_graph(k);
     return GP5_OK;
 }
 
@@ -1706,16 +1826,110 @@ int gp5_batch_advance(gp5_batch *b, const gp5_functional_desc *f, double dt_fact
     return GP5_OK;
 }
 
+// ---- autotuner --------------------------------------------------------------
+// Per class: the register-plan variant of each fast role (x lines, y columns, z
+// pass) and the fused x pass are chosen by timing `steps` steps of each
+// candidate on the class's own state (coordinate descent, role by role); axes on
+// the generic kernels get their radix order and tile.  The register-plan and
+// fusion choices are cached in a JSON file keyed by device architecture and
+// grid size (GP5_TUNE_CACHE, default $HOME/.cache/gp5_amd/fft_plans.json;
+// "off" disables it); a cached class is not timed again.
+struct TuneEntry {
+    int var[3];
+    int fuse;
+    double ms;
+};
+
+static std::string tune_cache_path()
+{
+    const char *e = getenv("GP5_TUNE_CACHE");
+    if (e) return (strcmp(e, "off") == 0 || !e[0]) ? std::string() : std::string(e);
+    const char *home = getenv("HOME");
+    if (!home || !home[0]) return std::string();
+    return std::string(home) + "/.cache/gp5_amd/fft_plans.json";
+}
+
+static std::string device_arch(int device)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    const size_t colon = a.find(':');
+    return colon == std::string::npos ? a : a.substr(0, colon);
+}
+
+// The cache file: one flat JSON object, "arch/n": [x, y, z, fused, ms] per line.
+static std::vector<std::pair<std::string, TuneEntry>> tune_cache_read(const std::string &path)
+{
+    std::vector<std::pair<std::string, TuneEntry>> out;
+    FILE *fp = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (!fp) return out;
+    char line[512];
+    while (fgets(line, sizeof line, fp)) {
+        char key[128];
+        TuneEntry t;
+        if (sscanf(line, " \"%127[^\"]\" : [ %d , %d , %d , %d , %lf ]", key, &t.var[0], &t.var[1], &t.var[2], &t.fuse,
+                   &t.ms) == 6)
+            out.push_back({key, t});
+    }
+    fclose(fp);
+    return out;
+}
+
+static void tune_cache_write(const std::string &path, const std::vector<std::pair<std::string, TuneEntry>> &all)
+{
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        // create the directory chain (mkdir -p)
+        std::string dir = path.substr(0, slash);
+        for (size_t k = 1; k <= dir.size(); ++k)
+            if (k == dir.size() || dir[k] == '/') (void)mkdir(dir.substr(0, k).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp";
+    FILE *fp = fopen(tmp.c_str(), "w");
+    if (!fp) return;
+    fprintf(fp, "{\n");
+    for (size_t k = 0; k < all.size(); ++k) {
+        const TuneEntry &t = all[k].second;
+        fprintf(fp, "  \"%s\": [%d, %d, %d, %d, %.6f]%s\n", all[k].first.c_str(), t.var[0], t.var[1], t.var[2], t.fuse,
+                t.ms, k + 1 < all.size() ? "," : "");
+    }
+    fprintf(fp, "}\n");
+    fclose(fp);
+    (void)rename(tmp.c_str(), path.c_str());
+}
+
 int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, int steps, char *report, size_t report_len)
 {
     if (!b || !f) return fail(GP5_ERR_ARG, "NULL argument");
     if (steps < 1) return fail(GP5_ERR_ARG, "steps < 1");
     std::string rep;
+    const std::string cache_path = tune_cache_path();
+    auto cache = tune_cache_read(cache_path);
+    bool cache_dirty = false;
     for (BatchClass &k : b->cls) {
         if (k.active == 0) continue;
         gp5_ctx *c = k.ctx;
-        batch_drop_graph(k);
         gp5_stepper_desc s = batch_stepper(k, dt_factor);
+        char line[512];
+        const std::string key = device_arch(c->device) + "/" + std::to_string(k.n);
+        const bool tunable = c->fast[0] || c->fast[1] || c->fast[2];
+        const TuneEntry *hit = nullptr;
+        for (auto &e : cache)
+            if (e.first == key) hit = &e.second;
+        if (tunable && hit) {
+            int v[3] = {hit->var[0], hit->var[1], hit->var[2]};
+            if (gp5_ctx_set_fft_plans(c, v) == GP5_OK) {
+                c->fuse_x = hit->fuse != 0;
+                snprintf(line, sizeof line, "class n=%d L=%g instances=%d: cached (%s) x %s, y %s, z %s, fused x %s\n", k.n,
+                         k.L, k.active, key.c_str(), fast_plan_name(k.n, 0, v[0]).c_str(),
+                         fast_plan_name(k.n, 1, v[1]).c_str(), fast_plan_name(k.n, 2, v[2]).c_str(),
+                         c->fuse_x ? "on" : "off");
+                rep += line;
+                continue;
+            }
+        }
         // tune on a copy: the class state is restored afterwards
         const size_t bytes = (size_t)c->M * c->nc * sizeof(double2);
         double *save = nullptr;
@@ -1723,12 +1937,42 @@ int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor,
         HIPCHK(hipMemcpy(save, k.psi->d, bytes, hipMemcpyDeviceToDevice));
         float best = 0.f;
         int rc = batch_time(k, f, &s, steps, &best);
-        char line[256];
+        const float dflt = best;
         snprintf(line, sizeof line, "class n=%d L=%g instances=%d: default %.3f ms/step", k.n, k.L, k.active, best / steps);
         rep += line;
+        // register plans, role by role
+        for (int role = 0; rc == GP5_OK && role < 3; ++role) {
+            if (!c->fast[role]) continue;
+            const int cnt = fast_variants(c->n[role]);
+            const int keep = c->pvar[role];
+            int bestv = keep;
+            std::string tried;
+            for (int v = 0; rc == GP5_OK && v < cnt; ++v) {
+                if (v == keep) continue;
+                int vv[3] = {c->pvar[0], c->pvar[1], c->pvar[2]};
+                vv[role] = v;
+                if (gp5_ctx_set_fft_plans(c, vv) != GP5_OK) continue;  // e.g. blocks spanning components
+                float t = 0.f;
+                rc = batch_time(k, f, &s, steps, &t);
+                snprintf(line, sizeof line, "%s%s %.3f", tried.empty() ? "" : ", ", fast_plan_name(k.n, role, v).c_str(),
+                         t / steps);
+                tried += line;
+                if (rc == GP5_OK && t < best) {
+                    best = t;
+                    bestv = v;
+                }
+                vv[role] = bestv;
+                (void)gp5_ctx_set_fft_plans(c, vv);
+            }
+            static const char *role_name[3] = {"x", "y", "z"};
+            snprintf(line, sizeof line, "; %s plan %s%s", role_name[role], fast_plan_name(k.n, role, c->pvar[role]).c_str(),
+                     bestv != keep ? " (tuned)" : "");
+            rep += line;
+            if (!tried.empty()) rep += " [tried " + tried + "]";
+        }
+        // generic-kernel axes: radix order and tile
         for (int d = 0; rc == GP5_OK && d < 3; ++d) {
             if (c->fast[d]) continue;
-            // radix order of the generic plan
             AxisPlan keep = c->plan[d];
             int best_order = -1;
             for (int o = 0; rc == GP5_OK && o < kPlanOrders; ++o) {
@@ -1746,7 +1990,6 @@ int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor,
                 }
             }
             c->plan[d] = keep;
-            // tile: lines per block
             const int b0 = c->geom[d].B;
             int best_B = b0;
             for (int B = 1; rc == GP5_OK && B <= 64; B *= 2) {
@@ -1779,18 +2022,99 @@ int gp5_batch_tune(gp5_batch *b, const gp5_functional_desc *f, double dt_factor,
             snprintf(line, sizeof line, "; fused x %s", c->fuse_x ? "on" : "off");
             rep += line;
         }
-        snprintf(line, sizeof line, "; tuned %.3f ms/step\n", best / steps);
+        snprintf(line, sizeof line, "; tuned %.3f ms/step (default %.3f)\n", best / steps, dflt / steps);
         rep += line;
         hipError_t e = hipMemcpy(k.psi->d, save, bytes, hipMemcpyDeviceToDevice);
         (void)hipFree(save);
         RCHK(rc);
         HIPCHK(e);
+        if (tunable) {
+            TuneEntry t = {{c->pvar[0], c->pvar[1], c->pvar[2]}, c->fuse_x ? 1 : 0, (double)best / steps};
+            bool found = false;
+            for (auto &en : cache)
+                if (en.first == key) {
+                    en.second = t;
+                    found = true;
+                }
+            if (!found) cache.push_back({key, t});
+            cache_dirty = true;
+        }
     }
+    if (cache_dirty) tune_cache_write(cache_path, cache);
     if (report && report_len) {
         strncpy(report, rep.c_str(), report_len - 1);
         report[report_len - 1] = 0;
     }
     return GP5_OK;
+}
+
+// One replay of check_every steps of class k on its stream, then the
+// convergence check and compaction (capturable: kernels only).
+static int batch_enqueue_replay(gp5_batch *b, BatchClass &k, const gp5_functional_desc *f, double dt_factor,
+                                double tol, int check_every, long long max_steps)
+{
+    gp5_ctx *c = k.ctx;
+    gp5_stepper_desc s = batch_stepper(k, dt_factor);
+    double *const home = k.psi->d;
+    RCHK(advance_enqueue(c, f, &s, k.psi, check_every, true));
+    if (k.psi->d != home) return fail(GP5_ERR_HIP, "a replay must return the state to its buffer (even step count)");
+    BatchCheckArgs a;
+    memset(&a, 0, sizeof a);
+    a.st = k.dev;
+    a.cap = k.cap;
+    a.check_every = check_every;
+    a.tol = tol;
+    a.max_steps = max_steps;
+    a.slot_inst = k.d_slot_inst;
+    a.prevE = k.d_prevE;
+    a.checks = k.d_checks;
+    a.log_row = c->dlog + (size_t)(check_every - 1) * kRedN * c->nc;
+    a.results = c->results;
+    a.nnorm = c->nnorm;
+    a.lhy_ab = c->lhy_ab;
+    a.res_steps = b->res_steps_dev;
+    a.res_energy = b->res_energy_dev;
+    a.copy_slot = k.d_copy_slot;
+    a.copy_inst = k.d_copy_inst;
+    a.move_src = k.d_move_src;
+    a.move_dst = k.d_move_dst;
+    HIPCHK(launch_batch_check(c->st, a));
+    HIPCHK(launch_batch_compact(c->st, (double2 *)k.psi->d, c->Vdev, c->M, a, b->res_ptr_dev));
+    return GP5_OK;
+}
+
+// Graph of one replay of the classes in `which`: joined (every class on its own
+// stream, forked from and joined to `origin`) when origin is given, else the one
+// class captured on its own stream.
+static int batch_capture(gp5_batch *b, const std::vector<int> &which, hipStream_t origin, const gp5_functional_desc *f,
+                         double dt_factor, double tol, int check_every, long long max_steps, hipGraphExec_t *exec)
+{
+    *exec = nullptr;
+    hipStream_t cap_st = origin ? origin : b->cls[which[0]].ctx->st;
+    if (hipStreamBeginCapture(cap_st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(GP5_ERR_HIP, "stream capture failed to start");
+    int rc = GP5_OK;
+    if (origin && hipEventRecord(b->fork, origin) != hipSuccess) rc = fail(GP5_ERR_HIP, "fork event");
+    for (size_t q = 0; q < which.size() && rc == GP5_OK; ++q) {
+        BatchClass &k = b->cls[which[q]];
+        if (origin && hipStreamWaitEvent(k.ctx->st, b->fork, 0) != hipSuccess) {
+            rc = fail(GP5_ERR_HIP, "fork wait");
+            break;
+        }
+        rc = batch_enqueue_replay(b, k, f, dt_factor, tol, check_every, max_steps);
+        if (rc == GP5_OK && origin &&
+            (hipEventRecord(k.join, k.ctx->st) != hipSuccess || hipStreamWaitEvent(origin, k.join, 0) != hipSuccess))
+            rc = fail(GP5_ERR_HIP, "join events");
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cap_st, &g);
+    if (rc == GP5_OK && e != hipSuccess) rc = fail(GP5_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+    if (rc == GP5_OK && hipGraphInstantiate(exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        *exec = nullptr;
+        rc = fail(GP5_ERR_HIP, "graph instantiation failed");
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
 }
 
 int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt_factor, double tol, int check_every,
@@ -1801,107 +2125,121 @@ int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt
     if (f->kind == GP5_FUNC_LHY_UNLOCKED) return fail(GP5_ERR_UNSUPPORTED, "batches hold uncoupled instances");
     if (check_every < 2 || check_every % 2) return fail(GP5_ERR_ARG, "check_every must be even and >= 2 (got %d)", check_every);
     if (!(tol > 0.0)) return fail(GP5_ERR_ARG, "tol must be > 0");
-    hipEvent_t t0, t1;
-    HIPCHK(hipEventCreate(&t0));
-    HIPCHK(hipEventCreate(&t1));
-    HIPCHK(hipEventRecord(t0, b->cls[0].ctx->st));
-    HIPCHK(hipStreamSynchronize(b->cls[0].ctx->st));
     const auto wall0 = std::chrono::steady_clock::now();
+    const int ncls = (int)b->cls.size();
+    std::vector<int> live;
+    for (int ki = 0; ki < ncls; ++ki) {
+        BatchClass &k = b->cls[ki];
+        HIPCHK(hipStreamSynchronize(k.ctx->st));
+        RCHK(batch_push_state(b, k));
+        if (k.active == 0) continue;
+        gp5_stepper_desc s = batch_stepper(k, dt_factor);
+        RCHK(advance_prepare(k.ctx, f, &s, k.psi, check_every, true));  // allocations happen here, not in replays
+        live.push_back(ki);
+    }
+    // units of launch: one joined graph of every live class (default), or one graph
+    // per class replayed at its own pace (GP5_BATCH_GRAPH=classes)
+    const char *mode = getenv("GP5_BATCH_GRAPH");
+    const bool per_class = mode && strcmp(mode, "classes") == 0;
+    struct Unit {
+        std::vector<int> cls;
+        hipStream_t st = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        int slot[2] = {0, 0};
+        int head = 0, inflight = 0, launched = 0;
+        bool stop = false;
+    };
+    std::vector<Unit> units;
+    if (per_class) {
+        for (int ki : live) {
+            units.emplace_back();
+            units.back().cls = {ki};
+            units.back().st = b->cls[ki].ctx->st;
+        }
+    } else if (!live.empty()) {
+        units.emplace_back();
+        units.back().cls = live;
+        units.back().st = b->st;
+    }
     int rc = GP5_OK;
-    for (;;) {
-        bool any = false;
-        // launch every class's replay on its own stream, then collect
-        for (BatchClass &k : b->cls) {
-            if (k.active == 0 || rc != GP5_OK) continue;
-            any = true;
-            gp5_ctx *c = k.ctx;
-            gp5_stepper_desc s = batch_stepper(k, dt_factor);
-            if (!k.exec || k.graph_nc != k.active || k.graph_steps != check_every) {
-                batch_drop_graph(k);
-                rc = advance_prepare(c, f, &s, k.psi, check_every, true);
-                if (rc != GP5_OK) break;
-                if (hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                    rc = fail(GP5_ERR_HIP, "stream capture failed to start");
+    for (size_t u = 0; u < units.size() && rc == GP5_OK; ++u) {
+        Unit &U = units[u];
+        rc = batch_capture(b, U.cls, per_class ? nullptr : b->st, f, dt_factor, tol, check_every, max_steps, &U.exec);
+        for (int q = 0; q < 2 && rc == GP5_OK; ++q)
+            if (hipEventCreateWithFlags(&U.ev[q], hipEventDisableTiming) != hipSuccess) {
+                U.ev[q] = nullptr;
+                rc = fail(GP5_ERR_HIP, "batch events");
+            }
+    }
+    // replay loop: every unit keeps up to two replays queued; the state of a
+    // replay is read (pinned, one replay late) when its event has completed
+    const size_t one = sizeof(BatchClassDev);
+    int err_inst = -1;
+    while (rc == GP5_OK) {
+        bool busy = false, progressed = false;
+        for (size_t u = 0; u < units.size() && rc == GP5_OK; ++u) {
+            Unit &U = units[u];
+            while (!U.stop && U.inflight < 2 && rc == GP5_OK) {
+                const int q = (U.head + U.inflight) % 2;
+                BatchClassDev *dst = b->status_host + (size_t)q * ncls;
+                hipError_t e = hipGraphLaunch(U.exec, U.st);
+                if (e == hipSuccess) {
+                    if (per_class) e = hipMemcpyAsync(dst + U.cls[0], b->status_dev + U.cls[0], one, hipMemcpyDeviceToHost, U.st);
+                    else e = hipMemcpyAsync(dst, b->status_dev, ncls * one, hipMemcpyDeviceToHost, U.st);
+                }
+                if (e == hipSuccess) e = hipEventRecord(U.ev[q], U.st);
+                if (e != hipSuccess) rc = fail(GP5_ERR_HIP, "batch replay launch: %s", hipGetErrorString(e));
+                U.inflight++;
+                U.launched++;
+                progressed = true;
+            }
+            while (U.inflight > 0 && rc == GP5_OK) {
+                const int q = U.head;
+                hipError_t e = hipEventQuery(U.ev[q]);
+                if (e == hipErrorNotReady) break;
+                if (e != hipSuccess) {
+                    rc = fail(GP5_ERR_HIP, "batch replay: %s", hipGetErrorString(e));
                     break;
                 }
-                double *d0 = k.psi->d;
-                rc = advance_enqueue(c, f, &s, k.psi, check_every, true);
-                hipGraph_t g = nullptr;
-                hipError_t e = hipStreamEndCapture(c->st, &g);
-                if (rc == GP5_OK && e != hipSuccess) rc = fail(GP5_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
-                if (rc == GP5_OK && k.psi->d != d0) rc = fail(GP5_ERR_HIP, "graph does not return the state to its buffer");
-                if (rc == GP5_OK && hipGraphInstantiate(&k.exec, g, nullptr, nullptr, 0) != hipSuccess)
-                    rc = fail(GP5_ERR_HIP, "graph instantiation failed");
-                k.graph = g;
-                if (rc != GP5_OK) break;
-                k.graph_nc = k.active;
-                k.graph_steps = check_every;
+                U.head = (U.head + 1) % 2;
+                U.inflight--;
+                progressed = true;
+                const BatchClassDev *h = b->status_host + (size_t)q * ncls;
+                int act = 0;
+                for (int ki : U.cls) {
+                    act += h[ki].active;
+                    if (h[ki].err && rc == GP5_OK) {
+                        err_inst = h[ki].err_inst;
+                        U.stop = true;
+                        rc = GP5_ERR_NONFINITE;
+                    }
+                }
+                if (act == 0) U.stop = true;
             }
-            if (hipGraphLaunch(k.exec, c->st) != hipSuccess) {
-                rc = fail(GP5_ERR_HIP, "graph launch failed");
-                break;
-            }
+            if (!U.stop || U.inflight > 0) busy = true;
         }
-        if (rc != GP5_OK || !any) break;
-        for (BatchClass &k : b->cls) {
-            if (k.active == 0 || k.graph_nc != k.active) continue;
-            gp5_ctx *c = k.ctx;
-            const int nc = k.active;
-            k.host_log.resize((size_t)kRedN * nc + nc);
-            // last step's row of the log: [quantity][slot]; then the last output norms
-            const double *row = c->dlog + (size_t)(check_every - 1) * kRedN * nc;
-            hipError_t e = hipMemcpyAsync(k.host_log.data(), row, (size_t)kRedN * nc * sizeof(double),
-                                          hipMemcpyDeviceToHost, c->st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(k.host_log.data() + (size_t)kRedN * nc, c->results, nc * sizeof(double),
-                                   hipMemcpyDeviceToHost, c->st);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-            if (e != hipSuccess) {
-                rc = fail(GP5_ERR_HIP, "batch read-back: %s", hipGetErrorString(e));
-                break;
-            }
-            k.steps_done += check_every;
-            std::vector<int> done;
-            std::vector<double> done_e;
-            for (int sl = 0; sl < nc; ++sl) {
-                const double E = k.host_log[(size_t)nc + sl];
-                const double nrm = k.host_log[(size_t)kRedN * nc + sl];
-                if (!std::isfinite(E) || !std::isfinite(nrm)) {
-                    rc = fail(GP5_ERR_NONFINITE, "instance %d: energy %g, norm %g", k.slot_inst[sl], E, nrm);
-                    break;
-                }
-                if (k.checks[sl] > 0 && std::fabs(E - k.prevE[sl]) <= tol * std::fabs(E)) {
-                    done.push_back(sl);
-                    done_e.push_back(E);
-                }
-                k.prevE[sl] = E;
-                k.checks[sl]++;
-            }
-            if (rc != GP5_OK) break;
-            const bool out_of_steps = max_steps > 0 && k.steps_done >= max_steps;
-            if (out_of_steps) {
-                // retire everything left; unconverged instances report negative steps
-                for (int sl = k.active - 1; sl >= 0 && rc == GP5_OK; --sl) {
-                    const bool conv = std::find(done.begin(), done.end(), sl) != done.end();
-                    rc = batch_retire(b, k, sl, conv ? k.steps_done : -k.steps_done, k.prevE[sl]);
-                }
-            } else {
-                // retire from the highest slot down so moved slots are not revisited
-                for (int j = (int)done.size() - 1; j >= 0 && rc == GP5_OK; --j)
-                    rc = batch_retire(b, k, done[j], k.steps_done, done_e[j]);
-            }
-            if (rc != GP5_OK) break;
-            if (k.active != k.graph_nc) {
-                batch_drop_graph(k);
-                batch_shrink(k);
-            }
-        }
-        if (rc != GP5_OK) break;
+        if (!busy || rc != GP5_OK) break;
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    // drain, then bring the bookkeeping back to the host
+    for (Unit &U : units) (void)hipStreamSynchronize(U.st);
+    for (int ki : live) (void)hipStreamSynchronize(b->cls[ki].ctx->st);
+    for (Unit &U : units) {
+        if (U.exec) (void)hipGraphExecDestroy(U.exec);
+        for (hipEvent_t e : U.ev)
+            if (e) (void)hipEventDestroy(e);
     }
     const double wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    if (rc == GP5_ERR_NONFINITE) return fail(GP5_ERR_NONFINITE, "instance %d: non-finite energy or norm", err_inst);
     RCHK(rc);
+    std::vector<BatchClassDev> st(ncls);
+    std::vector<long long> rs(b->count);
+    std::vector<double> re(b->count);
+    HIPCHK(hipMemcpy(st.data(), b->status_dev, ncls * one, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rs.data(), b->res_steps_dev, b->count * sizeof(long long), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(re.data(), b->res_energy_dev, b->count * sizeof(double), hipMemcpyDeviceToHost));
+    for (int ki : live) RCHK(batch_pull_state(b, b->cls[ki], st[ki], rs, re));
     for (int i = 0; i < b->count; ++i) {
         if (steps_out) steps_out[i] = b->steps[i];
         if (energy_out) energy_out[i] = b->energy[i];

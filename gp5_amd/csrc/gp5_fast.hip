@@ -19,6 +19,7 @@
 #include "gp5_device.h"
 
 #include <algorithm>
+#include <string>
 #include <type_traits>
 
 namespace gp5 {
@@ -133,8 +134,51 @@ GP5_ZPLAN_OVERRIDE(192, GP5_Z192_PLAN)
 #endif
 #undef GP5_ZPLAN_OVERRIDE
 
-template <int N, bool ROWS, int MODE>
-using PlanOf = std::conditional_t<ROWS, FPlan<N, true>, std::conditional_t<MODE == 1, FPlanZ<N>, FPlan<N, false>>>;
+// Alternative plans, chosen per context at run time by the autotuner
+// (gp5_batch_tune): variant 0 is the default above, variants 1 and 2 reorder the
+// stages (and for 64 and 128 change the values per thread).  Lengths without
+// alternatives have one variant.  Per role: x lines (FPlanV<N, true, V>), y
+// columns (FPlanV<N, false, V>) and the fused z pass (FPlanZV<N, V>).
+template <int N> constexpr int kVariants = (N == 64 || N == 96 || N == 128 || N == 192) ? 3 : 1;
+template <int N, bool ROWS, int V> struct FPlanV : FPlan<N, ROWS> {};
+template <int N, int V> struct FPlanZV : FPlanZ<N> {};
+#define GP5_PLAN_V(NN, RW, V, A, B, C, D, EE)                                                                \
+    template <> struct FPlanV<NN, RW, V> {                                                                  \
+        static constexpr int NP = NN, R0 = A, R1 = B, R2 = C, R3 = D, E = EE, COLS = 0;                     \
+    }
+#define GP5_ZPLAN_V(NN, V, A, B, C, D, EE, CC)                                                               \
+    template <> struct FPlanZV<NN, V> {                                                                     \
+        static constexpr int NP = NN, R0 = A, R1 = B, R2 = C, R3 = D, E = EE, COLS = CC;                    \
+    }
+GP5_PLAN_V(192, true, 1, 12, 4, 4, 0, 12);
+GP5_PLAN_V(192, true, 2, 4, 4, 12, 0, 12);
+GP5_PLAN_V(192, false, 1, 12, 4, 4, 0, 12);
+GP5_PLAN_V(192, false, 2, 4, 12, 4, 0, 12);
+GP5_ZPLAN_V(192, 1, 4, 4, 12, 0, 12, 16);
+GP5_ZPLAN_V(192, 2, 12, 4, 4, 0, 12, 8);
+GP5_PLAN_V(96, true, 1, 12, 4, 2, 0, 12);
+GP5_PLAN_V(96, true, 2, 4, 2, 12, 0, 12);
+GP5_PLAN_V(96, false, 1, 12, 2, 4, 0, 12);
+GP5_PLAN_V(96, false, 2, 12, 4, 2, 0, 12);
+GP5_ZPLAN_V(96, 1, 12, 4, 2, 0, 12, 16);
+GP5_ZPLAN_V(96, 2, 12, 2, 4, 0, 12, 16);
+GP5_PLAN_V(128, true, 1, 4, 8, 4, 0, 8);
+GP5_PLAN_V(128, true, 2, 4, 4, 8, 0, 8);
+GP5_PLAN_V(128, false, 1, 8, 16, 0, 0, 16);
+GP5_PLAN_V(128, false, 2, 8, 4, 4, 0, 8);
+GP5_ZPLAN_V(128, 1, 8, 16, 0, 0, 16, 16);
+GP5_ZPLAN_V(128, 2, 8, 4, 4, 0, 8, 16);
+GP5_PLAN_V(64, true, 1, 4, 4, 4, 0, 4);
+GP5_PLAN_V(64, true, 2, 16, 4, 0, 0, 16);
+GP5_PLAN_V(64, false, 1, 16, 4, 0, 0, 16);
+GP5_PLAN_V(64, false, 2, 4, 16, 0, 0, 16);
+GP5_ZPLAN_V(64, 1, 16, 4, 0, 0, 16, 16);
+GP5_ZPLAN_V(64, 2, 4, 16, 0, 0, 16, 16);
+#undef GP5_PLAN_V
+#undef GP5_ZPLAN_V
+
+template <int N, bool ROWS, int MODE, int V = 0>
+using PlanOf = std::conditional_t<ROWS, FPlanV<N, true, V>, std::conditional_t<MODE == 1, FPlanZV<N, V>, FPlanV<N, false, V>>>;
 
 // Block shapes.  rows: x lines, LPB lines per 128-thread block, LDS line stride
 // N + N/16 (one pad element every 16 keeps stride-16 accesses conflict free).
@@ -155,7 +199,7 @@ template <class P, bool ROWS> struct FShapeP {
     static constexpr int COLS = P::COLS > 0 ? P::COLS : (TL >= 64 ? 8 : GP5_COLS);
     static constexpr int COL_THREADS = COLS * TL;
 };
-template <int N, bool ROWS> using FShape = FShapeP<FPlan<N, ROWS>, ROWS>;
+template <int N, bool ROWS, int V = 0> using FShape = FShapeP<FPlanV<N, ROWS, V>, ROWS>;
 
 #ifndef GP5_COL_HALVES
 #define GP5_COL_HALVES 1
@@ -303,10 +347,10 @@ template <class P> struct FirstLast {
 // MODE 1: forward, k-space multiplier, inverse (the last axis).
 // MODE 2: rows only: inverse, epilogue, optional forward of the epilogue output.
 // MODE 3: MODE 2 with the component-local epilogue (no coupling, no constraint).
-template <int N, bool ROWS, int MODE, int DIR>
+template <int N, bool ROWS, int MODE, int DIR, int V>
 __device__ __forceinline__ void fast_body(const FastArgs &a)
 {
-    using PL = PlanOf<N, ROWS, MODE>;
+    using PL = PlanOf<N, ROWS, MODE, V>;
     using S = FShapeP<PL, ROWS>;
     using FL = FirstLast<PL>;
     constexpr int E = S::E, TL = S::TL;
@@ -328,6 +372,8 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
         gstride = 1;
         c = valid ? (int)(line / a.Opc) : 0;
         rmap.base = l * S::ROW_STRIDE;
+        // retired batch instances: the block's first line decides (block-uniform)
+        if (a.active && (long long)blockIdx.x * S::LPB >= (long long)*a.active * a.Opc) return;
     } else {
         const int b = tid % S::COLS;
         t = tid / S::COLS;
@@ -353,6 +399,7 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
         gstride = a.I;
         c = (int)(o / a.Opc);
         cmap.b = b;
+        if (a.active && c >= *a.active) return;  // retired batch instance (block-uniform)
     }
     double2 v[E];
     // ---- load: inputs of the first stage of the first transform ----
@@ -486,12 +533,12 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
 #else
 #define GP5_KATTR
 #endif
-template <int N, bool ROWS, int MODE, int DIR>
-__global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::ROW_THREADS
-                                          : FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::COL_THREADS))
+template <int N, bool ROWS, int MODE, int DIR, int V>
+__global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE, V>, ROWS>::ROW_THREADS
+                                          : FShapeP<PlanOf<N, ROWS, MODE, V>, ROWS>::COL_THREADS))
     GP5_KATTR k_fast(FastArgs a)
 {
-    fast_body<N, ROWS, MODE, DIR>(a);
+    fast_body<N, ROWS, MODE, DIR, V>(a);
 }
 
 // The same pass held to 3 waves per SIMD (at most 168 VGPRs).  Used for the
@@ -502,12 +549,12 @@ __global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::
 // and measured 106.2 -> 109.6 us).  More waves do not help the 96 and 192 z
 // passes either: held to 5 waves per SIMD (no spills at 96) they measured
 // 17.7 -> 18.1 us and 47.3 -> 50.0 us.
-template <int N, bool ROWS, int MODE, int DIR>
-__global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::ROW_THREADS
-                                          : FShapeP<PlanOf<N, ROWS, MODE>, ROWS>::COL_THREADS))
+template <int N, bool ROWS, int MODE, int DIR, int V>
+__global__ void __launch_bounds__((ROWS ? FShapeP<PlanOf<N, ROWS, MODE, V>, ROWS>::ROW_THREADS
+                                          : FShapeP<PlanOf<N, ROWS, MODE, V>, ROWS>::COL_THREADS))
     __attribute__((amdgpu_waves_per_eu(3))) k_fast_w3(FastArgs a)
 {
-    fast_body<N, ROWS, MODE, DIR>(a);
+    fast_body<N, ROWS, MODE, DIR, V>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -528,32 +575,32 @@ static void col_shape(long long I, long long O, int nx, int xcount, unsigned &gr
 }
 
 // launch shape of a pass; mode selects the z-pass plan for columns (MODE 1)
-template <int N>
+template <int N, int V>
 static void fast_shape(bool rows, int mode, long long I, long long O, int nx, int xcount, unsigned &grid,
                        unsigned &threads, size_t &lds)
 {
     if (rows) {
-        using S = FShape<N, true>;
+        using S = FShape<N, true, V>;
         grid = (unsigned)((O + S::LPB - 1) / S::LPB);
         threads = S::ROW_THREADS;
         lds = sizeof(double2) * (size_t)S::LPB * S::ROW_STRIDE;
     } else if (mode == 1) {
-        col_shape<FShapeP<FPlanZ<N>, false>>(I, O, nx, xcount, grid, threads, lds);
+        col_shape<FShapeP<FPlanZV<N, V>, false>>(I, O, nx, xcount, grid, threads, lds);
     } else {
-        col_shape<FShape<N, false>>(I, O, nx, xcount, grid, threads, lds);
+        col_shape<FShape<N, false, V>>(I, O, nx, xcount, grid, threads, lds);
     }
     lds = std::max(lds, sizeof(double) * (size_t)kRedScratch);
 }
 
-template <int N>
-static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArgs &a)
+template <int N, int V>
+static hipError_t launch_fast_nv(hipStream_t st, int mode, int dir, const FastArgs &a)
 {
     const bool rows = (a.I == 1);
     unsigned grid, threads;
     size_t lds;
-    fast_shape<N>(rows, mode, a.I, a.O, a.nx, a.xcount, grid, threads, lds);
+    fast_shape<N, V>(rows, mode, a.I, a.O, a.nx, a.xcount, grid, threads, lds);
     dim3 g(grid), b(threads);
-#define GP5_FAST_LAUNCH(R, M, D) hipLaunchKernelGGL((k_fast<N, R, M, D>), g, b, lds, st, a)
+#define GP5_FAST_LAUNCH(R, M, D) hipLaunchKernelGGL((k_fast<N, R, M, D, V>), g, b, lds, st, a)
     if (rows) {
         if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(true, 0, -1);
         else if (mode == 0) GP5_FAST_LAUNCH(true, 0, 1);
@@ -564,7 +611,7 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
         if (mode == 0 && dir < 0) GP5_FAST_LAUNCH(false, 0, -1);
         else if (mode == 0) GP5_FAST_LAUNCH(false, 0, 1);
         else if (mode == 1) {
-            if constexpr (N == 256) hipLaunchKernelGGL((k_fast_w3<N, false, 1, -1>), g, b, lds, st, a);
+            if constexpr (N == 256) hipLaunchKernelGGL((k_fast_w3<N, false, 1, -1, V>), g, b, lds, st, a);
             else GP5_FAST_LAUNCH(false, 1, -1);
         } else return hipErrorInvalidValue;  // the epilogue pass needs x lines
     }
@@ -572,53 +619,124 @@ static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArg
     return hipGetLastError();
 }
 
-long long fast_rows_per_block(int n)
+template <int N>
+static hipError_t launch_fast_n(hipStream_t st, int mode, int dir, const FastArgs &a, int var)
 {
-    switch (n) {
-        case 16: return FShape<16, true>::LPB;
-        case 32: return FShape<32, true>::LPB;
-        case 64: return FShape<64, true>::LPB;
-        case 96: return FShape<96, true>::LPB;
-        case 128: return FShape<128, true>::LPB;
-        case 192: return FShape<192, true>::LPB;
-        case 256: return FShape<256, true>::LPB;
-        case 512: return FShape<512, true>::LPB;
-        default: return 1;
+    if constexpr (kVariants<N> > 1) {
+        if (var == 1) return launch_fast_nv<N, 1>(st, mode, dir, a);
+        if (var == 2) return launch_fast_nv<N, 2>(st, mode, dir, a);
     }
+    return launch_fast_nv<N, 0>(st, mode, dir, a);
 }
 
-unsigned fast_grid(int n, long long I, long long O)
+template <int N>
+static long long rows_per_block_n(int var)
+{
+    if constexpr (kVariants<N> > 1) {
+        if (var == 1) return FShape<N, true, 1>::LPB;
+        if (var == 2) return FShape<N, true, 2>::LPB;
+    }
+    return FShape<N, true, 0>::LPB;
+}
+
+template <int N>
+static unsigned grid_n(long long I, long long O, int var)
 {
     unsigned grid = 0, threads;
     size_t lds;
     const bool rows = (I == 1);
-    switch (n) {
-        case 16: fast_shape<16>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 32: fast_shape<32>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 64: fast_shape<64>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 96: fast_shape<96>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 128: fast_shape<128>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 192: fast_shape<192>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 256: fast_shape<256>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        case 512: fast_shape<512>(rows, 2, I, O, 1, 0, grid, threads, lds); break;
-        default: break;
+    if constexpr (kVariants<N> > 1) {
+        if (var == 1) {
+            fast_shape<N, 1>(rows, 2, I, O, 1, 0, grid, threads, lds);
+            return grid;
+        }
+        if (var == 2) {
+            fast_shape<N, 2>(rows, 2, I, O, 1, 0, grid, threads, lds);
+            return grid;
+        }
     }
+    fast_shape<N, 0>(rows, 2, I, O, 1, 0, grid, threads, lds);
     return grid;
 }
 
-hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a)
-{
-    switch (n) {
-        case 16: return launch_fast_n<16>(st, mode, dir, a);
-        case 32: return launch_fast_n<32>(st, mode, dir, a);
-        case 64: return launch_fast_n<64>(st, mode, dir, a);
-        case 96: return launch_fast_n<96>(st, mode, dir, a);
-        case 128: return launch_fast_n<128>(st, mode, dir, a);
-        case 192: return launch_fast_n<192>(st, mode, dir, a);
-        case 256: return launch_fast_n<256>(st, mode, dir, a);
-        case 512: return launch_fast_n<512>(st, mode, dir, a);
-        default: return hipErrorInvalidValue;
+#define GP5_FAST_SWITCH(n, EXPR_N) \
+    switch (n) {                   \
+        case 16: return EXPR_N(16); \
+        case 32: return EXPR_N(32); \
+        case 64: return EXPR_N(64); \
+        case 96: return EXPR_N(96); \
+        case 128: return EXPR_N(128); \
+        case 192: return EXPR_N(192); \
+        case 256: return EXPR_N(256); \
+        case 512: return EXPR_N(512); \
+        default: break;             \
     }
+
+long long fast_rows_per_block(int n, int var)
+{
+#define GP5_E(NN) rows_per_block_n<NN>(var)
+    GP5_FAST_SWITCH(n, GP5_E)
+#undef GP5_E
+    return 1;
 }
+
+unsigned fast_grid(int n, long long I, long long O, int var)
+{
+#define GP5_E(NN) grid_n<NN>(I, O, var)
+    GP5_FAST_SWITCH(n, GP5_E)
+#undef GP5_E
+    return 0;
+}
+
+hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a, int var)
+{
+#define GP5_E(NN) launch_fast_n<NN>(st, mode, dir, a, var)
+    GP5_FAST_SWITCH(n, GP5_E)
+#undef GP5_E
+    return hipErrorInvalidValue;
+}
+
+int fast_variants(int n)
+{
+#define GP5_E(NN) kVariants<NN>
+    GP5_FAST_SWITCH(n, GP5_E)
+#undef GP5_E
+    return 0;
+}
+
+// "R0xR1xR2/E" of a plan, for tuning reports (role 0: x lines, 1: y columns, 2: z pass)
+template <class P> static std::string plan_text()
+{
+    std::string t = std::to_string(P::R0);
+    for (int r : {P::R1, P::R2, P::R3})
+        if (r > 0) t += "x" + std::to_string(r);
+    t += "/" + std::to_string(P::E);
+    return t;
+}
+
+template <int N, int V> static std::string plan_name_nv(int role)
+{
+    if (role == 0) return plan_text<FPlanV<N, true, V>>();
+    if (role == 1) return plan_text<FPlanV<N, false, V>>() + " cols " + std::to_string(FShape<N, false, V>::COLS);
+    return plan_text<FPlanZV<N, V>>() + " cols " + std::to_string(FShapeP<FPlanZV<N, V>, false>::COLS);
+}
+
+template <int N> static std::string plan_name_n(int role, int var)
+{
+    if constexpr (kVariants<N> > 1) {
+        if (var == 1) return plan_name_nv<N, 1>(role);
+        if (var == 2) return plan_name_nv<N, 2>(role);
+    }
+    return plan_name_nv<N, 0>(role);
+}
+
+std::string fast_plan_name(int n, int role, int var)
+{
+#define GP5_E(NN) plan_name_n<NN>(role, var)
+    GP5_FAST_SWITCH(n, GP5_E)
+#undef GP5_E
+    return "generic";
+}
+#undef GP5_FAST_SWITCH
 
 }  // namespace gp5

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace gp5 {
 
 constexpr int kMaxRadices = 24;
@@ -31,6 +33,9 @@ struct AxisGeom {
     long long I;   // product of the dims before the axis (its element stride)
     long long O;   // product of the dims after the axis times nc
     int B;         // lines per tile
+    // batch classes: number of live components (device); tiles of components at
+    // or beyond it exit at once.  Null: every component is live.
+    const int *active;
 };
 
 // Spectral multiplier of the fused forward/inverse pass:
@@ -101,6 +106,7 @@ struct EpiArgs {
     double Nnorm[2];        // N[c] for up to two components ...
     const double *Nnorm_dev;  // ... or a device array of nc values when non-null
     double dV;
+    const int *active;      // batch classes: live components (device), or null (finalize skips the rest)
 };
 
 // Load pre-operation of a first pass: v = src * scale - (cfr + i cfi) * src2.
@@ -126,13 +132,61 @@ struct FastArgs {
     // column kernels restricted to x in [x0, x0 + xcount) (columns i = x + nx*r);
     // xcount == 0: all columns, packed COLS at a time
     int nx, x0, xcount;
+    // batch classes: live components (device); blocks whose (first) component is
+    // at or beyond it exit at once.  Null: every component is live.
+    const int *active;
 };
+
+// Convergence state of one class of a batch ground state (device memory, one
+// entry per class in one array so that the host reads every class at once).
+struct BatchClassDev {
+    int active;            // live slots: the class's kernels gate on it
+    int err;               // nonzero: a non-finite energy or norm was met
+    int err_inst;          // the instance that had it
+    int ncopy;             // slots retired at the last check (copied to their results)
+    int nmove;             // live slots moved into retired ones at the last check
+    int pad;
+    long long steps_done;  // steps the class has taken
+};
+
+// Arguments of the on-device convergence check of one class (gp5_batch.hip):
+// after every replay of check_every steps, an instance whose energy moved by at
+// most tol relative since the previous check retires; its field is copied to its
+// result and the class's last live slot takes its place.
+struct BatchCheckArgs {
+    BatchClassDev *st;
+    int cap;                 // slots (components of the class's context)
+    int check_every;
+    double tol;
+    long long max_steps;     // > 0: every live slot retires once steps_done reaches it
+    int *slot_inst;          // [cap] slot -> instance
+    double *prevE;           // [cap] energy at the previous check
+    int *checks;             // [cap] checks done
+    const double *log_row;   // [kRedN][cap] the replay's last per-step log row (energy at [1][slot])
+    const double *results;   // [kRedN][cap] the last output's norms at [0][slot]
+    double *nnorm;           // [cap] normalisation targets, moved with their slots
+    double *lhy_ab;          // [2 cap] LHY coefficient pairs, or null
+    long long *res_steps;    // [instances] steps at retirement (negative: max_steps reached unconverged)
+    double *res_energy;      // [instances] energy at retirement
+    int *copy_slot;          // [cap] retired slots of the last check ...
+    int *copy_inst;          // [cap] ... and their instances
+    int *move_src;           // [cap] original slots moved ...
+    int *move_dst;           // [cap] ... into these
+};
+hipError_t launch_batch_check(hipStream_t st, const BatchCheckArgs &a);
+// copy the retired slots' fields to their results, then move live slots into the holes
+hipError_t launch_batch_compact(hipStream_t st, double2 *psi, double *V, long long M, const BatchCheckArgs &a,
+                                double2 *const *res_ptr);
 
 int set_error(int code, const char *msg);  // sets gp5_last_error(), returns code
 bool fast_supported(int n);
-unsigned fast_grid(int n, long long I, long long O);
-long long fast_rows_per_block(int n);  // x lines per block of the row kernels
-hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a);
+// var: register-plan variant (0 default; fast_variants(n) of them): per context,
+// one for the x lines, one for the y columns and one for the fused z pass
+unsigned fast_grid(int n, long long I, long long O, int var = 0);
+long long fast_rows_per_block(int n, int var = 0);  // x lines per block of the row kernels
+hipError_t launch_fast(hipStream_t st, int n, int mode, int dir, const FastArgs &a, int var = 0);
+int fast_variants(int n);
+std::string fast_plan_name(int n, int role, int var);  // "R0xR1xR2/E" (role 0 x lines, 1 y, 2 z)
 // spectral resampling (gp5_resample.hip): per destination index d of one axis,
 // the source indices src[off[d] .. off[d+1]) with complex weights w (device)
 struct ResampleAxis {

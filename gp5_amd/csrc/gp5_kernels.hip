@@ -133,6 +133,8 @@ __global__ __launch_bounds__(kThreads) void k_axis(const double2 *__restrict__ s
         i0 = ((long long)blockIdx.x % tpo) * B;
         count = (int)min((long long)B, G.I - i0);
     }
+    // retired batch instances: the tile's first line decides (block-uniform)
+    if (G.active && (rows ? o0 : ofix) >= (long long)*G.active * Opc) return;
     const int tileN = B * n;
     // ---- load ----
     for (int e = tid; e < tileN; e += kThreads) {
@@ -230,6 +232,7 @@ __global__ __launch_bounds__(kFinalThreads) void k_finalize(EpiArgs E, unsigned 
 {
     __shared__ double sred[kRedScratch];
     const int c = blockIdx.x;
+    if (E.active && c >= *E.active) return;  // retired batch instance
     long long b0 = 0, b1 = nblocks;
     if (lpb > 0) {
         b0 = (long long)c * lpc / lpb;
@@ -269,9 +272,10 @@ hipError_t launch_finalize(hipStream_t st, const EpiArgs &E, unsigned nblocks, l
 
 // f *= s (host value s when sp is null; else sp[0] for one component, sp[c] for all)
 __global__ __launch_bounds__(kThreads) void k_scale(double2 *f, long long M, int nc, int comp, double s,
-                                                    const double *sp)
+                                                    const double *sp, const int *active)
 {
-    const long long len = comp >= 0 ? M : M * nc;
+    // batch classes: the live components only
+    const long long len = comp >= 0 ? M : M * (active ? min(nc, *active) : nc);
     const long long base = comp >= 0 ? M * comp : 0;
     for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < len; i += (long long)gridDim.x * kThreads) {
         const long long g = base + i;
@@ -393,11 +397,12 @@ hipError_t launch_norm(hipStream_t st, const double2 *f, long long len, unsigned
     return hipGetLastError();
 }
 
-hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp)
+hipError_t launch_scale(hipStream_t st, double2 *f, long long M, int nc, int comp, double s, const double *sp,
+                        const int *active)
 {
     long long len = comp >= 0 ? M : M * nc;
     unsigned blocks = (unsigned)std::min<long long>((len + kThreads - 1) / kThreads, 8192);
-    hipLaunchKernelGGL(k_scale, dim3(blocks), dim3(kThreads), 0, st, f, M, nc, comp, s, sp);
+    hipLaunchKernelGGL(k_scale, dim3(blocks), dim3(kThreads), 0, st, f, M, nc, comp, s, sp, active);
     return hipGetLastError();
 }
 

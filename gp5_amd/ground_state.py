@@ -144,3 +144,58 @@ def multilevel(n: Sequence[int], L: Sequence[float], N: Sequence[float], f: capi
 def two_level(n, L, N, f, **kw):
     """multilevel with one coarse level (half the resolution)."""
     return multilevel(n, L, N, f, levels=2, **kw)
+
+
+def droplet_pair(n: int = 512, L: float = 64.0, N: float = 4000.0, x0: float = 8.0, k: float = 0.5,
+                 b: float = 2.5, tol: float = 1e-10, use_surrogate: bool = True, dt_factor: float = 0.1,
+                 check_every: int = 100):
+    """Initial state of the large collision workload: two LHY droplets of N atoms built
+    from one ground state of the periodic box [-L/2, L/2]^3 on n^3 (dx = L/n), computed
+    by two_level (coarse grid n/2, seeded by the learned warm start when its weights
+    are present, else by a Gaussian; FP64 imaginary time to `tol` relative energy
+    change per `check_every` steps), moved to x = -x0 and x = +x0 and given the
+    momenta +q and -q toward each other:
+        psi = g(x + x0, y, z) e^{+i q x} + g(x - x0, y, z) e^{-i q x},
+    q = k snapped to 2 pi m / L (periodic), x0 a whole number of cells (an exact
+    shift of the samples), the sum normalised to 2N.
+    Returns (shape, left, right, psi (n, n, n, 1) complex128 Fortran order, q, info)."""
+    import os
+
+    dx = L / n
+    shift = int(round(x0 / dx))
+    if abs(shift * dx - x0) > 1e-12 * L:
+        raise ValueError(f"x0 = {x0} is not a whole number of cells of {dx}")
+    q = 2.0 * np.pi / L * round(k * L / (2.0 * np.pi))
+    f = capi.functional_desc(capi.GP5_FUNC_LHY, lhy_a=-3.0, lhy_b=b)
+    kw = {}
+    start = "gaussian"
+    if use_surrogate:
+        from . import surrogate
+
+        if os.path.exists(surrogate.DEFAULT_WEIGHTS):
+            sur = surrogate.Surrogate()
+            kw["seed"] = lambda i, g, Lg, Ng: sur.seed(Ng, (0.0, 0.0, 0.0), b, g, Lg)
+            start = "learned warm start"
+    t0 = time.perf_counter()
+    fb, rep = two_level([n], [L], [N], f, dt_factor=dt_factor, tol=tol, check_every=check_every, **kw)
+    try:
+        g = fb.download(0)
+    finally:
+        fb.close()
+    seconds = time.perf_counter() - t0
+    x = -0.5 * L + (np.arange(n) + 0.5) * dx
+    ep, em = np.exp(1j * q * x), np.exp(-1j * q * x)
+    psi = np.empty((n, n, n, 1), dtype=np.complex128, order="F")
+    # g(x + x0) at sample i is g's sample i + shift (periodic), g(x - x0) the sample i - shift
+    psi[..., 0] = np.roll(g, -shift, axis=0) * ep[:, None, None] + np.roll(g, shift, axis=0) * em[:, None, None]
+    del g
+    norm = 0.0
+    for kz in range(n):
+        norm += float(np.sum(np.abs(psi[:, :, kz, 0]) ** 2))
+    psi *= np.sqrt(2.0 * N / (norm * dx ** 3))
+    info = {"ground_state": f"one N={N:g} droplet on {n}^3 (L={L:g}), two-level from {rep['coarse_grids'][0]}^3, "
+                            f"start: {start}, FP64 imaginary-time RK4 to {tol:g}",
+            "seconds": seconds, "coarse_steps": int(np.abs(rep["coarse_steps"][0])),
+            "fine_steps": int(np.abs(rep["steps"][0])), "converged": bool(rep["steps"][0] > 0),
+            "energy_per_atom": float(rep["energy"][0] / N), "overlap_norm_factor": float(2.0 * N / (norm * dx ** 3))}
+    return (n, n, n), (-0.5 * L,) * 3, (0.5 * L,) * 3, psi, q, info

@@ -81,6 +81,17 @@ int gp5_ctx_sync(gp5_ctx *ctx);
 /* Points per component (Nx*Ny*Nz). */
 size_t gp5_ctx_points(const gp5_ctx *ctx);
 
+/* Register plans of the FFT passes (stage order, values per thread, columns
+ * per block), per role: 0 x lines (with the fused epilogue), 1 y columns,
+ * 2 the fused z pass.  The reference leaves its plan choice to FFTW_MEASURE
+ * (src/fourierTransform.cpp:166-171); here the autotuner (gp5_batch_tune)
+ * picks a variant per role, and callers may set one.  count[r] = variants of
+ * role r (0: the axis runs on the generic kernels); variant 0 is the default.
+ * Every variant computes the same DFT. */
+int gp5_ctx_fft_plans(const gp5_ctx *ctx, int variant[3], int count[3]);
+int gp5_ctx_set_fft_plans(gp5_ctx *ctx, const int variant[3]);
+const char *gp5_ctx_fft_plan_name(const gp5_ctx *ctx, int role); /* e.g. "4x12x4/12" */
+
 /* Device buffers (the Eigen tensors of the reference live here). */
 int gp5_buf_alloc(gp5_ctx *ctx, size_t n_doubles, gp5_buf **out);
 int gp5_buf_free(gp5_buf *buf);

@@ -149,3 +149,35 @@ def test_batch_ground_states_64_96_confirmed_by_oracle(oracle_mod):
     err = rel_l2(out[i][..., None], ref)
     print(f"64^3 member after {int(steps[i])} steps: relative L2 vs oracle {err:.2e}", flush=True)
     assert err <= 1e-11
+
+
+@pytest.mark.timeout(600)
+def test_batch_graph_replays_equal_eager_steps_at_config3_classes(monkeypatch):
+    """The ground-state graph (every class forked from one launch, and one graph per
+    class) stopped by max_steps equals eager steps of the same batch, at the 64^3,
+    96^3, 128^3 and 192^3 classes."""
+    from gp5_amd import capi
+
+    n, L, N, psi0, pick = _class_members(2)
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    ref = capi.Batch(n, L, N)
+    for i, a in enumerate(psi0):
+        ref.upload(i, a)
+    ref.advance(f, 0.1, 20)
+    want = [ref.download(i) for i in range(len(n))]
+    ref.close()
+    got = {}
+    for mode in ("joined", "classes"):
+        monkeypatch.setenv("GP5_BATCH_GRAPH", mode)
+        b = capi.Batch(n, L, N)
+        for i, a in enumerate(psi0):
+            b.upload(i, a)
+        steps, en, ms = b.ground_state(f, 0.1, tol=1e-300, check_every=10, max_steps=20)
+        print(f"{mode}: steps {steps.tolist()} in {ms:.0f} ms", flush=True)
+        assert np.all(steps == -20)
+        got[mode] = [b.download(i) for i in range(len(n))]
+        b.close()
+        for i in range(len(n)):
+            assert rel_l2(got[mode][i], want[i]) < 1e-14, (mode, pick[i])
+    for i in range(len(n)):
+        np.testing.assert_array_equal(got["joined"][i], got["classes"][i])
