@@ -202,14 +202,35 @@ __device__ __forceinline__ void block_sum(double (&v)[kRedQ], double *sred, doub
     }
 }
 
-__device__ __forceinline__ void reduce_block(double (&v)[kRedQ], double *partials, double *sred)
+// bid: the block's index in the launch's tile order (blockIdx.x unless remapped)
+__device__ __forceinline__ void reduce_block(double (&v)[kRedQ], double *partials, double *sred, long long bid)
 {
     double s[kRedQ];
     block_sum(v, sred, s);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < kRedQ; ++q) partials[(size_t)blockIdx.x * kRedQ + q] = s[q];
+        for (int q = 0; q < kRedQ; ++q) partials[(size_t)bid * kRedQ + q] = s[q];
     }
+}
+
+__device__ __forceinline__ void reduce_block(double (&v)[kRedQ], double *partials, double *sred)
+{
+    reduce_block(v, partials, sred, (long long)blockIdx.x);
+}
+
+// XCD-aware tile order.  The dispatcher deals workgroups to the 8 XCDs of the chip
+// round-robin (block b runs on XCD b mod 8); with remap on, the b-th block takes
+// tile (b mod 8) * (G / 8) + b / 8, so that each XCD walks one contiguous eighth
+// of the tiles (its L2 and TLB see neighbouring lines and columns) instead of
+// every eighth tile of the whole field.  The last G mod 8 blocks keep their index,
+// so the map is a bijection on [0, G).
+__device__ __forceinline__ long long xcd_tile(bool remap)
+{
+    const long long b = blockIdx.x;
+    if (!remap) return b;
+    const long long per = (long long)gridDim.x / 8;
+    if (b >= per * 8) return b;
+    return (b % 8) * per + b / 8;
 }
 
 // ---------------------------------------------------------------------------

@@ -99,14 +99,16 @@ GP5_PLAN(512, true, 8, 8, 8, 0, 8);
 
 // Plans of the z pass (MODE 1 columns: forward transform, k-space multiplier,
 // inverse): the column plan unless specialised.  At 512 the z columns lie 4 MiB
-// apart; 16 values per thread over 8 columns (256-thread blocks, plan 16.8.4)
-// measured 11 % faster there than the 8.8.8 column plan, which stays the better
-// one for the y pass.  At 192 the radix-12 stage first (12.4.4 over 16 columns)
+// apart; 16 values per thread over 16 columns (512-thread blocks, plan 8.8.8,
+// 128 VGPRs, 4 waves per SIMD) measured 970 us per pass against 1104 us for
+// 16.8.4 over 8 columns and 1020 us for 8.8.8 with 32 values over 16 columns
+// (both with the XCD-aware column order); 8.8.8 with 8 values over 8 columns
+// stays the better plan for the y pass.  At 192 the radix-12 stage first (12.4.4 over 16 columns)
 // measured 3.5 % faster than the 4.4.12 column plan.
 template <int N> struct FPlanZ : FPlan<N, false> {};
 #ifndef GP5_Z512_DEFAULT
 #ifndef GP5_Z512_PLAN
-#define GP5_Z512_PLAN 16, 8, 4, 0, 16, 8  // R0, R1, R2, R3, E, COLS
+#define GP5_Z512_PLAN 8, 8, 8, 0, 16, 16  // R0, R1, R2, R3, E, COLS
 #endif
 template <> struct FPlanZ<512> {
     static constexpr int p[6] = {GP5_Z512_PLAN};
@@ -139,7 +141,7 @@ GP5_ZPLAN_OVERRIDE(192, GP5_Z192_PLAN)
 // stages (and for 64 and 128 change the values per thread).  Lengths without
 // alternatives have one variant.  Per role: x lines (FPlanV<N, true, V>), y
 // columns (FPlanV<N, false, V>) and the fused z pass (FPlanZV<N, V>).
-template <int N> constexpr int kVariants = (N == 64 || N == 96 || N == 128 || N == 192) ? 3 : 1;
+template <int N> constexpr int kVariants = (N == 64 || N == 96 || N == 128 || N == 192 || N == 512) ? 3 : 1;
 template <int N, bool ROWS, int V> struct FPlanV : FPlan<N, ROWS> {};
 template <int N, int V> struct FPlanZV : FPlanZ<N> {};
 #define GP5_PLAN_V(NN, RW, V, A, B, C, D, EE)                                                                \
@@ -174,6 +176,12 @@ GP5_PLAN_V(64, false, 1, 16, 4, 0, 0, 16);
 GP5_PLAN_V(64, false, 2, 4, 16, 0, 0, 16);
 GP5_ZPLAN_V(64, 1, 16, 4, 0, 0, 16, 16);
 GP5_ZPLAN_V(64, 2, 4, 16, 0, 0, 16, 16);
+GP5_PLAN_V(512, true, 1, 8, 8, 8, 0, 16);
+GP5_PLAN_V(512, true, 2, 16, 8, 4, 0, 16);
+GP5_PLAN_V(512, false, 1, 16, 8, 4, 0, 16);
+GP5_PLAN_V(512, false, 2, 8, 8, 8, 0, 16);
+GP5_ZPLAN_V(512, 1, 8, 8, 8, 0, 32, 16);
+GP5_ZPLAN_V(512, 2, 16, 8, 4, 0, 16, 8);
 #undef GP5_PLAN_V
 #undef GP5_ZPLAN_V
 
@@ -363,33 +371,35 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
     long long colidx = 0;
     RowMap rmap{0};
     ColMap cmap{0, S::COLS};
+    // tile of this block: XCD-aware order when a.xcd asks for it (bit 0 columns, bit 1 lines)
+    const long long bid = xcd_tile(ROWS ? (a.xcd & 2) != 0 : (a.xcd & 1) != 0);
     if constexpr (ROWS) {
         const int l = tid / TL;
         t = tid - l * TL;
-        const long long line = (long long)blockIdx.x * S::LPB + l;
+        const long long line = bid * S::LPB + l;
         valid = line < a.O;
         gbase = line * N;
         gstride = 1;
         c = valid ? (int)(line / a.Opc) : 0;
         rmap.base = l * S::ROW_STRIDE;
         // retired batch instances: the block's first line decides (block-uniform)
-        if (a.active && (long long)blockIdx.x * S::LPB >= (long long)*a.active * a.Opc) return;
+        if (a.active && bid * S::LPB >= (long long)*a.active * a.Opc) return;
     } else {
         const int b = tid % S::COLS;
         t = tid / S::COLS;
         long long o;
         if (a.xcount == 0) {
             const long long tpo = (a.I + S::COLS - 1) / S::COLS;
-            o = (long long)blockIdx.x / tpo;
-            colidx = ((long long)blockIdx.x - o * tpo) * S::COLS + b;
+            o = bid / tpo;
+            colidx = (bid - o * tpo) * S::COLS + b;
             valid = colidx < a.I;
         } else {
             // x slab [x0, x0 + xcount) of every row r of columns i = x + nx*r
             const long long tpr = (a.xcount + S::COLS - 1) / S::COLS;
             const long long rows_per_o = a.I / a.nx;
             const long long per_o = tpr * rows_per_o;
-            o = (long long)blockIdx.x / per_o;
-            const long long rem = (long long)blockIdx.x - o * per_o;
+            o = bid / per_o;
+            const long long rem = bid - o * per_o;
             const long long r = rem / tpr;
             const int xo = (int)(rem - r * tpr) * S::COLS + b;
             valid = xo < a.xcount;
@@ -520,10 +530,10 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
             double red[kRedQ];
 #pragma unroll
             for (int q = 0; q < kRedQ; ++q) red[q] = 0.0;
-            const int cb = (int)((long long)blockIdx.x * S::LPB / a.Opc);  // block's first component
+            const int cb = (int)(bid * S::LPB / a.Opc);  // block's first component
             acc_scatter(acc, c - cb, red);
             __syncthreads();
-            reduce_block(red, a.epi.partials, reinterpret_cast<double *>(lds));
+            reduce_block(red, a.epi.partials, reinterpret_cast<double *>(lds), bid);
         }
     }
 }

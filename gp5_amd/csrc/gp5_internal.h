@@ -135,6 +135,7 @@ struct FastArgs {
     // batch classes: live components (device); blocks whose (first) component is
     // at or beyond it exit at once.  Null: every component is live.
     const int *active;
+    int xcd;  // XCD-aware tile order: bit 0 column passes, bit 1 x-line passes
 };
 
 // Convergence state of one class of a batch ground state (device memory, one

@@ -199,3 +199,121 @@ def droplet_pair(n: int = 512, L: float = 64.0, N: float = 4000.0, x0: float = 8
             "fine_steps": int(np.abs(rep["steps"][0])), "converged": bool(rep["steps"][0] > 0),
             "energy_per_atom": float(rep["energy"][0] / N), "overlap_norm_factor": float(2.0 * N / (norm * dx ** 3))}
     return (n, n, n), (-0.5 * L,) * 3, (0.5 * L,) * 3, psi, q, info
+
+
+def seed_error_curve(per_class: int = 4, grids=(64, 96), eps=(0.3, 0.1, 0.03, 0.01, 3e-3, 1e-3, 3e-4, 1e-4, 1e-5),
+                     dt_factor: float = 0.1, tol: float = 1e-10, check_every: int = 100):
+    """Steps to convergence against the error of the start.  For droplets of the
+    batched workload (per_class of each grid in `grids`), a tight ground state g
+    (1e-12) is computed cold; starts s = (1 - e) g + e c (c: the cold Gaussian seed),
+    renormalised, span relative L2 errors from ~e down to 1e-5 of g.  All starts, the
+    cold seeds and the network's seeds run in one batch to `tol`.  Returns per start:
+    the median relative L2 error |s - g| / |g|, the total steps and the speed-up in
+    steps over the cold starts."""
+    import os
+
+    from . import synth
+
+    n0, L0, N0, psi0 = synth.droplet_batch(64, 3)
+    pick = []
+    for g in grids:
+        pick += [i for i in range(len(n0)) if n0[i] == g][:per_class]
+    n, L, N = [n0[i] for i in pick], [L0[i] for i in pick], [N0[i] for i in pick]
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    b = capi.Batch(n, L, N)
+    try:
+        for i, k in enumerate(pick):
+            b.upload(i, psi0[k])
+        b.ground_state(f, dt_factor, 1e-12, check_every, 200000)
+        gs = [b.download(i) for i in range(len(n))]
+    finally:
+        b.close()
+
+    def unit(a, Nk, h):
+        return a * np.sqrt(Nk / (np.sum(np.abs(a) ** 2) * h ** 3))
+
+    starts = {"cold": [], "network": []}
+    for e in eps:
+        starts[f"{e:g}"] = []
+    sur = None
+    from . import surrogate
+
+    if os.path.exists(surrogate.DEFAULT_WEIGHTS):
+        sur = surrogate.Surrogate()
+    for i, k in enumerate(pick):
+        h = L[i] / n[i]
+        c = unit(psi0[k], N[i], h)
+        starts["cold"].append(c)
+        for e in eps:
+            starts[f"{e:g}"].append(unit((1 - e) * gs[i] + e * c, N[i], h))
+        if sur is not None:
+            starts["network"].append(sur.seed(N[i], (0.0, 0.0, 0.0), 2.5, n[i], L[i]))
+    if sur is None:
+        del starts["network"]
+    names = list(starts)
+    inst = [(name, i) for name in names for i in range(len(n))]
+    bb = capi.Batch([n[i] for _, i in inst], [L[i] for _, i in inst], [N[i] for _, i in inst])
+    try:
+        for j, (name, i) in enumerate(inst):
+            bb.upload(j, starts[name][i])
+        steps, en, ms = bb.ground_state(f, dt_factor, tol, check_every, 200000)
+    finally:
+        bb.close()
+    out = {"instances": [f"{n[i]}^3 N={N[i]:.0f}" for i in range(len(n))], "tol": tol, "rows": []}
+    cold_steps = None
+    for name in names:
+        idx = [j for j, (nm, _) in enumerate(inst) if nm == name]
+        err = [float(np.linalg.norm(unit(starts[name][i], N[i], L[i] / n[i]) - gs[i]) / np.linalg.norm(gs[i]))
+               for _, i in (inst[j] for j in idx)]
+        tot = int(np.abs(steps[idx]).sum())
+        if name == "cold":
+            cold_steps = tot
+        out["rows"].append({"start": name, "rel_l2_median": float(np.median(err)), "rel_l2_max": float(np.max(err)),
+                            "steps_total": tot, "converged": int((steps[idx] > 0).sum())})
+    for r in out["rows"]:
+        r["speedup_in_steps_vs_cold"] = cold_steps / r["steps_total"]
+    return out
+
+
+def _main(argv=None) -> int:
+    """python -m gp5_amd.ground_state advance [--count 64] [--steps 10]: the batched
+    droplet workload (synth.droplet_batch) advanced eagerly, class by class, after
+    tuning; prints one JSON line with each class's time per step.  A fixed, short
+    workload of the batch kernels for rocprofv3 (kernel traces and counters)."""
+    import argparse
+    import json
+
+    from . import synth
+
+    ap = argparse.ArgumentParser(prog="python -m gp5_amd.ground_state")
+    ap.add_argument("what", choices=["advance", "curve"])
+    ap.add_argument("--count", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--dt-factor", type=float, default=0.1)
+    args = ap.parse_args(argv)
+    if args.what == "curve":
+        print(json.dumps(seed_error_curve(dt_factor=args.dt_factor)))
+        return 0
+    n, L, N, psi0 = synth.droplet_batch(args.count, 3)
+    b = capi.Batch(n, L, N)
+    try:
+        for i, a in enumerate(psi0):
+            b.upload(i, a)
+        f = capi.functional_desc(capi.GP5_FUNC_LHY)
+        report = b.tune(f, args.dt_factor, 2)
+        b.advance(f, args.dt_factor, 2)  # warm-up
+        t0 = time.perf_counter()
+        b.advance(f, args.dt_factor, args.steps)
+        wall = time.perf_counter() - t0
+    finally:
+        b.close()
+    print(json.dumps({"workload": f"{args.count} droplets (seed 3) on " + ", ".join(
+        f"{g}^3 x{sum(1 for k in n if k == g)}" for g in synth.BATCH_GRIDS) + f", {args.steps} eager RK4 steps",
+                      "seconds": wall, "ms_per_step": 1e3 * wall / args.steps, "tuning": report.strip().splitlines()}))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+
+    sys.exit(_main())

@@ -1,7 +1,7 @@
 """Register-plan variants of the FFT passes and the autotuner that picks them.
 
-Every variant (stage order, values per thread, columns per block) of the 64-,
-96-, 128- and 192-point passes computes the same unnormalised DFT
+Every variant (stage order, values per thread, columns per block) of the 64-, 96-,
+128-, 192- and 512-point passes computes the same unnormalised DFT
 (src/fourierTransform.cpp:184-214) and, inside a step, the same RK4 update
 (src/stepper.cpp:46-70): each is held to the oracle on the role it serves (x
 lines with the fused epilogue, y columns, the fused z multiplier pass).  The
@@ -15,7 +15,7 @@ from conftest import rel_l2, seeded_field
 capi = pytest.importorskip("gp5_amd.capi")
 
 ROLE_AXIS = {0: "x", 1: "y", 2: "z"}
-CASES = [(n, role, v) for n in (64, 96, 128, 192) for role in (0, 1, 2) for v in (1, 2)]
+CASES = [(n, role, v) for n in (64, 96, 128, 192, 512) for role in (0, 1, 2) for v in (1, 2)]
 
 
 @pytest.mark.gpu
@@ -44,11 +44,14 @@ def test_plan_variant_matches_oracle(oracle_mod, n, role, v):
     g = np.exp(-synth.r2_grid(shape, left, right) / 3.0)[..., None] * (1 + 0.05 * a)
     g = np.asfortranarray(g)
     geo = O.Geometry(shape, left, right, 1)
+    # explicit RK4 needs dt k_max^2 / 2 inside its stability range: 0.6 here
+    kmax = np.pi * n / 8.0
+    dt = min(2e-4, 1.2 / kmax ** 2)
     for imag, renorm in ((False, None), (True, [2.0])):
-        ref = O.advance(O.Functional(kind=O.OR_LHY), g, geo, stepper="RK4", dt=2e-4, imaginary=imag, nsteps=6,
+        ref = O.advance(O.Functional(kind=O.OR_LHY), g, geo, stepper="RK4", dt=dt, imaginary=imag, nsteps=6,
                         renormalize=renorm)
         psi = ctx.field(g)
-        s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=2e-4, imaginary=imag, renormalize=renorm)
+        s = capi.stepper_desc(capi.GP5_STEP_RK4, dt=dt, imaginary=imag, renormalize=renorm)
         capi.advance(ctx, capi.functional_desc(capi.GP5_FUNC_LHY), s, psi, 6)
         assert rel_l2(psi.download_complex(), ref) <= 1e-12
 

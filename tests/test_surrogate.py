@@ -116,3 +116,32 @@ def test_batched_predictions_match_single():
     for (N, w, b), psi in zip(items, seeds):
         ref = sur.seed(N, w, b, 32, 24.0)
         assert np.max(np.abs(psi - ref)) <= 2e-2 * np.max(np.abs(ref)), (N, w, b)
+
+
+@pytest.mark.gpu
+def test_network_coarse_starts_beat_gaussian_coarse_starts():
+    """What the trained network gains over the method without it: two-level ground
+    states (coarse grid n/2, then n) of the same droplets, coarse level started from the
+    network's seeds or from Gaussians (gp5_amd.ground_state.two_level).  The network's
+    coarse level takes fewer imaginary-time steps in total, and both reach the same
+    fine-grid energies within the convergence tolerance."""
+    from gp5_amd import capi, ground_state, synth
+
+    assert os.path.exists(S.DEFAULT_WEIGHTS)
+    n, L, N, _ = synth.droplet_batch(64, 3)
+    pick = [i for i in range(len(n)) if n[i] in (64, 96)][:12]
+    n, L, N = [n[i] for i in pick], [L[i] for i in pick], [N[i] for i in pick]
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    sur = S.Surrogate(batch=4)
+    runs = {}
+    for name, kw in (("gaussian", {}), ("network", {"seed_many": lambda items: sur.seed_many(
+            [(Nk, (0.0, 0.0, 0.0), 2.5, k, Lk) for _, k, Lk, Nk in items])})):
+        fb, rep = ground_state.two_level(n, L, N, f, tol=1e-10, tune=False, **kw)
+        fb.close()
+        runs[name] = rep
+        print(name, "coarse steps", int(np.abs(rep["coarse_steps"]).sum()), "fine steps",
+              int(np.abs(rep["steps"]).sum()), f"{rep['total_s']:.2f} s", flush=True)
+    g, w = runs["gaussian"], runs["network"]
+    assert np.all(g["steps"] > 0) and np.all(w["steps"] > 0)
+    assert np.abs(w["coarse_steps"]).sum() < np.abs(g["coarse_steps"]).sum()
+    assert np.all(np.abs(w["energy"] - g["energy"]) <= 5e-10 * np.abs(g["energy"]))
