@@ -2240,8 +2240,9 @@ int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt
             if (e) (void)hipEventDestroy(e);
     }
     const double wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    if (rc == GP5_ERR_NONFINITE) return fail(GP5_ERR_NONFINITE, "instance %d: non-finite energy or norm", err_inst);
-    RCHK(rc);
+    // the device may have retired and moved instances before any error: bring its
+    // bookkeeping back in every case, so that the host's slot map stays true
+    std::string err_msg = rc != GP5_OK ? std::string(gp5_last_error()) : std::string();
     std::vector<BatchClassDev> st(ncls);
     std::vector<long long> rs(b->count);
     std::vector<double> re(b->count);
@@ -2249,6 +2250,8 @@ int gp5_batch_ground_state(gp5_batch *b, const gp5_functional_desc *f, double dt
     HIPCHK(hipMemcpy(rs.data(), b->res_steps_dev, b->count * sizeof(long long), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(re.data(), b->res_energy_dev, b->count * sizeof(double), hipMemcpyDeviceToHost));
     for (int ki : live) RCHK(batch_pull_state(b, b->cls[ki], st[ki], rs, re));
+    if (rc == GP5_ERR_NONFINITE) return fail(GP5_ERR_NONFINITE, "instance %d: non-finite energy or norm", err_inst);
+    if (rc != GP5_OK) return fail(rc, "%s", err_msg.c_str());
     for (int i = 0; i < b->count; ++i) {
         if (steps_out) steps_out[i] = b->steps[i];
         if (energy_out) energy_out[i] = b->energy[i];

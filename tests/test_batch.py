@@ -205,3 +205,28 @@ def test_batch_ground_state_converges_and_matches_oracle(oracle_mod):
         a = O.advance(F, a, geo, stepper="RK4", dt=dt, imaginary=True, nsteps=1, renormalize=[N[i]])
         assert rel_l2(b.download(i)[..., None], a) < 1e-11, i
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_error_keeps_the_slot_map_true():
+    """A non-finite instance stops a ground-state run with GP5_ERR_NONFINITE naming it.
+    Instances of the other class may retire (and move slots on the device) in the replay
+    that runs while the host reads the error; afterwards every instance still downloads
+    its own field (each normalised to its own N)."""
+    n = [16, 16, 16, 24, 24]
+    L = [10.0, 10.0, 10.0, 12.0, 12.0]
+    N = [1.0, 2.0, 3.0, 4.0, 5.0]
+    b = capi.Batch(n, L, N)
+    for i in range(4):
+        b.upload(i, _gauss(n[i], L[i], 1.5 + 0.1 * i, 50 + i))
+    b.upload(4, np.full((24, 24, 24), np.nan, dtype=np.complex128))
+    f = capi.functional_desc(capi.GP5_FUNC_LHY)
+    with pytest.raises(capi.GP5Error) as e:
+        b.ground_state(f, 0.1, tol=5e-2, check_every=10, max_steps=2000)
+    assert e.value.code == capi.GP5_ERR_NONFINITE
+    assert "instance 4" in str(e.value)
+    for i in range(4):
+        a = b.download(i)
+        dV = (L[i] / n[i]) ** 3
+        assert np.sum(np.abs(a) ** 2) * dV == pytest.approx(N[i], rel=1e-10), i
+    b.close()
