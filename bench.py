@@ -94,8 +94,48 @@ def cpu_baseline(psi0, shape, left, right, budget_s: float = 25.0):
     }
 
 
+def cpu_ground_state_subset(n, L, N, psi0, gpu_steps, gpu_en, dt_factor, tol, check_every, budget_s: float = 90.0):
+    """A bounded CPU subset of the batched ground states: the 64^3 instance that the GPU
+    converged in the fewest steps, run cold by the CPU oracle (OpenMP, up to 16 cores)
+    with the same rule (energy at the start of the last step of every check_every
+    steps, converged when it moved by at most tol relative since the previous check).
+    Reports the CPU time and steps/s, and the GPU's steps and energy for the same
+    instance (they must agree: same step count, energy to 1e-10)."""
+    from oracle import oracle as O
+
+    O.build()
+    cores = min(_affinity(), 16)
+    O.set_threads(cores)
+    cand = [i for i in range(len(n)) if n[i] == 64 and gpu_steps[i] > 0]
+    if not cand:
+        return None
+    i = min(cand, key=lambda k: gpu_steps[k])
+    dx = L[i] / n[i]
+    geo = O.Geometry((n[i],) * 3, (-L[i] / 2,) * 3, (L[i] / 2,) * 3, 1)
+    F = O.Functional(kind=O.OR_LHY)
+    kw = dict(stepper="RK4", dt=dt_factor * dx * dx, imaginary=True, renormalize=[N[i]])
+    a = np.asfortranarray(psi0[i][..., None])
+    t0 = time.perf_counter()
+    steps, prev, E, converged = 0, None, None, False
+    while time.perf_counter() - t0 < budget_s:
+        a = O.advance(F, a, geo, nsteps=check_every - 1, **kw)
+        E = O.energy(F, a, geo)
+        a = O.advance(F, a, geo, nsteps=1, **kw)
+        steps += check_every
+        if prev is not None and abs(E - prev) <= tol * abs(E):
+            converged = True
+            break
+        prev = E
+    el = time.perf_counter() - t0
+    return {"sample": f"one 64^3 droplet of the batch (instance {i}, N = {N[i]:.1f}) to {tol:g} on the CPU oracle "
+                      f"with OpenMP over {cores} threads",
+            "cores": cores, "seconds": el, "steps": steps, "converged": converged,
+            "steps_per_s": steps / el, "energy": E, "gpu_steps": int(gpu_steps[i]), "gpu_energy": float(gpu_en[i]),
+            "energy_rel_diff_vs_gpu": abs(E - gpu_en[i]) / abs(gpu_en[i])}
+
+
 def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: float = 1e-10,
-                       check_every: int = 100, max_steps: int = 200000):
+                       check_every: int = 100, max_steps: int = 200000, cpu: bool = True):
     """Cold-started batched LHY droplet ground states: `count` instances with N log-uniform
     in [50, 5000] (seeded), grid per instance from {64, 96, 128, 192}^3 by the predicted
     radius (box = 6 radii at dx = 0.35), Gaussian seeds, imaginary-time RK4 with per-stage
@@ -116,6 +156,7 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
     steps, en, ms = b.ground_state(f, dt_factor, tol, check_every, max_steps)
     b.close()
     _progress(f"cold: {ms / 1e3:.1f} s; surrogate-seeded run")
+    cpu_gs = cpu_ground_state_subset(n, L, N, psi0, steps, en, dt_factor, tol, check_every) if cpu else None
     seeded = surrogate_seeded(n, L, N, f, dt_factor, tol, check_every, max_steps, ms, steps, en)
     _progress("two-level runs")
     two = two_level_legs(n, L, N, f, dt_factor, tol, check_every, max_steps, ms, steps, en)
@@ -139,6 +180,7 @@ def ground_state_batch(count: int, seed: int = 3, dt_factor: float = 0.1, tol: f
         "tune_seconds": tune_s,
         "tuning": report.strip().splitlines(),
         "start": "cold (Gaussian seeds)",
+        "cpu_subset": cpu_gs,
         "surrogate_seeded": seeded,
         "two_level": two,
     }
@@ -509,7 +551,7 @@ def main() -> int:
         _progress("config 1: 64^3 harmonic trap, Euler imaginary time")
         out["config1"] = config1_harmonic(cpu=not args.no_cpu)
     if args.gs_count > 0:
-        out["ground_state"] = ground_state_batch(args.gs_count)
+        out["ground_state"] = ground_state_batch(args.gs_count, cpu=not args.no_cpu)
     if args.large_steps > 0:
         out["large"] = collision_large(args.large_steps)
     if args.nn_batch > 0:

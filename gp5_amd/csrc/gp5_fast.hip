@@ -419,6 +419,13 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
     // stores), column passes GP5_NT_COLS
     constexpr bool NTL = (MODE >= 2) ? (GP5_NT_EPI & 1) != 0 : (!ROWS && GP5_NT_COLS != 0);
     constexpr bool NTS = (MODE >= 2) ? (GP5_NT_EPI & 2) != 0 : (!ROWS && GP5_NT_COLS != 0);
+    // the epilogue pass's forward x transform of the next stage input (T, read next by
+    // the y pass): GP5_NT_TOUT, else the epilogue's store hint
+#ifdef GP5_NT_TOUT
+    constexpr bool NTT = GP5_NT_TOUT != 0;
+#else
+    constexpr bool NTT = NTS;
+#endif
     {
         const double2 cf = make_double2(a.cfr, a.cfi);
 #pragma unroll
@@ -523,7 +530,7 @@ __device__ __forceinline__ void fast_body(const FastArgs &a)
 #pragma unroll
                 for (int i = 0; i < E / RF; ++i)
 #pragma unroll
-                    for (int k = 0; k < RF; ++k) stg<NTS>(a.t_out + gbase + (t + TL * i) + (N / RF) * k, v[i * RF + k]);
+                    for (int k = 0; k < RF; ++k) stg<NTT>(a.t_out + gbase + (t + TL * i) + (N / RF) * k, v[i * RF + k]);
             }
         }
         if (a.epi.want_red) {
