@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 #include <mutex>
@@ -1210,6 +1211,43 @@ __global__ void k_adam(float *p, const float *g, float *m, float *v, long long n
     }
 }
 
+// gp5_nn_adam_multi: blockIdx.y selects the tensor (a block-uniform index into
+// the argument table), the blocks along x stride over its elements.
+struct AdamMulti {
+    gp5_nn_adam_seg s[GP5_NN_ADAM_MAX_SEGS];
+    float lr, b1, b2, eps, c1, c2;
+};
+
+__global__ __launch_bounds__(256) void k_adam_multi(AdamMulti a)
+{
+    const gp5_nn_adam_seg &sg = a.s[blockIdx.y];
+    float *p = sg.p;
+    const float *g = sg.g;
+    float *m = sg.m, *v = sg.v;
+    uint16_t *wf = (uint16_t *)sg.wf, *wb = (uint16_t *)sg.wb;
+    const long long n = sg.n;
+    const int Cin = sg.cin, Cout = sg.cout;
+    const float lr = a.lr, b1 = a.b1, b2 = a.b2, eps = a.eps, c1 = a.c1, c2 = a.c2;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        float pi = p[i];
+        pi -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        if (wf || wb) {
+            const int co = (int)(i % Cout);
+            const int ci = (int)((i / Cout) % Cin);
+            const int tap = (int)(i / ((long long)Cin * Cout));
+            const uint16_t h = f2bf(pi);
+            if (wb) wb[i] = h;
+            if (wf) wf[((size_t)tap * Cout + co) * Cin + ci] = h;
+        }
+    }
+}
+
 __global__ void k_relayout(const float *W, uint16_t *Wf, uint16_t *Wb, int Cin, int Cout)
 {
     const long long n = 27LL * Cin * Cout;
@@ -1714,6 +1752,42 @@ int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
     hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, 0, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
     NNCHK(hipGetLastError(), "adam");
+    return GP5_OK;
+}
+
+int gp5_nn_adam_multi(const gp5_nn_adam_seg *segs, int nseg, float lr, float b1, float b2, float eps, int t)
+{
+    if (t < 1) return set_error(GP5_ERR_ARG, "adam step count starts at 1");
+    char buf[160];
+    if (!segs || nseg < 1 || nseg > GP5_NN_ADAM_MAX_SEGS) {
+        snprintf(buf, sizeof buf, "adam_multi takes 1 to %d tensors, got %d", GP5_NN_ADAM_MAX_SEGS, nseg);
+        return set_error(GP5_ERR_ARG, buf);
+    }
+    AdamMulti a;
+    memset(&a, 0, sizeof a);
+    long long most = 0;
+    for (int k = 0; k < nseg; ++k) {
+        const gp5_nn_adam_seg &sg = segs[k];
+        if (!sg.p || !sg.g || !sg.m || !sg.v || sg.n < 0) {
+            snprintf(buf, sizeof buf, "adam_multi: tensor %d is incomplete", k);
+            return set_error(GP5_ERR_ARG, buf);
+        }
+        if ((sg.wf || sg.wb) && (sg.cin < 1 || sg.cout < 1 || 27LL * sg.cin * sg.cout != sg.n)) {
+            snprintf(buf, sizeof buf, "adam_multi: tensor %d has %lld values, not 27 x %d x %d", k, sg.n, sg.cin, sg.cout);
+            return set_error(GP5_ERR_ARG, buf);
+        }
+        a.s[k] = sg;
+        most = std::max(most, sg.n);
+    }
+    a.lr = lr;
+    a.b1 = b1;
+    a.b2 = b2;
+    a.eps = eps;
+    a.c1 = 1.f - powf(b1, (float)t);
+    a.c2 = 1.f - powf(b2, (float)t);
+    if (most == 0) return GP5_OK;
+    hipLaunchKernelGGL(k_adam_multi, dim3(std::min(grid_for(most), 1024u), (unsigned)nseg), dim3(256), 0, 0, a);
+    NNCHK(hipGetLastError(), "adam_multi");
     return GP5_OK;
 }
 

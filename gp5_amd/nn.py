@@ -57,6 +57,7 @@ def _lib():
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
+        "gp5_nn_adam_multi": [P, I, F, F, F, F, I],
         "gp5_nn_droplet_features": [P, P, I, ctypes.c_double, F, P, P],
         "gp5_nn_predict_out": [P, I, P, P, LL],
     }
@@ -72,6 +73,13 @@ def _lib():
 
 def _ck(rc):
     capi.check(rc)
+
+
+class AdamSeg(ctypes.Structure):
+    """gp5_nn_adam_seg (include/gp5_nn.h)."""
+    _fields_ = [("p", ctypes.c_void_p), ("g", ctypes.c_void_p), ("m", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("wf", ctypes.c_void_p), ("wb", ctypes.c_void_p), ("n", ctypes.c_longlong), ("cin", ctypes.c_int),
+                ("cout", ctypes.c_int)]
 
 
 class DevBuf:
@@ -364,15 +372,24 @@ class UNet:
         self._wgrad("c1", t["X"], t["h16"], D)
         return float(loss.value)
 
+    def _adam_segs(self):
+        if getattr(self, "_segs", None) is None:
+            segs = []
+            for name, e in self.w.items():
+                for key, n in (("W", 27 * e["ci"] * e["co"]), ("b", e["co"])):
+                    w = key == "W"
+                    segs.append(AdamSeg(e[key].ptr, e[key + "g"].ptr, e[key + "m"].ptr, e[key + "v"].ptr,
+                                        e["Wf"].ptr if w else None, e["Wb"].ptr if w else None, n,
+                                        e["ci"] if w else 0, e["co"] if w else 0))
+            self._segs = (AdamSeg * len(segs))(*segs)
+        return self._segs
+
     def step(self, lr: float, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8) -> None:
-        """Adam update of the float masters, then fresh bf16 copies."""
-        L = _lib()
+        """Adam update of the float masters and fresh bf16 copies, every tensor in one
+        launch (gp5_nn_adam_multi: the arithmetic of gp5_nn_adam and gp5_nn_relayout)."""
         self.step_count += 1
-        for name, e in self.w.items():
-            for key, n in (("W", 27 * e["ci"] * e["co"]), ("b", e["co"])):
-                _ck(L.gp5_nn_adam(e[key].ptr, e[key + "g"].ptr, e[key + "m"].ptr, e[key + "v"].ptr, n, lr, b1, b2,
-                                  eps, self.step_count))
-        self._relayout()
+        segs = self._adam_segs()
+        _ck(_lib().gp5_nn_adam_multi(ctypes.cast(segs, ctypes.c_void_p), len(segs), lr, b1, b2, eps, self.step_count))
 
     def sync(self):
         _ck(_lib().gp5_nn_sync())

@@ -95,6 +95,23 @@ int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float
                 int t);
 /* bf16 copies of W[27][Cin][Cout]: Wf [27][Cout][Cin] (forward), Wb = same layout as W. */
 int gp5_nn_relayout(const float *W, void *Wf, void *Wb, int Cin, int Cout);
+/* One tensor of gp5_nn_adam_multi: p, g, m, v as in gp5_nn_adam; when wf or wb is
+ * non-null the tensor is W[27][cin][cout] and its updated values also go to the
+ * bf16 copies of gp5_nn_relayout (either may be null). */
+typedef struct gp5_nn_adam_seg {
+    float *p;
+    const float *g;
+    float *m, *v;
+    void *wf, *wb;
+    long long n;
+    int cin, cout;
+} gp5_nn_adam_seg;
+#define GP5_NN_ADAM_MAX_SEGS 16
+/* Adam step on up to GP5_NN_ADAM_MAX_SEGS tensors in one launch, with the bf16
+ * copies of the weight tensors refreshed in the same pass: element by element
+ * the arithmetic of gp5_nn_adam followed by gp5_nn_relayout, so the results
+ * are identical to those calls (which take two launches per tensor). */
+int gp5_nn_adam_multi(const gp5_nn_adam_seg *segs, int nseg, float lr, float b1, float b2, float eps, int t);
 
 #ifdef __cplusplus
 }

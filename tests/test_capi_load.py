@@ -133,3 +133,13 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     # mse_bf16: C a multiple of 8, H required
     assert L.gp5_nn_mse_bf16(fake, 12, fake, fake, None, fake, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_bf16(fake, 16, fake, fake, None, None, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
+    # adam_multi: 1..16 tensors, complete ones, weight copies only of 27 x cin x cout tensors
+    seg = nn.AdamSeg(fake, fake, fake, fake, None, None, 10, 0, 0)
+    assert L.gp5_nn_adam_multi(ctypes.byref(seg), 0, 1e-3, 0.9, 0.999, 1e-8, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_adam_multi(ctypes.byref(seg), 17, 1e-3, 0.9, 0.999, 1e-8, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_adam_multi(ctypes.byref(seg), 1, 1e-3, 0.9, 0.999, 1e-8, 0) == capi.GP5_ERR_ARG
+    bad = nn.AdamSeg(fake, fake, fake, fake, fake, None, 27 * 16 * 16 + 1, 16, 16)
+    assert L.gp5_nn_adam_multi(ctypes.byref(bad), 1, 1e-3, 0.9, 0.999, 1e-8, 1) == capi.GP5_ERR_ARG
+    assert b"27 x 16 x 16" in L.gp5_last_error()
+    partial = nn.AdamSeg(fake, None, fake, fake, None, None, 10, 0, 0)
+    assert L.gp5_nn_adam_multi(ctypes.byref(partial), 1, 1e-3, 0.9, 0.999, 1e-8, 1) == capi.GP5_ERR_ARG

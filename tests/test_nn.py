@@ -4,6 +4,8 @@ oracle/nn_ref.py: convolution forward, data and weight gradients on bf16 inputs
 Adam, and the whole network's loss and gradients (bf16 activations: agreement to
 the bf16 rounding level)."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -96,6 +98,51 @@ def test_pool_up_act_adam():
         v = 0.999 * v + 0.001 * g * g
         p = p - 1e-2 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
     assert np.allclose(pb.download(np.float32, p.shape), p, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_adam_multi_equals_per_tensor_adam_and_relayout():
+    """One launch over weight tensors (with both bf16 copies, one copy or none) and
+    a bias equals gp5_nn_adam + gp5_nn_relayout per tensor, bit for bit, over three
+    steps."""
+    rng = np.random.default_rng(13)
+    L = nn._lib()
+    shapes = [(16, 32, "both"), (32, 16, "wf"), (16, 16, "none"), (None, 48, "bias")]
+    ref, got = [], []
+    for ci, co, kind in shapes:
+        n = 27 * ci * co if ci else co
+        p = rng.standard_normal(n).astype(np.float32)
+        for dst in (ref, got):
+            t = {"p": _buf(p), "m": nn.DevBuf(4 * n), "v": nn.DevBuf(4 * n), "g": nn.DevBuf(4 * n), "n": n,
+                 "ci": ci or 0, "co": co, "kind": kind,
+                 "wf": nn.DevBuf(2 * n) if kind in ("both", "wf") else None,
+                 "wb": nn.DevBuf(2 * n) if kind == "both" else None}
+            t["m"].zero()
+            t["v"].zero()
+            for k in ("wf", "wb"):
+                if t[k] is not None:
+                    t[k].zero()
+            dst.append(t)
+    for step in (1, 2, 3):
+        for a, b in zip(ref, got):
+            g = rng.standard_normal(a["n"]).astype(np.float32)
+            a["g"].upload(g)
+            b["g"].upload(g)
+        for a in ref:
+            nn._ck(L.gp5_nn_adam(a["p"].ptr, a["g"].ptr, a["m"].ptr, a["v"].ptr, a["n"], 3e-3, 0.9, 0.999, 1e-8, step))
+            if a["kind"] != "bias" and a["kind"] != "none":
+                nn._ck(L.gp5_nn_relayout(a["p"].ptr, a["wf"].ptr if a["wf"] else None, a["wb"].ptr if a["wb"] else None,
+                                         a["ci"], a["co"]))
+        segs = (nn.AdamSeg * len(got))(*[
+            nn.AdamSeg(b["p"].ptr, b["g"].ptr, b["m"].ptr, b["v"].ptr, b["wf"].ptr if b["wf"] else None,
+                       b["wb"].ptr if b["wb"] else None, b["n"], b["ci"], b["co"]) for b in got])
+        nn._ck(L.gp5_nn_adam_multi(ctypes.cast(segs, ctypes.c_void_p), len(got), 3e-3, 0.9, 0.999, 1e-8, step))
+    for a, b in zip(ref, got):
+        for k in ("p", "m", "v"):
+            np.testing.assert_array_equal(a[k].download(np.uint32, (a["n"],)), b[k].download(np.uint32, (a["n"],)))
+        for k in ("wf", "wb"):
+            if a[k] is not None:
+                np.testing.assert_array_equal(a[k].download(np.uint16, (a["n"],)), b[k].download(np.uint16, (a["n"],)))
 
 
 def _cos(a, b):
