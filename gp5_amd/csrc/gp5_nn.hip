@@ -849,6 +849,196 @@ __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, 
     }
 }
 
+// Weight gradient straight from the channel-last X and dY, for cube sides D =
+// 16 .. 128 (a power of two), on v_mfma_f32_16x16x32_bf16 (M = 16 input
+// channels, N = 16 output channels, K = 32 positions).  A workgroup of 3 waves
+// owns one (16 ci, 16 co) tile pair and Rb = 256 / D rows of ZL consecutive
+// planes of one sample; wave w takes the 9 taps of dz = w - 1 and keeps their
+// accumulators in registers for the whole walk.  LDS holds a ring of three X
+// slabs (planes z-1, z, z+1; Rb + 2 rows with zeros outside the cube) and the
+// dY plane z, both channel-major, so that a lane's 8 consecutive positions of
+// one channel are one aligned 16-byte read.  The slabs are transposed on their
+// way in: a thread loads 8 channels of two x-neighbours (two 16-byte loads) and
+// writes 8 dwords, one per channel.  Row layout: x at element x + 8 of a row of
+// D + 8 elements; elements 0 .. 7 stay zero, so x = -1 reads element 7 and
+// x = D reads the next row's element 0 (or the zero padding after the last
+// row).  The x-shifted taps come from the aligned read and the dword on either
+// side by funnel shifts.  The global loads of plane z+2 (X) and z+1 (dY) are in
+// flight while the waves run plane z.  Per-workgroup sums go to
+// ws[blk][tap][ci][co] for the chunk reduction.  With up = 1, X has side D/2 and
+// position (z, y, x) reads its parent cell (the weight gradient of the
+// convolution of the nearest upsampling).
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
+
+// channel stride of a staged slab (elements): at least `elems`, a multiple of 8,
+// and (in dwords) 4 mod 32, so that the 16 channels of one read spread over the banks
+__host__ __device__ constexpr int wz_chstride(int elems)
+{
+    int dw = elems / 2 + 1;  // at least two elements of padding
+    dw += ((4 - dw) % 32 + 32) % 32;
+    return 2 * dw;
+}
+constexpr int kWzThreads = 192;
+
+__global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                         float *ws, int P, int D, int Cin, int Cout, int ZL, int up)
+{
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int Rb = 256 / D, XS = D + 8;
+    const int CSX = wz_chstride((Rb + 2) * XS), CSY = wz_chstride(256);
+    const int SLOT = 16 * CSX;  // elements per X slab
+    uint16_t *LX = reinterpret_cast<uint16_t *>(sx);
+    uint16_t *LY = LX + 3 * SLOT;
+    const int nco = Cout / 16;
+    const int cib = (int)blockIdx.y / nco, cob = (int)blockIdx.y % nco;
+    const int nyb = D / Rb, nzs = D / ZL;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * ZL;
+    const int Ds = D >> up;
+    const int sb = smp * Ds * Ds * Ds;  // stored position of this sample's origin (X)
+    const int s0 = smp * D * D * D;     // position of this sample's origin (dY)
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> (3 * up)) * Cin * 2);
+    const __amdgpu_buffer_rsrc_t yr = make_rsrc(dY, P * Cout * 2);
+    // zero the whole staging area once: the row pads are never written again
+    {
+        const int n16 = (3 * SLOT + 16 * CSY) / 8;
+        for (int i = tid; i < n16; i += kWzThreads) reinterpret_cast<uint4 *>(sx)[i] = make_uint4(0, 0, 0, 0);
+    }
+    // staging items: X slab (Rb + 2) rows x D/2 x-pairs x 2 channel halves; dY plane Rb rows x D/2 x 2
+    const int NXI = (Rb + 2) * D, NYI = Rb * D;  // items (both are at most 2.67 per thread)
+    constexpr int MXI = 3, MYI = 2;
+    uint4 xv[MXI][2], yv[MYI][2];
+    auto load_x = [&](int zz) {
+#pragma unroll
+        for (int k = 0; k < MXI; ++k) {
+            const int it = tid + kWzThreads * k;
+            const int pr = it % (D / 2), h = (it / (D / 2)) & 1, row = it / D;
+            const int x = 2 * pr, y = y0 - 1 + row;
+            const bool in = it < NXI && zz >= 0 && zz < D && y >= 0 && y < D;
+            const int c8 = cib * 16 + 8 * h;
+            const int o0 = in ? ((sb + ((zz >> up) * Ds + (y >> up)) * Ds + (x >> up)) * Cin + c8) * 2 : kOffNone;
+            const int o1 = in ? ((sb + ((zz >> up) * Ds + (y >> up)) * Ds + ((x + 1) >> up)) * Cin + c8) * 2 : kOffNone;
+            xv[k][0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, o0, 0, 0));
+            xv[k][1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, o1, 0, 0));
+        }
+    };
+    auto load_y = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < MYI; ++k) {
+            const int it = tid + kWzThreads * k;
+            const int pr = it % (D / 2), h = (it / (D / 2)) & 1, row = it / D;
+            const int x = 2 * pr;
+            const bool in = it < NYI && z < D;
+            const int c8 = cob * 16 + 8 * h;
+            const int o0 = in ? ((s0 + (z * D + y0 + row) * D + x) * Cout + c8) * 2 : kOffNone;
+            yv[k][0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yr, o0, 0, 0));
+            yv[k][1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yr, in ? o0 + Cout * 2 : kOffNone, 0, 0));
+        }
+    };
+    // dword of channel c (0..7 of the half) for the x-pair: low half from position x, high from x + 1
+    auto pair_dw = [](const uint4 &a, const uint4 &b, int c) -> uint32_t {
+        const uint32_t wa = c < 2 ? a.x : (c < 4 ? a.y : (c < 6 ? a.z : a.w));
+        const uint32_t wb = c < 2 ? b.x : (c < 4 ? b.y : (c < 6 ? b.z : b.w));
+        // byte selector of v_perm_b32(wb, wa, sel): bytes 0-3 from wa, 4-7 from wb
+        return (c & 1) ? __builtin_amdgcn_perm(wb, wa, 0x07060302u) : __builtin_amdgcn_perm(wb, wa, 0x05040100u);
+    };
+    auto store_x = [&](int slot) {
+        uint16_t *base = LX + slot * SLOT;
+#pragma unroll
+        for (int k = 0; k < MXI; ++k) {
+            const int it = tid + kWzThreads * k;
+            if (it >= NXI) continue;
+            const int pr = it % (D / 2), h = (it / (D / 2)) & 1, row = it / D;
+            uint16_t *p = base + (8 * h) * CSX + row * XS + 2 * pr + 8;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) *reinterpret_cast<uint32_t *>(p + c * CSX) = pair_dw(xv[k][0], xv[k][1], c);
+        }
+    };
+    auto store_y = [&]() {
+#pragma unroll
+        for (int k = 0; k < MYI; ++k) {
+            const int it = tid + kWzThreads * k;
+            if (it >= NYI) continue;
+            const int pr = it % (D / 2), h = (it / (D / 2)) & 1, row = it / D;
+            uint16_t *p = LY + (8 * h) * CSY + row * D + 2 * pr;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) *reinterpret_cast<uint32_t *>(p + c * CSY) = pair_dw(yv[k][0], yv[k][1], c);
+        }
+    };
+    __syncthreads();  // the zero fill is complete
+    load_x(zb - 1);
+    store_x(0);
+    load_x(zb);
+    store_x(1);
+    load_x(zb + 1);
+    store_x(2);
+    load_y(zb);
+    store_y();
+    __syncthreads();
+    const int dz = wave - 1;
+    f32x4 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;
+        if (zi + 1 < ZL) {  // in flight during this plane's MFMAs
+            load_x(z + 2);
+            load_y(z + 1);
+        }
+        const int slot = (zi + 1 + dz) % 3;  // ring slot of plane z + dz
+        const uint16_t *xs = LX + slot * SLOT + r * CSX;
+        const uint16_t *ys = LY + r * CSY;
+#pragma unroll 2
+        for (int ck = 0; ck < 8; ++ck) {
+            const int q = 32 * ck + 8 * g;  // plane-local position of this lane group's 8 positions
+            const int row = q / D, x = q - row * D;
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(__builtin_assume_aligned(ys + row * D + x, 16));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {  // dy = k - 1
+                const uint16_t *xp = xs + (row + k) * XS + x + 8;
+                const uint4 c = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(xp, 16));
+                const uint32_t pv = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(xp - 2, 4));
+                const uint32_t nx = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(xp + 8, 4));
+                const uint32_t a1 = __builtin_amdgcn_alignbit(c.x, pv, 16), a2 = __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                               a3 = __builtin_amdgcn_alignbit(c.z, c.y, 16), a4 = __builtin_amdgcn_alignbit(c.w, c.z, 16),
+                               a5 = __builtin_amdgcn_alignbit(nx, c.w, 16);
+                const bf16x8m bm = __builtin_bit_cast(bf16x8m, b);
+                acc[3 * k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8m, make_uint4(a1, a2, a3, a4)), bm, acc[3 * k], 0, 0, 0);
+                acc[3 * k + 1] =
+                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8m, c), bm, acc[3 * k + 1], 0, 0, 0);
+                acc[3 * k + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    __builtin_bit_cast(bf16x8m, make_uint4(a2, a3, a4, a5)), bm, acc[3 * k + 2], 0, 0, 0);
+            }
+        }
+        if (zi + 1 < ZL) {
+            __syncthreads();  // every wave is done with the slab of plane z-1 and the dY plane z
+            store_x(zi % 3);  // plane z + 2 takes the slot of plane z - 1
+            store_y();
+            __syncthreads();
+        }
+    }
+    // D[m = ci 4g+i][n = co r]
+    const int n = 27 * Cin * Cout;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int tap = (dz + 1) * 9 + t;
+        float *o = ws + (size_t)blockIdx.x * n + ((size_t)tap * Cin + cib * 16) * Cout + cob * 16 + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) o[(size_t)(4 * g + i) * Cout] = acc[t][i];
+    }
+}
+
+// LDS bytes of k_wgrad_zs for cube side D
+__host__ inline size_t wgrad_zs_lds(int D)
+{
+    const int Rb = 256 / D;
+    return (size_t)(3 * 16 * wz_chstride((Rb + 2) * (D + 8)) + 16 * wz_chstride(256)) * 2;
+}
+
 // Sum of the chunk partials in two fixed-order levels: kRedGroups groups of
 // consecutive chunks per output first (one thread each), then the group sums.
 constexpr int kRedGroups = 32;
@@ -1477,13 +1667,38 @@ int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A,
     return conv3d_run(Xh, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 1);
 }
 
+// Plan of k_wgrad_zs for a cube side D (a power of two in [16, 128]): the longest
+// plane run ZL (a power of two, at least 2) that keeps 1024 workgroups; returns
+// the number of position blocks (the chunks of the reduction), 0 when the
+// kernel does not apply (or GP5_WGRAD_ZS=0 turns it off).
+static long long wgrad_zs_plan(int B, int D, int Cin, int Cout, int *ZL)
+{
+    const char *env = getenv("GP5_WGRAD_ZS");
+    if (env && env[0] == '0') return 0;
+    if (D < 16 || D > 128 || (D & (D - 1)) != 0) return 0;
+    const long long per_z = (long long)B * (D / (256 / D)), tiles = (long long)(Cin / 16) * (Cout / 16);
+    int zl = 2;
+    for (int c = D; c >= 2; c /= 2)
+        if (per_z * (D / c) * tiles >= 1024) {
+            zl = c;
+            break;
+        }
+    *ZL = zl;
+    return per_z * (D / zl);
+}
+
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
 {
     const long long P = (long long)B * D * D * D;
     const long long chunk = wgrad_chunk(P, Cin, Cout);
     const long long chunks = (P + chunk - 1) / chunk;
-    // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums
-    return (size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * 27 * Cin * Cout * sizeof(float);
+    int zl = 0;
+    const long long zchunks = wgrad_zs_plan(B, D, Cin, Cout, &zl);
+    // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums;
+    // or the partial sums of the position blocks of k_wgrad_zs
+    const size_t n = (size_t)27 * Cin * Cout * sizeof(float);
+    return std::max((size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * n,
+                    (size_t)(zchunks + kRedGroups) * n);
 }
 
 // up = 1: X has side D/2 and the gradient is that of a convolution of its
@@ -1498,6 +1713,28 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
     if (rc == GP5_OK) rc = check_channels(Cout, "Cout");
     if (rc != GP5_OK) return rc;
     const long long P = (long long)B * D * D * D;
+    const long long n = 27LL * Cin * Cout;
+    int ZL = 0;
+    const long long zblocks = wgrad_zs_plan(B, D, Cin, Cout, &ZL);
+    if (zblocks > 0) {
+        const size_t shm = wgrad_zs_lds(D);
+        static size_t shm_set = 65536;  // the default limit of dynamic LDS
+        if (shm > shm_set) {
+            NNCHK(hipFuncSetAttribute((const void *)k_wgrad_zs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                  "wgrad LDS size");
+            shm_set = shm;
+        }
+        hipLaunchKernelGGL(k_wgrad_zs, dim3((unsigned)zblocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWzThreads),
+                           shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, ZL, up);
+        NNCHK(hipGetLastError(), "wgrad launch");
+        float *part = ws + (size_t)zblocks * n;
+        hipLaunchKernelGGL(k_wgrad_reduce_groups, dim3((unsigned)((n + 255) / 256), kRedGroups), dim3(256), 0, 0, ws,
+                           part, n, (int)zblocks);
+        NNCHK(hipGetLastError(), "wgrad reduce launch");
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, part, dW, n);
+        NNCHK(hipGetLastError(), "wgrad reduce launch");
+        return GP5_OK;
+    }
     const long long chunk = wgrad_chunk(P, Cin, Cout);
     const long long chunks = (P + chunk - 1) / chunk;
     uint16_t *XT = reinterpret_cast<uint16_t *>(ws);
@@ -1528,7 +1765,6 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
                            Cin, Cout, chunk);
     }
     NNCHK(hipGetLastError(), "wgrad launch");
-    const long long n = 27LL * Cin * Cout;
     float *part = part0 + (size_t)chunks * n;  // after the chunk partials
     hipLaunchKernelGGL(k_wgrad_reduce_groups, dim3((unsigned)((n + 255) / 256), kRedGroups), dim3(256), 0, 0, part0,
                        part, n, (int)chunks);

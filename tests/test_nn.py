@@ -282,6 +282,32 @@ def test_conv_z_sliding_path(D, Cin, Cout, B, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 16, 2), (64, 32, 16, 1), (32, 64, 32, 1), (128, 16, 16, 1),
+                                          (16, 128, 64, 2), (16, 16, 32, 1)])
+def test_wgrad_z_sliding_path(D, Cin, Cout, B, monkeypatch):
+    """Cube sides 16..128 take the weight gradient that stages channel-last X and dY
+    into LDS while it walks plane runs (16x16x32 MFMAs).  It agrees with the oracle
+    and with the kernel on the blocked channel-major copies (GP5_WGRAD_ZS=0), which
+    sums in another order (float accumulation: agreement to rounding)."""
+    rng = np.random.default_rng(D * 3 + Cin + 5 * Cout + B)
+    X = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    L = nn._lib()
+    xb, gb = _buf(X, True), _buf(G, True)
+    grads = []
+    for zs in ("1", "0"):
+        monkeypatch.setenv("GP5_WGRAD_ZS", zs)
+        ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+        dW = nn.DevBuf(27 * Cin * Cout * 4)
+        nn._ck(L.gp5_nn_conv3d_wgrad(xb.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+        grads.append(dW.download(np.float32, (27, Cin, Cout)))
+    ref = R.conv_wgrad(X, G)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(grads[0] - ref)) <= 1e-4 * scale
+    assert np.max(np.abs(grads[0] - grads[1])) <= 1e-5 * scale
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B", [(64, 32, 16, 1), (32, 64, 32, 1), (16, 128, 64, 2), (8, 64, 32, 2)])
 def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B):
     """gp5_nn_conv3d_up / gp5_nn_conv3d_wgrad_up on a cube of side D/2 give bit for
