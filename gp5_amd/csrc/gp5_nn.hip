@@ -862,8 +862,11 @@ __global__ __launch_bounds__(64) void k_wgrad8(const uint16_t *__restrict__ XT, 
 // writes 8 dwords, one per channel.  Row layout: x at element x + 8 of a row of
 // D + 8 elements; elements 0 .. 7 stay zero, so x = -1 reads element 7 and
 // x = D reads the next row's element 0 (or the zero padding after the last
-// row).  The x-shifted taps come from the aligned read and the dword on either
-// side by funnel shifts.  The global loads of plane z+2 (X) and z+1 (dY) are in
+// row).  The x shift of a tap moves to dY: sum_p X[p + dx] dY[p] =
+// sum_q X[q] dY[q - dx] over the cube (dY is zero outside it), so each 32-
+// position chunk builds its three x-shifted dY fragments once, from the aligned
+// read and the dword on either side by funnel shifts, and every X fragment is
+// one aligned read.  The global loads of plane z+2 (X) and z+1 (dY) are in
 // flight while the waves run plane z.  Per-workgroup sums go to
 // ws[blk][tap][ci][co] for the chunk reduction.  With up = 1, X has side D/2 and
 // position (z, y, x) reads its parent cell (the weight gradient of the
@@ -888,7 +891,7 @@ __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restr
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
     const int Rb = 256 / D, XS = D + 8;
-    const int CSX = wz_chstride((Rb + 2) * XS), CSY = wz_chstride(256);
+    const int CSX = wz_chstride((Rb + 2) * XS), CSY = wz_chstride(Rb * XS);
     const int SLOT = 16 * CSX;  // elements per X slab
     uint16_t *LX = reinterpret_cast<uint16_t *>(sx);
     uint16_t *LY = LX + 3 * SLOT;
@@ -963,7 +966,7 @@ __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restr
             const int it = tid + kWzThreads * k;
             if (it >= NYI) continue;
             const int pr = it % (D / 2), h = (it / (D / 2)) & 1, row = it / D;
-            uint16_t *p = LY + (8 * h) * CSY + row * D + 2 * pr;
+            uint16_t *p = LY + (8 * h) * CSY + row * XS + 2 * pr + 8;
 #pragma unroll
             for (int c = 0; c < 8; ++c) *reinterpret_cast<uint32_t *>(p + c * CSY) = pair_dw(yv[k][0], yv[k][1], c);
         }
@@ -995,23 +998,24 @@ __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restr
         for (int ck = 0; ck < 8; ++ck) {
             const int q = 32 * ck + 8 * g;  // plane-local position of this lane group's 8 positions
             const int row = q / D, x = q - row * D;
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(__builtin_assume_aligned(ys + row * D + x, 16));
+            // dY[q - dx] for dx = -1, 0, +1: positions q+1 .. q+8, q .. q+7, q-1 .. q+6
+            const uint16_t *yp = ys + row * XS + x + 8;
+            const uint4 c = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(yp, 16));
+            const uint32_t pv = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(yp - 2, 4));
+            const uint32_t nx = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(yp + 8, 4));
+            const uint32_t a1 = __builtin_amdgcn_alignbit(c.x, pv, 16), a2 = __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                           a3 = __builtin_amdgcn_alignbit(c.z, c.y, 16), a4 = __builtin_amdgcn_alignbit(c.w, c.z, 16),
+                           a5 = __builtin_amdgcn_alignbit(nx, c.w, 16);
+            const bf16x8m bm = __builtin_bit_cast(bf16x8m, make_uint4(a2, a3, a4, a5));
+            const bf16x8m b0 = __builtin_bit_cast(bf16x8m, c);
+            const bf16x8m bp = __builtin_bit_cast(bf16x8m, make_uint4(a1, a2, a3, a4));
 #pragma unroll
             for (int k = 0; k < 3; ++k) {  // dy = k - 1
-                const uint16_t *xp = xs + (row + k) * XS + x + 8;
-                const uint4 c = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(xp, 16));
-                const uint32_t pv = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(xp - 2, 4));
-                const uint32_t nx = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(xp + 8, 4));
-                const uint32_t a1 = __builtin_amdgcn_alignbit(c.x, pv, 16), a2 = __builtin_amdgcn_alignbit(c.y, c.x, 16),
-                               a3 = __builtin_amdgcn_alignbit(c.z, c.y, 16), a4 = __builtin_amdgcn_alignbit(c.w, c.z, 16),
-                               a5 = __builtin_amdgcn_alignbit(nx, c.w, 16);
-                const bf16x8m bm = __builtin_bit_cast(bf16x8m, b);
-                acc[3 * k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(bf16x8m, make_uint4(a1, a2, a3, a4)), bm, acc[3 * k], 0, 0, 0);
-                acc[3 * k + 1] =
-                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8m, c), bm, acc[3 * k + 1], 0, 0, 0);
-                acc[3 * k + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    __builtin_bit_cast(bf16x8m, make_uint4(a2, a3, a4, a5)), bm, acc[3 * k + 2], 0, 0, 0);
+                const bf16x8m a =
+                    *reinterpret_cast<const bf16x8m *>(__builtin_assume_aligned(xs + (row + k) * XS + x + 8, 16));
+                acc[3 * k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bm, acc[3 * k], 0, 0, 0);
+                acc[3 * k + 1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[3 * k + 1], 0, 0, 0);
+                acc[3 * k + 2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bp, acc[3 * k + 2], 0, 0, 0);
             }
         }
         if (zi + 1 < ZL) {
@@ -1036,7 +1040,7 @@ __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restr
 __host__ inline size_t wgrad_zs_lds(int D)
 {
     const int Rb = 256 / D;
-    return (size_t)(3 * 16 * wz_chstride((Rb + 2) * (D + 8)) + 16 * wz_chstride(256)) * 2;
+    return (size_t)(3 * 16 * wz_chstride((Rb + 2) * (D + 8)) + 16 * wz_chstride(Rb * (D + 8))) * 2;
 }
 
 // Sum of the chunk partials in two fixed-order levels: kRedGroups groups of
