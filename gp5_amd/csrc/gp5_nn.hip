@@ -125,7 +125,11 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
 constexpr int kEpiStride = 20;                       // floats per scratch row (16 channels, conflict-free)
 constexpr int kEpiScratch = 32 * kEpiStride * 4;     // bytes per wave
 
-template <int NB, int MT>
+// With WS the scratch is wave-private and the exchange needs no workgroup
+// barrier: a wave's LDS writes and reads execute in issue order, and the wave
+// barrier keeps the compiler from moving them across each other.  The caller
+// then owns the workgroup barrier before the scratch is reused by other waves.
+template <int NB, int MT, bool WS = false>
 __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
                                                   const float *__restrict__ bias, int act, uint16_t *A,
                                                   const uint16_t *skip, uint16_t *Y, float *Yf, unsigned char *scr)
@@ -148,7 +152,8 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
                     if (act == 1) v = v > 0.f ? v : kLeak * v;
                     T[(16 * tt + 4 * g + i) * kEpiStride + r] = v;
                 }
-            __syncthreads();
+            if constexpr (WS) __builtin_amdgcn_wave_barrier();
+            else __syncthreads();
             const int pb = base + 32 * h;  // first position of the pair
             if (Yf) {
 #pragma unroll
@@ -184,7 +189,8 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
                     }
                 }
             }
-            __syncthreads();  // the scratch is rewritten by the next pair
+            if constexpr (WS) __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next pair
+            else __syncthreads();
         }
     }
 }
@@ -319,10 +325,13 @@ __device__ __forceinline__ int swz_frag(int g, int ix)
 // which each lane's second read hits the banks of the next lane's first (2-way
 // conflicts, measured 40 % of the kernel time); single reads of 16 consecutive
 // positions cover 512 or 1024 consecutive bytes without conflicts.
-template <int LV>
+// VOL = false uses a non-volatile asm: the offset stays opaque (no pairing) but
+// the read may be scheduled freely, e.g. one tap ahead of its MFMAs.
+template <int LV, bool VOL = true>
 __device__ __forceinline__ Frag<LV> lds_frag(const unsigned char *sx, int off)
 {
-    asm volatile("" : "+v"(off));
+    if constexpr (VOL) asm volatile("" : "+v"(off));
+    else asm("" : "+v"(off));
     if constexpr (LV == 4) {
         return *reinterpret_cast<const Frag<LV> *>(sx + off);
     } else {  // 16-byte pieces, 16-byte aligned (off is a multiple of 2 LV)
@@ -466,8 +475,11 @@ __host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 1
 // wave's NB output blocks stay in registers for the whole walk.  Same K
 // permutation and accumulation order (tap, step) as k_conv with KC = CIN, so
 // the two agree bit for bit.  With up = 1, X has side D/2 and each slab cell is
-// read from its parent cell (the nearest upsampling of X).
-template <int CIN, int NB>
+// read from its parent cell (the nearest upsampling of X).  PF reads the
+// fragments of tap + 1 while the MFMAs of tap run (scheduling barriers keep
+// the compiler from sinking the reads to their uses); WS gives the epilogue
+// wave-private barriers.  Neither changes any accumulation order.
+template <int CIN, int NB, bool PF = false, bool WS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                  uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
@@ -555,13 +567,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             for (int j = 0; j < NB; ++j) acc[t][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         // slots of planes z-1, z, z+1
         const int sl[3] = {zi % 3, (zi + 1) % 3, (zi + 2) % 3};
-#pragma unroll
-        for (int tap = 0; tap < 27; ++tap) {
+        auto load_a = [&](int tap, Frag<LV> (&dst)[MT]) {
             const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
             const int toff = sl[dz + 1] * SU * 16 + dy * RX * CIN * 2;
-            Frag<LV> a[MT];
 #pragma unroll
-            for (int t = 0; t < MT; ++t) a[t] = lds_frag<LV>(sx, lo[t][dx + 1] + toff);
+            for (int t = 0; t < MT; ++t) dst[t] = lds_frag<LV, !PF>(sx, lo[t][dx + 1] + toff);
+        };
+        auto mfmas = [&](int tap, const Frag<LV> (&a)[MT]) {
 #pragma unroll
             for (int st = 0; st < LV / 4; ++st)
 #pragma unroll
@@ -570,11 +582,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     for (int t = 0; t < MT; ++t)
                         acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t].v[st], w[tap][j].v[st], acc[t][j],
                                                                               0, 0, 0);
+        };
+        if constexpr (PF) {
+            // the fragments of tap + 1 are read while the MFMAs of tap run
+            Frag<LV> a[2][MT];
+            load_a(0, a[0]);
+#pragma unroll
+            for (int tap = 0; tap < 27; ++tap) {
+                if (tap + 1 < 27) load_a(tap + 1, a[(tap + 1) & 1]);
+                // keeps the scheduler from pulling the reads of tap + 1 down to their MFMAs
+                __builtin_amdgcn_sched_barrier(0);
+                mfmas(tap, a[tap & 1]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+#pragma unroll
+            for (int tap = 0; tap < 27; ++tap) {
+                Frag<LV> a[MT];
+                load_a(tap, a);
+                mfmas(tap, a);
+            }
         }
         __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
-        conv_epilogue_lds<NB, MT>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf,
-                                  sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
+        conv_epilogue_lds<NB, MT, WS>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf,
+                                      sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
         if (zi + 1 < ZL) {
+            if constexpr (WS) __syncthreads();  // every wave is done with its scratch in the slot
             store_slab(sl[0]);
             __syncthreads();
         }
@@ -1093,9 +1126,12 @@ __global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ 
 // Backward of the leaky ReLU with the bias gradient folded in: dP = dA act'(A)
 // goes out as its bf16 copy H, and each of the kBiasBlocks blocks sums dP over a
 // contiguous range of positions per channel (threads take 4 channels of one
-// position; C / 4 divides 256), for k_bias_final.
+// position; C / 4 divides 256), for k_bias_final.  With G set, the incoming
+// gradient is dA + scale * G at the parent cell (G has side D/2: the adjoint of
+// pooling added on the fly, the same fused multiply-add as k_up2_add4).
 __global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__restrict__ A, const float *__restrict__ dA,
-                                                           uint16_t *__restrict__ H, float *part, long long P, int C)
+                                                           uint16_t *__restrict__ H, float *part, long long P, int C,
+                                                           const float *__restrict__ G, float scale, int D)
 {
     __shared__ float4 s[256];
     const int C4 = C / 4, per_pos = 256 / C4;
@@ -1106,7 +1142,14 @@ __global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__res
     for (long long p = p0 + sub; p < p1; p += per_pos) {
         const long long i = p * C4 + c4;
         const uint2 av = reinterpret_cast<const uint2 *>(A)[i];
-        const float4 d = reinterpret_cast<const float4 *>(dA)[i];
+        float4 d = reinterpret_cast<const float4 *>(dA)[i];
+        if (G) {
+            const int pp = (int)p, Dh = D / 2;  // P = B D^3 < 2^31
+            const int x = pp % D, y = (pp / D) % D, z = (pp / (D * D)) % D, b = pp / (D * D * D);
+            const float4 v = reinterpret_cast<const float4 *>(G)[(((b * Dh + z / 2) * Dh + y / 2) * Dh + x / 2) * C4 + c4];
+            d = make_float4(__builtin_fmaf(scale, v.x, d.x), __builtin_fmaf(scale, v.y, d.y),
+                            __builtin_fmaf(scale, v.z, d.z), __builtin_fmaf(scale, v.w, d.w));
+        }
         const uint16_t *ah = reinterpret_cast<const uint16_t *>(&av);
         const float o[4] = {d.x * (bf2f(ah[0]) > 0.f ? 1.f : kLeak), d.y * (bf2f(ah[1]) > 0.f ? 1.f : kLeak),
                             d.z * (bf2f(ah[2]) > 0.f ? 1.f : kLeak), d.w * (bf2f(ah[3]) > 0.f ? 1.f : kLeak)};
@@ -1273,7 +1316,8 @@ __global__ void k_up2_add4(const float *__restrict__ X, float *Y, int B, int D, 
         const long long q = ((b * D + z / 2) * D + y / 2) * D + x / 2;
         const float4 v = reinterpret_cast<const float4 *>(X)[q * C4 + c4];
         float4 &o = reinterpret_cast<float4 *>(Y)[i];
-        o = make_float4(o.x + scale * v.x, o.y + scale * v.y, o.z + scale * v.z, o.w + scale * v.w);
+        o = make_float4(__builtin_fmaf(scale, v.x, o.x), __builtin_fmaf(scale, v.y, o.y), __builtin_fmaf(scale, v.z, o.z),
+                        __builtin_fmaf(scale, v.w, o.w));
     }
 }
 
@@ -1488,6 +1532,23 @@ int check_channels(int c, const char *what)
 
 using namespace gp5;
 
+template <int C, int K, bool W, bool S>
+static int zs_launch(dim3 grid, size_t shm, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a,
+                     const uint16_t *sk, uint16_t *y, float *Yf, int P, int D, int Cout, int mirror, int act, int ZL,
+                     int up)
+{
+    static size_t shm_set = 65536;  // the default limit of dynamic LDS
+    if (shm > shm_set) {
+        NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K, W, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm),
+              "conv3d LDS size");
+        shm_set = shm;
+    }
+    hipLaunchKernelGGL((k_conv_zs<C, K, W, S>), grid, dim3(256), shm, 0, x, w, bias, a, sk, y, Yf, P, D, Cout, mirror,
+                       act, ZL, up);
+    return GP5_OK;
+}
+
 extern "C" {
 
 int gp5_nn_malloc(void **p, size_t bytes)
@@ -1591,20 +1652,22 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         if (ZL && SU <= 256 * zs_units_per_thread(Cin) && SU * 16 >= 4 * kEpiScratch) {
             const dim3 grid((unsigned)(per_z / (nblk / NBz) * (D / ZL)), (unsigned)(nblk / NBz));
             const size_t shm = (size_t)3 * SU * 16;
-#define GP5_CONV_ZS_CASE(C, K)                                                                                   \
-    if (Cin == C && NBz == K) {                                                                                  \
-        static size_t shm_set = 65536; /* the default limit of dynamic LDS */                                    \
-        if (shm > shm_set) {                                                                                     \
-            NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                                      (int)shm),                                                                 \
-                  "conv3d LDS size");                                                                            \
-            shm_set = shm;                                                                                       \
-        }                                                                                                        \
-        hipLaunchKernelGGL((k_conv_zs<C, K>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
-                           act, ZL, up);                                                                         \
-    }
-            GP5_CONV_ZS_CASE(16, 1) GP5_CONV_ZS_CASE(16, 2) GP5_CONV_ZS_CASE(32, 1)
-#undef GP5_CONV_ZS_CASE
+            // GP5_CONV_ZSV overrides the variant: bit 0 wave-private epilogue barriers, bit 1
+            // fragment reads one tap ahead.  Defaults (measured at 64^3, batch 8): both for
+            // 16 -> 16 (70.9 -> 62.5 us), the barriers only for 16 -> 32 and 32 input channels
+            // (reads ahead: 87.9 -> 98.6 us on 32 -> 16)
+            const char *v_env = getenv("GP5_CONV_ZSV");
+            const int Vz = v_env ? atoi(v_env) & 3 : (Cin == 16 && NBz == 1 ? 3 : 1);
+#define GP5_CONV_ZS_CASE(C, K, V)                                                                                \
+    if (Cin == C && NBz == K && Vz == V)                                                                         \
+        rc = zs_launch<C, K, (V & 2) != 0, (V & 1) != 0>(grid, shm, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
+                                                          act, ZL, up);
+#define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
+    GP5_CONV_ZS_CASE(C, K, 3)
+            int rc = GP5_OK;
+            GP5_CONV_ZS_V(16, 1) GP5_CONV_ZS_V(16, 2) GP5_CONV_ZS_V(32, 1)
+#undef GP5_CONV_ZS_V
+            if (rc != GP5_OK) return rc;
             NNCHK(hipGetLastError(), "conv3d launch");
             return GP5_OK;
         }
@@ -1876,7 +1939,8 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
     return GP5_OK;
 }
 
-int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db, long long P, int C)
+static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
+                            long long P, int C)
 {
     if (!A || !dA || !H || !db) return set_error(GP5_ERR_ARG, "NULL argument");
     if (C < 4 || C % 4 || 256 % (C / 4)) return set_error(GP5_ERR_ARG, "act_bwd_cast_bias: C / 4 must divide 256");
@@ -1884,11 +1948,26 @@ int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db,
     const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
     if (rc != GP5_OK) return rc;
     hipLaunchKernelGGL(k_act_bwd_cast_bias, dim3(kBiasBlocks), dim3(256), 0, 0, (const uint16_t *)A, dA, (uint16_t *)H,
-                       part, P, C);
+                       part, P, C, G, scale, D);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     return GP5_OK;
+}
+
+int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db, long long P, int C)
+{
+    return act_bwd_bias_run(A, dA, nullptr, 0.f, 0, H, db, P, C);
+}
+
+int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float *G, float scale, void *H, float *db,
+                                     int B, int D, int C)
+{
+    if (!G) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (B < 1 || D < 2 || D % 2) return set_error(GP5_ERR_ARG, "up2_add_act_bwd_cast_bias: D must be even");
+    if ((long long)B * D * D * D * C >= 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "up2_add_act_bwd_cast_bias: tensors must stay below 2^31 elements");
+    return act_bwd_bias_run(A, dA, G, scale, D, H, db, (long long)B * D * D * D, C);
 }
 
 int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale)

@@ -54,6 +54,7 @@ def _lib():
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
         "gp5_nn_mse_bf16": [P, I, P, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
+        "gp5_nn_up2_add_act_bwd_cast_bias": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -320,6 +321,14 @@ class UNet:
         _ck(_lib().gp5_nn_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[h].ptr, e["bg"].ptr,
                                             self.B * D ** 3, e["co"]))
 
+    def _up_add_act_bwd(self, name, a, g, gh, h, D):
+        """_act_bwd with the incoming gradient t[g] + t[gh] / 8 at the parent cell
+        (t[gh] of side D/2: the adjoint of the pooling that fed the next level), in
+        one pass that does not write t[g] back."""
+        e = self.w[name]
+        _ck(_lib().gp5_nn_up2_add_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[gh].ptr, 0.125,
+                                                    self.t[h].ptr, e["bg"].ptr, self.B, D, e["co"]))
+
     def _dgrad(self, name, dz_bf, out_f, D):
         # data gradient: forward kernel on the [27][ci][co] copy with mirrored taps
         e = self.w[name]
@@ -356,19 +365,16 @@ class UNet:
         self._act_bwd("c4", "e4", "g128", "h128", D // 8)
         self._wgrad("c4", t["p3"], t["h128"], D // 8)
         self._dgrad("c4", t["h128"], t["gp64"], D // 8)
-        _ck(L.gp5_nn_up2_add_f32(t["gp64"].ptr, t["g64a"].ptr, B, D // 8, 64, 0.125))  # de3 total
-        # c3
-        self._act_bwd("c3", "e3", "g64a", "h64", D // 4)
+        # c3 (de3 total = dd3 + the adjoint of the pooling into c4)
+        self._up_add_act_bwd("c3", "e3", "g64a", "gp64", "h64", D // 4)
         self._wgrad("c3", t["p2"], t["h64"], D // 4)
         self._dgrad("c3", t["h64"], t["gp32"], D // 4)
-        _ck(L.gp5_nn_up2_add_f32(t["gp32"].ptr, t["g32a"].ptr, B, D // 4, 32, 0.125))  # de2 total
-        # c2
-        self._act_bwd("c2", "e2", "g32a", "h32", D // 2)
+        # c2 (de2 total)
+        self._up_add_act_bwd("c2", "e2", "g32a", "gp32", "h32", D // 2)
         self._wgrad("c2", t["p1"], t["h32"], D // 2)
         self._dgrad("c2", t["h32"], t["gp16"], D // 2)
-        _ck(L.gp5_nn_up2_add_f32(t["gp16"].ptr, t["g16a"].ptr, B, D // 2, 16, 0.125))  # de1 total
-        # c1
-        self._act_bwd("c1", "e1", "g16a", "h16", D)
+        # c1 (de1 total)
+        self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16", D)
         self._wgrad("c1", t["X"], t["h16"], D)
         return float(loss.value)
 

@@ -70,6 +70,11 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
 /* H = bf16(dA act'(A)) for P positions of C channels (C / 4 divides 256) and the
  * bias gradient db[c] = sum over positions of dA act'(A) (float), in one pass. */
 int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db, long long P, int C);
+/* The same with the incoming gradient dA + scale * G at the parent cell, G of
+ * side D/2 (A, dA, H of side D, B samples): gp5_nn_up2_add_f32 into dA followed
+ * by gp5_nn_act_bwd_cast_bias, bit for bit, without writing dA. */
+int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float *G, float scale, void *H, float *db,
+                                     int B, int D, int C);
 /* Y (side 2D) += scale * X at the parent cell (the adjoint of pooling added into
  * Y); C a multiple of 4. */
 int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale);

@@ -130,6 +130,10 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     # act_bwd_cast_bias: C / 4 must divide 256
     assert L.gp5_nn_act_bwd_cast_bias(fake, fake, fake, fake, 64, 48) == capi.GP5_ERR_ARG
     assert L.gp5_nn_act_bwd_cast_bias(None, fake, fake, fake, 64, 16) == capi.GP5_ERR_ARG
+    # up2_add_act_bwd_cast_bias: G required, D even
+    assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, None, 0.125, fake, fake, 1, 8, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, fake, 0.125, fake, fake, 1, 7, 16) == capi.GP5_ERR_ARG
+    assert b"even" in L.gp5_last_error()
     # mse_bf16: C a multiple of 8, H required
     assert L.gp5_nn_mse_bf16(fake, 12, fake, fake, None, fake, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_bf16(fake, 16, fake, fake, None, None, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG

@@ -101,6 +101,34 @@ def test_pool_up_act_adam():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,D,C", [(2, 16, 16), (1, 8, 64), (3, 4, 32)])
+def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
+    """gp5_nn_up2_add_act_bwd_cast_bias gives bit for bit what gp5_nn_up2_add_f32
+    into dA followed by gp5_nn_act_bwd_cast_bias gives (H and the bias gradient),
+    and H matches bf16((dA + G/8 at the parent) * act'(A)) computed in numpy."""
+    rng = np.random.default_rng(B * 100 + D + C)
+    L = nn._lib()
+    A = _rb(rng.standard_normal((B, D, D, D, C)))
+    dA = rng.standard_normal((B, D, D, D, C)).astype(np.float32)
+    G = rng.standard_normal((B, D // 2, D // 2, D // 2, C)).astype(np.float32)
+    ab, gb = _buf(A, True), _buf(G)
+    da1, da2 = _buf(dA), _buf(dA)
+    H1, H2 = nn.DevBuf(dA.size * 2), nn.DevBuf(dA.size * 2)
+    db1, db2 = nn.DevBuf(C * 4), nn.DevBuf(C * 4)
+    nn._ck(L.gp5_nn_up2_add_act_bwd_cast_bias(ab.ptr, da1.ptr, gb.ptr, 0.125, H1.ptr, db1.ptr, B, D, C))
+    nn._ck(L.gp5_nn_up2_add_f32(gb.ptr, da2.ptr, B, D // 2, C, 0.125))
+    nn._ck(L.gp5_nn_act_bwd_cast_bias(ab.ptr, da2.ptr, H2.ptr, db2.ptr, B * D ** 3, C))
+    h1 = H1.download(np.uint16, dA.shape)
+    assert np.array_equal(h1, H2.download(np.uint16, dA.shape))
+    assert np.array_equal(db1.download(np.float32, (C,)), db2.download(np.float32, (C,)))
+    assert np.array_equal(da1.download(np.float32, dA.shape), dA)  # dA is not written
+    tot = dA.astype(np.float64) + 0.125 * R.up2(G.astype(np.float64))
+    ref = tot * R.lrelu_grad(A)
+    assert np.max(np.abs(nn.from_bf16(h1) - ref)) <= 2.0 ** -8 * np.max(np.abs(ref))
+    assert np.allclose(db1.download(np.float32, (C,)), ref.sum(axis=(0, 1, 2, 3)), rtol=1e-4, atol=1e-3)
+
+
+@pytest.mark.gpu
 def test_adam_multi_equals_per_tensor_adam_and_relayout():
     """One launch over weight tensors (with both bf16 copies, one copy or none) and
     a bias equals gp5_nn_adam + gp5_nn_relayout per tensor, bit for bit, over three
