@@ -274,14 +274,18 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     """MFMA throughput of the warm-start network (3D U-Net, bf16 MFMA with float
     accumulation) at D^3 with `batch` samples: forward passes and training steps
     (forward, backward, Adam) on synthetic inputs.  Flops: the sum over the eight
-    3x3x3 convolutions of 2 * 27 * Cin * Cout * D_layer^3 * batch for a forward pass,
-    three times that for a training step (data and weight gradients); pooling,
-    upsampling and the elementwise kernels count 0.  Wall time between device
-    synchronisations, so launch costs are included."""
+    3x3x3 convolutions of 2 * 27 * Cin * Cout * D_layer^3 * batch for a forward pass;
+    a training step adds the weight gradients of all eight and the data gradients of
+    c2..c8 (the input layer's data gradient is never computed), so three times the
+    forward count less c1's share; pooling, upsampling and the elementwise kernels
+    count 0.  Wall time between device synchronisations (the loss is read back every
+    step, as training does), so launch costs are included."""
     from gp5_amd import nn
 
     div = {"c1": 1, "c2": 2, "c3": 4, "c4": 8, "c5": 4, "c6": 2, "c7": 1, "c8": 1}
-    fwd_flops = sum(2.0 * 27 * ci * co * (D // div[name]) ** 3 * batch for name, ci, co in nn.SHAPES)
+    layer = {name: 2.0 * 27 * ci * co * (D // div[name]) ** 3 * batch for name, ci, co in nn.SHAPES}
+    fwd_flops = sum(layer.values())
+    train_flops = 3 * fwd_flops - layer["c1"]
     net = nn.UNet(batch, D, seed=0)
     rng = np.random.default_rng(0)
     X = (0.5 * rng.standard_normal((batch, D, D, D, nn.C0))).astype(np.float32)
@@ -305,11 +309,11 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     net.sync()
     train_s = (time.perf_counter() - t0) / steps
     fwd_tf = fwd_flops / fwd_s / 1e12
-    train_tf = 3 * fwd_flops / train_s / 1e12
+    train_tf = train_flops / train_s / 1e12
     return {
         "workload": f"3D U-Net 16-32-64-128 channels, {D}^3, batch {batch}, bf16 MFMA, float accumulation",
         "forward_flops": fwd_flops, "forward_ms": 1e3 * fwd_s, "forward_TFLOPs": fwd_tf,
-        "train_step_flops": 3 * fwd_flops, "train_step_ms": 1e3 * train_s, "train_TFLOPs": train_tf,
+        "train_step_flops": train_flops, "train_step_ms": 1e3 * train_s, "train_TFLOPs": train_tf,
         "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS},
     }

@@ -37,6 +37,7 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 
 constexpr float kLeak = 0.1f;
+constexpr int kActYf0 = 2;  // act flag of the convolutions: the float output keeps channel 0 only
 
 // Positions are linear indices p = ((b*D + z)*D + y)*D + x.
 
@@ -86,8 +87,9 @@ __device__ __forceinline__ Frag<LV> bld(__amdgpu_buffer_rsrc_t rs, int voff, int
 
 // Convolution epilogue: lane holds D[4g + i][r] of each block, i.e. position
 // base + 16 t + 4g + i and output channel 16 (nb0 + j) + r: bias, leaky ReLU
-// (act = 1), then the float output Yf, the bf16 output A and the bf16 output Y
-// (plus the skip input when given), each only when its pointer is set.
+// (act bit 0), then the float output Yf, the bf16 output A and the bf16 output Y
+// (plus the skip input when given), each only when its pointer is set.  With
+// act bit 1 (kActYf0) Yf holds channel 0 only, one float per position.
 template <int NB, int MT>
 __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
                                               const float *__restrict__ bias, int act, uint16_t *A,
@@ -95,6 +97,8 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
 {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
+    const bool yf0 = (act & kActYf0) != 0;
+    act &= 1;
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
@@ -107,7 +111,8 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
                 float v = acc[t][j][i] + (bias ? bias[co] : 0.f);
                 if (act == 1) v = v > 0.f ? v : kLeak * v;
                 const size_t o = (size_t)po * Cout + co;
-                if (Yf) Yf[o] = v;
+                if (Yf && !yf0) Yf[o] = v;
+                if (Yf && yf0 && co == 0) Yf[po] = v;
                 const uint16_t h = f2bf(v);
                 if (A) A[o] = h;
                 if (Y) Y[o] = skip ? f2bf(v + bf2f(skip[o])) : h;
@@ -137,6 +142,8 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
     static_assert(MT % 2 == 0, "tiles go through the scratch in pairs");
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
+    const bool yf0 = (act & kActYf0) != 0;
+    act &= 1;
     float *T = reinterpret_cast<float *>(scr);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -155,7 +162,7 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
             if constexpr (WS) __builtin_amdgcn_wave_barrier();
             else __syncthreads();
             const int pb = base + 32 * h;  // first position of the pair
-            if (Yf) {
+            if (Yf && !yf0) {
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
                     const int pos = lane / 4 + 16 * k, c4 = 4 * (lane & 3);
@@ -163,6 +170,8 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
                         *reinterpret_cast<float4 *>(Yf + (size_t)(pb + pos) * Cout + cb + c4) =
                             *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c4);
                 }
+            } else if (Yf && cb == 0 && lane < 32) {  // channel 0 of the 32 positions
+                if (pb + lane < P) Yf[pb + lane] = T[lane * kEpiStride];
             }
             if (A || Y) {
                 const int pos = lane / 2, c8 = 8 * (lane & 1);
@@ -1392,17 +1401,18 @@ __global__ void k_fill0(uint32_t *p, long long n)
         p[i] = 0u;
 }
 
-// dYf and H (each optional) receive the gradient rows in float and bf16;
-// partial[b] and partial[gridDim.x + b] the block sums of e^2 and of the
+// The prediction is y = base + Yf[p * ldy] (channel 0 of rows of ldy floats).
+// dYf and H (each optional) receive the gradient rows of C channels in float and
+// bf16; partial[b] and partial[gridDim.x + b] the block sums of e^2 and of the
 // channel-0 gradient 2 e / P (in double)
-__global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float *base, const float *target, float *dYf,
-                                             uint16_t *H, long long P, double *partial)
+__global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, const float *base, const float *target,
+                                             float *dYf, uint16_t *H, long long P, double *partial)
 {
     __shared__ double s[256], sg[256];
     double a = 0.0, ag = 0.0;
     const float inv = 2.0f / (float)P;
     for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < P; p += (long long)gridDim.x * 256) {
-        const float y = base[p] + Yf[(size_t)p * C];
+        const float y = base[p] + Yf[(size_t)p * ldy];
         const float e = y - target[p];
         a += (double)e * e;
         ag += (double)(inv * e);
@@ -1434,6 +1444,33 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int C, const float
         partial[blockIdx.x] = s[0];
         partial[gridDim.x + blockIdx.x] = sg[0];
     }
+}
+
+// one block: the block sums of k_mse added in a fixed order (double); the loss
+// (their mean) to loss[0], and the output layer's bias gradient (channel 0 the
+// gradient sum, the other C - 1 channels 0) to db when set
+__global__ __launch_bounds__(256) void k_mse_final(const double *partial, int blocks, long long P, double *loss, float *db,
+                                                   int C)
+{
+    __shared__ double s[256], sg[256];
+    double a = 0.0, ag = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+        a += partial[b];
+        ag += partial[blocks + b];
+    }
+    s[threadIdx.x] = a;
+    sg[threadIdx.x] = ag;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[threadIdx.x] += s[threadIdx.x + w];
+            sg[threadIdx.x] += sg[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = s[0] / (double)P;
+    if (db)
+        for (int c = threadIdx.x; c < C; c += 256) db[c] = c == 0 ? (float)sg[0] : 0.f;
 }
 
 __global__ void k_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps,
@@ -1660,14 +1697,14 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             const int Vz = v_env ? atoi(v_env) & 3 : (Cin == 16 && NBz == 1 ? 3 : 1);
 #define GP5_CONV_ZS_CASE(C, K, V)                                                                                \
     if (Cin == C && NBz == K && Vz == V)                                                                         \
-        rc = zs_launch<C, K, (V & 2) != 0, (V & 1) != 0>(grid, shm, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
+        zrc = zs_launch<C, K, (V & 2) != 0, (V & 1) != 0>(grid, shm, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
                                                           act, ZL, up);
 #define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
     GP5_CONV_ZS_CASE(C, K, 3)
-            int rc = GP5_OK;
+            int zrc = GP5_OK;
             GP5_CONV_ZS_V(16, 1) GP5_CONV_ZS_V(16, 2) GP5_CONV_ZS_V(32, 1)
 #undef GP5_CONV_ZS_V
-            if (rc != GP5_OK) return rc;
+            if (zrc != GP5_OK) return zrc;
             NNCHK(hipGetLastError(), "conv3d launch");
             return GP5_OK;
         }
@@ -1725,12 +1762,20 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
 int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                   int B, int D, int Cin, int Cout, int mirror, int act)
 {
+    if (act != 0 && act != 1) return set_error(GP5_ERR_ARG, "conv3d: act is 0 (none) or 1 (leaky ReLU)");
     return conv3d_run(X, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 0);
+}
+
+int gp5_nn_conv3d_out0(const void *X, const void *Wk, const float *bias, float *Yf0, int B, int D, int Cin, int Cout)
+{
+    if (!Yf0) return set_error(GP5_ERR_ARG, "NULL argument");
+    return conv3d_run(X, Wk, bias, nullptr, nullptr, nullptr, Yf0, B, D, Cin, Cout, 0, kActYf0, 0);
 }
 
 int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                      int B, int D, int Cin, int Cout, int mirror, int act)
 {
+    if (act != 0 && act != 1) return set_error(GP5_ERR_ARG, "conv3d: act is 0 (none) or 1 (leaky ReLU)");
     return conv3d_run(Xh, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 1);
 }
 
@@ -2020,49 +2065,65 @@ int gp5_nn_cast_bf16(const float *src, void *dst, long long n)
     return GP5_OK;
 }
 
-// db (optional, device, C floats): the bias gradient of the output layer,
-// sum over positions of the gradient rows: channel 0 from the double block sums,
-// 0 elsewhere
-static int mse_run(const float *Yf, int C, const float *base, const float *target, float *dYf, uint16_t *H, float *db,
-                   long long P, double *loss)
+// Scratch of the loss: block sums and the loss, in a process-wide buffer that
+// only grows (the kernels of successive calls run in order on the null stream).
+static int mse_scratch(size_t blocks, double **out)
+{
+    static std::mutex mu;
+    static double *buf = nullptr;
+    static size_t cap = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    const size_t need = 2 * blocks + 1;
+    if (cap < need) {
+        if (buf) NNCHK(hipFree(buf), "mse scratch");
+        buf = nullptr;
+        cap = 0;
+        NNCHK(hipMalloc(&buf, need * sizeof(double)), "mse scratch");
+        cap = need;
+    }
+    *out = buf;
+    return GP5_OK;
+}
+
+// db (optional, device, C floats): the bias gradient of the output layer, sum
+// over positions of the gradient rows: channel 0 from the double block sums, 0
+// elsewhere.  The sums finish on the device; the one host copy is the loss.
+static int mse_run(const float *Yf, int ldy, int C, const float *base, const float *target, float *dYf, uint16_t *H,
+                   float *db, long long P, double *loss)
 {
     if (!Yf || !base || !target || !loss) return set_error(GP5_ERR_ARG, "NULL argument");
     if (H && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
+    if (C < 1 || P < 1) return set_error(GP5_ERR_ARG, "mse: bad size");
     const unsigned blocks = (unsigned)std::min<long long>((P + 255) / 256, 1024);
     double *part = nullptr;
-    NNCHK(hipMalloc(&part, 2 * blocks * sizeof(double)), "mse scratch");
-    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, C, base, target, dYf, H, P, part);
-    hipError_t e = hipGetLastError();
-    std::vector<double> h(2 * blocks);
-    if (e == hipSuccess) e = hipMemcpy(h.data(), part, 2 * blocks * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(part);
-    if (e != hipSuccess) return hip_fail(e, "mse");
-    double s = 0.0, sg = 0.0;
-    for (unsigned b = 0; b < blocks; ++b) {
-        s += h[b];
-        sg += h[blocks + b];
-    }
-    *loss = s / (double)P;
-    if (db) {
-        std::vector<float> g(C, 0.f);
-        g[0] = (float)sg;
-        e = hipMemcpy(db, g.data(), C * sizeof(float), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_fail(e, "mse bias gradient");
-    }
+    const int rc = mse_scratch(blocks, &part);
+    if (rc != GP5_OK) return rc;
+    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, ldy, C, base, target, dYf, H, P, part);
+    NNCHK(hipGetLastError(), "mse");
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(256), 0, 0, part, (int)blocks, P, part + 2 * blocks, db, C);
+    NNCHK(hipGetLastError(), "mse");
+    NNCHK(hipMemcpy(loss, part + 2 * blocks, sizeof(double), hipMemcpyDeviceToHost), "mse loss");
     return GP5_OK;
 }
 
 int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, float *dYf, long long P, double *loss)
 {
     if (!dYf) return set_error(GP5_ERR_ARG, "NULL argument");
-    return mse_run(Yf, C, base, target, dYf, nullptr, nullptr, P, loss);
+    return mse_run(Yf, C, C, base, target, dYf, nullptr, nullptr, P, loss);
 }
 
 int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, float *db,
                     long long P, double *loss)
 {
     if (!H) return set_error(GP5_ERR_ARG, "NULL argument");
-    return mse_run(Yf, C, base, target, dYf, (uint16_t *)H, db, P, loss);
+    return mse_run(Yf, C, C, base, target, dYf, (uint16_t *)H, db, P, loss);
+}
+
+int gp5_nn_mse_out0(const float *y0, const float *base, const float *target, void *H, int C, float *db, long long P,
+                    double *loss)
+{
+    if (!H) return set_error(GP5_ERR_ARG, "NULL argument");
+    return mse_run(y0, 1, C, base, target, nullptr, (uint16_t *)H, db, P, loss);
 }
 
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps, int t)

@@ -53,6 +53,8 @@ def _lib():
         "gp5_nn_conv3d_up": [P, P, P, P, P, P, P, I, I, I, I, I, I],
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
         "gp5_nn_mse_bf16": [P, I, P, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_mse_out0": [P, P, P, P, I, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_up2_add_act_bwd_cast_bias": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
@@ -177,7 +179,7 @@ class UNet:
             "u4": (n(D // 4, 128), 2), "a5": (n(D // 4, 64), 2), "d3": (n(D // 4, 64), 2),
             "u3": (n(D // 2, 64), 2), "a6": (n(D // 2, 32), 2), "d2": (n(D // 2, 32), 2),
             "u2": (n(D, 32), 2), "a7": (n(D, 16), 2), "d1": (n(D, 16), 2),
-            "out": (n(D, 16), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
+            "out": (n(D, 1), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
             "h_out": (n(D, 16), 2),
             "g16a": (n(D, 16), 4), "h16": (n(D, 16), 2),
@@ -271,7 +273,8 @@ class UNet:
         self._conv_up("c5", "e4", "u4", D // 4, A=t["a5"], skip=t["e3"], Y=t["d3"])
         self._conv_up("c6", "d3", "u3", D // 2, A=t["a6"], skip=t["e2"], Y=t["d2"])
         self._conv_up("c7", "d2", "u2", D, A=t["a7"], skip=t["e1"], Y=t["d1"])
-        self._conv("c8", t["d1"], Yf=t["out"], D=D, act=0)
+        e = self.w["c8"]  # output layer: channel 0 of its float output
+        _ck(L.gp5_nn_conv3d_out0(t["d1"].ptr, e["Wf"].ptr, e["b"].ptr, t["out"].ptr, B, D, e["ci"], e["co"]))
 
     def set_droplet_features(self, s: float, a, consts, ub: float, index: int = 0):
         """Features of one droplet built on the device (gp5_nn_droplet_features) as
@@ -288,16 +291,16 @@ class UNet:
         _ck(_lib().gp5_nn_droplet_features(X, base, self.D, float(ub), float(s), af, cf))
 
     def output0(self) -> np.ndarray:
-        """The prediction base + out[..., 0] as [B, D, D, D] float, combined on the device."""
+        """The prediction base + out (channel 0 of c8) as [B, D, D, D] float, combined on the device."""
         B, D = self.B, self.D
         if "y" not in self.t:
             self.t["y"] = DevBuf(B * D ** 3 * 4)
-        _ck(_lib().gp5_nn_predict_out(self.t["out"].ptr, 16, self.t["base"].ptr, self.t["y"].ptr, B * D ** 3))
+        _ck(_lib().gp5_nn_predict_out(self.t["out"].ptr, 1, self.t["base"].ptr, self.t["y"].ptr, B * D ** 3))
         return self.t["y"].download(np.float32, (B, D, D, D))
 
     def output(self) -> np.ndarray:
         B, D = self.B, self.D
-        out = self.t["out"].download(np.float32, (B, D, D, D, 16))[..., 0]
+        out = self.t["out"].download(np.float32, (B, D, D, D))
         return self.t["base"].download(np.float32, (B, D, D, D)) + out
 
     def _wgrad(self, name, x, dz_bf, D):
@@ -341,7 +344,7 @@ class UNet:
         n = lambda d, c: B * d ** 3 * c  # noqa: E731
         loss = ctypes.c_double()
         # loss, the bf16 output gradient and c8's bias gradient in one pass
-        _ck(L.gp5_nn_mse_bf16(t["out"].ptr, 16, t["base"].ptr, t["target"].ptr, None, t["h_out"].ptr,
+        _ck(L.gp5_nn_mse_out0(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
                               self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss)))
         # c8
         self._wgrad("c8", t["d1"], t["h_out"], D)

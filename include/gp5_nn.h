@@ -39,6 +39,10 @@ int gp5_nn_sync(void);
  * Yf float = act(...) (for the last layer and for gradients). */
 int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                   int B, int D, int Cin, int Cout, int mirror, int act);
+/* The output layer's form: no activation, and the float output Yf0 keeps only
+ * channel 0 (one float per position, B*D^3 floats); the other channels are
+ * computed (the MFMA tiles are 16 wide) and dropped. */
+int gp5_nn_conv3d_out0(const void *X, const void *Wk, const float *bias, float *Yf0, int B, int D, int Cin, int Cout);
 /* The same convolution of the nearest upsampling of Xh (side D/2, Cin channels)
  * without forming it: output side D, a power of two in [8, 128]. */
 int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
@@ -88,6 +92,11 @@ int gp5_nn_mse(const float *Yf, int C, const float *base, const float *target, f
  * over positions of the gradient rows (channel 0; the others are 0). */
 int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *target, float *dYf, void *H, float *db,
                     long long P, double *loss);
+/* gp5_nn_mse_bf16 for a prediction y = base + y0 with y0 one float per position
+ * (gp5_nn_conv3d_out0's output); H rows have C channels (a multiple of 8),
+ * channel 0 the gradient 2 (y - t) / P, the others 0; db (C floats) as above. */
+int gp5_nn_mse_out0(const float *y0, const float *base, const float *target, void *H, int C, float *db, long long P,
+                    double *loss);
 /* Droplet input channels (one cube of side D on the scaled box [-ub, ub]^3, C-ordered
  * [D][D][D][16] bf16): 0 = 0.5 (1 - tanh((r - 1) s)), 1 = a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
  * 2..6 = c[0..4], 7 = r / ub, 8..15 = 0; base (float, D^3) = channel 0.  float32

@@ -134,6 +134,12 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, None, 0.125, fake, fake, 1, 8, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, fake, 0.125, fake, fake, 1, 7, 16) == capi.GP5_ERR_ARG
     assert b"even" in L.gp5_last_error()
+    # conv3d: act 0 or 1 only; out0 needs its output
+    assert L.gp5_nn_conv3d(fake, fake, None, None, None, None, fake, 1, 8, 16, 16, 0, 2) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_out0(fake, fake, None, None, 1, 8, 16, 16) == capi.GP5_ERR_ARG
+    # mse_out0: H required, C a multiple of 8
+    assert L.gp5_nn_mse_out0(fake, fake, fake, None, 16, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_mse_out0(fake, fake, fake, fake, 12, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     # mse_bf16: C a multiple of 8, H required
     assert L.gp5_nn_mse_bf16(fake, 12, fake, fake, None, fake, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_bf16(fake, 16, fake, fake, None, None, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG

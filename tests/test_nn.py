@@ -129,6 +129,41 @@ def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D,B", [(64, 2), (16, 1), (4, 2)])
+def test_output_layer_channel0_and_its_loss(D, B):
+    """gp5_nn_conv3d_out0 writes, bit for bit, channel 0 of gp5_nn_conv3d's float
+    output (z-sliding, LDS-staged and direct kernels by cube side), and
+    gp5_nn_mse_out0 on it gives the loss, the bf16 gradient rows and the bias
+    gradient of gp5_nn_mse_bf16 on the 16-channel output, bit for bit."""
+    rng = np.random.default_rng(D + B)
+    L = nn._lib()
+    C = 16
+    X = _rb(rng.standard_normal((B, D, D, D, C)))
+    W = _rb(rng.standard_normal((27, C, C)) * 0.1)
+    b = rng.standard_normal(C).astype(np.float32)
+    xb, wf, bb = _buf(X, True), _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(b)
+    P = B * D ** 3
+    Yf, y0 = nn.DevBuf(P * C * 4), nn.DevBuf(P * 4)
+    nn._ck(L.gp5_nn_conv3d(xb.ptr, wf.ptr, bb.ptr, None, None, None, Yf.ptr, B, D, C, C, 0, 0))
+    nn._ck(L.gp5_nn_conv3d_out0(xb.ptr, wf.ptr, bb.ptr, y0.ptr, B, D, C, C))
+    full = Yf.download(np.float32, (P, C))
+    assert np.array_equal(y0.download(np.float32, (P,)), full[:, 0])
+    base = rng.standard_normal(P).astype(np.float32)
+    tgt = (base + 0.1 * rng.standard_normal(P)).astype(np.float32)
+    baseb, tb = _buf(base), _buf(tgt)
+    H1, H2 = nn.DevBuf(P * C * 2), nn.DevBuf(P * C * 2)
+    db1, db2 = nn.DevBuf(C * 4), nn.DevBuf(C * 4)
+    l1, l2 = ctypes.c_double(), ctypes.c_double()
+    nn._ck(L.gp5_nn_mse_bf16(Yf.ptr, C, baseb.ptr, tb.ptr, None, H1.ptr, db1.ptr, P, ctypes.byref(l1)))
+    nn._ck(L.gp5_nn_mse_out0(y0.ptr, baseb.ptr, tb.ptr, H2.ptr, C, db2.ptr, P, ctypes.byref(l2)))
+    assert l1.value == l2.value
+    assert np.array_equal(H1.download(np.uint16, (P, C)), H2.download(np.uint16, (P, C)))
+    assert np.array_equal(db1.download(np.float32, (C,)), db2.download(np.float32, (C,)))
+    e = (base.astype(np.float64) + full[:, 0]) - tgt
+    assert abs(l2.value - np.mean(e * e)) <= 1e-6 * np.mean(e * e)
+
+
+@pytest.mark.gpu
 def test_adam_multi_equals_per_tensor_adam_and_relayout():
     """One launch over weight tensors (with both bf16 copies, one copy or none) and
     a bias equals gp5_nn_adam + gp5_nn_relayout per tensor, bit for bit, over three
