@@ -488,7 +488,16 @@ __host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 1
 // fragments of tap + 1 while the MFMAs of tap run (scheduling barriers keep
 // the compiler from sinking the reads to their uses); WS gives the epilogue
 // wave-private barriers.  Neither changes any accumulation order.
-template <int CIN, int NB, bool PF = false, bool WS = false>
+//
+// POOL (D = 64 or 128, no bias or activation, float output only): Yf receives,
+// at side D/2, the sum over the 2x2x2 children of each parent cell (the
+// adjoint of nearest upsampling applied to the convolution), in the order
+// k = 0..7 of k_pool2 (z, y, x bits), so that it equals the convolution
+// followed by gp5_nn_pool2_f32 with scale 8 bit for bit.  A parent's x pair
+// lies in one lane's registers, its y pair in two waves (the even row's wave
+// owns the parent and reads the odd row's values through LDS), its z pair in
+// consecutive planes of the walk (the even plane's sum waits in registers).
+template <int CIN, int NB, bool PF = false, bool WS = false, bool POOL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                  uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
@@ -561,6 +570,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     load_slab(zb + 1);
     store_slab(2);
     __syncthreads();
+    float S[POOL ? MT : 1][POOL ? NB : 1][2];  // POOL: the even plane's partial parent sums
     for (int zi = 0; zi < ZL; ++zi) {
         const int z = zb + zi;
         // keeps the 27 MT tap offsets from being hoisted out of this loop into registers
@@ -613,8 +623,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
-        conv_epilogue_lds<NB, MT, WS>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf,
-                                      sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
+        if constexpr (POOL) {
+            float *xs = reinterpret_cast<float *>(sx + (size_t)sl[0] * SU * 16);  // 8 KB of the free slot
+            const int wpr = D / 64;                                              // waves per row
+            const bool owner = ((wave / wpr) & 1) == 0;                          // even row
+            const int pair = wave / (2 * wpr) * wpr + wave % wpr;
+            const int Dh = D / 2;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (!owner)
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) xs[((pair * 16) + t * 4 + i) * 64 + lane] = acc[t][j][i];
+                __syncthreads();
+                if (owner) {
+#pragma unroll
+                    for (int t = 0; t < MT; ++t) {
+                        const int q0 = 64 * wave + 16 * t;  // first plane-local position of the tile
+                        const int yp = (y0 + q0 / D) / 2, xb = (q0 % D + 4 * g) / 2;
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const float a0 = acc[t][j][2 * h], a1 = acc[t][j][2 * h + 1];
+                            const float b0 = xs[((pair * 16) + t * 4 + 2 * h) * 64 + lane];
+                            const float b1 = xs[((pair * 16) + t * 4 + 2 * h + 1) * 64 + lane];
+                            if ((zi & 1) == 0) {
+                                S[t][j][h] = ((a0 + a1) + b0) + b1;
+                            } else {
+                                const float v = (((S[t][j][h] + a0) + a1) + b0) + b1;
+                                const int pp = ((smp * Dh + z / 2) * Dh + yp) * Dh + xb + h;
+                                Yf[(size_t)pp * Cout + 16 * (nb0 + j) + r] = v;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+            conv_epilogue_lds<NB, MT, WS>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y,
+                                          Yf, sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
+        }
         if (zi + 1 < ZL) {
             if constexpr (WS) __syncthreads();  // every wave is done with its scratch in the slot
             store_slab(sl[0]);
@@ -1100,6 +1148,40 @@ __global__ void k_wgrad_reduce_groups(const float *ws, float *part, long long n,
     part[(size_t)grp * n + i] = s;
 }
 
+// The same sum in one pass: a workgroup owns OPB consecutive outputs, its Q =
+// 256 / OPB thread groups take the chunks c = q, q + Q, q + 2Q, ... (q the
+// group) in increasing order, and the Q group sums are added in the order
+// q = 0 .. Q-1.  OPB = 16 keeps small layers spread over enough workgroups.
+template <int OPB>
+__global__ __launch_bounds__(256) void k_wgrad_reduce1(const float *__restrict__ ws, float *dW, long long n, int chunks)
+{
+    constexpr int Q = 256 / OPB;
+    __shared__ float s[Q][OPB];
+    const int o = threadIdx.x % OPB, q = threadIdx.x / OPB;
+    const long long i = (long long)blockIdx.x * OPB + o;
+    float a = 0.f;
+    if (i < n) {
+        int c = q;
+        for (; c + 3 * Q < chunks; c += 4 * Q) {  // four loads in flight
+            const float v0 = ws[(size_t)c * n + i], v1 = ws[(size_t)(c + Q) * n + i],
+                        v2 = ws[(size_t)(c + 2 * Q) * n + i], v3 = ws[(size_t)(c + 3 * Q) * n + i];
+            a += v0;
+            a += v1;
+            a += v2;
+            a += v3;
+        }
+        for (; c < chunks; c += Q) a += ws[(size_t)c * n + i];
+    }
+    s[q][o] = a;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        float t = s[0][o];
+#pragma unroll
+        for (int k = 1; k < Q; ++k) t += s[k][o];
+        dW[i] = t;
+    }
+}
+
 __global__ void k_wgrad_reduce(const float *part, float *dW, long long n)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -1569,19 +1651,19 @@ int check_channels(int c, const char *what)
 
 using namespace gp5;
 
-template <int C, int K, bool W, bool S>
+template <int C, int K, bool W, bool S, bool PL = false>
 static int zs_launch(dim3 grid, size_t shm, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a,
                      const uint16_t *sk, uint16_t *y, float *Yf, int P, int D, int Cout, int mirror, int act, int ZL,
                      int up)
 {
     static size_t shm_set = 65536;  // the default limit of dynamic LDS
     if (shm > shm_set) {
-        NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K, W, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K, W, S, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm),
               "conv3d LDS size");
         shm_set = shm;
     }
-    hipLaunchKernelGGL((k_conv_zs<C, K, W, S>), grid, dim3(256), shm, 0, x, w, bias, a, sk, y, Yf, P, D, Cout, mirror,
+    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), grid, dim3(256), shm, 0, x, w, bias, a, sk, y, Yf, P, D, Cout, mirror,
                        act, ZL, up);
     return GP5_OK;
 }
@@ -1636,8 +1718,10 @@ int gp5_nn_sync(void)
 
 // up = 1: X has side D/2 and the convolution reads its nearest upsampling (only
 // the LDS-staged and z-sliding kernels do that)
+// pool = 1: Yf (side D/2) receives the 2x2x2 child sums of the convolution (the
+// POOL form of k_conv_zs; only where that kernel applies, else an error)
 static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y,
-                      float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up)
+                      float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up, int pool = 0)
 {
     if (!X || !Wk) return set_error(GP5_ERR_ARG, "NULL input or weights");
     if (skip && !Y) return set_error(GP5_ERR_ARG, "a skip input needs the Y output");
@@ -1702,6 +1786,16 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
 #define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
     GP5_CONV_ZS_CASE(C, K, 3)
             int zrc = GP5_OK;
+            if (pool) {
+                if (Cin != 16 || NBz != 2 || D < 64 || bias || act || A || Y || !Yf)
+                    return set_error(GP5_ERR_ARG, "conv3d_pool: needs 16 input channels, a multiple of 32 output "
+                                                  "channels, D = 64 or 128, and only the float output");
+                zrc = zs_launch<16, 2, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
+                                                           (int)P, D, Cout, mirror, 0, ZL, up);
+                if (zrc != GP5_OK) return zrc;
+                NNCHK(hipGetLastError(), "conv3d launch");
+                return GP5_OK;
+            }
             GP5_CONV_ZS_V(16, 1) GP5_CONV_ZS_V(16, 2) GP5_CONV_ZS_V(32, 1)
 #undef GP5_CONV_ZS_V
             if (zrc != GP5_OK) return zrc;
@@ -1709,6 +1803,7 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             return GP5_OK;
         }
     }
+    if (pool) return set_error(GP5_ERR_ARG, "conv3d_pool: the shape does not take the z-sliding kernel");
     // LDS-staged kernel: D a power of two in [8, 128] (workgroups of 256 positions
     // are whole rows of one sample), channel chunk KC = 32 or 16 with a staged
     // region of at most 64 KB, output blocks per wave NB = 4, 2 or 1, the widest
@@ -1764,6 +1859,12 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
 {
     if (act != 0 && act != 1) return set_error(GP5_ERR_ARG, "conv3d: act is 0 (none) or 1 (leaky ReLU)");
     return conv3d_run(X, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 0);
+}
+
+int gp5_nn_conv3d_pool(const void *X, const void *Wk, float *Yh, int B, int D, int Cin, int Cout, int mirror)
+{
+    if (!Yh) return set_error(GP5_ERR_ARG, "NULL argument");
+    return conv3d_run(X, Wk, nullptr, nullptr, nullptr, nullptr, Yh, B, D, Cin, Cout, mirror, 0, 0, 1);
 }
 
 int gp5_nn_conv3d_out0(const void *X, const void *Wk, const float *bias, float *Yf0, int B, int D, int Cin, int Cout)
@@ -1839,11 +1940,12 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
         hipLaunchKernelGGL(k_wgrad_zs, dim3((unsigned)zblocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWzThreads),
                            shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, ZL, up);
         NNCHK(hipGetLastError(), "wgrad launch");
-        float *part = ws + (size_t)zblocks * n;
-        hipLaunchKernelGGL(k_wgrad_reduce_groups, dim3((unsigned)((n + 255) / 256), kRedGroups), dim3(256), 0, 0, ws,
-                           part, n, (int)zblocks);
-        NNCHK(hipGetLastError(), "wgrad reduce launch");
-        hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, part, dW, n);
+        if (n >= 64 * 256)
+            hipLaunchKernelGGL(k_wgrad_reduce1<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, 0, ws, dW, n,
+                               (int)zblocks);
+        else
+            hipLaunchKernelGGL(k_wgrad_reduce1<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, 0, ws, dW, n,
+                               (int)zblocks);
         NNCHK(hipGetLastError(), "wgrad reduce launch");
         return GP5_OK;
     }

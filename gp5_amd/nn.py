@@ -28,6 +28,14 @@ SHAPES = [("c1", 16, 16), ("c2", 16, 32), ("c3", 32, 64), ("c4", 64, 128),
 C0 = 16  # input feature channels
 
 
+def _pool_fused(d: int, ci: int, co: int) -> bool:
+    """Whether the data gradient of the layer of side d that reads the upsampled
+    input (ci input channels of the layer, co output) comes out already summed over
+    each cell's children (gp5_nn_conv3d_pool: 16 gradient channels in, a multiple
+    of 32 out, side 64 or 128)."""
+    return d in (64, 128) and co == 16 and ci % 32 == 0
+
+
 def _up_fused(d: int) -> bool:
     """Whether the convolutions of side d read their upsampled input in place
     (gp5_nn_conv3d_up / gp5_nn_conv3d_wgrad_up: a power of two in [8, 128])."""
@@ -55,6 +63,7 @@ def _lib():
         "gp5_nn_mse_bf16": [P, I, P, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_mse_out0": [P, P, P, P, I, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
+        "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_up2_add_act_bwd_cast_bias": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
@@ -192,6 +201,8 @@ class UNet:
         for name, d in (("u4", D // 4), ("u3", D // 2), ("u2", D)):
             if _up_fused(d):  # read in place from the half-side tensor
                 del sizes[name]
+        if _pool_fused(D, 32, 16):  # c7's data gradient arrives summed over the children
+            del sizes["gu32"]
         self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
         L = _lib()
         ws = max(L.gp5_nn_wgrad_ws(B, d, ci, co) for d, ci, co in
@@ -352,8 +363,12 @@ class UNet:
         # c7: dz7 = dd1 * act'(a7)
         self._act_bwd("c7", "a7", "g16a", "h16", D)
         self._wgrad_up("c7", "d2", "u2", t["h16"], D)
-        self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
-        _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
+        if _pool_fused(D, 32, 16):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
+            e = self.w["c7"]
+            _ck(L.gp5_nn_conv3d_pool(t["h16"].ptr, e["Wb"].ptr, t["g32a"].ptr, B, D, e["co"], e["ci"], 1))
+        else:
+            self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
+            _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
         # c6
         self._act_bwd("c6", "a6", "g32a", "h32", D // 2)
         self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2)

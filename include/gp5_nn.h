@@ -39,6 +39,14 @@ int gp5_nn_sync(void);
  * Yf float = act(...) (for the last layer and for gradients). */
 int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                   int B, int D, int Cin, int Cout, int mirror, int act);
+/* The float output of the convolution (no bias, no activation; mirror as in
+ * gp5_nn_conv3d) summed over the 2x2x2 children of each cell of side D/2: Yh
+ * (B*(D/2)^3*Cout floats) equals gp5_nn_conv3d into Yf followed by
+ * gp5_nn_pool2_f32(Yf, Yh, B, D, Cout, 8), bit for bit, without writing Yf (the
+ * data gradient of a layer that reads the nearest upsampling of its input).
+ * Cin = 16, Cout a multiple of 32, D = 64 or 128; other shapes return
+ * GP5_ERR_ARG. */
+int gp5_nn_conv3d_pool(const void *X, const void *Wk, float *Yh, int B, int D, int Cin, int Cout, int mirror);
 /* The output layer's form: no activation, and the float output Yf0 keeps only
  * channel 0 (one float per position, B*D^3 floats); the other channels are
  * computed (the MFMA tiles are 16 wide) and dropped. */

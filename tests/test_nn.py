@@ -129,6 +129,31 @@ def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D,Cout,B", [(64, 32, 2), (128, 32, 1), (64, 64, 1)])
+def test_conv_pool_equals_conv_then_pool(D, Cout, B):
+    """gp5_nn_conv3d_pool (the data gradient summed over each cell's 2x2x2
+    children, mirrored taps) equals gp5_nn_conv3d into a float tensor followed by
+    gp5_nn_pool2_f32 with scale 8, bit for bit, and agrees with the oracle."""
+    rng = np.random.default_rng(D + Cout + B)
+    L = nn._lib()
+    Cin = 16
+    G = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cout, Cin)) * 0.1)  # [27][ci][co] of the layer: its Cout = 16 is Cin here
+    gb, wb = _buf(G, True), _buf(W, True)
+    full = nn.DevBuf(B * D ** 3 * Cout * 4)
+    ref_h, got_h = nn.DevBuf(B * D ** 3 * Cout // 2), nn.DevBuf(B * D ** 3 * Cout // 2)
+    nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, full.ptr, B, D, Cin, Cout, 1, 0))
+    nn._ck(L.gp5_nn_pool2_f32(full.ptr, ref_h.ptr, B, D, Cout, 8.0))
+    nn._ck(L.gp5_nn_conv3d_pool(gb.ptr, wb.ptr, got_h.ptr, B, D, Cin, Cout, 1))
+    shape = (B, D // 2, D // 2, D // 2, Cout)
+    got = got_h.download(np.float32, shape)
+    assert np.array_equal(got, ref_h.download(np.float32, shape))
+    if D <= 64:  # the oracle at 128^3 takes minutes; there the two GPU paths are compared
+        ref = 8.0 * R.pool2(R.conv_dgrad(G, W))
+        assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,B", [(64, 2), (16, 1), (4, 2)])
 def test_output_layer_channel0_and_its_loss(D, B):
     """gp5_nn_conv3d_out0 writes, bit for bit, channel 0 of gp5_nn_conv3d's float
