@@ -489,14 +489,15 @@ __host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 1
 // the compiler from sinking the reads to their uses); WS gives the epilogue
 // wave-private barriers.  Neither changes any accumulation order.
 //
-// POOL (D = 64 or 128, no bias or activation, float output only): Yf receives,
+// POOL (D = 32, 64 or 128, no bias or activation, float output only): Yf receives,
 // at side D/2, the sum over the 2x2x2 children of each parent cell (the
 // adjoint of nearest upsampling applied to the convolution), in the order
 // k = 0..7 of k_pool2 (z, y, x bits), so that it equals the convolution
 // followed by gp5_nn_pool2_f32 with scale 8 bit for bit.  A parent's x pair
-// lies in one lane's registers, its y pair in two waves (the even row's wave
-// owns the parent and reads the odd row's values through LDS), its z pair in
-// consecutive planes of the walk (the even plane's sum waits in registers).
+// lies in one lane's registers, its y pair in one wave at D = 32 (tiles t and
+// t + 2) and in two waves at D >= 64 (the even row's wave owns the parent and
+// reads the odd row's values through LDS), its z pair in consecutive planes of
+// the walk (the even plane's sum waits in registers).
 template <int CIN, int NB, bool PF = false, bool WS = false, bool POOL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
@@ -624,6 +625,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
         __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
         if constexpr (POOL) {
+          if (D == 32) {  // a wave holds rows 2w (tiles 0, 1) and 2w + 1 (tiles 2, 3): its own y pairs
+            const int Dh = 16, yp = (y0 + 2 * wave) / 2;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float a0 = acc[t][j][2 * h], a1 = acc[t][j][2 * h + 1];
+                        const float b0 = acc[t + 2][j][2 * h], b1 = acc[t + 2][j][2 * h + 1];
+                        if ((zi & 1) == 0) {
+                            S[t][j][h] = ((a0 + a1) + b0) + b1;
+                        } else {
+                            const float v = (((S[t][j][h] + a0) + a1) + b0) + b1;
+                            const int pp = ((smp * Dh + z / 2) * Dh + yp) * Dh + (16 * t + 4 * g) / 2 + h;
+                            Yf[(size_t)pp * Cout + 16 * (nb0 + j) + r] = v;
+                        }
+                    }
+          } else {
             float *xs = reinterpret_cast<float *>(sx + (size_t)sl[0] * SU * 16);  // 8 KB of the free slot
             const int wpr = D / 64;                                              // waves per row
             const bool owner = ((wave / wpr) & 1) == 0;                          // even row
@@ -659,6 +679,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 }
                 __syncthreads();
             }
+          }
         } else {
             conv_epilogue_lds<NB, MT, WS>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y,
                                           Yf, sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
@@ -1133,22 +1154,8 @@ __host__ inline size_t wgrad_zs_lds(int D)
     return (size_t)(3 * 16 * wz_chstride((Rb + 2) * (D + 8)) + 16 * wz_chstride(Rb * (D + 8))) * 2;
 }
 
-// Sum of the chunk partials in two fixed-order levels: kRedGroups groups of
-// consecutive chunks per output first (one thread each), then the group sums.
-constexpr int kRedGroups = 32;
-__global__ void k_wgrad_reduce_groups(const float *ws, float *part, long long n, int chunks)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int grp = blockIdx.y;
-    if (i >= n) return;
-    const int per = (chunks + kRedGroups - 1) / kRedGroups;
-    const int c0 = grp * per, c1 = min(chunks, c0 + per);
-    float s = 0.f;
-    for (int c = c0; c < c1; ++c) s += ws[(size_t)c * n + i];
-    part[(size_t)grp * n + i] = s;
-}
-
-// The same sum in one pass: a workgroup owns OPB consecutive outputs, its Q =
+// Sum of the chunk partials in one fixed-order pass: a workgroup owns OPB
+// consecutive outputs, its Q =
 // 256 / OPB thread groups take the chunks c = q, q + Q, q + 2Q, ... (q the
 // group) in increasing order, and the Q group sums are added in the order
 // q = 0 .. Q-1.  OPB = 16 keeps small layers spread over enough workgroups.
@@ -1179,15 +1186,6 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce1(const float *__restrict__
 #pragma unroll
         for (int k = 1; k < Q; ++k) t += s[k][o];
         dW[i] = t;
-    }
-}
-
-__global__ void k_wgrad_reduce(const float *part, float *dW, long long n)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int g = 0; g < kRedGroups; ++g) s += part[(size_t)g * n + i];
-        dW[i] = s;
     }
 }
 
@@ -1787,11 +1785,18 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
     GP5_CONV_ZS_CASE(C, K, 3)
             int zrc = GP5_OK;
             if (pool) {
-                if (Cin != 16 || NBz != 2 || D < 64 || bias || act || A || Y || !Yf)
-                    return set_error(GP5_ERR_ARG, "conv3d_pool: needs 16 input channels, a multiple of 32 output "
-                                                  "channels, D = 64 or 128, and only the float output");
-                zrc = zs_launch<16, 2, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
-                                                           (int)P, D, Cout, mirror, 0, ZL, up);
+                if (D < 32 || bias || act || A || Y || !Yf)
+                    return set_error(GP5_ERR_ARG, "conv3d_pool: needs 16 or 32 input channels, D = 32, 64 or 128, "
+                                                  "and only the float output");
+                if (Cin == 16 && NBz == 2)
+                    zrc = zs_launch<16, 2, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
+                                                               (int)P, D, Cout, mirror, 0, ZL, up);
+                else if (Cin == 32)
+                    zrc = zs_launch<32, 1, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
+                                                               (int)P, D, Cout, mirror, 0, ZL, up);
+                else
+                    return set_error(GP5_ERR_ARG, "conv3d_pool: 16 input channels need a multiple of 32 output "
+                                                  "channels");
                 if (zrc != GP5_OK) return zrc;
                 NNCHK(hipGetLastError(), "conv3d launch");
                 return GP5_OK;
@@ -1910,8 +1915,18 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
     // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums;
     // or the partial sums of the position blocks of k_wgrad_zs
     const size_t n = (size_t)27 * Cin * Cout * sizeof(float);
-    return std::max((size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)(chunks + kRedGroups) * n,
-                    (size_t)(zchunks + kRedGroups) * n);
+    return std::max((size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)chunks * n, (size_t)zchunks * n);
+}
+
+// dW = the sum of `chunks` partials of n values each (fixed order)
+static int wgrad_reduce(const float *ws, float *dW, long long n, int chunks)
+{
+    if (n >= 64 * 256)
+        hipLaunchKernelGGL(k_wgrad_reduce1<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, 0, ws, dW, n, chunks);
+    else
+        hipLaunchKernelGGL(k_wgrad_reduce1<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, 0, ws, dW, n, chunks);
+    NNCHK(hipGetLastError(), "wgrad reduce launch");
+    return GP5_OK;
 }
 
 // up = 1: X has side D/2 and the gradient is that of a convolution of its
@@ -1940,14 +1955,7 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
         hipLaunchKernelGGL(k_wgrad_zs, dim3((unsigned)zblocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWzThreads),
                            shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, ZL, up);
         NNCHK(hipGetLastError(), "wgrad launch");
-        if (n >= 64 * 256)
-            hipLaunchKernelGGL(k_wgrad_reduce1<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, 0, ws, dW, n,
-                               (int)zblocks);
-        else
-            hipLaunchKernelGGL(k_wgrad_reduce1<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, 0, ws, dW, n,
-                               (int)zblocks);
-        NNCHK(hipGetLastError(), "wgrad reduce launch");
-        return GP5_OK;
+        return wgrad_reduce(ws, dW, n, (int)zblocks);
     }
     const long long chunk = wgrad_chunk(P, Cin, Cout);
     const long long chunks = (P + chunk - 1) / chunk;
@@ -1979,13 +1987,7 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
                            Cin, Cout, chunk);
     }
     NNCHK(hipGetLastError(), "wgrad launch");
-    float *part = part0 + (size_t)chunks * n;  // after the chunk partials
-    hipLaunchKernelGGL(k_wgrad_reduce_groups, dim3((unsigned)((n + 255) / 256), kRedGroups), dim3(256), 0, 0, part0,
-                       part, n, (int)chunks);
-    NNCHK(hipGetLastError(), "wgrad reduce launch");
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid_for(n)), dim3(256), 0, 0, part, dW, n);
-    NNCHK(hipGetLastError(), "wgrad reduce launch");
-    return GP5_OK;
+    return wgrad_reduce(part0, dW, n, (int)chunks);
 }
 
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)

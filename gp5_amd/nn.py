@@ -28,12 +28,17 @@ SHAPES = [("c1", 16, 16), ("c2", 16, 32), ("c3", 32, 64), ("c4", 64, 128),
 C0 = 16  # input feature channels
 
 
-def _pool_fused(d: int, ci: int, co: int) -> bool:
+def _pool_fused(d: int, ci: int, co: int, B: int) -> bool:
     """Whether the data gradient of the layer of side d that reads the upsampled
     input (ci input channels of the layer, co output) comes out already summed over
-    each cell's children (gp5_nn_conv3d_pool: 16 gradient channels in, a multiple
-    of 32 out, side 64 or 128)."""
-    return d in (64, 128) and co == 16 and ci % 32 == 0
+    each cell's children (gp5_nn_conv3d_pool: co = 16 gradient channels in with ci a
+    multiple of 32 out, or co = 32; side 32, 64 or 128; enough workgroups for the
+    z-sliding kernel)."""
+    if d not in (32, 64, 128) or not ((co == 16 and ci % 32 == 0) or co == 32):
+        return False
+    nbz = 2 if co == 16 else 1
+    per_z = B * (d // (256 // d)) * (ci // 16 // nbz)
+    return per_z * (d // 2) >= 512
 
 
 def _up_fused(d: int) -> bool:
@@ -201,8 +206,10 @@ class UNet:
         for name, d in (("u4", D // 4), ("u3", D // 2), ("u2", D)):
             if _up_fused(d):  # read in place from the half-side tensor
                 del sizes[name]
-        if _pool_fused(D, 32, 16):  # c7's data gradient arrives summed over the children
+        if _pool_fused(D, 32, 16, B):  # c7's and c6's data gradients arrive summed over the children
             del sizes["gu32"]
+        if _pool_fused(D // 2, 64, 32, B):
+            del sizes["gu64"]
         self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
         L = _lib()
         ws = max(L.gp5_nn_wgrad_ws(B, d, ci, co) for d, ci, co in
@@ -363,7 +370,7 @@ class UNet:
         # c7: dz7 = dd1 * act'(a7)
         self._act_bwd("c7", "a7", "g16a", "h16", D)
         self._wgrad_up("c7", "d2", "u2", t["h16"], D)
-        if _pool_fused(D, 32, 16):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
+        if _pool_fused(D, 32, 16, B):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
             e = self.w["c7"]
             _ck(L.gp5_nn_conv3d_pool(t["h16"].ptr, e["Wb"].ptr, t["g32a"].ptr, B, D, e["co"], e["ci"], 1))
         else:
@@ -372,8 +379,12 @@ class UNet:
         # c6
         self._act_bwd("c6", "a6", "g32a", "h32", D // 2)
         self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2)
-        self._dgrad("c6", t["h32"], t["gu64"], D // 2)
-        _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
+        if _pool_fused(D // 2, 64, 32, B):  # dd3 = de3 (skip part)
+            e = self.w["c6"]
+            _ck(L.gp5_nn_conv3d_pool(t["h32"].ptr, e["Wb"].ptr, t["g64a"].ptr, B, D // 2, e["co"], e["ci"], 1))
+        else:
+            self._dgrad("c6", t["h32"], t["gu64"], D // 2)
+            _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
         # c5
         self._act_bwd("c5", "a5", "g64a", "h64", D // 4)
         self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4)

@@ -44,8 +44,8 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
  * (B*(D/2)^3*Cout floats) equals gp5_nn_conv3d into Yf followed by
  * gp5_nn_pool2_f32(Yf, Yh, B, D, Cout, 8), bit for bit, without writing Yf (the
  * data gradient of a layer that reads the nearest upsampling of its input).
- * Cin = 16, Cout a multiple of 32, D = 64 or 128; other shapes return
- * GP5_ERR_ARG. */
+ * Cin = 16 with Cout a multiple of 32, or Cin = 32; D = 32, 64 or 128; other
+ * shapes return GP5_ERR_ARG. */
 int gp5_nn_conv3d_pool(const void *X, const void *Wk, float *Yh, int B, int D, int Cin, int Cout, int mirror);
 /* The output layer's form: no activation, and the float output Yf0 keeps only
  * channel 0 (one float per position, B*D^3 floats); the other channels are
@@ -56,10 +56,9 @@ int gp5_nn_conv3d_out0(const void *X, const void *Wk, const float *bias, float *
 int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A, const void *skip, void *Y, float *Yf,
                      int B, int D, int Cin, int Cout, int mirror, int act);
 /* dW[tap][ci][co] = sum over positions of X[p + off(tap)][ci] dY[p][co]
- * (bf16 inputs, float result); ws: device workspace of at least
- * gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes (channel-major copies of X and dY, in
- * runs of 8 positions per channel when D is a multiple of 8, and the partial
- * sums). */
+ * (bf16 inputs, float result, summed in a fixed order); ws: device workspace of
+ * at least gp5_nn_wgrad_ws(B, D, Cin, Cout) bytes (per-workgroup partial sums;
+ * for cube sides outside 16..128 also channel-major copies of X and dY). */
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout);
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
 /* The weight gradient of gp5_nn_conv3d_up: X is Xh (side D/2), read upsampled;

@@ -134,9 +134,10 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, None, 0.125, fake, fake, 1, 8, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, fake, 0.125, fake, fake, 1, 7, 16) == capi.GP5_ERR_ARG
     assert b"even" in L.gp5_last_error()
-    # conv3d_pool: 16 input channels, a multiple of 32 output channels, side 64 or 128
-    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 32, 16, 32, 1) == capi.GP5_ERR_ARG
-    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 32, 32, 1) == capi.GP5_ERR_ARG
+    # conv3d_pool: 16 input channels with a multiple of 32 outputs, or 32 inputs; side 32, 64 or 128
+    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 16, 16, 32, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 16, 16, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 64, 32, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, None, 1, 64, 16, 32, 1) == capi.GP5_ERR_ARG
     # conv3d: act 0 or 1 only; out0 needs its output
     assert L.gp5_nn_conv3d(fake, fake, None, None, None, None, fake, 1, 8, 16, 16, 0, 2) == capi.GP5_ERR_ARG

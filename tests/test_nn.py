@@ -129,16 +129,16 @@ def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("D,Cout,B", [(64, 32, 2), (128, 32, 1), (64, 64, 1)])
-def test_conv_pool_equals_conv_then_pool(D, Cout, B):
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 32, 2), (128, 16, 32, 1), (64, 16, 64, 1), (32, 32, 64, 4),
+                                          (64, 32, 32, 1)])
+def test_conv_pool_equals_conv_then_pool(D, Cin, Cout, B):
     """gp5_nn_conv3d_pool (the data gradient summed over each cell's 2x2x2
     children, mirrored taps) equals gp5_nn_conv3d into a float tensor followed by
     gp5_nn_pool2_f32 with scale 8, bit for bit, and agrees with the oracle."""
-    rng = np.random.default_rng(D + Cout + B)
+    rng = np.random.default_rng(D + Cin + Cout + B)
     L = nn._lib()
-    Cin = 16
     G = _rb(rng.standard_normal((B, D, D, D, Cin)))
-    W = _rb(rng.standard_normal((27, Cout, Cin)) * 0.1)  # [27][ci][co] of the layer: its Cout = 16 is Cin here
+    W = _rb(rng.standard_normal((27, Cout, Cin)) * 0.1)  # [27][ci][co] of the layer: its co is Cin here
     gb, wb = _buf(G, True), _buf(W, True)
     full = nn.DevBuf(B * D ** 3 * Cout * 4)
     ref_h, got_h = nn.DevBuf(B * D ** 3 * Cout // 2), nn.DevBuf(B * D ** 3 * Cout // 2)
