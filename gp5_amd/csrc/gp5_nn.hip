@@ -1266,6 +1266,36 @@ __global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__res
 
 // one block per channel: each thread sums a strided subset of the block partials,
 // then a fixed tree over the threads
+// the block partials of one channel in a fixed order: each thread a strided
+// subset, then a fixed tree over the threads
+__device__ __forceinline__ float bias_channel_sum(const float *part, int nblk, int C, int c)
+{
+    __shared__ float s[256];
+    float t = 0.f;
+    for (int b = threadIdx.x; b < nblk; b += 256) t += part[(size_t)b * C + c];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    return s[0];
+}
+
+// gp5_nn_bias_final_multi: workgroup b finishes channel b of the segment whose
+// channel range holds b (the same sum as k_bias_final)
+struct BiasMulti {
+    gp5_nn_bias_seg s[GP5_NN_BIAS_MAX_SEGS];
+    int n;
+};
+__global__ __launch_bounds__(256) void k_bias_final_multi(BiasMulti a)
+{
+    int b = blockIdx.x, k = 0;
+    while (k + 1 < a.n && b >= a.s[k].C) b -= a.s[k++].C;  // block-uniform
+    const float v = bias_channel_sum(a.s[k].part, kBiasBlocks, a.s[k].C, b);
+    if (threadIdx.x == 0) a.s[k].db[b] = v;
+}
+
 __global__ __launch_bounds__(256) void k_bias_final(const float *part, float *db, int nblk, int C)
 {
     __shared__ float s[256];
@@ -2088,17 +2118,22 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
     return GP5_OK;
 }
 
+// db set: the bias gradient is finished here (partials in the shared scratch);
+// else the partials go to the caller's `part` (kBiasBlocks * C floats)
 static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
-                            long long P, int C)
+                            long long P, int C, float *out_part = nullptr)
 {
-    if (!A || !dA || !H || !db) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (!A || !dA || !H || (!db && !out_part)) return set_error(GP5_ERR_ARG, "NULL argument");
     if (C < 4 || C % 4 || 256 % (C / 4)) return set_error(GP5_ERR_ARG, "act_bwd_cast_bias: C / 4 must divide 256");
-    float *part = nullptr;
-    const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
-    if (rc != GP5_OK) return rc;
+    float *part = out_part;
+    if (!part) {
+        const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
+        if (rc != GP5_OK) return rc;
+    }
     hipLaunchKernelGGL(k_act_bwd_cast_bias, dim3(kBiasBlocks), dim3(256), 0, 0, (const uint16_t *)A, dA, (uint16_t *)H,
                        part, P, C, G, scale, D);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
+    if (!db) return GP5_OK;
     hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     return GP5_OK;
@@ -2117,6 +2152,39 @@ int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float
     if ((long long)B * D * D * D * C >= 0x7fffffffLL)
         return set_error(GP5_ERR_ARG, "up2_add_act_bwd_cast_bias: tensors must stay below 2^31 elements");
     return act_bwd_bias_run(A, dA, G, scale, D, H, db, (long long)B * D * D * D, C);
+}
+
+size_t gp5_nn_bias_part_floats(int C)
+{
+    return (size_t)kBiasBlocks * (size_t)(C > 0 ? C : 0);
+}
+
+int gp5_nn_act_bwd_part(const void *A, const float *dA, const float *G, float scale, void *H, float *part, int B, int D,
+                        int C)
+{
+    if (!part) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (B < 1 || D < 1 || (G && D % 2)) return set_error(GP5_ERR_ARG, "act_bwd_part: D must be even when G is set");
+    if ((long long)B * D * D * D * C >= 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "act_bwd_part: tensors must stay below 2^31 elements");
+    return act_bwd_bias_run(A, dA, G, scale, D, H, nullptr, (long long)B * D * D * D, C, part);
+}
+
+int gp5_nn_bias_final_multi(const gp5_nn_bias_seg *segs, int nseg)
+{
+    if (!segs || nseg < 1 || nseg > GP5_NN_BIAS_MAX_SEGS)
+        return set_error(GP5_ERR_ARG, "bias_final_multi: 1 .. GP5_NN_BIAS_MAX_SEGS segments");
+    BiasMulti a{};
+    a.n = nseg;
+    int blocks = 0;
+    for (int k = 0; k < nseg; ++k) {
+        if (!segs[k].part || !segs[k].db || segs[k].C < 1)
+            return set_error(GP5_ERR_ARG, "bias_final_multi: incomplete segment");
+        a.s[k] = segs[k];
+        blocks += segs[k].C;
+    }
+    hipLaunchKernelGGL(k_bias_final_multi, dim3(blocks), dim3(256), 0, 0, a);
+    NNCHK(hipGetLastError(), "bias_final_multi");
+    return GP5_OK;
 }
 
 int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale)

@@ -69,6 +69,8 @@ def _lib():
         "gp5_nn_mse_out0": [P, P, P, P, I, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
+        "gp5_nn_act_bwd_part": [P, P, P, F, P, P, I, I, I],
+        "gp5_nn_bias_final_multi": [P, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_up2_add_act_bwd_cast_bias": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
@@ -84,6 +86,8 @@ def _lib():
         fn.restype = I
     L.gp5_nn_wgrad_ws.argtypes = [I, I, I, I]
     L.gp5_nn_wgrad_ws.restype = S
+    L.gp5_nn_bias_part_floats.argtypes = [I]
+    L.gp5_nn_bias_part_floats.restype = S
     L._gp5_nn_sigs = True
     return L
 
@@ -97,6 +101,11 @@ class AdamSeg(ctypes.Structure):
     _fields_ = [("p", ctypes.c_void_p), ("g", ctypes.c_void_p), ("m", ctypes.c_void_p), ("v", ctypes.c_void_p),
                 ("wf", ctypes.c_void_p), ("wb", ctypes.c_void_p), ("n", ctypes.c_longlong), ("cin", ctypes.c_int),
                 ("cout", ctypes.c_int)]
+
+
+class BiasSeg(ctypes.Structure):
+    """gp5_nn_bias_seg (include/gp5_nn.h)."""
+    _fields_ = [("part", ctypes.c_void_p), ("db", ctypes.c_void_p), ("C", ctypes.c_int)]
 
 
 class DevBuf:
@@ -216,6 +225,9 @@ class UNet:
                  [(D, C0, 16), (D // 2, 16, 32), (D // 4, 32, 64), (D // 8, 64, 128), (D // 4, 128, 64),
                   (D // 2, 64, 32), (D, 32, 16), (D, 16, 16)])
         self.ws = DevBuf(ws)
+        # bias-gradient block partials of c1..c7, finished together at the end of backward
+        self.bpart = {name: DevBuf(L.gp5_nn_bias_part_floats(co) * 4) for name, ci, co in SHAPES if name != "c8"}
+        self._bsegs = None
 
     def _relayout(self):
         L = _lib()
@@ -337,18 +349,24 @@ class UNet:
     def _act_bwd(self, name, a, g, h, D):
         """Backward of layer name's leaky ReLU from its output t[a] and the incoming
         gradient t[g] (float): t[h] = bf16 gradient of the pre-activation, and the
-        layer's bias gradient, in one pass."""
-        e = self.w[name]
-        _ck(_lib().gp5_nn_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[h].ptr, e["bg"].ptr,
-                                            self.B * D ** 3, e["co"]))
+        block partials of the layer's bias gradient (finished by _bias_final), in one
+        pass."""
+        _ck(_lib().gp5_nn_act_bwd_part(self.t[a].ptr, self.t[g].ptr, None, 0.0, self.t[h].ptr,
+                                       self.bpart[name].ptr, self.B, D, self.w[name]["co"]))
 
     def _up_add_act_bwd(self, name, a, g, gh, h, D):
         """_act_bwd with the incoming gradient t[g] + t[gh] / 8 at the parent cell
         (t[gh] of side D/2: the adjoint of the pooling that fed the next level), in
         one pass that does not write t[g] back."""
-        e = self.w[name]
-        _ck(_lib().gp5_nn_up2_add_act_bwd_cast_bias(self.t[a].ptr, self.t[g].ptr, self.t[gh].ptr, 0.125,
-                                                    self.t[h].ptr, e["bg"].ptr, self.B, D, e["co"]))
+        _ck(_lib().gp5_nn_act_bwd_part(self.t[a].ptr, self.t[g].ptr, self.t[gh].ptr, 0.125, self.t[h].ptr,
+                                       self.bpart[name].ptr, self.B, D, self.w[name]["co"]))
+
+    def _bias_final(self):
+        """The bias gradients of c1..c7 from their block partials, one launch."""
+        if self._bsegs is None:
+            segs = [BiasSeg(self.bpart[n].ptr, self.w[n]["bg"].ptr, self.w[n]["co"]) for n in self.bpart]
+            self._bsegs = (BiasSeg * len(segs))(*segs)
+        _ck(_lib().gp5_nn_bias_final_multi(ctypes.cast(self._bsegs, ctypes.c_void_p), len(self._bsegs)))
 
     def _dgrad(self, name, dz_bf, out_f, D):
         # data gradient: forward kernel on the [27][ci][co] copy with mirrored taps
@@ -405,6 +423,7 @@ class UNet:
         # c1 (de1 total)
         self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16", D)
         self._wgrad("c1", t["X"], t["h16"], D)
+        self._bias_final()
         return float(loss.value)
 
     def _adam_segs(self):

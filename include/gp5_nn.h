@@ -86,6 +86,20 @@ int gp5_nn_act_bwd_cast_bias(const void *A, const float *dA, void *H, float *db,
  * by gp5_nn_act_bwd_cast_bias, bit for bit, without writing dA. */
 int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float *G, float scale, void *H, float *db,
                                      int B, int D, int C);
+/* The activation backward of the two calls above (G may be NULL) with the bias
+ * gradient left as block partials in `part` (device, gp5_nn_bias_part_floats(C)
+ * floats), so that several layers' bias gradients finish in one launch of
+ * gp5_nn_bias_final_multi (same values as the calls above, bit for bit). */
+size_t gp5_nn_bias_part_floats(int C);
+int gp5_nn_act_bwd_part(const void *A, const float *dA, const float *G, float scale, void *H, float *part, int B, int D,
+                        int C);
+typedef struct gp5_nn_bias_seg {
+    const float *part; /* partials of gp5_nn_act_bwd_part */
+    float *db;         /* C floats: the bias gradient */
+    int C;
+} gp5_nn_bias_seg;
+#define GP5_NN_BIAS_MAX_SEGS 16
+int gp5_nn_bias_final_multi(const gp5_nn_bias_seg *segs, int nseg);
 /* Y (side 2D) += scale * X at the parent cell (the adjoint of pooling added into
  * Y); C a multiple of 4. */
 int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scale);

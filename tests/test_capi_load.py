@@ -139,6 +139,11 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 16, 16, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 64, 32, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, None, 1, 64, 16, 32, 1) == capi.GP5_ERR_ARG
+    # act_bwd_part needs its partial buffer; bias_final_multi 1..16 complete segments
+    assert L.gp5_nn_act_bwd_part(fake, fake, None, 0.0, fake, None, 1, 8, 16) == capi.GP5_ERR_ARG
+    bseg = nn.BiasSeg(fake, None, 16)
+    assert L.gp5_nn_bias_final_multi(ctypes.byref(bseg), 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_bias_final_multi(ctypes.byref(bseg), 0) == capi.GP5_ERR_ARG
     # conv3d: act 0 or 1 only; out0 needs its output
     assert L.gp5_nn_conv3d(fake, fake, None, None, None, None, fake, 1, 8, 16, 16, 0, 2) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_out0(fake, fake, None, None, 1, 8, 16, 16) == capi.GP5_ERR_ARG

@@ -122,6 +122,14 @@ def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
     assert np.array_equal(h1, H2.download(np.uint16, dA.shape))
     assert np.array_equal(db1.download(np.float32, (C,)), db2.download(np.float32, (C,)))
     assert np.array_equal(da1.download(np.float32, dA.shape), dA)  # dA is not written
+    # the same with the bias partials kept and finished in one multi-segment launch
+    H3, db3 = nn.DevBuf(dA.size * 2), nn.DevBuf(C * 4)
+    part = nn.DevBuf(L.gp5_nn_bias_part_floats(C) * 4)
+    nn._ck(L.gp5_nn_act_bwd_part(ab.ptr, da1.ptr, gb.ptr, 0.125, H3.ptr, part.ptr, B, D, C))
+    seg = (nn.BiasSeg * 1)(nn.BiasSeg(part.ptr, db3.ptr, C))
+    nn._ck(L.gp5_nn_bias_final_multi(ctypes.cast(seg, ctypes.c_void_p), 1))
+    assert np.array_equal(H3.download(np.uint16, dA.shape), h1)
+    assert np.array_equal(db3.download(np.float32, (C,)), db1.download(np.float32, (C,)))
     tot = dA.astype(np.float64) + 0.125 * R.up2(G.astype(np.float64))
     ref = tot * R.lrelu_grad(A)
     assert np.max(np.abs(nn.from_bf16(h1) - ref)) <= 2.0 ** -8 * np.max(np.abs(ref))
