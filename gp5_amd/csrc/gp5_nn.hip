@@ -368,11 +368,15 @@ __device__ __forceinline__ Frag<LV> lds_frag(const unsigned char *sx, int off)
 // accumulation order as k_conv (chunk, tap, step), so for the same KC the two
 // kernels give bit-identical results.  With up = 1, X has side D/2 and each
 // staged cell is read from its parent cell (the nearest upsampling of X).
+//
+// Split over channel chunks (csplit > 0, grid z = the split): workgroup z takes
+// the chunks [z csplit, (z + 1) csplit) and writes its raw float sums (no bias,
+// no activation) to Yf + z P Cout; k_conv_split_finish adds the splits in order.
 template <int KC, int NB, int PD>
 __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                   const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                   uint16_t *Y, float *Yf, int P, int D, int Cin, int Cout, int mirror,
-                                                  int act, int Rb, int PZ, int up)
+                                                  int act, int Rb, int PZ, int up, int csplit)
 {
     constexpr int MT = 4, LV = KC / 4, U8 = KC / 8;  // U8: 16-byte units per staged position
     extern __shared__ __align__(16) unsigned char sx[];
@@ -412,8 +416,9 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
 #pragma unroll
         for (int j = 0; j < NB; ++j) b[slot][j] = bld<LV>(wr, wvo, wso + j * 16 * Cin * 2);
     };
-    for (int c0 = 0; c0 < Cin; c0 += KC) {
-        if (c0) __syncthreads();  // every wave is done with the previous chunk
+    const int cb = csplit ? (int)blockIdx.z * csplit * KC : 0, ce = csplit ? cb + csplit * KC : Cin;
+    for (int c0 = cb; c0 < ce; c0 += KC) {
+        if (c0 != cb) __syncthreads();  // every wave is done with the previous chunk
         // keeps the 27 MT tap offsets from being hoisted out of this loop into registers
 #pragma unroll
         for (int t = 0; t < MT; ++t)
@@ -465,7 +470,52 @@ __global__ __launch_bounds__(256) void k_conv_lds(const uint16_t *__restrict__ X
         }
     }
     __syncthreads();  // the staged region becomes the epilogue scratch
-    conv_epilogue_lds<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf, sx + wave * kEpiScratch);
+    if (csplit)
+        conv_epilogue_lds<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, nullptr, 0, nullptr, nullptr, nullptr,
+                                  Yf + (size_t)blockIdx.z * P * Cout, sx + wave * kEpiScratch);
+    else
+        conv_epilogue_lds<NB, MT>(acc, p0 + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y, Yf,
+                                  sx + wave * kEpiScratch);
+}
+
+// The splits of k_conv_lds added in order (z = 0, 1, ...), then the epilogue of
+// conv_epilogue: bias, activation (act bit 0; bit 1 keeps channel 0 of Yf), the
+// float output Yf, the bf16 output A and Y (plus skip); 4 channels per thread.
+__global__ __launch_bounds__(256) void k_conv_split_finish(const float *__restrict__ part, int splits, int P, int Cout,
+                                                           const float *__restrict__ bias, int act, uint16_t *A,
+                                                           const uint16_t *skip, uint16_t *Y, float *Yf)
+{
+    const bool yf0 = (act & kActYf0) != 0;
+    act &= 1;
+    const int C4 = Cout / 4;
+    const long long n = (long long)P * C4;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        float4 v = reinterpret_cast<const float4 *>(part)[i];
+        for (int z = 1; z < splits; ++z) {
+            const float4 w = reinterpret_cast<const float4 *>(part + (size_t)z * P * Cout)[i];
+            v = make_float4(v.x + w.x, v.y + w.y, v.z + w.z, v.w + w.w);
+        }
+        const int c = (int)(i % C4) * 4;
+        float o[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[k] += bias ? bias[c + k] : 0.f;
+            if (act == 1) o[k] = o[k] > 0.f ? o[k] : kLeak * o[k];
+        }
+        if (Yf && !yf0) reinterpret_cast<float4 *>(Yf)[i] = make_float4(o[0], o[1], o[2], o[3]);
+        if (Yf && yf0 && c == 0) Yf[i / C4] = o[0];
+        uint16_t h[4] = {f2bf(o[0]), f2bf(o[1]), f2bf(o[2]), f2bf(o[3])};
+        if (A) reinterpret_cast<uint2 *>(A)[i] = __builtin_bit_cast(uint2, h);
+        if (Y) {
+            if (skip) {
+                const uint2 s2 = reinterpret_cast<const uint2 *>(skip)[i];
+                const uint16_t *sk = reinterpret_cast<const uint16_t *>(&s2);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) h[k] = f2bf(o[k] + bf2f(sk[k]));
+            }
+            reinterpret_cast<uint2 *>(Y)[i] = __builtin_bit_cast(uint2, h);
+        }
+    }
 }
 
 // slab units (16 bytes) per thread of k_conv_zs: a slab has at most
@@ -1746,6 +1796,25 @@ int gp5_nn_sync(void)
 
 // up = 1: X has side D/2 and the convolution reads its nearest upsampling (only
 // the LDS-staged and z-sliding kernels do that)
+// The float partial sums of split convolutions, in a process-wide buffer that
+// only grows (the kernels of successive calls run in order on the null stream).
+static int conv_split_scratch(size_t floats, float **out)
+{
+    static std::mutex mu;
+    static float *buf = nullptr;
+    static size_t cap = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cap < floats) {
+        if (buf) NNCHK(hipFree(buf), "conv split scratch");
+        buf = nullptr;
+        cap = 0;
+        NNCHK(hipMalloc(&buf, floats * sizeof(float)), "conv split scratch");
+        cap = floats;
+    }
+    *out = buf;
+    return GP5_OK;
+}
+
 // pool = 1: Yf (side D/2) receives the 2x2x2 child sums of the convolution (the
 // POOL form of k_conv_zs; only where that kernel applies, else an error)
 static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y,
@@ -1855,16 +1924,39 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             int NBl = 1;
             for (int nb : {4, 2})
                 if (NBl == 1 && nblk % nb == 0 && wgs * (nblk / nb) >= 1024) NBl = nb;
-            const dim3 grid((unsigned)wgs, (unsigned)(nblk / NBl));
+            // fewer workgroups than CUs: split the channel chunks S ways (S a power of two
+            // dividing the chunk count, up to 256 workgroups), the splits' float sums added
+            // in order by k_conv_split_finish; GP5_CONV_SPLIT=0 turns it off.  Measured at
+            // 64^3 batch 8: c4 (8^3) data gradient 28.1 -> 15.3 us with 4 splits, forward
+            // 17.0 -> 15.2 us with 2; with 256-512 workgroups the split lost (c3 data
+            // gradient 19.0 -> 22.2 us, c5 forward 41.2 -> 51.6 us)
+            const char *sp_env = getenv("GP5_CONV_SPLIT");
+            const int chunks = Cin / KCl;
+            int S = 1;
+            if (!(sp_env && sp_env[0] == '0'))
+                while (chunks % (2 * S) == 0 && wgs * (nblk / NBl) * S < 256) S *= 2;
+            float *ws = nullptr;
+            if (S > 1) {
+                const int rc2 = conv_split_scratch((size_t)S * P * Cout, &ws);
+                if (rc2 != GP5_OK) return rc2;
+            }
+            const dim3 grid((unsigned)wgs, (unsigned)(nblk / NBl), (unsigned)S);
             const size_t shm = (size_t)region * KCl * 2;
+            const int csplit = S > 1 ? chunks / S : 0;
+            float *yf_k = S > 1 ? ws : Yf;
 #define GP5_CONV_LDS_CASE(C, K)                                                                                   \
     if (KCl == C && NBl == K)                                                                                     \
-        hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, 0, x, w, bias, a, sk, y, Yf, (int)P, D, Cin, \
-                           Cout, mirror, act, Rb, PZ, up);
+        hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, 0, x, w, bias, a, sk, y, yf_k, (int)P, D,    \
+                           Cin, Cout, mirror, act, Rb, PZ, up, csplit);
             GP5_CONV_LDS_CASE(16, 1) GP5_CONV_LDS_CASE(16, 2) GP5_CONV_LDS_CASE(16, 4)
             GP5_CONV_LDS_CASE(32, 1) GP5_CONV_LDS_CASE(32, 2) GP5_CONV_LDS_CASE(32, 4)
 #undef GP5_CONV_LDS_CASE
             NNCHK(hipGetLastError(), "conv3d launch");
+            if (S > 1) {
+                hipLaunchKernelGGL(k_conv_split_finish, dim3(grid_for(P * (Cout / 4))), dim3(256), 0, 0, ws, S, (int)P,
+                                   Cout, bias, act, a, sk, y, Yf);
+                NNCHK(hipGetLastError(), "conv3d split finish");
+            }
             return GP5_OK;
         }
     }

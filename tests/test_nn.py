@@ -378,6 +378,50 @@ def test_conv_z_sliding_path(D, Cin, Cout, B, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B,up,mirror", [(8, 64, 128, 8, 0, 0), (8, 128, 64, 8, 0, 1),
+                                                    (16, 64, 32, 2, 0, 1), (16, 128, 64, 1, 1, 0),
+                                                    (8, 64, 32, 2, 0, 0)])
+def test_conv_split_over_channel_chunks(D, Cin, Cout, B, up, mirror, monkeypatch):
+    """Small cubes split the LDS-staged convolution over channel chunks (the splits'
+    float sums added in order by a finishing pass, with bias, activation, skip and
+    the bf16 outputs there).  Same results as the unsplit kernel (GP5_CONV_SPLIT=0)
+    to float rounding, and as the oracle."""
+    rng = np.random.default_rng(D * 5 + Cin + Cout + B + up)
+    L = nn._lib()
+    Ds = D // 2 if up else D
+    X = _rb(rng.standard_normal((B, Ds, Ds, Ds, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    skip = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    xb, bb, sb = _buf(X, True), _buf(b), _buf(skip, True)
+    # forward: the [27][co][ci] copy of W; data gradient (mirror): the [27][ci][co] layout of the layer
+    # Wl = W^T (ci = this call's Cout), which is the same array
+    wk = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True)
+    f = L.gp5_nn_conv3d_up if up else L.gp5_nn_conv3d
+    shape = (B, D, D, D, Cout)
+    outs = []
+    for split in ("1", "0"):
+        monkeypatch.setenv("GP5_CONV_SPLIT", split)
+        Yf, A, Y = nn.DevBuf(B * D ** 3 * Cout * 4), nn.DevBuf(B * D ** 3 * Cout * 2), nn.DevBuf(B * D ** 3 * Cout * 2)
+        if mirror:
+            nn._ck(f(xb.ptr, wk.ptr, None, None, None, None, Yf.ptr, B, D, Cin, Cout, 1, 0))
+        else:
+            nn._ck(f(xb.ptr, wk.ptr, bb.ptr, A.ptr, sb.ptr, Y.ptr, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        outs.append((Yf.download(np.float32, shape), A.download(np.uint16, shape), Y.download(np.uint16, shape)))
+    scale = np.max(np.abs(outs[1][0]))
+    assert np.max(np.abs(outs[0][0] - outs[1][0])) <= 1e-6 * scale
+    if mirror:
+        # data gradient of a layer with Cout input channels: the oracle's conv_dgrad with W^T
+        ref = R.conv_dgrad(X, np.ascontiguousarray(W.transpose(0, 2, 1)))
+    else:
+        U = X.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3) if up else X
+        ref = R.lrelu(R.conv(U, W, b.astype(np.float64)))
+        for k in (1, 2):  # bf16 outputs: within one rounding step of each other
+            assert np.max(np.abs(nn.from_bf16(outs[0][k]) - nn.from_bf16(outs[1][k]))) <= 2.0 ** -7 * scale
+    assert np.max(np.abs(outs[0][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 16, 2), (64, 32, 16, 1), (32, 64, 32, 1), (128, 16, 16, 1),
                                           (16, 128, 64, 2), (16, 16, 32, 1)])
 def test_wgrad_z_sliding_path(D, Cin, Cout, B, monkeypatch):
