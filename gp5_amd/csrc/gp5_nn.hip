@@ -552,7 +552,7 @@ template <int CIN, int NB, bool PF = false, bool WS = false, bool POOL = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                  uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
-                                                 int ZL, int up)
+                                                 int nzs, int up)
 {
     constexpr int MT = 4, LV = CIN / 4, U8 = CIN / 8;
     constexpr int NU = zs_units_per_thread(CIN);
@@ -563,9 +563,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int Rb = 256 / D, RY = Rb + 2, RX = D + 2;
     const int UR = RX * U8, SU = RY * UR;  // 16-byte units per slab row, per slab
     const float invUR = 1.f / (float)UR;
-    const int nyb = D / Rb, nzs = D / ZL;
+    const int nyb = D / Rb;
     const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
-    const int y0 = yb * Rb, zb = zsg * ZL;
+    // nzs runs of planes per cube side, ZL = D / nzs or one more plane per run
+    const int y0 = yb * Rb, zb = zsg * D / nzs, ZL = (zsg + 1) * D / nzs - zb;
     const int s0 = smp * D * D * D;
     // up = 1: X is stored at side Ds = D/2 and read at the parent cell (nearest upsampling)
     const int Ds = D >> up;
@@ -1046,7 +1047,7 @@ __host__ __device__ constexpr int wz_chstride(int elems)
 constexpr int kWzThreads = 192;
 
 __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
-                                                         float *ws, int P, int D, int Cin, int Cout, int ZL, int up)
+                                                         float *ws, int P, int D, int Cin, int Cout, int nzs, int up)
 {
     extern __shared__ __align__(16) unsigned char sx[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1058,9 +1059,9 @@ __global__ __launch_bounds__(kWzThreads) void k_wgrad_zs(const uint16_t *__restr
     uint16_t *LY = LX + 3 * SLOT;
     const int nco = Cout / 16;
     const int cib = (int)blockIdx.y / nco, cob = (int)blockIdx.y % nco;
-    const int nyb = D / Rb, nzs = D / ZL;
+    const int nyb = D / Rb;
     const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
-    const int y0 = yb * Rb, zb = zsg * ZL;
+    const int y0 = yb * Rb, zb = zsg * D / nzs, ZL = (zsg + 1) * D / nzs - zb;  // nzs runs of planes
     const int Ds = D >> up;
     const int sb = smp * Ds * Ds * Ds;  // stored position of this sample's origin (X)
     const int s0 = smp * D * D * D;     // position of this sample's origin (dY)
@@ -1729,20 +1730,41 @@ int check_channels(int c, const char *what)
 
 using namespace gp5;
 
+// Launch of k_conv_zs with `per_run` x `gy` workgroups per plane run.  The number of
+// plane runs per cube side is the one that fills the device's workgroup slots once
+// (CUs x resident workgroups of this instance), between 1 and D/2, and for the
+// pooled form one that pairs planes; `runs` > 0 overrides it.  Measured at 64^3,
+// batch 8 (one full round of workgroups against the 1024-workgroup rule of before):
+// 16 -> 16 62.3 -> 57.9 us at 6 runs, 32 -> 16 upsampled 91.1 -> 85.4 us at 4 runs.
 template <int C, int K, bool W, bool S, bool PL = false>
-static int zs_launch(dim3 grid, size_t shm, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a,
-                     const uint16_t *sk, uint16_t *y, float *Yf, int P, int D, int Cout, int mirror, int act, int ZL,
-                     int up)
+static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const uint16_t *w, const float *bias,
+                     uint16_t *a, const uint16_t *sk, uint16_t *y, float *Yf, int P, int D, int Cout, int mirror,
+                     int act, int runs, int up)
 {
     static size_t shm_set = 65536;  // the default limit of dynamic LDS
+    const void *fn = (const void *)k_conv_zs<C, K, W, S, PL>;
     if (shm > shm_set) {
-        NNCHK(hipFuncSetAttribute((const void *)k_conv_zs<C, K, W, S, PL>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)shm),
-              "conv3d LDS size");
+        NNCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm), "conv3d LDS size");
         shm_set = shm;
     }
-    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), grid, dim3(256), shm, 0, x, w, bias, a, sk, y, Yf, P, D, Cout, mirror,
-                       act, ZL, up);
+    int nzs = runs;
+    if (nzs <= 0) {
+        static size_t occ_shm = 0;
+        static int slots = 0;
+        if (occ_shm != shm) {
+            int dev = 0, cus = 0, per_cu = 0;
+            NNCHK(hipGetDevice(&dev), "conv3d device");
+            NNCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "conv3d CU count");
+            NNCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, shm), "conv3d occupancy");
+            slots = std::max(1, cus * per_cu);
+            occ_shm = shm;
+        }
+        nzs = std::max(1, std::min(D / 2, slots / (per_run * gy)));
+        if (PL)
+            while (nzs > 1 && D % (2 * nzs)) --nzs;
+    }
+    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, 0, x,
+                       w, bias, a, sk, y, Yf, P, D, Cout, mirror, act, nzs, up);
     return GP5_OK;
 }
 
@@ -1868,7 +1890,13 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         if (!ZL && per_z * (D / 2) >= 512) ZL = 2;
         const int SU = (Rb + 2) * (D + 2) * (Cin / 8);
         if (ZL && SU <= 256 * zs_units_per_thread(Cin) && SU * 16 >= 4 * kEpiScratch) {
-            const dim3 grid((unsigned)(per_z / (nblk / NBz) * (D / ZL)), (unsigned)(nblk / NBz));
+            // plane runs per cube side: from the occupancy in zs_launch, or GP5_ZS_RUNS
+            // (measurement; for the pooled form only counts whose runs pair planes)
+            int nzs = 0;
+            const char *runs = getenv("GP5_ZS_RUNS");
+            if (runs && atoi(runs) >= 1 && atoi(runs) <= D / 2 && (!pool || (D % (2 * atoi(runs)) == 0)))
+                nzs = atoi(runs);
+            const int per_run = (int)(per_z / (nblk / NBz)), gy = nblk / NBz;
             const size_t shm = (size_t)3 * SU * 16;
             // GP5_CONV_ZSV overrides the variant: bit 0 wave-private epilogue barriers, bit 1
             // fragment reads one tap ahead.  Defaults (measured at 64^3, batch 8): both for
@@ -1878,8 +1906,8 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             const int Vz = v_env ? atoi(v_env) & 3 : (Cin == 16 && NBz == 1 ? 3 : 1);
 #define GP5_CONV_ZS_CASE(C, K, V)                                                                                \
     if (Cin == C && NBz == K && Vz == V)                                                                         \
-        zrc = zs_launch<C, K, (V & 2) != 0, (V & 1) != 0>(grid, shm, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, mirror, \
-                                                          act, ZL, up);
+        zrc = zs_launch<C, K, (V & 2) != 0, (V & 1) != 0>(per_run, gy, shm, x, w, bias, a, sk, y, Yf, (int)P, D, Cout, \
+                                                          mirror, act, nzs, up);
 #define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
     GP5_CONV_ZS_CASE(C, K, 3)
             int zrc = GP5_OK;
@@ -1888,11 +1916,11 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
                     return set_error(GP5_ERR_ARG, "conv3d_pool: needs 16 or 32 input channels, D = 32, 64 or 128, "
                                                   "and only the float output");
                 if (Cin == 16 && NBz == 2)
-                    zrc = zs_launch<16, 2, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
-                                                               (int)P, D, Cout, mirror, 0, ZL, up);
+                    zrc = zs_launch<16, 2, false, true, true>(per_run, gy, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
+                                                               (int)P, D, Cout, mirror, 0, nzs, up);
                 else if (Cin == 32)
-                    zrc = zs_launch<32, 1, false, true, true>(grid, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
-                                                               (int)P, D, Cout, mirror, 0, ZL, up);
+                    zrc = zs_launch<32, 1, false, true, true>(per_run, gy, shm, x, w, nullptr, nullptr, nullptr, nullptr, Yf,
+                                                               (int)P, D, Cout, mirror, 0, nzs, up);
                 else
                     return set_error(GP5_ERR_ARG, "conv3d_pool: 16 input channels need a multiple of 32 output "
                                                   "channels");
@@ -2008,23 +2036,29 @@ int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A,
 }
 
 // Plan of k_wgrad_zs for a cube side D (a power of two in [16, 128]): the longest
-// plane run ZL (a power of two, at least 2) that keeps 1024 workgroups; returns
+// plane run ZL (a power of two, at least 2) that keeps 512 workgroups; returns
 // the number of position blocks (the chunks of the reduction), 0 when the
 // kernel does not apply (or GP5_WGRAD_ZS=0 turns it off).
-static long long wgrad_zs_plan(int B, int D, int Cin, int Cout, int *ZL)
+static long long wgrad_zs_plan(int B, int D, int Cin, int Cout, int *nzs_out)
 {
     const char *env = getenv("GP5_WGRAD_ZS");
     if (env && env[0] == '0') return 0;
     if (D < 16 || D > 128 || (D & (D - 1)) != 0) return 0;
     const long long per_z = (long long)B * (D / (256 / D)), tiles = (long long)(Cin / 16) * (Cout / 16);
+    // 512 workgroups (two per CU) against 1024 before, measured at batch 8: 64^3 16 x 16
+    // 63.8 -> 57.7 us, 64^3 32 x 16 upsampled 103.6 -> 96.9, 32^3 16 x 32 36.1 -> 26.8,
+    // 16^3 128 x 64 upsampled 44.4 -> 38.2 (fewer, longer runs halve the overlap planes)
     int zl = 2;
     for (int c = D; c >= 2; c /= 2)
-        if (per_z * (D / c) * tiles >= 1024) {
+        if (per_z * (D / c) * tiles >= 512) {
             zl = c;
             break;
         }
-    *ZL = zl;
-    return per_z * (D / zl);
+    int nzs = D / zl;
+    const char *runs = getenv("GP5_WZ_RUNS");  // plane runs per cube side (measurement)
+    if (runs && atoi(runs) >= 1 && atoi(runs) <= D / 2) nzs = atoi(runs);
+    *nzs_out = nzs;
+    return per_z * nzs;
 }
 
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
@@ -2032,8 +2066,8 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
     const long long P = (long long)B * D * D * D;
     const long long chunk = wgrad_chunk(P, Cin, Cout);
     const long long chunks = (P + chunk - 1) / chunk;
-    int zl = 0;
-    const long long zchunks = wgrad_zs_plan(B, D, Cin, Cout, &zl);
+    int nzs = 0;
+    const long long zchunks = wgrad_zs_plan(B, D, Cin, Cout, &nzs);
     // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums;
     // or the partial sums of the position blocks of k_wgrad_zs
     const size_t n = (size_t)27 * Cin * Cout * sizeof(float);
@@ -2064,8 +2098,8 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
     if (rc != GP5_OK) return rc;
     const long long P = (long long)B * D * D * D;
     const long long n = 27LL * Cin * Cout;
-    int ZL = 0;
-    const long long zblocks = wgrad_zs_plan(B, D, Cin, Cout, &ZL);
+    int nzs = 0;
+    const long long zblocks = wgrad_zs_plan(B, D, Cin, Cout, &nzs);
     if (zblocks > 0) {
         const size_t shm = wgrad_zs_lds(D);
         static size_t shm_set = 65536;  // the default limit of dynamic LDS
@@ -2075,7 +2109,7 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
             shm_set = shm;
         }
         hipLaunchKernelGGL(k_wgrad_zs, dim3((unsigned)zblocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWzThreads),
-                           shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, ZL, up);
+                           shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, nzs, up);
         NNCHK(hipGetLastError(), "wgrad launch");
         return wgrad_reduce(ws, dW, n, (int)zblocks);
     }

@@ -502,3 +502,62 @@ def test_bias_gradient_entries(C):
     dP = dA * np.where(nn.from_bf16(A) > 0, np.float32(1.0), np.float32(0.1))
     assert np.array_equal(H.download(np.uint16, (P, C)), nn.to_bf16(dP))
     assert np.allclose(db.download(np.float32, (C,)), dP.astype(np.float64).sum(axis=0), rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B,up", [(64, 16, 16, 1, 0), (64, 32, 16, 1, 1), (32, 16, 32, 2, 0)])
+def test_plane_run_counts_leave_results_unchanged(D, Cin, Cout, B, up, monkeypatch):
+    """The number of plane runs per cube side of the z-sliding kernels (chosen from
+    the occupancy, overridden by GP5_ZS_RUNS / GP5_WZ_RUNS) includes counts that do
+    not divide the side (runs of unequal length).  The convolution sums each output
+    in the same order whatever the count: bit for bit the same outputs; the pooled
+    data gradient likewise over counts that pair planes; the weight gradient sums
+    per-run partials (float: agreement to rounding, and to the oracle)."""
+    rng = np.random.default_rng(D + Cin + 3 * Cout + B + up)
+    L = nn._lib()
+    S = D // 2 if up else D
+    X = _rb(rng.standard_normal((B, S, S, S, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    b = rng.standard_normal(Cout).astype(np.float32)
+    xb, gb, bb = _buf(X, True), _buf(G, True), _buf(b)
+    wf = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True)
+    conv = L.gp5_nn_conv3d_up if up else L.gp5_nn_conv3d
+    shape = (B, D, D, D, Cout)
+    outs, grads = [], []
+    for runs in (None, "3", "5", "7", str(D // 2)):
+        if runs is None:
+            monkeypatch.delenv("GP5_ZS_RUNS", raising=False)
+            monkeypatch.delenv("GP5_WZ_RUNS", raising=False)
+        else:
+            monkeypatch.setenv("GP5_ZS_RUNS", runs)
+            monkeypatch.setenv("GP5_WZ_RUNS", runs)
+        Yf = nn.DevBuf(B * D ** 3 * Cout * 4)
+        nn._ck(conv(xb.ptr, wf.ptr, bb.ptr, None, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        outs.append(Yf.download(np.float32, shape))
+        ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+        dW = nn.DevBuf(27 * Cin * Cout * 4)
+        wgrad = L.gp5_nn_conv3d_wgrad_up if up else L.gp5_nn_conv3d_wgrad
+        nn._ck(wgrad(xb.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+        grads.append(dW.download(np.float32, (27, Cin, Cout)))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    U = X.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3) if up else X
+    ref = R.conv_wgrad(U, G)
+    scale = np.max(np.abs(ref))
+    for g in grads:
+        assert np.max(np.abs(g - ref)) <= 1e-4 * scale
+    if Cin == 16 and Cout == 16 and not up:  # pooled data gradient (16 -> 32 channels of its layer)
+        Wp = _rb(rng.standard_normal((27, 32, 16)) * 0.1)
+        wp = _buf(Wp, True)
+        pooled = []
+        for runs in (None, "2", "4", "8"):
+            if runs is None:
+                monkeypatch.delenv("GP5_ZS_RUNS", raising=False)
+            else:
+                monkeypatch.setenv("GP5_ZS_RUNS", runs)
+            h = nn.DevBuf(B * D ** 3 * 32 // 2)
+            nn._ck(L.gp5_nn_conv3d_pool(gb.ptr, wp.ptr, h.ptr, B, D, 16, 32, 1))
+            pooled.append(h.download(np.float32, (B, D // 2, D // 2, D // 2, 32)))
+        for p in pooled[1:]:
+            assert np.array_equal(p, pooled[0])
