@@ -304,7 +304,7 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     t0 = time.perf_counter()
     for _ in range(steps):
         net.forward()
-        net.backward()
+        net.backward(read_loss=False)  # the loss stays on the device (summed there)
         net.step(1e-4)
     net.sync()
     train_s = (time.perf_counter() - t0) / steps

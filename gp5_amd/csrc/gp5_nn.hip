@@ -39,6 +39,11 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 constexpr float kLeak = 0.1f;
 constexpr int kActYf0 = 2;  // act flag of the convolutions: the float output keeps channel 0 only
 
+// The stream that the calling thread's launches and copies go to
+// (gp5_nn_set_stream; the null stream until set)
+thread_local hipStream_t g_stream = nullptr;
+inline hipStream_t nn_stream() { return g_stream; }
+
 // Positions are linear indices p = ((b*D + z)*D + y)*D + x.
 
 // 8-byte load of 4 consecutive bf16 (the caller guarantees 8-byte alignment:
@@ -1608,10 +1613,11 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, co
 }
 
 // one block: the block sums of k_mse added in a fixed order (double); the loss
-// (their mean) to loss[0], and the output layer's bias gradient (channel 0 the
-// gradient sum, the other C - 1 channels 0) to db when set
+// (their mean) to loss[0], also added to acc[0] when set, and the output layer's
+// bias gradient (channel 0 the gradient sum, the other C - 1 channels 0) to db
+// when set
 __global__ __launch_bounds__(256) void k_mse_final(const double *partial, int blocks, long long P, double *loss, float *db,
-                                                   int C)
+                                                   int C, double *acc)
 {
     __shared__ double s[256], sg[256];
     double a = 0.0, ag = 0.0;
@@ -1629,7 +1635,10 @@ __global__ __launch_bounds__(256) void k_mse_final(const double *partial, int bl
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) loss[0] = s[0] / (double)P;
+    if (threadIdx.x == 0) {
+        loss[0] = s[0] / (double)P;
+        if (acc) acc[0] += s[0] / (double)P;
+    }
     if (db)
         for (int c = threadIdx.x; c < C; c += 256) db[c] = c == 0 ? (float)sg[0] : 0.f;
 }
@@ -1763,7 +1772,7 @@ static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const u
         if (PL)
             while (nzs > 1 && D % (2 * nzs)) --nzs;
     }
-    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, 0, x,
+    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(), x,
                        w, bias, a, sk, y, Yf, P, D, Cout, mirror, act, nzs, up);
     return GP5_OK;
 }
@@ -1783,21 +1792,75 @@ int gp5_nn_free(void *p)
     return GP5_OK;
 }
 
+// Copies run in order on the thread's stream; upload and download return when
+// theirs is done.
 int gp5_nn_upload(void *dst, const void *host, size_t bytes)
 {
-    NNCHK(hipMemcpy(dst, host, bytes, hipMemcpyHostToDevice), "gp5_nn_upload");
+    NNCHK(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, nn_stream()), "gp5_nn_upload");
+    NNCHK(hipStreamSynchronize(nn_stream()), "gp5_nn_upload");
     return GP5_OK;
 }
 
 int gp5_nn_download(void *host, const void *src, size_t bytes)
 {
-    NNCHK(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost), "gp5_nn_download");
+    NNCHK(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, nn_stream()), "gp5_nn_download");
+    NNCHK(hipStreamSynchronize(nn_stream()), "gp5_nn_download");
     return GP5_OK;
 }
 
 int gp5_nn_copy(void *dst, const void *src, size_t bytes)
 {
-    NNCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "gp5_nn_copy");
+    NNCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, nn_stream()), "gp5_nn_copy");
+    return GP5_OK;
+}
+
+int gp5_nn_stream_create(void **stream)
+{
+    if (!stream) return set_error(GP5_ERR_ARG, "NULL argument");
+    hipStream_t s = nullptr;
+    NNCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "gp5_nn_stream_create");
+    *stream = (void *)s;
+    return GP5_OK;
+}
+
+int gp5_nn_stream_destroy(void *stream)
+{
+    if (stream) NNCHK(hipStreamDestroy((hipStream_t)stream), "gp5_nn_stream_destroy");
+    return GP5_OK;
+}
+
+int gp5_nn_set_stream(void *stream)
+{
+    g_stream = (hipStream_t)stream;
+    return GP5_OK;
+}
+
+int gp5_nn_event_create(void **event)
+{
+    if (!event) return set_error(GP5_ERR_ARG, "NULL argument");
+    hipEvent_t e = nullptr;
+    NNCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming), "gp5_nn_event_create");
+    *event = (void *)e;
+    return GP5_OK;
+}
+
+int gp5_nn_event_destroy(void *event)
+{
+    if (event) NNCHK(hipEventDestroy((hipEvent_t)event), "gp5_nn_event_destroy");
+    return GP5_OK;
+}
+
+int gp5_nn_event_record(void *event, void *stream)
+{
+    if (!event) return set_error(GP5_ERR_ARG, "NULL argument");
+    NNCHK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "gp5_nn_event_record");
+    return GP5_OK;
+}
+
+int gp5_nn_stream_wait(void *stream, void *event)
+{
+    if (!event) return set_error(GP5_ERR_ARG, "NULL argument");
+    NNCHK(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0), "gp5_nn_stream_wait");
     return GP5_OK;
 }
 
@@ -1805,7 +1868,7 @@ int gp5_nn_zero(void *p, size_t bytes)
 {
     if (bytes % 4) return set_error(GP5_ERR_ARG, "gp5_nn_zero: size must be a multiple of 4 bytes");
     const long long n = (long long)(bytes / 4);
-    hipLaunchKernelGGL(k_fill0, dim3(grid_for(n)), dim3(256), 0, 0, (uint32_t *)p, n);
+    hipLaunchKernelGGL(k_fill0, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (uint32_t *)p, n);
     NNCHK(hipGetLastError(), "fill");
     return GP5_OK;
 }
@@ -1819,7 +1882,7 @@ int gp5_nn_sync(void)
 // up = 1: X has side D/2 and the convolution reads its nearest upsampling (only
 // the LDS-staged and z-sliding kernels do that)
 // The float partial sums of split convolutions, in a process-wide buffer that
-// only grows (the kernels of successive calls run in order on the null stream).
+// only grows (the kernels of successive calls run in order on one stream).
 static int conv_split_scratch(size_t floats, float **out)
 {
     static std::mutex mu;
@@ -1974,14 +2037,14 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             float *yf_k = S > 1 ? ws : Yf;
 #define GP5_CONV_LDS_CASE(C, K)                                                                                   \
     if (KCl == C && NBl == K)                                                                                     \
-        hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, 0, x, w, bias, a, sk, y, yf_k, (int)P, D,    \
+        hipLaunchKernelGGL((k_conv_lds<C, K, kConvPD>), grid, block, shm, nn_stream(), x, w, bias, a, sk, y, yf_k, (int)P, D,    \
                            Cin, Cout, mirror, act, Rb, PZ, up, csplit);
             GP5_CONV_LDS_CASE(16, 1) GP5_CONV_LDS_CASE(16, 2) GP5_CONV_LDS_CASE(16, 4)
             GP5_CONV_LDS_CASE(32, 1) GP5_CONV_LDS_CASE(32, 2) GP5_CONV_LDS_CASE(32, 4)
 #undef GP5_CONV_LDS_CASE
             NNCHK(hipGetLastError(), "conv3d launch");
             if (S > 1) {
-                hipLaunchKernelGGL(k_conv_split_finish, dim3(grid_for(P * (Cout / 4))), dim3(256), 0, 0, ws, S, (int)P,
+                hipLaunchKernelGGL(k_conv_split_finish, dim3(grid_for(P * (Cout / 4))), dim3(256), 0, nn_stream(), ws, S, (int)P,
                                    Cout, bias, act, a, sk, y, Yf);
                 NNCHK(hipGetLastError(), "conv3d split finish");
             }
@@ -1995,7 +2058,7 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         const int KC = Cin % 64 == 0 ? 64 : (Cin % 32 == 0 ? 32 : 16);
 #define GP5_CONV_CASE(C, K, T)                                                                                 \
     if (KC == C && NB == K && MT == T)                                                                         \
-        hipLaunchKernelGGL((k_conv<C, K, T, kConvPD>), dim3((unsigned)((P + 64 * T - 1) / (64 * T)), gy), block, 0, 0, x, w, \
+        hipLaunchKernelGGL((k_conv<C, K, T, kConvPD>), dim3((unsigned)((P + 64 * T - 1) / (64 * T)), gy), block, 0, nn_stream(), x, w, \
                            bias, a, sk, y, Yf, (int)P, D, Cin, Cout, mirror, act);
 #define GP5_CONV_KC(C)                                                                                         \
     GP5_CONV_CASE(C, 1, 4) GP5_CONV_CASE(C, 1, 1) GP5_CONV_CASE(C, 2, 2) GP5_CONV_CASE(C, 2, 1)                \
@@ -2078,9 +2141,9 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
 static int wgrad_reduce(const float *ws, float *dW, long long n, int chunks)
 {
     if (n >= 64 * 256)
-        hipLaunchKernelGGL(k_wgrad_reduce1<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, 0, ws, dW, n, chunks);
+        hipLaunchKernelGGL(k_wgrad_reduce1<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, nn_stream(), ws, dW, n, chunks);
     else
-        hipLaunchKernelGGL(k_wgrad_reduce1<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, 0, ws, dW, n, chunks);
+        hipLaunchKernelGGL(k_wgrad_reduce1<16>, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, nn_stream(), ws, dW, n, chunks);
     NNCHK(hipGetLastError(), "wgrad reduce launch");
     return GP5_OK;
 }
@@ -2109,7 +2172,7 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
             shm_set = shm;
         }
         hipLaunchKernelGGL(k_wgrad_zs, dim3((unsigned)zblocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWzThreads),
-                           shm, 0, (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, nzs, up);
+                           shm, nn_stream(), (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, nzs, up);
         NNCHK(hipGetLastError(), "wgrad launch");
         return wgrad_reduce(ws, dW, n, (int)zblocks);
     }
@@ -2121,25 +2184,25 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
     const dim3 grid((unsigned)((Cin / 16) * (Cout / 16)), (unsigned)chunks);
     if (D % 4 == 0) {
         if (D % 8 == 0) {  // P = B D^3 is a multiple of 512
-            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, 0,
+            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cin / 16)), dim3(256), 0, nn_stream(),
                                (const uint16_t *)X, XT, Cin, D, up);
             NNCHK(hipGetLastError(), "wgrad transpose");
-            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, 0,
+            hipLaunchKernelGGL(k_transpose_b8, dim3((unsigned)(P / 128), (unsigned)(Cout / 16)), dim3(256), 0, nn_stream(),
                                (const uint16_t *)dY, dYT, Cout, D, 0);
         } else {
             const unsigned tb = (unsigned)((P + 63) / 64);
-            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)X, XT, P, Cin);
+            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, nn_stream(), (const uint16_t *)X, XT, P, Cin);
             NNCHK(hipGetLastError(), "wgrad transpose");
-            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, 0, (const uint16_t *)dY, dYT, P, Cout);
+            hipLaunchKernelGGL(k_transpose_pc, dim3(tb), dim3(256), 0, nn_stream(), (const uint16_t *)dY, dYT, P, Cout);
         }
         NNCHK(hipGetLastError(), "wgrad transpose");
         if (D % 8 == 0)  // 32-bit offsets: P * C * 2 < 2^31 (checked above)
-            hipLaunchKernelGGL(k_wgrad8, dim3(grid.x, grid.y, 3), dim3(64), 0, 0, XT, dYT, part0, (int)P, D, Cin, Cout,
+            hipLaunchKernelGGL(k_wgrad8, dim3(grid.x, grid.y, 3), dim3(64), 0, nn_stream(), XT, dYT, part0, (int)P, D, Cin, Cout,
                                (int)chunk);
         else
-            hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, 0, XT, dYT, part0, P, D, Cin, Cout, chunk);
+            hipLaunchKernelGGL(k_wgrad, grid, dim3(64), 0, nn_stream(), XT, dYT, part0, P, D, Cin, Cout, chunk);
     } else {
-        hipLaunchKernelGGL(k_wgrad_gather, grid, dim3(64), 0, 0, (const uint16_t *)X, (const uint16_t *)dY, part0, P, D,
+        hipLaunchKernelGGL(k_wgrad_gather, grid, dim3(64), 0, nn_stream(), (const uint16_t *)X, (const uint16_t *)dY, part0, P, D,
                            Cin, Cout, chunk);
     }
     NNCHK(hipGetLastError(), "wgrad launch");
@@ -2157,8 +2220,8 @@ int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws,
 }
 
 // Block partials of the bias gradients in a process-wide scratch that only grows
-// (the kernels of successive calls run in order on the null stream, so they can
-// share it).
+// (the kernels of successive calls run in order on one stream, so they can share
+// it).
 static int bias_scratch(size_t need, float **out)
 {
     static std::mutex mu;
@@ -2182,9 +2245,9 @@ int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C)
     float *part = nullptr;
     const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
     if (rc != GP5_OK) return rc;
-    hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, 0, dY, part, P, C);
+    hipLaunchKernelGGL(k_bias_partial, dim3(kBiasBlocks), dim3(256), 0, nn_stream(), dY, part, P, C);
     NNCHK(hipGetLastError(), "bias grad launch");
-    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
+    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, nn_stream(), part, db, kBiasBlocks, C);
     NNCHK(hipGetLastError(), "bias grad launch");
     return GP5_OK;
 }
@@ -2194,7 +2257,7 @@ int gp5_nn_pool2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
     if (C % 4) return set_error(GP5_ERR_ARG, "pool2: C must be a multiple of 4");
     const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * (C / 4);
-    hipLaunchKernelGGL(k_pool2_bf16x4, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
+    hipLaunchKernelGGL(k_pool2_bf16x4, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)X, (uint16_t *)Y, B, D,
                        C, scale);
     NNCHK(hipGetLastError(), "pool2");
     return GP5_OK;
@@ -2204,7 +2267,7 @@ int gp5_nn_up2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
 {
     if (C % 4) return set_error(GP5_ERR_ARG, "up2: C must be a multiple of 4");
     const long long n = (long long)B * 8 * D * D * D * (C / 4);
-    hipLaunchKernelGGL(k_up2_bf16x4, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)X, (uint16_t *)Y, B, D,
+    hipLaunchKernelGGL(k_up2_bf16x4, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)X, (uint16_t *)Y, B, D,
                        C, scale);
     NNCHK(hipGetLastError(), "up2");
     return GP5_OK;
@@ -2214,7 +2277,7 @@ int gp5_nn_pool2_f32(const float *X, float *Y, int B, int D, int C, float scale)
 {
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
     const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * C;
-    hipLaunchKernelGGL(k_pool2<float>, dim3(grid_for(n)), dim3(256), 0, 0, X, Y, B, D, C, scale);
+    hipLaunchKernelGGL(k_pool2<float>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
     NNCHK(hipGetLastError(), "pool2");
     return GP5_OK;
 }
@@ -2222,14 +2285,14 @@ int gp5_nn_pool2_f32(const float *X, float *Y, int B, int D, int C, float scale)
 int gp5_nn_up2_f32(const float *X, float *Y, int B, int D, int C, float scale)
 {
     const long long n = (long long)B * 8 * D * D * D * C;
-    hipLaunchKernelGGL(k_up2<float>, dim3(grid_for(n)), dim3(256), 0, 0, X, Y, B, D, C, scale);
+    hipLaunchKernelGGL(k_up2<float>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
     NNCHK(hipGetLastError(), "up2");
     return GP5_OK;
 }
 
 int gp5_nn_act_bwd(const void *A, const float *dA, float *dPre, long long n)
 {
-    hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(n)), dim3(256), 0, 0, (const uint16_t *)A, dA, dPre, n);
+    hipLaunchKernelGGL(k_act_bwd, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)A, dA, dPre, n);
     NNCHK(hipGetLastError(), "act_bwd");
     return GP5_OK;
 }
@@ -2238,7 +2301,7 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
 {
     if (!A || !dA || !dPre || !H) return set_error(GP5_ERR_ARG, "NULL argument");
     if (n % 4) return set_error(GP5_ERR_ARG, "act_bwd_cast: n must be a multiple of 4");
-    hipLaunchKernelGGL(k_act_bwd_cast4, dim3(grid_for(n / 4)), dim3(256), 0, 0, (const uint16_t *)A, dA, dPre,
+    hipLaunchKernelGGL(k_act_bwd_cast4, dim3(grid_for(n / 4)), dim3(256), 0, nn_stream(), (const uint16_t *)A, dA, dPre,
                        (uint16_t *)H, n / 4);
     NNCHK(hipGetLastError(), "act_bwd_cast");
     return GP5_OK;
@@ -2256,11 +2319,11 @@ static int act_bwd_bias_run(const void *A, const float *dA, const float *G, floa
         const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
         if (rc != GP5_OK) return rc;
     }
-    hipLaunchKernelGGL(k_act_bwd_cast_bias, dim3(kBiasBlocks), dim3(256), 0, 0, (const uint16_t *)A, dA, (uint16_t *)H,
+    hipLaunchKernelGGL(k_act_bwd_cast_bias, dim3(kBiasBlocks), dim3(256), 0, nn_stream(), (const uint16_t *)A, dA, (uint16_t *)H,
                        part, P, C, G, scale, D);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     if (!db) return GP5_OK;
-    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, 0, part, db, kBiasBlocks, C);
+    hipLaunchKernelGGL(k_bias_final, dim3(C), dim3(256), 0, nn_stream(), part, db, kBiasBlocks, C);
     NNCHK(hipGetLastError(), "act_bwd_cast_bias");
     return GP5_OK;
 }
@@ -2308,7 +2371,7 @@ int gp5_nn_bias_final_multi(const gp5_nn_bias_seg *segs, int nseg)
         a.s[k] = segs[k];
         blocks += segs[k].C;
     }
-    hipLaunchKernelGGL(k_bias_final_multi, dim3(blocks), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_bias_final_multi, dim3(blocks), dim3(256), 0, nn_stream(), a);
     NNCHK(hipGetLastError(), "bias_final_multi");
     return GP5_OK;
 }
@@ -2318,14 +2381,14 @@ int gp5_nn_up2_add_f32(const float *X, float *Y, int B, int D, int C, float scal
     if (!X || !Y) return set_error(GP5_ERR_ARG, "NULL argument");
     if (C % 4) return set_error(GP5_ERR_ARG, "up2_add: C must be a multiple of 4");
     const long long n = (long long)B * 8 * D * D * D * (C / 4);
-    hipLaunchKernelGGL(k_up2_add4, dim3(grid_for(n)), dim3(256), 0, 0, X, Y, B, D, C, scale);
+    hipLaunchKernelGGL(k_up2_add4, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
     NNCHK(hipGetLastError(), "up2_add");
     return GP5_OK;
 }
 
 int gp5_nn_add_f32(float *dst, const float *src, long long n)
 {
-    hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(256), 0, 0, dst, src, n);
+    hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(256), 0, nn_stream(), dst, src, n);
     NNCHK(hipGetLastError(), "add");
     return GP5_OK;
 }
@@ -2343,7 +2406,7 @@ int gp5_nn_droplet_features(void *X, float *base, int D, double ub, float s, con
     f.a2 = a[2];
     for (int q = 0; q < 5; ++q) f.c[q] = c[q];
     const long long P = (long long)D * D * D;
-    hipLaunchKernelGGL(k_droplet_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, (uint16_t *)X, base, f);
+    hipLaunchKernelGGL(k_droplet_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, nn_stream(), (uint16_t *)X, base, f);
     NNCHK(hipGetLastError(), "droplet features");
     return GP5_OK;
 }
@@ -2351,20 +2414,20 @@ int gp5_nn_droplet_features(void *X, float *base, int D, double ub, float s, con
 int gp5_nn_predict_out(const float *Yf, int C, const float *base, float *y, long long P)
 {
     if (!Yf || !base || !y || C < 1 || P < 0) return set_error(GP5_ERR_ARG, "gp5_nn_predict_out: bad argument");
-    hipLaunchKernelGGL(k_predict_out, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, 0, Yf, C, base, y, P);
+    hipLaunchKernelGGL(k_predict_out, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, nn_stream(), Yf, C, base, y, P);
     NNCHK(hipGetLastError(), "predict out");
     return GP5_OK;
 }
 
 int gp5_nn_cast_bf16(const float *src, void *dst, long long n)
 {
-    hipLaunchKernelGGL(k_cast, dim3(grid_for(n)), dim3(256), 0, 0, src, (uint16_t *)dst, n);
+    hipLaunchKernelGGL(k_cast, dim3(grid_for(n)), dim3(256), 0, nn_stream(), src, (uint16_t *)dst, n);
     NNCHK(hipGetLastError(), "cast");
     return GP5_OK;
 }
 
 // Scratch of the loss: block sums and the loss, in a process-wide buffer that
-// only grows (the kernels of successive calls run in order on the null stream).
+// only grows (the kernels of successive calls run in order on one stream).
 static int mse_scratch(size_t blocks, double **out)
 {
     static std::mutex mu;
@@ -2385,22 +2448,26 @@ static int mse_scratch(size_t blocks, double **out)
 
 // db (optional, device, C floats): the bias gradient of the output layer, sum
 // over positions of the gradient rows: channel 0 from the double block sums, 0
-// elsewhere.  The sums finish on the device; the one host copy is the loss.
+// elsewhere.  The sums finish on the device; the one host copy is the loss
+// (none when acc, a device double the loss is added to, is given instead).
 static int mse_run(const float *Yf, int ldy, int C, const float *base, const float *target, float *dYf, uint16_t *H,
-                   float *db, long long P, double *loss)
+                   float *db, long long P, double *loss, double *acc = nullptr)
 {
-    if (!Yf || !base || !target || !loss) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (!Yf || !base || !target || (!loss && !acc)) return set_error(GP5_ERR_ARG, "NULL argument");
     if (H && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
     if (C < 1 || P < 1) return set_error(GP5_ERR_ARG, "mse: bad size");
     const unsigned blocks = (unsigned)std::min<long long>((P + 255) / 256, 1024);
     double *part = nullptr;
     const int rc = mse_scratch(blocks, &part);
     if (rc != GP5_OK) return rc;
-    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, 0, Yf, ldy, C, base, target, dYf, H, P, part);
+    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, nn_stream(), Yf, ldy, C, base, target, dYf, H, P, part);
     NNCHK(hipGetLastError(), "mse");
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(256), 0, 0, part, (int)blocks, P, part + 2 * blocks, db, C);
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(256), 0, nn_stream(), part, (int)blocks, P, part + 2 * blocks, db, C, acc);
     NNCHK(hipGetLastError(), "mse");
-    NNCHK(hipMemcpy(loss, part + 2 * blocks, sizeof(double), hipMemcpyDeviceToHost), "mse loss");
+    if (loss) {
+        NNCHK(hipMemcpyAsync(loss, part + 2 * blocks, sizeof(double), hipMemcpyDeviceToHost, nn_stream()), "mse loss");
+        NNCHK(hipStreamSynchronize(nn_stream()), "mse loss");
+    }
     return GP5_OK;
 }
 
@@ -2424,11 +2491,18 @@ int gp5_nn_mse_out0(const float *y0, const float *base, const float *target, voi
     return mse_run(y0, 1, C, base, target, nullptr, (uint16_t *)H, db, P, loss);
 }
 
+int gp5_nn_mse_out0_acc(const float *y0, const float *base, const float *target, void *H, int C, float *db,
+                        long long P, double *loss_acc)
+{
+    if (!H || !loss_acc) return set_error(GP5_ERR_ARG, "NULL argument");
+    return mse_run(y0, 1, C, base, target, nullptr, (uint16_t *)H, db, P, nullptr, loss_acc);
+}
+
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps, int t)
 {
     if (t < 1) return set_error(GP5_ERR_ARG, "adam step count starts at 1");
     const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
-    hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, 0, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, nn_stream(), p, g, m, v, n, lr, b1, b2, eps, c1, c2);
     NNCHK(hipGetLastError(), "adam");
     return GP5_OK;
 }
@@ -2464,7 +2538,7 @@ int gp5_nn_adam_multi(const gp5_nn_adam_seg *segs, int nseg, float lr, float b1,
     a.c1 = 1.f - powf(b1, (float)t);
     a.c2 = 1.f - powf(b2, (float)t);
     if (most == 0) return GP5_OK;
-    hipLaunchKernelGGL(k_adam_multi, dim3(std::min(grid_for(most), 1024u), (unsigned)nseg), dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(k_adam_multi, dim3(std::min(grid_for(most), 1024u), (unsigned)nseg), dim3(256), 0, nn_stream(), a);
     NNCHK(hipGetLastError(), "adam_multi");
     return GP5_OK;
 }
@@ -2472,7 +2546,7 @@ int gp5_nn_adam_multi(const gp5_nn_adam_seg *segs, int nseg, float lr, float b1,
 int gp5_nn_relayout(const float *W, void *Wf, void *Wb, int Cin, int Cout)
 {
     const long long n = 27LL * Cin * Cout;
-    hipLaunchKernelGGL(k_relayout, dim3(grid_for(n)), dim3(256), 0, 0, W, (uint16_t *)Wf, (uint16_t *)Wb, Cin, Cout);
+    hipLaunchKernelGGL(k_relayout, dim3(grid_for(n)), dim3(256), 0, nn_stream(), W, (uint16_t *)Wf, (uint16_t *)Wb, Cin, Cout);
     NNCHK(hipGetLastError(), "relayout");
     return GP5_OK;
 }

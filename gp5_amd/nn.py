@@ -17,6 +17,7 @@ Loss: mean squared error of y against the target field.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Dict, Optional
 
 import numpy as np
@@ -67,6 +68,7 @@ def _lib():
         "gp5_nn_conv3d_wgrad_up": [P, P, P, P, I, I, I, I],
         "gp5_nn_mse_bf16": [P, I, P, P, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_mse_out0": [P, P, P, P, I, P, LL, ctypes.POINTER(ctypes.c_double)],
+        "gp5_nn_mse_out0_acc": [P, P, P, P, I, P, LL, P],
         "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
         "gp5_nn_act_bwd_part": [P, P, P, F, P, P, I, I, I],
@@ -79,6 +81,9 @@ def _lib():
         "gp5_nn_adam_multi": [P, I, F, F, F, F, I],
         "gp5_nn_droplet_features": [P, P, I, ctypes.c_double, F, P, P],
         "gp5_nn_predict_out": [P, I, P, P, LL],
+        "gp5_nn_stream_create": [ctypes.POINTER(ctypes.c_void_p)], "gp5_nn_stream_destroy": [P],
+        "gp5_nn_set_stream": [P], "gp5_nn_event_create": [ctypes.POINTER(ctypes.c_void_p)],
+        "gp5_nn_event_destroy": [P], "gp5_nn_event_record": [P, P], "gp5_nn_stream_wait": [P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -167,6 +172,8 @@ class UNet:
     """Weights (float masters, Adam moments, bf16 copies) and the activations of
     one batch shape (B, D)."""
 
+    _main = None  # the stream of every pass but the side stream's weight gradients (None: the null stream)
+
     def __init__(self, B: int, D: int, params: Optional[Dict[str, np.ndarray]] = None, seed: int = 0):
         if D % 8:
             raise ValueError("D must be divisible by 8")
@@ -209,6 +216,9 @@ class UNet:
             "g32a": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
             "g64a": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
             "g128": (n(D // 8, 128), 4), "h128": (n(D // 8, 128), 2),
+            # the encoder layers' bf16 gradients (their own buffers: the decoder
+            # layers' weight gradients may still read h64 / h32 / h16)
+            "h64e": (n(D // 4, 64), 2), "h32e": (n(D // 2, 32), 2), "h16e": (n(D, 16), 2),
             "gu32": (n(D, 32), 4), "gu64": (n(D // 2, 64), 4), "gu128": (n(D // 4, 128), 4),
             "gp16": (n(D // 2, 16), 4), "gp32": (n(D // 4, 32), 4), "gp64": (n(D // 8, 64), 4),
         }
@@ -220,6 +230,8 @@ class UNet:
         if _pool_fused(D // 2, 64, 32, B):
             del sizes["gu64"]
         self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
+        self.t["loss_acc"] = DevBuf(8)  # sum of the losses of backward(read_loss=False) calls
+        self.t["loss_acc"].zero()
         L = _lib()
         ws = max(L.gp5_nn_wgrad_ws(B, d, ci, co) for d, ci, co in
                  [(D, C0, 16), (D // 2, 16, 32), (D // 4, 32, 64), (D // 8, 64, 128), (D // 4, 128, 64),
@@ -354,6 +366,65 @@ class UNet:
         _ck(_lib().gp5_nn_act_bwd_part(self.t[a].ptr, self.t[g].ptr, None, 0.0, self.t[h].ptr,
                                        self.bpart[name].ptr, self.B, D, self.w[name]["co"]))
 
+    # --- the side stream of the weight gradients ----------------------------
+    def _side_setup(self):
+        """A non-blocking side stream and two events per layer, made on first use
+        (GP5_NN_OVERLAP=0 keeps every pass on the null stream)."""
+        if getattr(self, "_aux", None) is not None or os.environ.get("GP5_NN_OVERLAP", "1") == "0":
+            return getattr(self, "_aux", None) is not None
+        L = _lib()
+        aux = ctypes.c_void_p()
+        _ck(L.gp5_nn_stream_create(ctypes.byref(aux)))
+        self._aux, self._ev = aux, {}
+        for name, _, _ in SHAPES:
+            for kind in ("dz", "w"):
+                ev = ctypes.c_void_p()
+                _ck(L.gp5_nn_event_create(ctypes.byref(ev)))
+                self._ev[name, kind] = ev
+        return True
+
+    def _side(self, name, *fns):
+        """Queues fns (weight gradients) on the side stream behind the work queued
+        so far on the main stream, which has produced their inputs; the
+        data-gradient passes that follow on the main stream run beside them."""
+        if not self._overlapping:
+            for fn in fns:
+                fn()
+            return
+        L = _lib()
+        _ck(L.gp5_nn_event_record(self._ev[name, "dz"], self._main))
+        _ck(L.gp5_nn_stream_wait(self._aux, self._ev[name, "dz"]))
+        _ck(L.gp5_nn_set_stream(self._aux))
+        try:
+            for fn in fns:
+                fn()
+            _ck(L.gp5_nn_event_record(self._ev[name, "w"], self._aux))
+        finally:
+            _ck(L.gp5_nn_set_stream(self._main))
+
+    def _after_side(self, name):
+        """Later work on the main stream waits for layer name's weight gradient
+        (and the side stream's earlier work)."""
+        if self._overlapping:
+            _ck(_lib().gp5_nn_stream_wait(self._main, self._ev[name, "w"]))
+
+    def _side_free(self):
+        aux = getattr(self, "_aux", None)
+        if aux is None:
+            return
+        L = _lib()
+        L.gp5_nn_sync()
+        for ev in self._ev.values():
+            L.gp5_nn_event_destroy(ev)
+        L.gp5_nn_stream_destroy(aux)
+        self._aux = None
+
+    def __del__(self):
+        try:
+            self._side_free()
+        except Exception:
+            pass
+
     def _up_add_act_bwd(self, name, a, g, gh, h, D):
         """_act_bwd with the incoming gradient t[g] + t[gh] / 8 at the parent cell
         (t[gh] of side D/2: the adjoint of the pooling that fed the next level), in
@@ -374,20 +445,31 @@ class UNet:
         _ck(_lib().gp5_nn_conv3d(dz_bf.ptr, e["Wb"].ptr, None, None, None, None, out_f.ptr, self.B, D, e["co"],
                                  e["ci"], 1, 0))
 
-    def backward(self) -> float:
-        """MSE loss against the target; fills every weight and bias gradient."""
+    def backward(self, read_loss: bool = True) -> Optional[float]:
+        """MSE loss against the target; fills every weight and bias gradient.
+        Returns the loss (one device-to-host copy, which waits for the forward
+        pass), or with read_loss=False adds it to a device sum that take_loss_sum()
+        reads, and returns None without synchronising."""
         L, t, B, D = _lib(), self.t, self.B, self.D
-        n = lambda d, c: B * d ** 3 * c  # noqa: E731
         loss = ctypes.c_double()
         # loss, the bf16 output gradient and c8's bias gradient in one pass
-        _ck(L.gp5_nn_mse_out0(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
-                              self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss)))
+        if read_loss:
+            _ck(L.gp5_nn_mse_out0(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
+                                  self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss)))
+        else:
+            _ck(L.gp5_nn_mse_out0_acc(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
+                                      self.w["c8"]["bg"].ptr, B * D ** 3, t["loss_acc"].ptr))
+        # Weight gradients run on a side stream beside the data-gradient chain
+        # (_side), each as soon as its dz is ready (grouping them, fewer forks,
+        # measured slower); nothing on the main stream overwrites what they read
+        # before the join at the end.
+        self._overlapping = self._side_setup()
         # c8
-        self._wgrad("c8", t["d1"], t["h_out"], D)
+        self._side("c8", lambda: self._wgrad("c8", t["d1"], t["h_out"], D))
         self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
         # c7: dz7 = dd1 * act'(a7)
         self._act_bwd("c7", "a7", "g16a", "h16", D)
-        self._wgrad_up("c7", "d2", "u2", t["h16"], D)
+        self._side("c7", lambda: self._wgrad_up("c7", "d2", "u2", t["h16"], D))
         if _pool_fused(D, 32, 16, B):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
             e = self.w["c7"]
             _ck(L.gp5_nn_conv3d_pool(t["h16"].ptr, e["Wb"].ptr, t["g32a"].ptr, B, D, e["co"], e["ci"], 1))
@@ -396,7 +478,7 @@ class UNet:
             _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
         # c6
         self._act_bwd("c6", "a6", "g32a", "h32", D // 2)
-        self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2)
+        self._side("c6", lambda: self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2))
         if _pool_fused(D // 2, 64, 32, B):  # dd3 = de3 (skip part)
             e = self.w["c6"]
             _ck(L.gp5_nn_conv3d_pool(t["h32"].ptr, e["Wb"].ptr, t["g64a"].ptr, B, D // 2, e["co"], e["ci"], 1))
@@ -405,26 +487,34 @@ class UNet:
             _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
         # c5
         self._act_bwd("c5", "a5", "g64a", "h64", D // 4)
-        self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4)
+        self._side("c5", lambda: self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4))
         self._dgrad("c5", t["h64"], t["gu128"], D // 4)
         _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4
         # c4
         self._act_bwd("c4", "e4", "g128", "h128", D // 8)
-        self._wgrad("c4", t["p3"], t["h128"], D // 8)
+        self._side("c4", lambda: self._wgrad("c4", t["p3"], t["h128"], D // 8))
         self._dgrad("c4", t["h128"], t["gp64"], D // 8)
         # c3 (de3 total = dd3 + the adjoint of the pooling into c4)
-        self._up_add_act_bwd("c3", "e3", "g64a", "gp64", "h64", D // 4)
-        self._wgrad("c3", t["p2"], t["h64"], D // 4)
-        self._dgrad("c3", t["h64"], t["gp32"], D // 4)
+        self._up_add_act_bwd("c3", "e3", "g64a", "gp64", "h64e", D // 4)
+        self._side("c3", lambda: self._wgrad("c3", t["p2"], t["h64e"], D // 4))
+        self._dgrad("c3", t["h64e"], t["gp32"], D // 4)
         # c2 (de2 total)
-        self._up_add_act_bwd("c2", "e2", "g32a", "gp32", "h32", D // 2)
-        self._wgrad("c2", t["p1"], t["h32"], D // 2)
-        self._dgrad("c2", t["h32"], t["gp16"], D // 2)
+        self._up_add_act_bwd("c2", "e2", "g32a", "gp32", "h32e", D // 2)
+        self._side("c2", lambda: self._wgrad("c2", t["p1"], t["h32e"], D // 2))
+        self._dgrad("c2", t["h32e"], t["gp16"], D // 2)
         # c1 (de1 total)
-        self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16", D)
-        self._wgrad("c1", t["X"], t["h16"], D)
+        self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16e", D)
+        self._side("c1", lambda: self._wgrad("c1", t["X"], t["h16e"], D))
         self._bias_final()
-        return float(loss.value)
+        self._after_side("c1")  # every weight gradient is done before anything reads them
+        return float(loss.value) if read_loss else None
+
+    def take_loss_sum(self) -> float:
+        """The sum of the losses of the backward(read_loss=False) calls since the
+        last take (in call order, double), and a fresh zero sum."""
+        v = float(self.t["loss_acc"].download(np.float64, (1,))[0])
+        self.t["loss_acc"].zero()
+        return v
 
     def _adam_segs(self):
         if getattr(self, "_segs", None) is None:

@@ -174,14 +174,13 @@ def train(N, w, b, phi, epochs: int = 12, batch: int = 4, lr: float = 1e-3, seed
     it = 0
     for ep in range(epochs):
         order = rng.permutation(k)
-        acc = 0.0
         for j in range(k // batch):
             net.load_batch(Xs, bases, targets, order[j * batch:(j + 1) * batch])
             net.forward()
-            acc += net.backward()
+            net.backward(read_loss=False)  # losses summed on the device, read once per epoch
             net.step(0.5 * lr * (1.0 + np.cos(np.pi * it / steps_total)))  # cosine decay
             it += 1
-        hist.append(acc / (k // batch))
+        hist.append(net.take_loss_sum() / (k // batch))
         if verbose:
             print(f"  epoch {ep + 1}/{epochs}: mse {hist[-1]:.3e} ({time.time() - t0:.0f} s)", flush=True)
     return net.params(), hist

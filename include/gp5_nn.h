@@ -10,8 +10,13 @@
  * of a 3x3x3 "same" convolution are float masters W[tap][ci][co]
  * (tap = 9 dz + 3 dy + dx over offsets -1..1) with bfloat16 copies in two
  * layouts for the forward and data-gradient kernels.  Channel counts are
- * multiples of 16.  All kernels run on the null stream; every call returns
- * GP5_OK or an error code with gp5_last_error() set (include/gp5.h). */
+ * multiples of 16.  Kernels and copies go to the calling thread's stream
+ * (gp5_nn_set_stream; the null stream until set); every call returns GP5_OK or
+ * an error code with gp5_last_error() set (include/gp5.h).  The library's own
+ * scratch (split convolutions, the loss, the bias-gradient partials of
+ * gp5_nn_bias_grad and gp5_nn_act_bwd_cast_bias) is one per process: calls that
+ * use it must not run on two streams at once.  The weight gradients use only
+ * the caller's workspace. */
 #ifndef GP5_NN_H
 #define GP5_NN_H
 
@@ -21,14 +26,29 @@
 extern "C" {
 #endif
 
-/* device memory and transfers (synchronous) */
+/* device memory and transfers (upload and download return when done; copy and
+ * zero are queued on the thread's stream) */
 int gp5_nn_malloc(void **p, size_t bytes);
 int gp5_nn_free(void *p);
 int gp5_nn_upload(void *dst, const void *host, size_t bytes);
 int gp5_nn_download(void *host, const void *src, size_t bytes);
 int gp5_nn_copy(void *dst, const void *src, size_t bytes); /* device to device */
 int gp5_nn_zero(void *p, size_t bytes); /* fill kernel, not a memset node */
-int gp5_nn_sync(void);
+int gp5_nn_sync(void); /* waits for every stream of the device */
+
+/* Streams and events, so that independent passes of a training step (a layer's
+ * weight gradient beside the data gradients that follow it) run side by side.
+ * Streams are non-blocking (they do not wait for the null stream) at the default
+ * priority; events carry no timing.  gp5_nn_set_stream(NULL) returns the calling
+ * thread to the null stream.  gp5_nn_event_record marks the work queued so far
+ * on a stream; gp5_nn_stream_wait makes later work on a stream wait for it. */
+int gp5_nn_stream_create(void **stream);
+int gp5_nn_stream_destroy(void *stream);
+int gp5_nn_set_stream(void *stream);
+int gp5_nn_event_create(void **event);
+int gp5_nn_event_destroy(void *event);
+int gp5_nn_event_record(void *event, void *stream);
+int gp5_nn_stream_wait(void *stream, void *event);
 
 /* Y = act(conv(X, W) + bias) (+ skip).  Implicit GEMM on bf16 MFMA tiles with
  * float accumulation.  Wk: bf16 [27][Cout][Cin] (forward layout); with
@@ -118,6 +138,11 @@ int gp5_nn_mse_bf16(const float *Yf, int C, const float *base, const float *targ
  * channel 0 the gradient 2 (y - t) / P, the others 0; db (C floats) as above. */
 int gp5_nn_mse_out0(const float *y0, const float *base, const float *target, void *H, int C, float *db, long long P,
                     double *loss);
+/* gp5_nn_mse_out0 that adds the loss to loss_acc[0] (a device double) instead of
+ * copying it to the host: no synchronisation, so a training loop reads the sum of
+ * its losses when it needs them (one loss per call, added in call order). */
+int gp5_nn_mse_out0_acc(const float *y0, const float *base, const float *target, void *H, int C, float *db,
+                        long long P, double *loss_acc);
 /* Droplet input channels (one cube of side D on the scaled box [-ub, ub]^3, C-ordered
  * [D][D][D][16] bf16): 0 = 0.5 (1 - tanh((r - 1) s)), 1 = a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
  * 2..6 = c[0..4], 7 = r / ub, 8..15 = 0; base (float, D^3) = channel 0.  float32

@@ -150,6 +150,14 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     # mse_out0: H required, C a multiple of 8
     assert L.gp5_nn_mse_out0(fake, fake, fake, None, 16, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_out0(fake, fake, fake, fake, 12, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_mse_out0_acc(fake, fake, fake, fake, 16, None, 64, None) == capi.GP5_ERR_ARG
+    # streams and events: NULL handles out and events in
+    assert L.gp5_nn_stream_create(None) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_event_create(None) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_event_record(None, None) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_stream_wait(None, None) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_set_stream(None) == capi.GP5_OK
+    assert L.gp5_nn_mse_out0_acc(fake, fake, fake, fake, 12, None, 64, fake) == capi.GP5_ERR_ARG
     # mse_bf16: C a multiple of 8, H required
     assert L.gp5_nn_mse_bf16(fake, 12, fake, fake, None, fake, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_bf16(fake, 16, fake, fake, None, None, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG

@@ -292,6 +292,67 @@ def test_training_steps_reduce_the_loss():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,D", [(1, 64), (2, 16)])
+def test_side_stream_weight_gradients_change_nothing(B, D, monkeypatch):
+    """The weight gradients on the side stream (GP5_NN_OVERLAP, on by default)
+    give, bit for bit, the gradients and weights of the all-null-stream order over
+    two training steps (64^3 takes the pooled data gradients, 16^3 the LDS-staged
+    and side-8 kernels)."""
+    rng = np.random.default_rng(B + D)
+    X = rng.standard_normal((B, D, D, D, 16)).astype(np.float32) * 0.5
+    base = rng.standard_normal((B, D, D, D)).astype(np.float32)
+    target = (base + 0.1 * rng.standard_normal((B, D, D, D))).astype(np.float32)
+    res = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("GP5_NN_OVERLAP", flag)
+        net = nn.UNet(B, D, seed=4)
+        net.set_batch(X, base, target)
+        losses = []
+        for _ in range(2):
+            net.forward()
+            losses.append(net.backward())
+            net.step(5e-4)
+        assert (getattr(net, "_aux", None) is not None) == (flag == "1")
+        res.append((losses, net.grads(), net.params()))
+        net._side_free()
+    assert res[0][0] == res[1][0]
+    for name in res[0][1]:
+        assert np.array_equal(res[0][1][name], res[1][1][name]), name
+    for key in res[0][2]:
+        assert np.array_equal(res[0][2][key], res[1][2][key]), key
+
+
+@pytest.mark.gpu
+def test_device_loss_sum_equals_the_returned_losses():
+    """backward(read_loss=False) (gp5_nn_mse_out0_acc) adds each loss to a device
+    double in call order: after three training steps the sum equals, bit for bit,
+    the Python sum of the losses that backward() returns for the same steps, and
+    the weights come out the same."""
+    rng = np.random.default_rng(21)
+    B, D = 2, 16
+    X = rng.standard_normal((B, D, D, D, 16)).astype(np.float32) * 0.5
+    base = rng.standard_normal((B, D, D, D)).astype(np.float32)
+    target = (base + 0.1 * rng.standard_normal((B, D, D, D))).astype(np.float32)
+    nets = [nn.UNet(B, D, seed=3), nn.UNet(B, D, seed=3)]
+    for net in nets:
+        net.set_batch(X, base, target)
+    total = 0.0
+    for _ in range(3):
+        for k, net in enumerate(nets):
+            net.forward()
+            if k == 0:
+                total += net.backward()
+            else:
+                assert net.backward(read_loss=False) is None
+            net.step(5e-4)
+    assert nets[1].take_loss_sum() == total
+    assert nets[1].take_loss_sum() == 0.0
+    p0, p1 = nets[0].params(), nets[1].params()
+    for key in p0:
+        assert np.array_equal(p0[key], p1[key]), key
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B,exact", [(64, 16, 16, 1, True), (32, 32, 32, 1, True), (16, 32, 16, 2, True),
                                                 (16, 128, 64, 2, False), (8, 64, 128, 1, False)])
 def test_conv_lds_staged_path(D, Cin, Cout, B, exact, monkeypatch):
