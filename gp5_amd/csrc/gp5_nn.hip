@@ -38,6 +38,8 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 constexpr float kLeak = 0.1f;
 constexpr int kActYf0 = 2;  // act flag of the convolutions: the float output keeps channel 0 only
+// block partials of a bias gradient: rows of C floats, summed in a fixed order
+constexpr int kBiasBlocks = 2048;
 
 // The stream that the calling thread's launches and copies go to
 // (gp5_nn_set_stream; the null stream until set)
@@ -139,10 +141,16 @@ constexpr int kEpiScratch = 32 * kEpiStride * 4;     // bytes per wave
 // barrier: a wave's LDS writes and reads execute in issue order, and the wave
 // barrier keeps the compiler from moving them across each other.  The caller
 // then owns the workgroup barrier before the scratch is reused by other waves.
-template <int NB, int MT, bool WS = false>
-__device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
-                                                  const float *__restrict__ bias, int act, uint16_t *A,
-                                                  const uint16_t *skip, uint16_t *Y, float *Yf, unsigned char *scr)
+//
+// ABWD (the data gradient of a layer followed by the backward of the leaky ReLU
+// below it): A is that layer's activation (read), Y receives H = bf16(v act'(A))
+// in the arithmetic of k_act_bwd_cast_bias, and bs[j][e] (this lane's channels
+// 16 (nb0 + j) + 8 (lane & 1) + e) adds up v act'(A) over its positions.
+template <int NB, int MT, bool WS = false, bool ABWD = false>
+__device__ __forceinline__ void conv_epilogue_lds_impl(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
+                                                       const float *__restrict__ bias, int act, uint16_t *A,
+                                                       const uint16_t *skip, uint16_t *Y, float *Yf, unsigned char *scr,
+                                                       float (&bs)[NB][8])
 {
     static_assert(MT % 2 == 0, "tiles go through the scratch in pairs");
     const int lane = threadIdx.x & 63;
@@ -178,7 +186,25 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
             } else if (Yf && cb == 0 && lane < 32) {  // channel 0 of the 32 positions
                 if (pb + lane < P) Yf[pb + lane] = T[lane * kEpiStride];
             }
-            if (A || Y) {
+            if constexpr (ABWD) {
+                const int pos = lane / 2, c8 = 8 * (lane & 1);
+                if (pb + pos < P) {
+                    const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
+                    const float4 hi = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8 + 4);
+                    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    const size_t o = (size_t)(pb + pos) * Cout + cb + c8;
+                    const uint4 a4 = *reinterpret_cast<const uint4 *>(A + o);
+                    const uint16_t *ah = reinterpret_cast<const uint16_t *>(&a4);
+                    uint16_t hb[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float ov = v[e] * (bf2f(ah[e]) > 0.f ? 1.f : kLeak);
+                        hb[e] = f2bf(ov);
+                        bs[j][e] += ov;
+                    }
+                    *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
+                }
+            } else if (A || Y) {
                 const int pos = lane / 2, c8 = 8 * (lane & 1);
                 if (pb + pos < P) {
                     const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
@@ -207,6 +233,15 @@ __device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], in
             else __syncthreads();
         }
     }
+}
+
+template <int NB, int MT, bool WS = false>
+__device__ __forceinline__ void conv_epilogue_lds(const f32x4 (&acc)[MT][NB], int base, int nb0, int P, int Cout,
+                                                  const float *__restrict__ bias, int act, uint16_t *A,
+                                                  const uint16_t *skip, uint16_t *Y, float *Yf, unsigned char *scr)
+{
+    float bs[NB][8];  // unused without ABWD
+    conv_epilogue_lds_impl<NB, MT, WS, false>(acc, base, nb0, P, Cout, bias, act, A, skip, Y, Yf, scr, bs);
 }
 
 // Forward / data-gradient convolution.  A workgroup of 4 waves covers 64 * MT
@@ -553,12 +588,20 @@ __host__ __device__ constexpr int zs_units_per_thread(int cin) { return cin == 1
 // t + 2) and in two waves at D >= 64 (the even row's wave owns the parent and
 // reads the odd row's values through LDS), its z pair in consecutive planes of
 // the walk (the even plane's sum waits in registers).
-template <int CIN, int NB, bool PF = false, bool WS = false, bool POOL = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
+//
+// ABWD (no bias, activation or skip): the epilogue also forms the backward of the
+// leaky ReLU below (conv_epilogue_lds_impl: A read, H = Y written), and at the end
+// workgroup x writes its bias-gradient partial, the sum over its positions, to row
+// x of bpart (kBiasBlocks rows of Cout floats, gridDim.x <= kBiasBlocks) and zeros
+// to rows x + gridDim.x, x + 2 gridDim.x, ..., so that the rows add up to the bias
+// gradient in k_bias_final's fixed order.
+template <int CIN, int NB, bool PF = false, bool WS = false, bool POOL = false, bool ABWD = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ABWD && CIN == 16 ? 3 : 2))) void k_conv_zs(const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk,
                                                  const float *__restrict__ bias, uint16_t *A, const uint16_t *skip,
                                                  uint16_t *Y, float *Yf, int P, int D, int Cout, int mirror, int act,
-                                                 int nzs, int up)
+                                                 int nzs, int up, float *bpart)
 {
+    static_assert(!(ABWD && POOL), "one epilogue form at a time");
     constexpr int MT = 4, LV = CIN / 4, U8 = CIN / 8;
     constexpr int NU = zs_units_per_thread(CIN);
     extern __shared__ __align__(16) unsigned char sx[];
@@ -628,6 +671,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     store_slab(2);
     __syncthreads();
     float S[POOL ? MT : 1][POOL ? NB : 1][2];  // POOL: the even plane's partial parent sums
+    float bs[NB][8];                           // ABWD: this lane's bias-gradient sums
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bs[j][e] = 0.f;
     for (int zi = 0; zi < ZL; ++zi) {
         const int z = zb + zi;
         // keeps the 27 MT tap offsets from being hoisted out of this loop into registers
@@ -737,13 +785,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
           }
         } else {
-            conv_epilogue_lds<NB, MT, WS>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act, A, skip, Y,
-                                          Yf, sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch);
+            conv_epilogue_lds_impl<NB, MT, WS, ABWD>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, bias, act,
+                                                     A, skip, Y, Yf, sx + (size_t)sl[0] * SU * 16 + wave * kEpiScratch,
+                                                     bs);
         }
         if (zi + 1 < ZL) {
             if constexpr (WS) __syncthreads();  // every wave is done with its scratch in the slot
             store_slab(sl[0]);
             __syncthreads();
+        }
+    }
+    if constexpr (ABWD) {
+        // channel c of block j: the lanes with (lane & 1) == c / 8 of the four waves,
+        // in wave and lane order
+        __syncthreads();  // every wave is done with the LDS
+        float *red = reinterpret_cast<float *>(sx);  // NB x 8 x 256 floats
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) red[(j * 8 + e) * 256 + tid] = bs[j][e];
+        __syncthreads();
+        if (tid < 16 * NB) {
+            const int j = tid / 16, c = tid % 16;
+            const float *rr = red + (j * 8 + c % 8) * 256;
+            float t = 0.f;
+            for (int w = 0; w < 4; ++w)
+                for (int l = c / 8; l < 64; l += 2) t += rr[64 * w + l];
+            const int co = 16 * (nb0 + j) + c;
+            bpart[(size_t)blockIdx.x * Cout + co] = t;
+            for (int row = (int)blockIdx.x + (int)gridDim.x; row < kBiasBlocks; row += (int)gridDim.x)
+                bpart[(size_t)row * Cout + co] = 0.f;
         }
     }
 }
@@ -1248,7 +1319,6 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce1(const float *__restrict__
 // db[c] = sum_p dY[p][c] in two fixed-order passes: kBiasBlocks blocks each sum a
 // contiguous range of positions for every channel (consecutive threads read
 // consecutive channels), then one thread per channel adds the block sums in order.
-constexpr int kBiasBlocks = 2048;
 __global__ __launch_bounds__(256) void k_bias_partial(const float *__restrict__ dY, float *part, long long P, int C)
 {
     __shared__ float s[256];
@@ -1787,13 +1857,13 @@ using namespace gp5;
 // pooled form one that pairs planes; `runs` > 0 overrides it.  Measured at 64^3,
 // batch 8 (one full round of workgroups against the 1024-workgroup rule of before):
 // 16 -> 16 62.3 -> 57.9 us at 6 runs, 32 -> 16 upsampled 91.1 -> 85.4 us at 4 runs.
-template <int C, int K, bool W, bool S, bool PL = false>
+template <int C, int K, bool W, bool S, bool PL = false, bool AB = false>
 static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const uint16_t *w, const float *bias,
                      uint16_t *a, const uint16_t *sk, uint16_t *y, float *Yf, int P, int D, int Cout, int mirror,
-                     int act, int runs, int up)
+                     int act, int runs, int up, float *bpart = nullptr)
 {
     static size_t shm_set = 65536;  // the default limit of dynamic LDS
-    const void *fn = (const void *)k_conv_zs<C, K, W, S, PL>;
+    const void *fn = (const void *)k_conv_zs<C, K, W, S, PL, AB>;
     if (shm > shm_set) {
         NNCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm), "conv3d LDS size");
         shm_set = shm;
@@ -1814,8 +1884,12 @@ static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const u
         if (PL)
             while (nzs > 1 && D % (2 * nzs)) --nzs;
     }
-    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(), x,
-                       w, bias, a, sk, y, Yf, P, D, Cout, mirror, act, nzs, up);
+    if (AB) {  // one bias partial row per workgroup
+        if (per_run > kBiasBlocks) return set_error(GP5_ERR_ARG, "conv3d_act_bwd: too many workgroups per plane run");
+        nzs = std::max(1, std::min(nzs, kBiasBlocks / per_run));
+    }
+    hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL, AB>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm,
+                       nn_stream(), x, w, bias, a, sk, y, Yf, P, D, Cout, mirror, act, nzs, up, bpart);
     return GP5_OK;
 }
 
@@ -1944,8 +2018,12 @@ static int conv_split_scratch(size_t floats, float **out)
 
 // pool = 1: Yf (side D/2) receives the 2x2x2 child sums of the convolution (the
 // POOL form of k_conv_zs; only where that kernel applies, else an error)
+// bpart set (gp5_nn_conv3d_act_bwd): where the z-sliding kernel takes the shape
+// with one output block per wave, its ABWD form runs (A read, Y = H written, the
+// bias partials to bpart) and *fused is set; otherwise only Yf is written.
 static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y,
-                      float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up, int pool = 0)
+                      float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up, int pool = 0,
+                      float *bpart = nullptr, int *fused = nullptr)
 {
     if (!X || !Wk) return set_error(GP5_ERR_ARG, "NULL input or weights");
     if (skip && !Y) return set_error(GP5_ERR_ARG, "a skip input needs the Y output");
@@ -2016,6 +2094,24 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
 #define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
     GP5_CONV_ZS_CASE(C, K, 3)
             int zrc = GP5_OK;
+            if (bpart && NBz == 1) {
+                if (bias || act || skip || !A || !Y)
+                    return set_error(GP5_ERR_ARG, "conv3d_act_bwd: no bias, activation or skip");
+                if (Cin == 16)
+                    zrc = zs_launch<16, 1, true, true, false, true>(per_run, gy, shm, x, w, nullptr, a, nullptr, y, Yf,
+                                                                     (int)P, D, Cout, mirror, 0, nzs, up, bpart);
+                else
+                    zrc = zs_launch<32, 1, false, true, false, true>(per_run, gy, shm, x, w, nullptr, a, nullptr, y, Yf,
+                                                                      (int)P, D, Cout, mirror, 0, nzs, up, bpart);
+                if (zrc != GP5_OK) return zrc;
+                NNCHK(hipGetLastError(), "conv3d launch");
+                *fused = 1;
+                return GP5_OK;
+            }
+            if (bpart) {  // the plain data gradient below: only Yf
+                a = nullptr;
+                y = nullptr;
+            }
             if (pool) {
                 if (D < 32 || bias || act || A || Y || !Yf)
                     return set_error(GP5_ERR_ARG, "conv3d_pool: needs 16 or 32 input channels, D = 32, 64 or 128, "
@@ -2041,6 +2137,10 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         }
     }
     if (pool) return set_error(GP5_ERR_ARG, "conv3d_pool: the shape does not take the z-sliding kernel");
+    if (bpart) {  // not fused: only Yf
+        a = nullptr;
+        y = nullptr;
+    }
     // LDS-staged kernel: D a power of two in [8, 128] (workgroups of 256 positions
     // are whole rows of one sample), channel chunk KC = 32 or 16 with a staged
     // region of at most 64 KB, output blocks per wave NB = 4, 2 or 1, the widest
@@ -2138,6 +2238,21 @@ int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A,
 {
     if (act != 0 && act != 1) return set_error(GP5_ERR_ARG, "conv3d: act is 0 (none) or 1 (leaky ReLU)");
     return conv3d_run(Xh, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 1);
+}
+
+static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
+                            long long P, int C, float *out_part = nullptr);
+
+int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+                          int D, int Cin, int Cout)
+{
+    if (!dY || !Wk || !Ab || !Yf || !H || !part) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (Cout % 4 || 256 % (Cout / 4)) return set_error(GP5_ERR_ARG, "conv3d_act_bwd: Cout / 4 must divide 256");
+    int fused = 0;
+    const int rc = conv3d_run(dY, Wk, nullptr, const_cast<void *>(Ab), nullptr, H, Yf, B, D, Cin, Cout, 1, 0, 0, 0,
+                              part, &fused);
+    if (rc != GP5_OK || fused) return rc;
+    return act_bwd_bias_run(Ab, Yf, nullptr, 0.f, D, H, nullptr, (long long)B * D * D * D, Cout, part);
 }
 
 // Plan of k_wgrad_zs for a cube side D (a power of two in [16, 128]): the longest
@@ -2352,7 +2467,7 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
 // db set: the bias gradient is finished here (partials in the shared scratch);
 // else the partials go to the caller's `part` (kBiasBlocks * C floats)
 static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
-                            long long P, int C, float *out_part = nullptr)
+                            long long P, int C, float *out_part)
 {
     if (!A || !dA || !H || (!db && !out_part)) return set_error(GP5_ERR_ARG, "NULL argument");
     if (C < 4 || C % 4 || 256 % (C / 4)) return set_error(GP5_ERR_ARG, "act_bwd_cast_bias: C / 4 must divide 256");

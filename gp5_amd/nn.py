@@ -71,6 +71,7 @@ def _lib():
         "gp5_nn_mse_out0_acc": [P, P, P, P, I, P, LL, P],
         "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
+        "gp5_nn_conv3d_act_bwd": [P, P, P, P, P, P, I, I, I, I],
         "gp5_nn_act_bwd_part": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_bias_final_multi": [P, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
@@ -464,11 +465,13 @@ class UNet:
         # measured slower); nothing on the main stream overwrites what they read
         # before the join at the end.
         self._overlapping = self._side_setup()
-        # c8
+        # c8, c7
         self._side("c8", lambda: self._wgrad("c8", t["d1"], t["h_out"], D))
-        self._dgrad("c8", t["h_out"], t["g16a"], D)            # g16a = dd1 = de1 (skip part)
-        # c7: dz7 = dd1 * act'(a7)
-        self._act_bwd("c7", "a7", "g16a", "h16", D)
+        # c8's data gradient g16a = dd1 = de1 (skip part) and c7's activation backward
+        # dz7 = dd1 * act'(a7) (h16, c7's bias partials) in one pass where the shape allows
+        e = self.w["c8"]
+        _ck(L.gp5_nn_conv3d_act_bwd(t["h_out"].ptr, e["Wb"].ptr, t["a7"].ptr, t["g16a"].ptr, t["h16"].ptr,
+                                    self.bpart["c7"].ptr, B, D, e["co"], e["ci"]))
         self._side("c7", lambda: self._wgrad_up("c7", "d2", "u2", t["h16"], D))
         if _pool_fused(D, 32, 16, B):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
             e = self.w["c7"]

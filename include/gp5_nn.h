@@ -67,6 +67,18 @@ int gp5_nn_conv3d(const void *X, const void *Wk, const float *bias, void *A, con
  * Cin = 16 with Cout a multiple of 32, or Cin = 32; D = 32, 64 or 128; other
  * shapes return GP5_ERR_ARG. */
 int gp5_nn_conv3d_pool(const void *X, const void *Wk, float *Yh, int B, int D, int Cin, int Cout, int mirror);
+/* The data gradient of a layer (gp5_nn_conv3d with mirrored taps, no bias or
+ * activation: dY of Cin channels, Wk its [27][ci][co] copy, Yf float of Cout
+ * channels) followed by the backward of the leaky ReLU of the layer below, whose
+ * output is Ab (bf16, Cout channels): H = bf16(Yf act'(Ab)) and the block partials
+ * of that layer's bias gradient in part (gp5_nn_bias_part_floats(Cout) floats,
+ * finished by gp5_nn_bias_final_multi), as gp5_nn_act_bwd_part(Ab, Yf, NULL, 0,
+ * H, part, ...) would give after the convolution (Yf and H bit for bit; the
+ * partials are the z-sliding kernel's per-workgroup sums where it takes the shape
+ * with one 16-channel output block per wave, one pass; otherwise the two passes
+ * run).  Cout / 4 must divide 256. */
+int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+                          int D, int Cin, int Cout);
 /* The output layer's form: no activation, and the float output Yf0 keeps only
  * channel 0 (one float per position, B*D^3 floats); the other channels are
  * computed (the MFMA tiles are 16 wide) and dropped. */

@@ -139,6 +139,9 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 16, 16, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 64, 32, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, None, 1, 64, 16, 32, 1) == capi.GP5_ERR_ARG
+    # conv3d_act_bwd: every pointer, and Cout / 4 dividing 256
+    assert L.gp5_nn_conv3d_act_bwd(fake, fake, None, fake, fake, fake, 1, 16, 16, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_act_bwd(fake, fake, fake, fake, fake, fake, 1, 16, 16, 48) == capi.GP5_ERR_ARG
     # act_bwd_part needs its partial buffer; bias_final_multi 1..16 complete segments
     assert L.gp5_nn_act_bwd_part(fake, fake, None, 0.0, fake, None, 1, 8, 16) == capi.GP5_ERR_ARG
     bseg = nn.BiasSeg(fake, None, 16)

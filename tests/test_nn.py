@@ -622,3 +622,41 @@ def test_plane_run_counts_leave_results_unchanged(D, Cin, Cout, B, up, monkeypat
             pooled.append(h.download(np.float32, (B, D // 2, D // 2, D // 2, 32)))
         for p in pooled[1:]:
             assert np.array_equal(p, pooled[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,D,Cin,Cout", [(2, 64, 16, 16), (1, 64, 32, 16), (2, 16, 16, 16), (1, 8, 64, 32)])
+def test_conv_act_bwd_equals_the_two_passes(B, D, Cin, Cout):
+    """gp5_nn_conv3d_act_bwd (a data gradient with the activation backward of the
+    layer below in the z-sliding epilogue; the first two shapes take it, the others
+    run the two passes) gives Yf and H bit for bit as gp5_nn_conv3d (mirrored)
+    followed by gp5_nn_act_bwd_part, and the bias gradient from its partials to
+    float rounding."""
+    rng = np.random.default_rng(B + D + Cin + Cout)
+    L = nn._lib()
+    dY = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cout, Cin)) * 0.1)  # [27][ci][co] of the layer: its co is Cin here
+    Ab = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    gb, wb, ab = _buf(dY, True), _buf(W, True), _buf(Ab, True)
+    P = B * D ** 3
+    out = []
+    for fused in (False, True):
+        Yf, H = nn.DevBuf(P * Cout * 4), nn.DevBuf(P * Cout * 2)
+        part, db = nn.DevBuf(L.gp5_nn_bias_part_floats(Cout) * 4), nn.DevBuf(Cout * 4)
+        if fused:
+            nn._ck(L.gp5_nn_conv3d_act_bwd(gb.ptr, wb.ptr, ab.ptr, Yf.ptr, H.ptr, part.ptr, B, D, Cin, Cout))
+        else:
+            nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, Yf.ptr, B, D, Cin, Cout, 1, 0))
+            nn._ck(L.gp5_nn_act_bwd_part(ab.ptr, Yf.ptr, None, 0.0, H.ptr, part.ptr, B, D, Cout))
+        seg = (nn.BiasSeg * 1)(nn.BiasSeg(part.ptr, db.ptr, Cout))
+        nn._ck(L.gp5_nn_bias_final_multi(ctypes.cast(seg, ctypes.c_void_p), 1))
+        out.append((Yf.download(np.float32, (P, Cout)), H.download(np.uint16, (P, Cout)),
+                    db.download(np.float32, (Cout,))))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(ab.download(np.uint16, Ab.shape), nn.to_bf16(Ab))  # Ab is only read
+    ov = out[0][0] * np.where(Ab.reshape(P, Cout) > 0, 1.0, 0.1).astype(np.float32)
+    ref = ov.astype(np.float64).sum(axis=0)
+    tol = 1e-5 * np.abs(ov).astype(np.float64).sum(axis=0) + 1e-6
+    assert np.all(np.abs(out[1][2] - ref) <= tol), (out[1][2], ref)
+    assert np.all(np.abs(out[0][2] - ref) <= tol)
