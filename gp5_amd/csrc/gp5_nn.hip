@@ -1884,10 +1884,7 @@ static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const u
         if (PL)
             while (nzs > 1 && D % (2 * nzs)) --nzs;
     }
-    if (AB) {  // one bias partial row per workgroup
-        if (per_run > kBiasBlocks) return set_error(GP5_ERR_ARG, "conv3d_act_bwd: too many workgroups per plane run");
-        nzs = std::max(1, std::min(nzs, kBiasBlocks / per_run));
-    }
+    if (AB) nzs = std::max(1, std::min(nzs, kBiasBlocks / per_run));  // one bias partial row per workgroup
     hipLaunchKernelGGL((k_conv_zs<C, K, W, S, PL, AB>), dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm,
                        nn_stream(), x, w, bias, a, sk, y, Yf, P, D, Cout, mirror, act, nzs, up, bpart);
     return GP5_OK;
@@ -2019,8 +2016,9 @@ static int conv_split_scratch(size_t floats, float **out)
 // pool = 1: Yf (side D/2) receives the 2x2x2 child sums of the convolution (the
 // POOL form of k_conv_zs; only where that kernel applies, else an error)
 // bpart set (gp5_nn_conv3d_act_bwd): where the z-sliding kernel takes the shape
-// with one output block per wave, its ABWD form runs (A read, Y = H written, the
-// bias partials to bpart) and *fused is set; otherwise only Yf is written.
+// with one output block per wave and at most kBiasBlocks workgroups per plane run,
+// its ABWD form runs (A read, Y = H written, the bias partials to bpart) and
+// *fused is set; otherwise only Yf is written.
 static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A, const void *skip, void *Y,
                       float *Yf, int B, int D, int Cin, int Cout, int mirror, int act, int up, int pool = 0,
                       float *bpart = nullptr, int *fused = nullptr)
@@ -2094,7 +2092,7 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
 #define GP5_CONV_ZS_V(C, K) GP5_CONV_ZS_CASE(C, K, 0) GP5_CONV_ZS_CASE(C, K, 1) GP5_CONV_ZS_CASE(C, K, 2) \
     GP5_CONV_ZS_CASE(C, K, 3)
             int zrc = GP5_OK;
-            if (bpart && NBz == 1) {
+            if (bpart && NBz == 1 && per_run <= kBiasBlocks) {  // one bias partial row per workgroup
                 if (bias || act || skip || !A || !Y)
                     return set_error(GP5_ERR_ARG, "conv3d_act_bwd: no bias, activation or skip");
                 if (Cin == 16)
