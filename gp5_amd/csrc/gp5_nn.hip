@@ -2142,7 +2142,7 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
     // LDS-staged kernel: D a power of two in [8, 128] (workgroups of 256 positions
     // are whole rows of one sample), channel chunk KC = 32 or 16 with a staged
     // region of at most 64 KB, output blocks per wave NB = 4, 2 or 1, the widest
-    // that still gives 1024 workgroups.  GP5_CONV_LDS=0 selects the direct kernel.
+    // that still gives 512 workgroups.  GP5_CONV_LDS=0 selects the direct kernel.
     const char *lds_env = getenv("GP5_CONV_LDS");
     if (pow2 && !(lds_env && lds_env[0] == '0')) {
         const int Rb = std::min(D, 256 / D), PZ = 256 / (D * Rb);
@@ -2153,8 +2153,14 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         if (KCl) {
             const long long wgs = P / 256;
             int NBl = 1;
+            // (512 workgroups: c5's data gradient at 16^3, batch 8, 41.8 -> 37.4 us with 2
+            // blocks; the other layers' choices are unchanged, and near their best in a
+            // sweep of 1, 2, 4 blocks and 1, 2, 4 splits)
             for (int nb : {4, 2})
-                if (NBl == 1 && nblk % nb == 0 && wgs * (nblk / nb) >= 1024) NBl = nb;
+                if (NBl == 1 && nblk % nb == 0 && wgs * (nblk / nb) >= 512) NBl = nb;
+            const char *nb_env = getenv("GP5_CONV_LDS_NB");  // measurement: 1, 2 or 4
+            if (nb_env && (atoi(nb_env) == 1 || atoi(nb_env) == 2 || atoi(nb_env) == 4) && nblk % atoi(nb_env) == 0)
+                NBl = atoi(nb_env);
             // fewer workgroups than CUs: split the channel chunks S ways (S a power of two
             // dividing the chunk count, up to 256 workgroups), the splits' float sums added
             // in order by k_conv_split_finish; GP5_CONV_SPLIT=0 turns it off.  Measured at
@@ -2166,6 +2172,7 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
             int S = 1;
             if (!(sp_env && sp_env[0] == '0'))
                 while (chunks % (2 * S) == 0 && wgs * (nblk / NBl) * S < 256) S *= 2;
+            if (sp_env && atoi(sp_env) > 1 && chunks % atoi(sp_env) == 0) S = atoi(sp_env);  // measurement
             float *ws = nullptr;
             if (S > 1) {
                 const int rc2 = conv_split_scratch((size_t)S * P * Cout, &ws);
