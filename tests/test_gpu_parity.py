@@ -1,7 +1,7 @@
 """HIP path vs the CPU oracle, through the C ABI (libgp5.so via ctypes).
 
-Bar: relative L2 <= 1e-12 for stepped trajectories (floating point; the FP64
-tolerance of the north star), <= 1e-13 for single transforms.  The
+Bar: relative L2 <= 1e-12 for stepped trajectories (floating point; FP64 RK4
+at 256^3 must match the reference to 1e-12), <= 1e-13 for single transforms.  The
 reference's own known-answer tests are reproduced with their tolerances and
 with the tighter FP64 bar.
 """
