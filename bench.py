@@ -273,19 +273,27 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
 def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     """MFMA throughput of the warm-start network (3D U-Net, bf16 MFMA with float
     accumulation) at D^3 with `batch` samples: forward passes and training steps
-    (forward, backward, Adam) on synthetic inputs.  Flops: the sum over the eight
-    3x3x3 convolutions of 2 * 27 * Cin * Cout * D_layer^3 * batch for a forward pass;
-    a training step adds the weight gradients of all eight and the data gradients of
-    c2..c8 (the input layer's data gradient is never computed), so three times the
-    forward count less c1's share; pooling, upsampling and the elementwise kernels
-    count 0.  Wall time between device synchronisations (the loss is read back every
-    step, as training does), so launch costs are included."""
+    (forward, backward, Adam) on synthetic inputs.  Model flops: the sum over the
+    eight 3x3x3 convolutions of 2 * 27 * Cin * Cout * D_layer^3 * batch for a forward
+    pass; a training step adds the weight gradients of all eight and the data
+    gradients of c2..c8 (the input layer's data gradient is never computed), so three
+    times the forward count less c1's share; pooling, upsampling and the elementwise
+    kernels count 0.  The roofline fractions count the MFMA flops the kernels
+    execute: the forward of a decoder layer with 32 input channels at side 32..128
+    (c7) runs by output parity, 16 (coarse tap, x parity) pairs of hi and lo MFMAs
+    for two fine positions, 32/54 of its model count.  Wall time between device
+    synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
 
     div = {"c1": 1, "c2": 2, "c3": 4, "c4": 8, "c5": 4, "c6": 2, "c7": 1, "c8": 1}
     layer = {name: 2.0 * 27 * ci * co * (D // div[name]) ** 3 * batch for name, ci, co in nn.SHAPES}
-    fwd_flops = sum(layer.values())
-    train_flops = 3 * fwd_flops - layer["c1"]
+    model_fwd = sum(layer.values())
+    model_train = 3 * model_fwd - layer["c1"]
+    parity = os.environ.get("GP5_CONV_UP2P", "1") != "0"
+    by_parity = {name for name, ci, co in nn.SHAPES
+                 if parity and name in ("c5", "c6", "c7") and ci == 32 and D // div[name] in (32, 64, 128)}
+    fwd_flops = sum(f * (32.0 / 54.0 if name in by_parity else 1.0) for name, f in layer.items())
+    train_flops = fwd_flops + 2 * model_fwd - layer["c1"]
     net = nn.UNet(batch, D, seed=0)
     rng = np.random.default_rng(0)
     X = (0.5 * rng.standard_normal((batch, D, D, D, nn.C0))).astype(np.float32)
@@ -314,8 +322,12 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
         "workload": f"3D U-Net 16-32-64-128 channels, {D}^3, batch {batch}, bf16 MFMA, float accumulation",
         "forward_flops": fwd_flops, "forward_ms": 1e3 * fwd_s, "forward_TFLOPs": fwd_tf,
         "train_step_flops": train_flops, "train_step_ms": 1e3 * train_s, "train_TFLOPs": train_tf,
+        "model_forward_flops": model_fwd, "model_forward_TFLOPs": model_fwd / fwd_s / 1e12,
+        "model_train_step_flops": model_train, "model_train_TFLOPs": model_train / train_s / 1e12,
+        "by_output_parity": sorted(by_parity),
         "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS},
+                     "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
+                     "flops": "executed MFMA flops"},
     }
 
 

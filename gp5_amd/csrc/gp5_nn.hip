@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <utility>
 #include <mutex>
 #include <vector>
 
@@ -137,6 +138,51 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
 constexpr int kEpiStride = 20;                       // floats per scratch row (16 channels, conflict-free)
 constexpr int kEpiScratch = 32 * kEpiStride * 4;     // bytes per wave
 
+// The outputs of 32 consecutive positions pb .. pb + 31 (16 channels cb .. cb + 15
+// of each) from the float scratch T (position-major rows of kEpiStride floats):
+// Yf (all channels, or channel 0 only with yf0), A = bf16(v) and Y = bf16(v + skip)
+// or bf16(v), each when set, with 16-byte accesses (called by one whole wave).
+__device__ __forceinline__ void epi_write32(const float *T, int pb, int P, int Cout, int cb, int lane, uint16_t *A,
+                                            const uint16_t *skip, uint16_t *Y, float *Yf, bool yf0)
+{
+    if (Yf && !yf0) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int pos = lane / 4 + 16 * k, c4 = 4 * (lane & 3);
+            if (pb + pos < P)
+                *reinterpret_cast<float4 *>(Yf + (size_t)(pb + pos) * Cout + cb + c4) =
+                    *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c4);
+        }
+    } else if (Yf && cb == 0 && lane < 32) {  // channel 0 of the 32 positions
+        if (pb + lane < P) Yf[pb + lane] = T[lane * kEpiStride];
+    }
+    if (A || Y) {
+        const int pos = lane / 2, c8 = 8 * (lane & 1);
+        if (pb + pos < P) {
+            const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
+            const float4 hi = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8 + 4);
+            const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            const size_t o = (size_t)(pb + pos) * Cout + cb + c8;
+            uint16_t hb[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) hb[e] = f2bf(v[e]);
+            if (A) *reinterpret_cast<uint4 *>(A + o) = __builtin_bit_cast(uint4, hb);
+            if (Y) {
+                if (skip) {
+                    const uint4 sk4 = *reinterpret_cast<const uint4 *>(skip + o);
+                    const uint16_t *sk = reinterpret_cast<const uint16_t *>(&sk4);
+                    uint16_t yb[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) yb[e] = f2bf(v[e] + bf2f(sk[e]));
+                    *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, yb);
+                } else {
+                    *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
+                }
+            }
+        }
+    }
+}
+
 // With WS the scratch is wave-private and the exchange needs no workgroup
 // barrier: a wave's LDS writes and reads execute in issue order, and the wave
 // barrier keeps the compiler from moving them across each other.  The caller
@@ -175,18 +221,8 @@ __device__ __forceinline__ void conv_epilogue_lds_impl(const f32x4 (&acc)[MT][NB
             if constexpr (WS) __builtin_amdgcn_wave_barrier();
             else __syncthreads();
             const int pb = base + 32 * h;  // first position of the pair
-            if (Yf && !yf0) {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const int pos = lane / 4 + 16 * k, c4 = 4 * (lane & 3);
-                    if (pb + pos < P)
-                        *reinterpret_cast<float4 *>(Yf + (size_t)(pb + pos) * Cout + cb + c4) =
-                            *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c4);
-                }
-            } else if (Yf && cb == 0 && lane < 32) {  // channel 0 of the 32 positions
-                if (pb + lane < P) Yf[pb + lane] = T[lane * kEpiStride];
-            }
             if constexpr (ABWD) {
+                epi_write32(T, pb, P, Cout, cb, lane, nullptr, nullptr, nullptr, Yf, yf0);
                 const int pos = lane / 2, c8 = 8 * (lane & 1);
                 if (pb + pos < P) {
                     const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
@@ -204,30 +240,8 @@ __device__ __forceinline__ void conv_epilogue_lds_impl(const f32x4 (&acc)[MT][NB
                     }
                     *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
                 }
-            } else if (A || Y) {
-                const int pos = lane / 2, c8 = 8 * (lane & 1);
-                if (pb + pos < P) {
-                    const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
-                    const float4 hi = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8 + 4);
-                    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                    const size_t o = (size_t)(pb + pos) * Cout + cb + c8;
-                    uint16_t hb[8];
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) hb[e] = f2bf(v[e]);
-                    if (A) *reinterpret_cast<uint4 *>(A + o) = __builtin_bit_cast(uint4, hb);
-                    if (Y) {
-                        if (skip) {
-                            const uint4 sk4 = *reinterpret_cast<const uint4 *>(skip + o);
-                            const uint16_t *sk = reinterpret_cast<const uint16_t *>(&sk4);
-                            uint16_t yb[8];
-#pragma unroll
-                            for (int e = 0; e < 8; ++e) yb[e] = f2bf(v[e] + bf2f(sk[e]));
-                            *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, yb);
-                        } else {
-                            *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
-                        }
-                    }
-                }
+            } else {
+                epi_write32(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0);
             }
             if constexpr (WS) __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next pair
             else __syncthreads();
@@ -815,6 +829,217 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ABWD && CIN
             bpart[(size_t)blockIdx.x * Cout + co] = t;
             for (int row = (int)blockIdx.x + (int)gridDim.x; row < kBiasBlocks; row += (int)gridDim.x)
                 bpart[(size_t)row * Cout + co] = 0.f;
+        }
+    }
+}
+
+// Convolution of the nearest upsampling of X (side Dc = D/2, CIN = 32 channels)
+// by output parity.  Fine position 2q + p (p in {0,1}^3) sees, along each axis,
+// the coarse cells q + o with o in {p - 1, p}: the 3x3x3 fine taps t fold into
+// 2x2x2 coarse taps with weights W_p[o] = sum of W[t] over floor((p + t) / 2) = o.
+// A workgroup owns one (py, pz) and both x parities of 16 output channels: its
+// 12 coarse taps (oz, oy in the window, ox = -1, 0, 1) make 16 (tap, px) pairs
+// against 27 taps for each fine tile, and every A fragment (16 coarse positions
+// x 32 channels, one 16x16x32 MFMA operand) serves one or both x parities.  The
+// folded weights are summed in float from the bf16 ones and kept as bf16 hi + lo
+// (hi = bf16(s), lo = bf16(s - hi)), two MFMAs per pair, so that the result
+// agrees with the direct convolution to float rounding.  The walk over coarse
+// planes, the three-slab ring and the epilogue follow k_conv_zs (side Dc, no
+// upsampling in the staging); the epilogue interleaves the two x parities into
+// 32 consecutive fine positions.
+// f(std::integral_constant<int, K>) for K = 0 .. N - 1: indices known at compile time
+template <typename F, int... Ks>
+__device__ __forceinline__ void static_for(F &&f, std::integer_sequence<int, Ks...>)
+{
+    (f(std::integral_constant<int, Ks>{}), ...);
+}
+
+constexpr int up2p_pair(int k, int px)  // index of (tap k, x parity px) among the 16 pairs, or -1
+{
+    int n = 0;
+    for (int kk = 0; kk < k; ++kk) n += (kk % 3 == 1) ? 2 : 1;
+    const int ox = k % 3 - 1;
+    if (ox == -1) return px == 0 ? n : -1;
+    if (ox == 1) return px == 1 ? n : -1;
+    return n + px;
+}
+
+template <int CIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_up2p(
+    const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk, const float *__restrict__ bias, uint16_t *A,
+    const uint16_t *skip, uint16_t *Y, float *Yf, int P, int D, int Cout, int act, int nzs)
+{
+    static_assert(CIN == 32, "one 16x16x32 operand per position");
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    // MT = 2 tiles of 16 coarse positions per wave, 128 per workgroup and plane
+    // (Rb = 128 / Dc rows): the 32 folded weight fragments take 128 VGPRs
+    constexpr int MT = 2, U8 = CIN / 8, NU = 5;  // a slab has at most 4 x 66 x 4 units (Dc = 64)
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int Dc = D >> 1;
+    const int nb0 = (int)blockIdx.y >> 2, py = ((int)blockIdx.y >> 1) & 1, pz = (int)blockIdx.y & 1;
+    const int Rb = 128 / Dc, RY = Rb + 2, RX = Dc + 2;
+    const int UR = RX * U8, SU = RY * UR;
+    const float invUR = 1.f / (float)UR;
+    const int nyb = Dc / Rb;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * Dc / nzs, ZL = (zsg + 1) * Dc / nzs - zb;  // coarse rows and planes
+    const int sb = smp * Dc * Dc * Dc;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> 3) * CIN * 2);
+    const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wk, 27 * Cout * CIN * 2);
+    int lo[MT][3];  // LDS byte offset of this lane's fragment in tile t at coarse taps (0, 0, ox = d - 1), slot 0
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int q = 16 * MT * wave + 16 * t + r;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ix = (q & (Dc - 1)) + d;
+            lo[t][d] = (((q / Dc) + 1) * RX + ix) * CIN * 2 + swz_frag<CIN>(g, ix);
+        }
+    }
+    // folded weights of the 16 (tap, px) pairs: lane (r, g) holds output channel
+    // 16 nb0 + r, input channels 8g .. 8g + 7
+    uint4 whi[16], wlo[16];
+    {
+        const int wvo = ((nb0 * 16 + r) * CIN + g * 8) * 2;
+        auto fold = [&](auto kc, auto pc) {
+            constexpr int k = decltype(kc)::value, px = decltype(pc)::value, pi = up2p_pair(k, px);
+            if constexpr (pi >= 0) {
+                constexpr int ox = k % 3 - 1;
+                const int oz = pz - 1 + k / 6, oy = py - 1 + (k / 3) % 2;
+                float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int tz = -1; tz <= 1; ++tz)
+#pragma unroll
+                    for (int ty = -1; ty <= 1; ++ty)
+#pragma unroll
+                        for (int tx = -1; tx <= 1; ++tx) {
+                            if ((px + tx + 2) / 2 - 1 != ox) continue;  // compile time
+                            if ((pz + tz + 2) / 2 - 1 != oz || (py + ty + 2) / 2 - 1 != oy) continue;
+                            const int tap = 9 * (tz + 1) + 3 * (ty + 1) + (tx + 1);
+                            const uint4 wv = __builtin_bit_cast(
+                                uint4, __builtin_amdgcn_raw_buffer_load_b128(wr, wvo, tap * Cout * CIN * 2, 0));
+                            const uint32_t wd[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+                            for (int e = 0; e < 8; ++e)
+                                sum[e] += __uint_as_float(e & 1 ? wd[e / 2] & 0xffff0000u : wd[e / 2] << 16);
+                        }
+                uint32_t hd[4], ld[4];
+#pragma unroll
+                for (int e2 = 0; e2 < 4; ++e2) {
+                    const uint32_t h0 = f2bf(sum[2 * e2]), h1 = f2bf(sum[2 * e2 + 1]);
+                    const uint32_t l0 = f2bf(sum[2 * e2] - bf2f((uint16_t)h0)),
+                                   l1 = f2bf(sum[2 * e2 + 1] - bf2f((uint16_t)h1));
+                    hd[e2] = h0 | (h1 << 16);
+                    ld[e2] = l0 | (l1 << 16);
+                }
+                whi[pi] = make_uint4(hd[0], hd[1], hd[2], hd[3]);
+                wlo[pi] = make_uint4(ld[0], ld[1], ld[2], ld[3]);
+            }
+        };
+        static_for([&](auto kc) {
+            fold(kc, std::integral_constant<int, 0>{});
+            fold(kc, std::integral_constant<int, 1>{});
+        }, std::make_integer_sequence<int, 12>{});
+    }
+    uint4 v[NU];
+    int us[NU];  // swizzled unit of the LDS write
+    auto load_slab = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+            const int u = tid + 256 * k;
+            const int row = udiv_small(u, UR, invUR), e = u - row * UR;
+            const int ix = e / U8, c8 = e - ix * U8;
+            us[k] = u - c8 + (c8 ^ swz_unit<CIN>(ix));
+            const int y = y0 - 1 + row;
+            const bool in = u < SU && z >= 0 && z < Dc && y >= 0 && y < Dc && ix >= 1 && ix <= Dc;
+            const int voff = in ? ((sb + (z * Dc + y) * Dc + (ix - 1)) * CIN + 8 * c8) * 2 : kOffNone;
+            v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, 0, 0));
+        }
+    };
+    auto store_slab = [&](int slot) {
+        unsigned char *dst = sx + (size_t)slot * SU * 16;
+#pragma unroll
+        for (int k = 0; k < NU; ++k)
+            if (tid + 256 * k < SU) *reinterpret_cast<uint4 *>(dst + 16 * us[k]) = v[k];
+    };
+    load_slab(zb - 1);
+    store_slab(0);
+    load_slab(zb);
+    store_slab(1);
+    load_slab(zb + 1);
+    store_slab(2);
+    __syncthreads();
+    const int yf0 = (act & kActYf0) != 0;
+    const int a1 = act & 1;
+    const int cb = 16 * nb0;
+    const float bco = bias ? bias[cb + r] : 0.f;
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;  // coarse plane; the outputs are fine plane 2 z + pz
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) asm volatile("" : "+v"(lo[t][d]));
+        if (zi + 1 < ZL) load_slab(z + 2);  // in flight during this plane's MFMAs
+        f32x4 acc[MT][2];
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int px = 0; px < 2; ++px) acc[t][px] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // the slab of plane z + o is in slot (zi + 1 + o) % 3
+        static_for([&](auto kc) {
+            constexpr int k = decltype(kc)::value, ox = k % 3 - 1;
+            const int oz = pz - 1 + k / 6, oy = py - 1 + (k / 3) % 2;
+            const int toff = ((zi + 1 + oz) % 3) * SU * 16 + oy * RX * CIN * 2;
+            uint4 a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                int off = lo[t][ox + 1] + toff;
+                asm volatile("" : "+v"(off));
+                a[t] = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(sx + off, 16));
+            }
+            auto pass = [&](auto pc) {
+                constexpr int px = decltype(pc)::value, pi = up2p_pair(k, px);
+                if constexpr (pi >= 0) {
+                    const bf8 bh = __builtin_bit_cast(bf8, whi[pi]), bl = __builtin_bit_cast(bf8, wlo[pi]);
+#pragma unroll
+                    for (int t = 0; t < MT; ++t) {
+                        const bf8 av = __builtin_bit_cast(bf8, a[t]);
+                        acc[t][px] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bh, acc[t][px], 0, 0, 0);
+                        acc[t][px] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bl, acc[t][px], 0, 0, 0);
+                    }
+                }
+            };
+            pass(std::integral_constant<int, 0>{});
+            pass(std::integral_constant<int, 1>{});
+        }, std::make_integer_sequence<int, 12>{});
+        const int free_slot = zi % 3;  // plane z - 1
+        __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
+        float *T = reinterpret_cast<float *>(sx + (size_t)free_slot * SU * 16 + wave * kEpiScratch);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            // coarse tile: 16 positions from q0 (row q0 / Dc, columns q0 % Dc ..) -> fine
+            // positions 2 (q0 % Dc) .. + 31 of fine row 2 (y0 + q0 / Dc) + py, plane 2 z + pz
+#pragma unroll
+            for (int px = 0; px < 2; ++px)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float vv = acc[t][px][i] + bco;
+                    if (a1) vv = vv > 0.f ? vv : kLeak * vv;
+                    T[(2 * (4 * g + i) + px) * kEpiStride + r] = vv;
+                }
+            __builtin_amdgcn_wave_barrier();
+            const int q0 = 16 * MT * wave + 16 * t;
+            const int fy = 2 * (y0 + q0 / Dc) + py, fz = 2 * z + pz, fx = 2 * (q0 & (Dc - 1));
+            const int pb = ((smp * D + fz) * D + fy) * D + fx;
+            epi_write32(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0 != 0);
+            __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next tile
+        }
+        if (zi + 1 < ZL) {
+            __syncthreads();  // every wave is done with its scratch in the slot
+            store_slab(free_slot);
+            __syncthreads();
         }
     }
 }
@@ -1890,6 +2115,43 @@ static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const u
     return GP5_OK;
 }
 
+// Launch of k_conv_up2p (X of side D/2, 32 channels, D = 32, 64 or 128; slabs of
+// at most 5 x 256 units of 16 bytes): coarse
+// plane runs that fill the device's workgroup slots once, as for k_conv_zs, or
+// `runs` > 0 of them.
+static int up2p_launch(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a, const uint16_t *sk,
+                       uint16_t *y, float *Yf, int B, int P, int D, int Cout, int act, int runs)
+{
+    const int Dc = D / 2, Rb = 128 / Dc;
+    const size_t shm = (size_t)3 * (Rb + 2) * (Dc + 2) * 4 * 16;
+    const void *fn = (const void *)k_conv_up2p<32>;
+    static size_t shm_set = 65536;  // the default limit of dynamic LDS
+    if (shm > shm_set) {
+        NNCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm), "conv3d LDS size");
+        shm_set = shm;
+    }
+    const int per_run = B * (Dc / Rb), gy = (Cout / 16) * 4;
+    int nzs = runs;
+    if (nzs <= 0) {
+        static size_t occ_shm = 0;
+        static int slots = 0;
+        if (occ_shm != shm) {
+            int dev = 0, cus = 0, per_cu = 0;
+            NNCHK(hipGetDevice(&dev), "conv3d device");
+            NNCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "conv3d CU count");
+            NNCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, shm), "conv3d occupancy");
+            slots = std::max(1, cus * per_cu);
+            occ_shm = shm;
+        }
+        nzs = slots / (per_run * gy);
+    }
+    nzs = std::max(1, std::min(Dc / 2, nzs));
+    hipLaunchKernelGGL(k_conv_up2p<32>, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(), x,
+                       w, bias, a, sk, y, Yf, P, D, Cout, act, nzs);
+    NNCHK(hipGetLastError(), "conv3d launch");
+    return GP5_OK;
+}
+
 extern "C" {
 
 int gp5_nn_malloc(void **p, size_t bytes)
@@ -2060,6 +2322,15 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
     // z-sliding kernel: 16 or 32 input channels, D a power of two in [16, 128], the
     // longest plane run ZL (a power of two, at least 2) that keeps 1024 workgroups,
     // as long as there are at least 512.  GP5_CONV_ZS=0 turns it off.
+    // the upsampled input by output parity (k_conv_up2p): 32 input channels, D = 32,
+    // 64 or 128, forward taps; GP5_CONV_UP2P=0 turns it off, GP5_UP2P_RUNS sets the
+    // coarse plane runs (measurement)
+    const char *pp_env = getenv("GP5_CONV_UP2P");
+    if (up && !mirror && !pool && !bpart && Cin == 32 && (D == 32 || D == 64 || D == 128) &&
+        !(pp_env && pp_env[0] == '0')) {
+        const char *pr = getenv("GP5_UP2P_RUNS");
+        return up2p_launch(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, pr ? atoi(pr) : 0);
+    }
     const char *zs_env = getenv("GP5_CONV_ZS");
     if (pow2 && D >= 16 && (Cin == 16 || Cin == 32) && !(zs_env && zs_env[0] == '0')) {
         const int NBz = (Cin == 16 && nblk % 2 == 0) ? 2 : 1;

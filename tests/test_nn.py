@@ -510,9 +510,12 @@ def test_wgrad_z_sliding_path(D, Cin, Cout, B, monkeypatch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B", [(64, 32, 16, 1), (32, 64, 32, 1), (16, 128, 64, 2), (8, 64, 32, 2)])
-def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B):
+def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B, monkeypatch):
     """gp5_nn_conv3d_up / gp5_nn_conv3d_wgrad_up on a cube of side D/2 give bit for
-    bit what gp5_nn_conv3d / gp5_nn_conv3d_wgrad give on its nearest upsampling."""
+    bit what gp5_nn_conv3d / gp5_nn_conv3d_wgrad give on its nearest upsampling
+    (the kernels that read the parent cell; the output-parity form, which sums in
+    another order, is off here and has its own test)."""
+    monkeypatch.setenv("GP5_CONV_UP2P", "0")
     rng = np.random.default_rng(D * 7 + Cin)
     Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
     U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
@@ -660,3 +663,42 @@ def test_conv_act_bwd_equals_the_two_passes(B, D, Cin, Cout):
     tol = 1e-5 * np.abs(ov).astype(np.float64).sum(axis=0) + 1e-6
     assert np.all(np.abs(out[1][2] - ref) <= tol), (out[1][2], ref)
     assert np.all(np.abs(out[0][2] - ref) <= tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cout,B,skipped", [(64, 16, 1, True), (32, 32, 2, False), (128, 16, 1, True),
+                                              (32, 16, 1, True)])
+def test_upsampled_conv_by_output_parity(D, Cout, B, skipped, monkeypatch):
+    """gp5_nn_conv3d_up with 32 input channels at D = 32..128 runs the convolution
+    of the upsampled input by output parity (k_conv_up2p: 2x2x2 coarse taps with
+    folded weights held as bf16 hi + lo).  Its float output agrees with the
+    z-sliding kernel on the upsampled input (GP5_CONV_UP2P=0) to float rounding,
+    its bf16 outputs (A, and Y with the skip input) to one bf16 step, and with the
+    oracle to 1e-4."""
+    Cin = 32
+    rng = np.random.default_rng(D + Cout + B)
+    L = nn._lib()
+    Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    S = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    xh, wf, bb, sk = _buf(Xh, True), _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(b), _buf(S, True)
+    P = B * D ** 3
+    outs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GP5_CONV_UP2P", flag)
+        Yf = nn.DevBuf(P * Cout * 4)
+        nn._ck(L.gp5_nn_conv3d_up(xh.ptr, wf.ptr, bb.ptr, None, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        A, Y = nn.DevBuf(P * Cout * 2), nn.DevBuf(P * Cout * 2)
+        nn._ck(L.gp5_nn_conv3d_up(xh.ptr, wf.ptr, bb.ptr, A.ptr, sk.ptr if skipped else None, Y.ptr, None, B, D, Cin,
+                                  Cout, 0, 1))
+        outs.append((Yf.download(np.float32, (P, Cout)), nn.from_bf16(A.download(np.uint16, (P, Cout))),
+                     nn.from_bf16(Y.download(np.uint16, (P, Cout)))))
+    scale = np.max(np.abs(outs[1][0]))
+    assert np.max(np.abs(outs[0][0] - outs[1][0])) <= 1e-5 * scale
+    for k in (1, 2):
+        assert np.max(np.abs(outs[0][k] - outs[1][k])) <= 2.0 ** -7 * (scale + (np.max(np.abs(S)) if k == 2 else 0))
+    if D <= 64:
+        U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
+        ref = R.lrelu(R.conv(U, W, b.astype(np.float64))).reshape(P, Cout)
+        assert np.max(np.abs(outs[0][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
