@@ -279,9 +279,9 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     gradients of c2..c8 (the input layer's data gradient is never computed), so three
     times the forward count less c1's share; pooling, upsampling and the elementwise
     kernels count 0.  The roofline fractions count the MFMA flops the kernels
-    execute: the forward of a decoder layer with 32 input channels at side 32..128
-    (c7) runs by output parity, 16 (coarse tap, x parity) pairs of hi and lo MFMAs
-    for two fine positions, 32/54 of its model count.  Wall time between device
+    execute: the forward of a decoder layer with 32 or 64 input channels at side
+    32..128 (c6, c7) runs by output parity, 16 (coarse tap, x parity) pairs of hi and
+    lo MFMAs for two fine positions, 32/54 of its model count.  Wall time between device
     synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
 
@@ -291,7 +291,7 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     model_train = 3 * model_fwd - layer["c1"]
     parity = os.environ.get("GP5_CONV_UP2P", "1") != "0"
     by_parity = {name for name, ci, co in nn.SHAPES
-                 if parity and name in ("c5", "c6", "c7") and ci == 32 and D // div[name] in (32, 64, 128)}
+                 if parity and name in ("c5", "c6", "c7") and ci in (32, 64) and D // div[name] in (32, 64, 128)}
     fwd_flops = sum(f * (32.0 / 54.0 if name in by_parity else 1.0) for name, f in layer.items())
     train_flops = fwd_flops + 2 * model_fwd - layer["c1"]
     net = nn.UNet(batch, D, seed=0)

@@ -869,17 +869,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk, const float *__restrict__ bias, uint16_t *A,
     const uint16_t *skip, uint16_t *Y, float *Yf, int P, int D, int Cout, int act, int nzs)
 {
-    static_assert(CIN == 32, "one 16x16x32 operand per position");
+    static_assert(CIN == 32 || CIN == 64, "32-channel 16x16x32 operands");
     typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-    // MT = 2 tiles of 16 coarse positions per wave, 128 per workgroup and plane
-    // (Rb = 128 / Dc rows): the 32 folded weight fragments take 128 VGPRs
-    constexpr int MT = 2, U8 = CIN / 8, NU = 5;  // a slab has at most 4 x 66 x 4 units (Dc = 64)
+    // MT = 2 tiles of 16 coarse positions per wave: the 32 folded weight fragments
+    // of one 32-channel half take 128 VGPRs.  CIN = 64 splits the channels over two
+    // wave pairs (KS = 2: waves 0, 1 channels 0..31, waves 2, 3 channels 32..63, for
+    // the same positions), whose sums are added in that order through LDS before the
+    // epilogue of waves 0, 1.  A workgroup covers NPOS coarse positions per plane
+    // (Rb = NPOS / Dc rows).
+    constexpr int KS = CIN / 32, WG = 4 / KS, MT = 2, NPOS = 16 * MT * WG, U8 = CIN / 8;
+    constexpr int NU = CIN == 32 ? 5 : 7;  // slab units per thread: (Rb + 2) (Dc + 2) U8 <= 256 NU for Dc <= 64
     extern __shared__ __align__(16) unsigned char sx[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
+    const int wv = wave % WG, kh = wave / WG;  // position group, channel half
     const int Dc = D >> 1;
     const int nb0 = (int)blockIdx.y >> 2, py = ((int)blockIdx.y >> 1) & 1, pz = (int)blockIdx.y & 1;
-    const int Rb = 128 / Dc, RY = Rb + 2, RX = Dc + 2;
+    const int Rb = NPOS / Dc, RY = Rb + 2, RX = Dc + 2;
     const int UR = RX * U8, SU = RY * UR;
     const float invUR = 1.f / (float)UR;
     const int nyb = Dc / Rb;
@@ -891,18 +897,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     int lo[MT][3];  // LDS byte offset of this lane's fragment in tile t at coarse taps (0, 0, ox = d - 1), slot 0
 #pragma unroll
     for (int t = 0; t < MT; ++t) {
-        const int q = 16 * MT * wave + 16 * t + r;
+        const int q = 16 * MT * wv + 16 * t + r;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int ix = (q & (Dc - 1)) + d;
-            lo[t][d] = (((q / Dc) + 1) * RX + ix) * CIN * 2 + swz_frag<CIN>(g, ix);
+            // 16-byte unit 4 kh + g of the position, swizzled within its group of four
+            lo[t][d] = (((q / Dc) + 1) * RX + ix) * CIN * 2 + 16 * ((4 * kh + g) ^ swz_unit<32>(ix));
         }
     }
     // folded weights of the 16 (tap, px) pairs: lane (r, g) holds output channel
-    // 16 nb0 + r, input channels 8g .. 8g + 7
+    // 16 nb0 + r, input channels 32 kh + 8g .. + 7
     uint4 whi[16], wlo[16];
     {
-        const int wvo = ((nb0 * 16 + r) * CIN + g * 8) * 2;
+        const int wvo = ((nb0 * 16 + r) * CIN + 32 * kh + g * 8) * 2;
         auto fold = [&](auto kc, auto pc) {
             constexpr int k = decltype(kc)::value, px = decltype(pc)::value, pi = up2p_pair(k, px);
             if constexpr (pi >= 0) {
@@ -951,7 +958,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const int u = tid + 256 * k;
             const int row = udiv_small(u, UR, invUR), e = u - row * UR;
             const int ix = e / U8, c8 = e - ix * U8;
-            us[k] = u - c8 + (c8 ^ swz_unit<CIN>(ix));
+            us[k] = u - c8 + (c8 ^ swz_unit<32>(ix));
             const int y = y0 - 1 + row;
             const bool in = u < SU && z >= 0 && z < Dc && y >= 0 && y < Dc && ix >= 1 && ix <= Dc;
             const int voff = in ? ((sb + (z * Dc + y) * Dc + (ix - 1)) * CIN + 8 * c8) * 2 : kOffNone;
@@ -1016,25 +1023,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }, std::make_integer_sequence<int, 12>{});
         const int free_slot = zi % 3;  // plane z - 1
         __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
-        float *T = reinterpret_cast<float *>(sx + (size_t)free_slot * SU * 16 + wave * kEpiScratch);
+        unsigned char *freeb = sx + (size_t)free_slot * SU * 16;
+        if constexpr (KS == 2) {  // the second channel half's sums, added in order by waves 0, 1
+            float *red = reinterpret_cast<float *>(freeb);  // WG x MT x 2 x 4 x 64 floats (8 KB)
+            if (kh == 1)
 #pragma unroll
-        for (int t = 0; t < MT; ++t) {
-            // coarse tile: 16 positions from q0 (row q0 / Dc, columns q0 % Dc ..) -> fine
-            // positions 2 (q0 % Dc) .. + 31 of fine row 2 (y0 + q0 / Dc) + py, plane 2 z + pz
+                for (int t = 0; t < MT; ++t)
 #pragma unroll
-            for (int px = 0; px < 2; ++px)
+                    for (int px = 0; px < 2; ++px)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    float vv = acc[t][px][i] + bco;
-                    if (a1) vv = vv > 0.f ? vv : kLeak * vv;
-                    T[(2 * (4 * g + i) + px) * kEpiStride + r] = vv;
-                }
-            __builtin_amdgcn_wave_barrier();
-            const int q0 = 16 * MT * wave + 16 * t;
-            const int fy = 2 * (y0 + q0 / Dc) + py, fz = 2 * z + pz, fx = 2 * (q0 & (Dc - 1));
-            const int pb = ((smp * D + fz) * D + fy) * D + fx;
-            epi_write32(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0 != 0);
-            __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next tile
+                        for (int i = 0; i < 4; ++i) red[(((wv * MT + t) * 2 + px) * 4 + i) * 64 + lane] = acc[t][px][i];
+            __syncthreads();
+            if (kh == 0)
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+#pragma unroll
+                    for (int px = 0; px < 2; ++px)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) acc[t][px][i] += red[(((wv * MT + t) * 2 + px) * 4 + i) * 64 + lane];
+        }
+        if (kh == 0) {
+            float *T = reinterpret_cast<float *>(freeb + (KS == 2 ? 8192 : 0) + wv * kEpiScratch);
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                // coarse tile: 16 positions from q0 (row q0 / Dc, columns q0 % Dc ..) -> fine
+                // positions 2 (q0 % Dc) .. + 31 of fine row 2 (y0 + q0 / Dc) + py, plane 2 z + pz
+#pragma unroll
+                for (int px = 0; px < 2; ++px)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float vv = acc[t][px][i] + bco;
+                        if (a1) vv = vv > 0.f ? vv : kLeak * vv;
+                        T[(2 * (4 * g + i) + px) * kEpiStride + r] = vv;
+                    }
+                __builtin_amdgcn_wave_barrier();
+                const int q0 = 16 * MT * wv + 16 * t;
+                const int fy = 2 * (y0 + q0 / Dc) + py, fz = 2 * z + pz, fx = 2 * (q0 & (Dc - 1));
+                const int pb = ((smp * D + fz) * D + fy) * D + fx;
+                epi_write32(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0 != 0);
+                __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next tile
+            }
         }
         if (zi + 1 < ZL) {
             __syncthreads();  // every wave is done with its scratch in the slot
@@ -2115,16 +2143,17 @@ static int zs_launch(int per_run, int gy, size_t shm, const uint16_t *x, const u
     return GP5_OK;
 }
 
-// Launch of k_conv_up2p (X of side D/2, 32 channels, D = 32, 64 or 128; slabs of
-// at most 5 x 256 units of 16 bytes): coarse
+// Launch of k_conv_up2p (X of side D/2, 32 or 64 channels, D = 32, 64 or 128; slabs
+// of at most 5 or 7 x 256 units of 16 bytes): coarse
 // plane runs that fill the device's workgroup slots once, as for k_conv_zs, or
 // `runs` > 0 of them.
+template <int CIN>
 static int up2p_launch(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a, const uint16_t *sk,
                        uint16_t *y, float *Yf, int B, int P, int D, int Cout, int act, int runs)
 {
-    const int Dc = D / 2, Rb = 128 / Dc;
-    const size_t shm = (size_t)3 * (Rb + 2) * (Dc + 2) * 4 * 16;
-    const void *fn = (const void *)k_conv_up2p<32>;
+    const int Dc = D / 2, Rb = (CIN == 32 ? 128 : 64) / Dc;
+    const size_t shm = (size_t)3 * (Rb + 2) * (Dc + 2) * (CIN / 8) * 16;
+    const void *fn = (const void *)k_conv_up2p<CIN>;
     static size_t shm_set = 65536;  // the default limit of dynamic LDS
     if (shm > shm_set) {
         NNCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm), "conv3d LDS size");
@@ -2146,7 +2175,7 @@ static int up2p_launch(const uint16_t *x, const uint16_t *w, const float *bias, 
         nzs = slots / (per_run * gy);
     }
     nzs = std::max(1, std::min(Dc / 2, nzs));
-    hipLaunchKernelGGL(k_conv_up2p<32>, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(), x,
+    hipLaunchKernelGGL(k_conv_up2p<CIN>, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(), x,
                        w, bias, a, sk, y, Yf, P, D, Cout, act, nzs);
     NNCHK(hipGetLastError(), "conv3d launch");
     return GP5_OK;
@@ -2322,14 +2351,16 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
     // z-sliding kernel: 16 or 32 input channels, D a power of two in [16, 128], the
     // longest plane run ZL (a power of two, at least 2) that keeps 1024 workgroups,
     // as long as there are at least 512.  GP5_CONV_ZS=0 turns it off.
-    // the upsampled input by output parity (k_conv_up2p): 32 input channels, D = 32,
-    // 64 or 128, forward taps; GP5_CONV_UP2P=0 turns it off, GP5_UP2P_RUNS sets the
-    // coarse plane runs (measurement)
+    // the upsampled input by output parity (k_conv_up2p): 32 or 64 input channels,
+    // D = 32, 64 or 128, forward taps; GP5_CONV_UP2P=0 turns it off, GP5_UP2P_RUNS
+    // sets the coarse plane runs (measurement)
     const char *pp_env = getenv("GP5_CONV_UP2P");
-    if (up && !mirror && !pool && !bpart && Cin == 32 && (D == 32 || D == 64 || D == 128) &&
+    if (up && !mirror && !pool && !bpart && (Cin == 32 || Cin == 64) && (D == 32 || D == 64 || D == 128) &&
         !(pp_env && pp_env[0] == '0')) {
         const char *pr = getenv("GP5_UP2P_RUNS");
-        return up2p_launch(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, pr ? atoi(pr) : 0);
+        const int runs = pr ? atoi(pr) : 0;
+        if (Cin == 32) return up2p_launch<32>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
+        return up2p_launch<64>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
     }
     const char *zs_env = getenv("GP5_CONV_ZS");
     if (pow2 && D >= 16 && (Cin == 16 || Cin == 32) && !(zs_env && zs_env[0] == '0')) {

@@ -666,17 +666,18 @@ def test_conv_act_bwd_equals_the_two_passes(B, D, Cin, Cout):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("D,Cout,B,skipped", [(64, 16, 1, True), (32, 32, 2, False), (128, 16, 1, True),
-                                              (32, 16, 1, True)])
-def test_upsampled_conv_by_output_parity(D, Cout, B, skipped, monkeypatch):
-    """gp5_nn_conv3d_up with 32 input channels at D = 32..128 runs the convolution
-    of the upsampled input by output parity (k_conv_up2p: 2x2x2 coarse taps with
-    folded weights held as bf16 hi + lo).  Its float output agrees with the
-    z-sliding kernel on the upsampled input (GP5_CONV_UP2P=0) to float rounding,
-    its bf16 outputs (A, and Y with the skip input) to one bf16 step, and with the
-    oracle to 1e-4."""
-    Cin = 32
-    rng = np.random.default_rng(D + Cout + B)
+@pytest.mark.parametrize("D,Cin,Cout,B,skipped", [(64, 32, 16, 1, True), (32, 32, 32, 2, False),
+                                                  (128, 32, 16, 1, True), (32, 32, 16, 1, True),
+                                                  (32, 64, 32, 2, True), (64, 64, 16, 1, False),
+                                                  (128, 64, 16, 1, True)])
+def test_upsampled_conv_by_output_parity(D, Cin, Cout, B, skipped, monkeypatch):
+    """gp5_nn_conv3d_up with 32 or 64 input channels at D = 32..128 runs the
+    convolution of the upsampled input by output parity (k_conv_up2p: 2x2x2 coarse
+    taps with folded weights held as bf16 hi + lo; 64 channels in two halves added
+    through LDS).  Its float output agrees with the other kernels on the upsampled
+    input (GP5_CONV_UP2P=0) to float rounding, its bf16 outputs (A, and Y with the
+    skip input) to one bf16 step, and with the oracle to 1e-4."""
+    rng = np.random.default_rng(D + Cin + Cout + B)
     L = nn._lib()
     Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
     W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
