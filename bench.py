@@ -281,7 +281,9 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     kernels count 0.  The roofline fractions count the MFMA flops the kernels
     execute: the forward of a decoder layer with 32 or 64 input channels at side
     32..128 (c6, c7) runs by output parity, 16 (coarse tap, x parity) pairs of hi and
-    lo MFMAs for two fine positions, 32/54 of its model count.  Wall time between device
+    lo MFMAs for two fine positions, and the pooled data gradients of c6 and c7 run as
+    stride-2 convolutions, 64 folded taps of hi and lo MFMAs for the 8 children of a
+    cell: 32/54 of the model count for each.  Wall time between device
     synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
 
@@ -292,8 +294,12 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     parity = os.environ.get("GP5_CONV_UP2P", "1") != "0"
     by_parity = {name for name, ci, co in nn.SHAPES
                  if parity and name in ("c5", "c6", "c7") and ci in (32, 64) and D // div[name] in (32, 64, 128)}
+    stride2 = {name for name, ci, co in nn.SHAPES
+               if name in ("c5", "c6", "c7") and nn._pool_fused(D // div[name], ci, co, batch)
+               and nn._pool2p(D // div[name], co)}
     fwd_flops = sum(f * (32.0 / 54.0 if name in by_parity else 1.0) for name, f in layer.items())
-    train_flops = fwd_flops + 2 * model_fwd - layer["c1"]
+    dgrad_flops = sum(f * (32.0 / 54.0 if name in stride2 else 1.0) for name, f in layer.items() if name != "c1")
+    train_flops = fwd_flops + dgrad_flops + model_fwd
     net = nn.UNet(batch, D, seed=0)
     rng = np.random.default_rng(0)
     X = (0.5 * rng.standard_normal((batch, D, D, D, nn.C0))).astype(np.float32)
@@ -324,7 +330,7 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
         "train_step_flops": train_flops, "train_step_ms": 1e3 * train_s, "train_TFLOPs": train_tf,
         "model_forward_flops": model_fwd, "model_forward_TFLOPs": model_fwd / fwd_s / 1e12,
         "model_train_step_flops": model_train, "model_train_TFLOPs": model_train / train_s / 1e12,
-        "by_output_parity": sorted(by_parity),
+        "by_output_parity": sorted(by_parity), "data_gradient_stride2": sorted(stride2),
         "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
                      "flops": "executed MFMA flops"},

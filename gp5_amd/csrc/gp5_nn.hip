@@ -1072,6 +1072,177 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// Data gradient of the convolution of an upsampled input, summed over each cell's
+// children (what k_conv_zs's POOL form gives), as a stride-2 convolution of the
+// fine gradient: dX[q] = sum over d in {-1, 0, 1, 2}^3 of V[d]^T dY[2q + d], where
+// V[d] sums the fine weights W[t] with 0 <= d + t <= 1 along each axis (1, 2, 4 or
+// 8 of them), which is the adjoint of the fold that k_conv_up2p uses.  dY has CO =
+// 16 or 32 channels (the layer's outputs), dX CI (the layer's inputs, 16 per
+// blockIdx.y).  The four waves of a workgroup cover the same MT tiles of 16 coarse
+// positions, wave w the taps with dz = w - 1; the folded weights of its 16 taps are
+// summed in float from the bf16 ones and held as bf16 hi + lo, one 16x16x32 MFMA
+// operand per tap pair (CO = 16: x-neighbours) or tap (CO = 32).  The four partial
+// sums go through LDS and are added in dz order.  A ring of four fine planes (two
+// fetched during each coarse plane's MFMAs) feeds the walk over coarse planes.
+template <int CO>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_pool2p(
+    const uint16_t *__restrict__ dY, const uint16_t *__restrict__ Wb, float *dX, int P, int D, int CI, int nzs)
+{
+    static_assert(CO == 16 || CO == 32, "16 or 32 gradient channels");
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    constexpr int MT = CO == 16 ? 4 : 2, NPOS = 16 * MT, U = CO / 8;
+    constexpr int NS = CO == 16 ? 8 : 16;  // MFMA steps per wave: (dy, x pair) or (dy, dx)
+    constexpr int NUF = 9;                 // staging units per thread: two fine planes, at most 2304
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int dz = wave - 1;
+    const int Dc = D >> 1, Rb = NPOS / Dc;
+    const int FR = 2 * Rb + 2, FX = D + 2, SUF = FR * FX * U;  // fine rows, columns, 16-byte units of a plane
+    const int nb0 = (int)blockIdx.y;
+    const int nyb = Dc / Rb;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * Dc / nzs, ZL = (zsg + 1) * Dc / nzs - zb;  // coarse
+    const __amdgpu_buffer_rsrc_t yr = make_rsrc(dY, P * CO * 2);
+    const __amdgpu_buffer_rsrc_t wr = make_rsrc(Wb, 27 * CI * CO * 2);
+    float *red = reinterpret_cast<float *>(sx + (size_t)4 * SUF * 16);  // 4 x MT x 4 x 64 partial sums
+    auto swz = [&](int ix) { return (ix >> (U == 2 ? 3 : 2)) & (U - 1); };
+    // lane's unit and x offset: CO = 16, step (dy, xp): dx = 2 xp - 1 + g / 2, unit g % 2;
+    // CO = 32, step (dy, dx): unit g
+    const int cu = CO == 16 ? (g & 1) : g;
+    int lo[MT][CO == 16 ? 2 : 4];  // byte offset in a plane slab of tile t's fragment at dy = 0 and each x choice
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+        const int q = 16 * t + r;
+        const int row = 2 * (q / Dc) + 1, ixb = 2 * (q & (Dc - 1)) + 1;
+#pragma unroll
+        for (int xi = 0; xi < (CO == 16 ? 2 : 4); ++xi) {
+            const int ddx = CO == 16 ? 2 * xi - 1 + (g >> 1) : xi - 1;
+            const int ix = ixb + ddx;
+            lo[t][xi] = ((row * FX + ix) * U + (cu ^ swz(ix))) * 16;
+        }
+    }
+    // folded weights: lane (r, g) holds dX channel 16 nb0 + r and 8 dY channels
+    uint4 whi[NS], wlo[NS];
+    {
+        auto taps = [](int d, int k) { return k - d; };  // the fine taps t of offset d: t = k - d for k = 0, 1 in range
+        static_for([&](auto sc) {
+            constexpr int st = decltype(sc)::value;
+            const int ddy = CO == 16 ? st / 2 - 1 : st / 4 - 1;
+            const int ddx = CO == 16 ? 2 * (st % 2) - 1 + (g >> 1) : st % 4 - 1;
+            const int co0 = 8 * cu;
+            float sum[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int kz = 0; kz < 2; ++kz) {
+                const int tz = taps(dz, kz);
+                if (tz < -1 || tz > 1) continue;
+                for (int ky = 0; ky < 2; ++ky) {
+                    const int ty = taps(ddy, ky);
+                    if (ty < -1 || ty > 1) continue;
+                    for (int kx = 0; kx < 2; ++kx) {
+                        const int tx = taps(ddx, kx);
+                        if (tx < -1 || tx > 1) continue;
+                        const int tap = 9 * (tz + 1) + 3 * (ty + 1) + (tx + 1);
+                        const uint4 wv = __builtin_bit_cast(
+                            uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                       wr, ((tap * CI + 16 * nb0 + r) * CO + co0) * 2, 0, 0));
+                        const uint32_t wd[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            sum[e] += __uint_as_float(e & 1 ? wd[e / 2] & 0xffff0000u : wd[e / 2] << 16);
+                    }
+                }
+            }
+            uint32_t hd[4], ld[4];
+#pragma unroll
+            for (int e2 = 0; e2 < 4; ++e2) {
+                const uint32_t h0 = f2bf(sum[2 * e2]), h1 = f2bf(sum[2 * e2 + 1]);
+                const uint32_t l0 = f2bf(sum[2 * e2] - bf2f((uint16_t)h0)),
+                               l1 = f2bf(sum[2 * e2 + 1] - bf2f((uint16_t)h1));
+                hd[e2] = h0 | (h1 << 16);
+                ld[e2] = l0 | (l1 << 16);
+            }
+            whi[st] = make_uint4(hd[0], hd[1], hd[2], hd[3]);
+            wlo[st] = make_uint4(ld[0], ld[1], ld[2], ld[3]);
+        }, std::make_integer_sequence<int, NS>{});
+    }
+    // two fine planes f0, f0 + 1 through registers into ring slots (f + 4) % 4
+    uint4 v[NUF];
+    int us[NUF];
+    auto load_pair = [&](int f0) {
+#pragma unroll
+        for (int k = 0; k < NUF; ++k) {
+            const int u = tid + 256 * k;
+            const int pl = u >= SUF ? 1 : 0, uu = u - pl * SUF;
+            const int row = uu / (FX * U), rem = uu - row * FX * U;
+            const int ix = rem / U, c = rem - ix * U;
+            us[k] = pl * SUF + (row * FX + ix) * U + (c ^ swz(ix));
+            const int fz = f0 + pl, fy = 2 * y0 - 1 + row, fx = ix - 1;
+            const bool in = u < 2 * SUF && fz >= 0 && fz < D && fy >= 0 && fy < D && fx >= 0 && fx < D;
+            const int voff = in ? ((((smp * D + fz) * D + fy) * D + fx) * CO + 8 * c) * 2 : kOffNone;
+            v[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yr, voff, 0, 0));
+        }
+    };
+    auto store_pair = [&](int f0) {
+        const int s0 = (f0 + 4) & 3, s1 = (f0 + 5) & 3;
+#pragma unroll
+        for (int k = 0; k < NUF; ++k) {
+            const int u = tid + 256 * k;
+            if (u < 2 * SUF) {
+                const int pl = us[k] >= SUF ? 1 : 0;
+                *reinterpret_cast<uint4 *>(sx + ((size_t)(pl ? s1 : s0) * SUF + us[k] - pl * SUF) * 16) = v[k];
+            }
+        }
+    };
+    load_pair(2 * zb - 1);
+    store_pair(2 * zb - 1);
+    load_pair(2 * zb + 1);
+    store_pair(2 * zb + 1);
+    __syncthreads();
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;  // coarse plane: fine planes 2z - 1 .. 2z + 2 are in the ring
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int xi = 0; xi < (CO == 16 ? 2 : 4); ++xi) asm volatile("" : "+v"(lo[t][xi]));
+        if (zi + 1 < ZL) load_pair(2 * z + 3);  // in flight during this plane's MFMAs
+        f32x4 acc[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int pbase = ((2 * z + dz + 4) & 3) * SUF * 16;
+        static_for([&](auto sc) {
+            constexpr int st = decltype(sc)::value;
+            constexpr int ddy = CO == 16 ? st / 2 - 1 : st / 4 - 1, xi = CO == 16 ? st % 2 : st % 4;
+            const bf8 bh = __builtin_bit_cast(bf8, whi[st]), bl = __builtin_bit_cast(bf8, wlo[st]);
+#pragma unroll
+            for (int t = 0; t < MT; ++t) {
+                int off = pbase + ddy * FX * U * 16 + lo[t][xi];
+                asm volatile("" : "+v"(off));
+                const bf8 a = __builtin_bit_cast(bf8, *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(sx + off, 16)));
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl, acc[t], 0, 0, 0);
+            }
+        }, std::make_integer_sequence<int, NS>{});
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[((wave * MT + t) * 4 + i) * 64 + lane] = acc[t][i];
+        __syncthreads();  // every wave's MFMAs and partial sums are done
+        if (wave < MT) {  // wave w finishes tile w: the dz partials in order
+            const int t = wave, q0 = 16 * t;
+            const int y = y0 + q0 / Dc, xb = (q0 & (Dc - 1)) + 4 * g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float o = red[((0 * MT + t) * 4 + i) * 64 + lane];
+#pragma unroll
+                for (int w = 1; w < 4; ++w) o += red[((w * MT + t) * 4 + i) * 64 + lane];
+                dX[((size_t)((smp * Dc + z) * Dc + y) * Dc + xb + i) * CI + 16 * nb0 + r] = o;
+            }
+        }
+        if (zi + 1 < ZL) store_pair(2 * z + 3);  // into the slots of planes 2z - 1, 2z
+        __syncthreads();
+    }
+}
+
 // Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
 // positions) accumulates all 27 taps at once, so each X and dY element of the
 // chunk is fetched by one wave instead of 27; partial sums go to
@@ -2181,6 +2352,44 @@ static int up2p_launch(const uint16_t *x, const uint16_t *w, const float *bias, 
     return GP5_OK;
 }
 
+// Launch of k_conv_pool2p (dY of side D with CO = 16 or 32 channels, D = 32, 64 or
+// 128, and D <= 64 for CO = 32): coarse plane runs that fill the device's workgroup
+// slots once, or `runs` > 0 of them.
+template <int CO>
+static int pool2p_launch(const uint16_t *dy, const uint16_t *w, float *dX, int B, int P, int D, int CI, int runs)
+{
+    constexpr int NPOS = CO == 16 ? 64 : 32, U = CO / 8, MT = CO == 16 ? 4 : 2;
+    const int Dc = D / 2, Rb = NPOS / Dc;
+    const int SUF = (2 * Rb + 2) * (D + 2) * U;
+    const size_t shm = (size_t)4 * SUF * 16 + (size_t)4 * MT * 4 * 64 * 4;
+    const void *fn = (const void *)k_conv_pool2p<CO>;
+    static size_t shm_set = 65536;  // the default limit of dynamic LDS
+    if (shm > shm_set) {
+        NNCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm), "conv3d LDS size");
+        shm_set = shm;
+    }
+    const int per_run = B * (Dc / Rb), gy = CI / 16;
+    int nzs = runs;
+    if (nzs <= 0) {
+        static size_t occ_shm = 0;
+        static int slots = 0;
+        if (occ_shm != shm) {
+            int dev = 0, cus = 0, per_cu = 0;
+            NNCHK(hipGetDevice(&dev), "conv3d device");
+            NNCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "conv3d CU count");
+            NNCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, shm), "conv3d occupancy");
+            slots = std::max(1, cus * per_cu);
+            occ_shm = shm;
+        }
+        nzs = slots / (per_run * gy);
+    }
+    nzs = std::max(1, std::min(Dc / 2, nzs));
+    hipLaunchKernelGGL(k_conv_pool2p<CO>, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(),
+                       dy, w, dX, P, D, CI, nzs);
+    NNCHK(hipGetLastError(), "conv3d launch");
+    return GP5_OK;
+}
+
 extern "C" {
 
 int gp5_nn_malloc(void **p, size_t bytes)
@@ -2361,6 +2570,17 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
         const int runs = pr ? atoi(pr) : 0;
         if (Cin == 32) return up2p_launch<32>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
         return up2p_launch<64>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
+    }
+    // the pooled data gradient as a stride-2 convolution of dY (k_conv_pool2p): 16 or
+    // 32 dY channels, D = 32, 64 or 128 (at most 64 with 32 channels), mirrored taps;
+    // GP5_CONV_POOL2P=0 turns it off, GP5_POOL2P_RUNS sets the coarse plane runs
+    const char *p2_env = getenv("GP5_CONV_POOL2P");
+    if (pool && mirror == 1 && !bias && !act && !A && !Y && Yf && (Cin == 16 || Cin == 32) &&
+        (D == 32 || D == 64 || (D == 128 && Cin == 16)) && !(p2_env && p2_env[0] == '0')) {
+        const char *pr = getenv("GP5_POOL2P_RUNS");
+        const int runs = pr ? atoi(pr) : 0;
+        if (Cin == 16) return pool2p_launch<16>(x, w, Yf, B, (int)P, D, Cout, runs);
+        return pool2p_launch<32>(x, w, Yf, B, (int)P, D, Cout, runs);
     }
     const char *zs_env = getenv("GP5_CONV_ZS");
     if (pow2 && D >= 16 && (Cin == 16 || Cin == 32) && !(zs_env && zs_env[0] == '0')) {

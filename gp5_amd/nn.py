@@ -29,12 +29,22 @@ SHAPES = [("c1", 16, 16), ("c2", 16, 32), ("c3", 32, 64), ("c4", 64, 128),
 C0 = 16  # input feature channels
 
 
+def _pool2p(d: int, co: int) -> bool:
+    """Whether gp5_nn_conv3d_pool runs as the stride-2 convolution of the gradient
+    (k_conv_pool2p: co = 16 gradient channels at side 32..128, or 32 at 32..64;
+    GP5_CONV_POOL2P=0 turns it off)."""
+    return os.environ.get("GP5_CONV_POOL2P", "1") != "0" and (
+        (co == 16 and d in (32, 64, 128)) or (co == 32 and d in (32, 64)))
+
+
 def _pool_fused(d: int, ci: int, co: int, B: int) -> bool:
     """Whether the data gradient of the layer of side d that reads the upsampled
     input (ci input channels of the layer, co output) comes out already summed over
-    each cell's children (gp5_nn_conv3d_pool: co = 16 gradient channels in with ci a
-    multiple of 32 out, or co = 32; side 32, 64 or 128; enough workgroups for the
-    z-sliding kernel)."""
+    each cell's children (gp5_nn_conv3d_pool): the stride-2 kernel's shapes, or the
+    z-sliding kernel's (co = 16 gradient channels in with ci a multiple of 32 out,
+    or co = 32; side 32, 64 or 128; enough workgroups)."""
+    if _pool2p(d, co):
+        return True
     if d not in (32, 64, 128) or not ((co == 16 and ci % 32 == 0) or co == 32):
         return False
     nbz = 2 if co == 16 else 1

@@ -139,10 +139,13 @@ def test_up2_add_act_bwd_equals_the_two_passes(B, D, C):
 @pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 32, 2), (128, 16, 32, 1), (64, 16, 64, 1), (32, 32, 64, 4),
                                           (64, 32, 32, 1)])
-def test_conv_pool_equals_conv_then_pool(D, Cin, Cout, B):
+def test_conv_pool_equals_conv_then_pool(D, Cin, Cout, B, monkeypatch):
     """gp5_nn_conv3d_pool (the data gradient summed over each cell's 2x2x2
-    children, mirrored taps) equals gp5_nn_conv3d into a float tensor followed by
-    gp5_nn_pool2_f32 with scale 8, bit for bit, and agrees with the oracle."""
+    children, mirrored taps) in the z-sliding kernel's pooled epilogue equals
+    gp5_nn_conv3d into a float tensor followed by gp5_nn_pool2_f32 with scale 8, bit
+    for bit, and agrees with the oracle (the stride-2 form, which sums in another
+    order, is off here and has its own test)."""
+    monkeypatch.setenv("GP5_CONV_POOL2P", "0")
     rng = np.random.default_rng(D + Cin + Cout + B)
     L = nn._lib()
     G = _rb(rng.standard_normal((B, D, D, D, Cin)))
@@ -703,3 +706,30 @@ def test_upsampled_conv_by_output_parity(D, Cin, Cout, B, skipped, monkeypatch):
         U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
         ref = R.lrelu(R.conv(U, W, b.astype(np.float64))).reshape(P, Cout)
         assert np.max(np.abs(outs[0][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 16, 32, 2), (32, 32, 64, 4), (128, 16, 32, 1), (32, 16, 32, 1),
+                                          (64, 32, 32, 1), (64, 16, 64, 1)])
+def test_pooled_data_gradient_as_stride2_conv(D, Cin, Cout, B):
+    """gp5_nn_conv3d_pool with 16 or 32 gradient channels runs the pooled data
+    gradient as a stride-2 convolution of dY with folded weights (k_conv_pool2p):
+    it agrees with gp5_nn_conv3d (mirrored) followed by gp5_nn_pool2_f32 to float
+    rounding and with the oracle to 1e-4."""
+    rng = np.random.default_rng(D + 2 * Cin + Cout + B)
+    L = nn._lib()
+    G = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    W = _rb(rng.standard_normal((27, Cout, Cin)) * 0.1)  # [27][ci][co] of the layer: its co is Cin here
+    gb, wb = _buf(G, True), _buf(W, True)
+    shape = (B, D // 2, D // 2, D // 2, Cout)
+    h = nn.DevBuf(B * D ** 3 * Cout // 2)
+    nn._ck(L.gp5_nn_conv3d_pool(gb.ptr, wb.ptr, h.ptr, B, D, Cin, Cout, 1))
+    got = h.download(np.float32, shape)
+    full, two = nn.DevBuf(B * D ** 3 * Cout * 4), nn.DevBuf(B * D ** 3 * Cout // 2)
+    nn._ck(L.gp5_nn_conv3d(gb.ptr, wb.ptr, None, None, None, None, full.ptr, B, D, Cin, Cout, 1, 0))
+    nn._ck(L.gp5_nn_pool2_f32(full.ptr, two.ptr, B, D, Cout, 8.0))
+    ref2 = two.download(np.float32, shape)
+    assert np.max(np.abs(got - ref2)) <= 1e-5 * np.max(np.abs(ref2))
+    if D <= 64:
+        ref = 8.0 * R.pool2(R.conv_dgrad(G, W))
+        assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
