@@ -2795,9 +2795,11 @@ static long long wgrad_zs_plan(int B, int D, int Cin, int Cout, int *nzs_out)
     // 512 workgroups (two per CU) against 1024 before, measured at batch 8: 64^3 16 x 16
     // 63.8 -> 57.7 us, 64^3 32 x 16 upsampled 103.6 -> 96.9, 32^3 16 x 32 36.1 -> 26.8,
     // 16^3 128 x 64 upsampled 44.4 -> 38.2 (fewer, longer runs halve the overlap planes)
+    const char *tg = getenv("GP5_WZ_TARGET");  // measurement: workgroup target
+    const long long target = tg && atoll(tg) >= 16 ? atoll(tg) : 512;
     int zl = 2;
     for (int c = D; c >= 2; c /= 2)
-        if (per_z * (D / c) * tiles >= 512) {
+        if (per_z * (D / c) * tiles >= target) {
             zl = c;
             break;
         }
