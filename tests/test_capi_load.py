@@ -134,9 +134,10 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, None, 0.125, fake, fake, 1, 8, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_up2_add_act_bwd_cast_bias(fake, fake, fake, 0.125, fake, fake, 1, 7, 16) == capi.GP5_ERR_ARG
     assert b"even" in L.gp5_last_error()
-    # conv3d_pool: 16 input channels with a multiple of 32 outputs, or 32 inputs; side 32, 64 or 128
+    # conv3d_pool: 16 or 32 gradient channels, side 32, 64 or 128 (every call below is
+    # refused before any launch: the pointers are fake)
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 16, 16, 32, 1) == capi.GP5_ERR_ARG
-    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 16, 16, 1) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 16, 16, 16, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, fake, 1, 64, 64, 32, 1) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_pool(fake, fake, None, 1, 64, 16, 32, 1) == capi.GP5_ERR_ARG
     # conv3d_act_bwd: every pointer, and Cout / 4 dividing 256
