@@ -279,8 +279,8 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     gradients of c2..c8 (the input layer's data gradient is never computed), so three
     times the forward count less c1's share; pooling, upsampling and the elementwise
     kernels count 0.  The roofline fractions count the MFMA flops the kernels
-    execute: the forward of a decoder layer with 32 or 64 input channels at side
-    32..128 (c6, c7) runs by output parity, 16 (coarse tap, x parity) pairs of hi and
+    execute: the forward of a decoder layer (c5, c6, c7: 32 or 64 input channels at
+    side 32..128, 128 at 16..64) runs by output parity, 16 (coarse tap, x parity) pairs of hi and
     lo MFMAs for two fine positions, and the pooled data gradients of c6 and c7 run as
     stride-2 convolutions, 64 folded taps of hi and lo MFMAs for the 8 children of a
     cell: 32/54 of the model count for each.  Wall time between device
@@ -293,7 +293,8 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     model_train = 3 * model_fwd - layer["c1"]
     parity = os.environ.get("GP5_CONV_UP2P", "1") != "0"
     by_parity = {name for name, ci, co in nn.SHAPES
-                 if parity and name in ("c5", "c6", "c7") and ci in (32, 64) and D // div[name] in (32, 64, 128)}
+                 if parity and name in ("c5", "c6", "c7")
+                 and ((ci in (32, 64) and D // div[name] in (32, 64, 128)) or (ci == 128 and D // div[name] in (16, 32, 64)))}
     stride2 = {name for name, ci, co in nn.SHAPES
                if name in ("c5", "c6", "c7") and nn._pool_fused(D // div[name], ci, co, batch)
                and nn._pool2p(D // div[name], co)}

@@ -138,31 +138,34 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[MT][NB], int ba
 constexpr int kEpiStride = 20;                       // floats per scratch row (16 channels, conflict-free)
 constexpr int kEpiScratch = 32 * kEpiStride * 4;     // bytes per wave
 
-// The outputs of 32 consecutive positions pb .. pb + 31 (16 channels cb .. cb + 15
-// of each) from the float scratch T (position-major rows of kEpiStride floats):
-// Yf (all channels, or channel 0 only with yf0), A = bf16(v) and Y = bf16(v + skip)
-// or bf16(v), each when set, with 16-byte accesses (called by one whole wave).
+// The outputs of 32 positions (16 channels cb .. cb + 15 of each) from the float
+// scratch T (position-major rows of kEpiStride floats): rows 0..15 go to positions
+// pb .. pb + 15, rows 16..31 to pb + 16 .. pb + 31, or with SPLIT to pb2 .. pb2 + 15.  Yf (all channels, or channel 0 only with yf0), A = bf16(v) and
+// Y = bf16(v + skip) or bf16(v), each when set, with 16-byte accesses (called by one
+// whole wave).
+template <bool SPLIT = false>
 __device__ __forceinline__ void epi_write32(const float *T, int pb, int P, int Cout, int cb, int lane, uint16_t *A,
-                                            const uint16_t *skip, uint16_t *Y, float *Yf, bool yf0)
+                                            const uint16_t *skip, uint16_t *Y, float *Yf, bool yf0, int pb2 = 0)
 {
+    auto at = [&](int pos) { return SPLIT ? (pos < 16 ? pb + pos : pb2 + pos - 16) : pb + pos; };
     if (Yf && !yf0) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int pos = lane / 4 + 16 * k, c4 = 4 * (lane & 3);
-            if (pb + pos < P)
-                *reinterpret_cast<float4 *>(Yf + (size_t)(pb + pos) * Cout + cb + c4) =
+            if (at(pos) < P)
+                *reinterpret_cast<float4 *>(Yf + (size_t)at(pos) * Cout + cb + c4) =
                     *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c4);
         }
     } else if (Yf && cb == 0 && lane < 32) {  // channel 0 of the 32 positions
-        if (pb + lane < P) Yf[pb + lane] = T[lane * kEpiStride];
+        if (at(lane) < P) Yf[at(lane)] = T[lane * kEpiStride];
     }
     if (A || Y) {
         const int pos = lane / 2, c8 = 8 * (lane & 1);
-        if (pb + pos < P) {
+        if (at(pos) < P) {
             const float4 lo = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8);
             const float4 hi = *reinterpret_cast<const float4 *>(T + pos * kEpiStride + c8 + 4);
             const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-            const size_t o = (size_t)(pb + pos) * Cout + cb + c8;
+            const size_t o = (size_t)at(pos) * Cout + cb + c8;
             uint16_t hb[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) hb[e] = f2bf(v[e]);
@@ -869,16 +872,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wk, const float *__restrict__ bias, uint16_t *A,
     const uint16_t *skip, uint16_t *Y, float *Yf, int P, int D, int Cout, int act, int nzs)
 {
-    static_assert(CIN == 32 || CIN == 64, "32-channel 16x16x32 operands");
+    static_assert(CIN == 32 || CIN == 64 || CIN == 128, "32-channel 16x16x32 operands");
     typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
     // MT = 2 tiles of 16 coarse positions per wave: the 32 folded weight fragments
-    // of one 32-channel half take 128 VGPRs.  CIN = 64 splits the channels over two
-    // wave pairs (KS = 2: waves 0, 1 channels 0..31, waves 2, 3 channels 32..63, for
-    // the same positions), whose sums are added in that order through LDS before the
-    // epilogue of waves 0, 1.  A workgroup covers NPOS coarse positions per plane
-    // (Rb = NPOS / Dc rows).
+    // of one 32-channel part take 128 VGPRs.  CIN = 64 and 128 split the channels over
+    // KS = CIN / 32 wave groups (waves 0, 1 channels 0..31, waves 2, 3 channels 32..63
+    // with 64; one wave per 32 channels with 128), which cover the same positions and
+    // whose sums are added in channel order through LDS before the epilogue of the
+    // first group.  A workgroup covers NPOS coarse positions per plane (Rb = NPOS / Dc
+    // rows); with Dc = 8 a tile of 16 positions covers two coarse rows.
     constexpr int KS = CIN / 32, WG = 4 / KS, MT = 2, NPOS = 16 * MT * WG, U8 = CIN / 8;
-    constexpr int NU = CIN == 32 ? 5 : 7;  // slab units per thread: (Rb + 2) (Dc + 2) U8 <= 256 NU for Dc <= 64
+    constexpr int NU = CIN == 32 ? 5 : 7;  // slab units per thread: (Rb + 2) (Dc + 2) U8 <= 256 NU
     extern __shared__ __align__(16) unsigned char sx[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = lane & 15, g = lane >> 4;
@@ -1024,26 +1028,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int free_slot = zi % 3;  // plane z - 1
         __syncthreads();  // every wave is done with the slab of plane z-1: it becomes the epilogue scratch
         unsigned char *freeb = sx + (size_t)free_slot * SU * 16;
-        if constexpr (KS == 2) {  // the second channel half's sums, added in order by waves 0, 1
-            float *red = reinterpret_cast<float *>(freeb);  // WG x MT x 2 x 4 x 64 floats (8 KB)
-            if (kh == 1)
+        constexpr int RED = (KS - 1) * WG * MT * 2 * 4 * 64;  // floats of the other groups' sums
+        if constexpr (KS > 1) {  // the other channel parts' sums, added in order by the first group
+            float *red = reinterpret_cast<float *>(freeb);
+            if (kh > 0)
 #pragma unroll
                 for (int t = 0; t < MT; ++t)
 #pragma unroll
                     for (int px = 0; px < 2; ++px)
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) red[(((wv * MT + t) * 2 + px) * 4 + i) * 64 + lane] = acc[t][px][i];
+                        for (int i = 0; i < 4; ++i)
+                            red[((((kh - 1) * WG + wv) * MT + t) * 2 + px) * 4 * 64 + i * 64 + lane] = acc[t][px][i];
             __syncthreads();
             if (kh == 0)
 #pragma unroll
-                for (int t = 0; t < MT; ++t)
+                for (int k2 = 1; k2 < KS; ++k2)
 #pragma unroll
-                    for (int px = 0; px < 2; ++px)
+                    for (int t = 0; t < MT; ++t)
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) acc[t][px][i] += red[(((wv * MT + t) * 2 + px) * 4 + i) * 64 + lane];
+                        for (int px = 0; px < 2; ++px)
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                acc[t][px][i] += red[((((k2 - 1) * WG + wv) * MT + t) * 2 + px) * 4 * 64 + i * 64 + lane];
         }
         if (kh == 0) {
-            float *T = reinterpret_cast<float *>(freeb + (KS == 2 ? 8192 : 0) + wv * kEpiScratch);
+            float *T = reinterpret_cast<float *>(freeb + RED * 4 + wv * kEpiScratch);
 #pragma unroll
             for (int t = 0; t < MT; ++t) {
                 // coarse tile: 16 positions from q0 (row q0 / Dc, columns q0 % Dc ..) -> fine
@@ -1060,7 +1069,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const int q0 = 16 * MT * wv + 16 * t;
                 const int fy = 2 * (y0 + q0 / Dc) + py, fz = 2 * z + pz, fx = 2 * (q0 & (Dc - 1));
                 const int pb = ((smp * D + fz) * D + fy) * D + fx;
-                epi_write32(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0 != 0);
+                // Dc = 8: the tile's second coarse row is the fine row two below
+                epi_write32<CIN == 128>(T, pb, P, Cout, cb, lane, A, skip, Y, Yf, yf0 != 0,
+                                        Dc == 8 ? pb + 2 * D : pb + 16);
                 __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next tile
             }
         }
@@ -2322,7 +2333,7 @@ template <int CIN>
 static int up2p_launch(const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *a, const uint16_t *sk,
                        uint16_t *y, float *Yf, int B, int P, int D, int Cout, int act, int runs)
 {
-    const int Dc = D / 2, Rb = (CIN == 32 ? 128 : 64) / Dc;
+    const int Dc = D / 2, Rb = (CIN == 32 ? 128 : (CIN == 64 ? 64 : 32)) / Dc;
     const size_t shm = (size_t)3 * (Rb + 2) * (Dc + 2) * (CIN / 8) * 16;
     const void *fn = (const void *)k_conv_up2p<CIN>;
     static size_t shm_set = 65536;  // the default limit of dynamic LDS
@@ -2560,16 +2571,19 @@ static int conv3d_run(const void *X, const void *Wk, const float *bias, void *A,
     // z-sliding kernel: 16 or 32 input channels, D a power of two in [16, 128], the
     // longest plane run ZL (a power of two, at least 2) that keeps 1024 workgroups,
     // as long as there are at least 512.  GP5_CONV_ZS=0 turns it off.
-    // the upsampled input by output parity (k_conv_up2p): 32 or 64 input channels,
-    // D = 32, 64 or 128, forward taps; GP5_CONV_UP2P=0 turns it off, GP5_UP2P_RUNS
-    // sets the coarse plane runs (measurement)
+    // the upsampled input by output parity (k_conv_up2p): 32 or 64 input channels at
+    // D = 32, 64 or 128, or 128 at D = 16, 32 or 64 (a workgroup covers whole coarse
+    // rows), forward taps; GP5_CONV_UP2P=0 turns it off, GP5_UP2P_RUNS sets the coarse
+    // plane runs (measurement)
     const char *pp_env = getenv("GP5_CONV_UP2P");
-    if (up && !mirror && !pool && !bpart && (Cin == 32 || Cin == 64) && (D == 32 || D == 64 || D == 128) &&
-        !(pp_env && pp_env[0] == '0')) {
+    // (Dc = 8, a tile over two coarse rows, only with 128 channels)
+    if (up && !mirror && !pool && !bpart && !(pp_env && pp_env[0] == '0') &&
+        (((Cin == 32 || Cin == 64) && (D == 32 || D == 64 || D == 128)) || (Cin == 128 && (D == 16 || D == 32 || D == 64)))) {
         const char *pr = getenv("GP5_UP2P_RUNS");
         const int runs = pr ? atoi(pr) : 0;
         if (Cin == 32) return up2p_launch<32>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
-        return up2p_launch<64>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
+        if (Cin == 64) return up2p_launch<64>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
+        return up2p_launch<128>(x, w, bias, a, sk, y, Yf, B, (int)P, D, Cout, act, runs);
     }
     // the pooled data gradient as a stride-2 convolution of dY (k_conv_pool2p): 16 or
     // 32 dY channels, D = 32, 64 or 128 (at most 64 with 32 channels), mirrored taps;

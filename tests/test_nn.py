@@ -672,12 +672,14 @@ def test_conv_act_bwd_equals_the_two_passes(B, D, Cin, Cout):
 @pytest.mark.parametrize("D,Cin,Cout,B,skipped", [(64, 32, 16, 1, True), (32, 32, 32, 2, False),
                                                   (128, 32, 16, 1, True), (32, 32, 16, 1, True),
                                                   (32, 64, 32, 2, True), (64, 64, 16, 1, False),
-                                                  (128, 64, 16, 1, True)])
+                                                  (128, 64, 16, 1, True), (16, 128, 64, 2, True),
+                                                  (32, 128, 16, 1, False), (64, 128, 16, 1, True)])
 def test_upsampled_conv_by_output_parity(D, Cin, Cout, B, skipped, monkeypatch):
-    """gp5_nn_conv3d_up with 32 or 64 input channels at D = 32..128 runs the
-    convolution of the upsampled input by output parity (k_conv_up2p: 2x2x2 coarse
-    taps with folded weights held as bf16 hi + lo; 64 channels in two halves added
-    through LDS).  Its float output agrees with the other kernels on the upsampled
+    """gp5_nn_conv3d_up with 32 or 64 input channels at D = 32..128, or 128 at
+    D = 16..64, runs the convolution of the upsampled input by output parity
+    (k_conv_up2p: 2x2x2 coarse taps with folded weights held as bf16 hi + lo; 64 and
+    128 channels in 32-channel parts added through LDS; at D = 16 a tile covers two
+    coarse rows).  Its float output agrees with the other kernels on the upsampled
     input (GP5_CONV_UP2P=0) to float rounding, its bf16 outputs (A, and Y with the
     skip input) to one bf16 step, and with the oracle to 1e-4."""
     rng = np.random.default_rng(D + Cin + Cout + B)
