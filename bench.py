@@ -283,7 +283,9 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     side 32..128, 128 at 16..64) runs by output parity, 16 (coarse tap, x parity) pairs of hi and
     lo MFMAs for two fine positions, and the pooled data gradients of c6 and c7 run as
     stride-2 convolutions, 64 folded taps of hi and lo MFMAs for the 8 children of a
-    cell: 32/54 of the model count for each.  Wall time between device
+    cell: 32/54 of the model count for each; the output layer's weight gradient, whose
+    dY has one nonzero channel, runs the taps as the MFMA's N dimension (1024 of the
+    model's 13824 flops per position).  Wall time between device
     synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
 
@@ -300,7 +302,11 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
                and nn._pool2p(D // div[name], co)}
     fwd_flops = sum(f * (32.0 / 54.0 if name in by_parity else 1.0) for name, f in layer.items())
     dgrad_flops = sum(f * (32.0 / 54.0 if name in stride2 else 1.0) for name, f in layer.items() if name != "c1")
-    train_flops = fwd_flops + dgrad_flops + model_fwd
+    # the output layer's weight gradient from channel 0 (gp5_nn_conv3d_wgrad_out0): one
+    # 16x16x32 MFMA per 16 taps and 32 positions, 1024 flops per position against 13824
+    out0 = D in (32, 64, 128) and os.environ.get("GP5_WGRAD_OUT0", "1") != "0"
+    wgrad_flops = model_fwd - (layer["c8"] * (1.0 - 1024.0 / 13824.0) if out0 else 0.0)
+    train_flops = fwd_flops + dgrad_flops + wgrad_flops
     net = nn.UNet(batch, D, seed=0)
     rng = np.random.default_rng(0)
     X = (0.5 * rng.standard_normal((batch, D, D, D, nn.C0))).astype(np.float32)

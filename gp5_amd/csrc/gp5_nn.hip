@@ -1254,6 +1254,170 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
 }
 
+// Weight gradient of a layer whose output gradient dY is nonzero in channel 0 only
+// (the output layer, whose loss reads channel 0; dY rows of ldy channels, X of 16):
+// dW[t][ci][0] = sum_q X[q][ci] dY[q - t][0], one 16x16x32 MFMA per 16 taps and 32
+// positions (M = ci, N = the 27 taps, K = positions).  A workgroup (4 waves) owns
+// Rb = 256 / D rows (D >= 32) of a run of planes: X of plane z channel-major in LDS,
+// channel 0 of dY in a ring of three planes (rows y0 - 1 .. y0 + Rb, 8 zero
+// elements on either side of each row); lane (n, g) builds the tap n's shifted dY
+// fragment from an aligned read and its two funnel shifts.  Each workgroup's sums
+// (27 x 16 floats, the waves added in order) go to part[blk].
+constexpr int kWo0Pad = 8;
+__global__ __launch_bounds__(256) void k_wgrad_o0(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                  float *part, int P, int D, int ldy, int nzs)
+{
+    constexpr int CIN = 16;
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int Rb = 256 / D, HS = D + 2 * kWo0Pad, HR = Rb + 2;
+    uint16_t *XC = reinterpret_cast<uint16_t *>(sx);    // [16][Rb * D]
+    uint16_t *HY = XC + CIN * 256;                       // [3][HR][HS]
+    const int nyb = D / Rb;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * D / nzs, ZL = (zsg + 1) * D / nzs - zb;
+    const int s0 = smp * D * D * D;
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, P * CIN * 2);
+    const __amdgpu_buffer_rsrc_t yr = make_rsrc(dY, P * ldy * 2);
+    for (int i = tid; i < 3 * HR * HS / 8; i += 256) reinterpret_cast<uint4 *>(HY)[i] = make_uint4(0, 0, 0, 0);
+    // X: thread = (position pair, channel half); dY: (Rb + 2) x D elements of channel 0
+    const int pr = tid % (128), hx = tid / 128;  // pair 0..127 of the 256 positions, channel half
+    uint4 xa, xb;
+    uint16_t hv[2];  // at most (Rb + 2) D = 2 x 256 + ... elements: two per thread for D >= 32
+    auto load_x = [&](int z) {
+        const int q = 2 * pr, row = q / D, x = q - row * D;
+        const int o = ((s0 + (z * D + y0 + row) * D + x) * CIN + 8 * hx) * 2;
+        xa = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, o, 0, 0));
+        xb = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, o + CIN * 2, 0, 0));
+    };
+    auto store_x = [&]() {
+        const uint32_t wa[4] = {xa.x, xa.y, xa.z, xa.w}, wb[4] = {xb.x, xb.y, xb.z, xb.w};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            // channel 8 hx + c of positions 2 pr (low half) and 2 pr + 1 (high half)
+            const uint32_t d = (c & 1) ? __builtin_amdgcn_perm(wb[c / 2], wa[c / 2], 0x07060302u)
+                                       : __builtin_amdgcn_perm(wb[c / 2], wa[c / 2], 0x05040100u);
+            *reinterpret_cast<uint32_t *>(XC + (8 * hx + c) * 256 + 2 * pr) = d;
+        }
+    };
+    auto load_h = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = tid + 256 * k, row = e / D, x = e - row * D;
+            const int y = y0 - 1 + row;
+            const bool in = row < HR && z >= 0 && z < D && y >= 0 && y < D;
+            hv[k] = in ? dY[(size_t)(s0 + (z * D + y) * D + x) * ldy] : (uint16_t)0;
+        }
+    };
+    auto store_h = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = tid + 256 * k, row = e / D, x = e - row * D;
+            if (row < HR) HY[(slot * HR + row) * HS + kWo0Pad + x] = hv[k];
+        }
+    };
+    __syncthreads();  // the zero fill is complete
+    load_h(zb - 1);
+    store_h(0);
+    load_h(zb);
+    store_h(1);
+    load_h(zb + 1);
+    store_h(2);
+    load_x(zb);
+    store_x();
+    __syncthreads();
+    // this lane's taps t = 16 nt + r: (dz, dy, dx)
+    int tdz[2], tdy[2], tdx[2];
+    bool tok[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int t = 16 * nt + r;
+        tok[nt] = t < 27;
+        tdz[nt] = tok[nt] ? t / 9 - 1 : 0;
+        tdy[nt] = tok[nt] ? (t / 3) % 3 - 1 : 0;
+        tdx[nt] = tok[nt] ? t % 3 - 1 : 0;
+    }
+    f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;
+        if (zi + 1 < ZL) {  // in flight during this plane's MFMAs
+            load_x(z + 1);
+            load_h(z + 2);
+        }
+        for (int ck = wave; ck < 8; ck += 4) {  // 32-position chunks of the plane block
+            const int q0 = 32 * ck, row = q0 / D, x0 = q0 - row * D;
+            const bf8 a = __builtin_bit_cast(
+                bf8, *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(XC + r * 256 + q0 + 8 * g, 16)));
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                // dY[q - t] for the lane's 8 positions q: plane z - dz (slot), row - dy, x - dx
+                const int slot = (zi + 1 - tdz[nt]) % 3;
+                const uint16_t *hp = HY + (slot * HR + row + 1 - tdy[nt]) * HS + kWo0Pad + x0 + 8 * g;
+                const uint4 c = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(hp, 16));
+                const uint32_t pv = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(hp - 2, 4));
+                const uint32_t nx = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(hp + 8, 4));
+                const uint32_t a1 = __builtin_amdgcn_alignbit(c.x, pv, 16), a2 = __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                               a3 = __builtin_amdgcn_alignbit(c.z, c.y, 16), a4 = __builtin_amdgcn_alignbit(c.w, c.z, 16),
+                               a5 = __builtin_amdgcn_alignbit(nx, c.w, 16);
+                // dx = 1: positions q - 1 .. q + 6; dx = -1: q + 1 .. q + 8
+                uint4 bv = tdx[nt] == 0 ? c : (tdx[nt] == 1 ? make_uint4(a1, a2, a3, a4) : make_uint4(a2, a3, a4, a5));
+                if (!tok[nt]) bv = make_uint4(0, 0, 0, 0);
+                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf8, bv), acc[nt], 0, 0, 0);
+            }
+        }
+        if (zi + 1 < ZL) {
+            __syncthreads();  // every wave is done with X of plane z and dY of plane z - 1
+            store_x();
+            store_h((zi + 3) % 3);  // plane z + 2 takes the slot of plane z - 1
+            __syncthreads();
+        }
+    }
+    // D[m = ci 4g + i][n = tap 16 nt + r]; the four waves' sums added in order
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(sx);  // 4 x 2 x 4 x 64
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[((wave * 2 + nt) * 4 + i) * 64 + lane] = acc[nt][i];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int t = 16 * nt + r;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = red[((0 * 2 + nt) * 4 + i) * 64 + lane];
+#pragma unroll
+                for (int w = 1; w < 4; ++w) v += red[((w * 2 + nt) * 4 + i) * 64 + lane];
+                if (t < 27) part[((size_t)blockIdx.x * 27 + t) * CIN + 4 * g + i] = v;
+            }
+        }
+    }
+}
+
+// dW[t][ci][co] = sum over the blocks of part[blk][t][ci] for co = 0, 0 for the other
+// output channels.  Workgroup t: thread (ci = tid % 16, k = tid / 16) adds blocks k,
+// k + 16, ... in order, then a fixed tree over k.
+__global__ __launch_bounds__(256) void k_wgrad_o0_reduce(const float *__restrict__ part, float *dW, int blocks, int Cout)
+{
+    __shared__ float s[256];
+    const int t = blockIdx.x, ci = threadIdx.x & 15, k = threadIdx.x >> 4;
+    float v = 0.f;
+    for (int b = k; b < blocks; b += 16) v += part[((size_t)b * 27 + t) * 16 + ci];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 8; w > 0; w >>= 1) {
+        if (k < w) s[threadIdx.x] += s[threadIdx.x + 16 * w];
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < 16 * Cout; e += 256) {  // (ci, co) of tap t
+        const int c = e / Cout, co = e - c * Cout;
+        dW[(size_t)t * 16 * Cout + e] = co == 0 ? s[c] : 0.f;
+    }
+}
+
 // Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
 // positions) accumulates all 27 taps at once, so each X and dY element of the
 // chunk is fetched by one wave instead of 27; partial sums go to
@@ -2912,6 +3076,32 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
 int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
 {
     return wgrad_run(X, dY, dW, ws, B, D, Cin, Cout, 0);
+}
+
+int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+{
+    if (!X || !dY || !dW || !ws) return set_error(GP5_ERR_ARG, "NULL argument");
+    int nzs = 0;
+    const long long zblocks = (Cin == 16 && Cout % 16 == 0 && Cout >= 16 && (D == 32 || D == 64 || D == 128))
+                                  ? wgrad_zs_plan(B, D, Cin, 16, &nzs) : 0;
+    const char *env = getenv("GP5_WGRAD_OUT0");
+    if (zblocks <= 0 || (long long)B * D * D * D * Cout * 2 > 0x7fffffffLL || (env && env[0] == '0'))
+        return wgrad_run(X, dY, dW, ws, B, D, Cin, Cout, 0);  // every output channel (the others give 0)
+    const int Rb = 256 / D;
+    const size_t shm = (size_t)16 * 256 * 2 + (size_t)3 * (Rb + 2) * (D + 2 * kWo0Pad) * 2;
+    // D / 16 plane runs (measured at 64^3, batch 8: 4 runs 41.3 us, 8 53.4, 16 73.1, 32
+    // 110.8, the full weight gradient 58.2); the partials, 27 x 16 floats per block, fit
+    // the plan's workspace of 27 x 16 x 16 per block
+    const char *rn = getenv("GP5_WO0_RUNS");  // measurement
+    const int runs = std::max(1, std::min(D / 2, rn && atoi(rn) > 0 ? atoi(rn) : D / 16));
+    const long long blocks = (long long)B * (D / Rb) * runs;
+    if (blocks * 27 * 16 > zblocks * 27 * 16 * 16) return set_error(GP5_ERR_ARG, "wgrad out0: workspace");
+    hipLaunchKernelGGL(k_wgrad_o0, dim3((unsigned)blocks), dim3(256), std::max<size_t>(shm, 8192), nn_stream(),
+                       (const uint16_t *)X, (const uint16_t *)dY, ws, B * D * D * D, D, Cout, runs);
+    NNCHK(hipGetLastError(), "wgrad out0 launch");
+    hipLaunchKernelGGL(k_wgrad_o0_reduce, dim3(27), dim3(256), 0, nn_stream(), ws, dW, (int)blocks, Cout);
+    NNCHK(hipGetLastError(), "wgrad out0 reduce");
+    return GP5_OK;
 }
 
 int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)

@@ -82,6 +82,7 @@ def _lib():
         "gp5_nn_conv3d_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
         "gp5_nn_conv3d_act_bwd": [P, P, P, P, P, P, I, I, I, I],
+        "gp5_nn_conv3d_wgrad_out0": [P, P, P, P, I, I, I, I],
         "gp5_nn_act_bwd_part": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_bias_final_multi": [P, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
@@ -476,7 +477,9 @@ class UNet:
         # before the join at the end.
         self._overlapping = self._side_setup()
         # c8, c7
-        self._side("c8", lambda: self._wgrad("c8", t["d1"], t["h_out"], D))
+        e8 = self.w["c8"]  # h_out is nonzero in channel 0 only
+        self._side("c8", lambda: _ck(L.gp5_nn_conv3d_wgrad_out0(t["d1"].ptr, t["h_out"].ptr, e8["Wg"].ptr, self.ws.ptr,
+                                                                B, D, e8["ci"], e8["co"])))
         # c8's data gradient g16a = dd1 = de1 (skip part) and c7's activation backward
         # dz7 = dd1 * act'(a7) (h16, c7's bias partials) in one pass where the shape allows
         e = self.w["c8"]

@@ -96,6 +96,12 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
 /* The weight gradient of gp5_nn_conv3d_up: X is Xh (side D/2), read upsampled;
  * D a multiple of 8; the same workspace size. */
 int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
+/* gp5_nn_conv3d_wgrad for a dY that is nonzero in channel 0 only (the output layer's
+ * gradient rows from gp5_nn_mse_out0): dW[t][ci][0] from channel 0, the other output
+ * channels' gradients 0.  With 16 input channels and D = 32, 64 or 128 the 27 taps
+ * are the MFMA's N dimension (one 16x16x32 MFMA per 16 taps and 32 positions instead
+ * of 27); otherwise the full weight gradient runs (its other columns come out 0). */
+int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
 /* db[c] = sum over positions of dY[p][c] (float). */
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C);
 

@@ -735,3 +735,31 @@ def test_pooled_data_gradient_as_stride2_conv(D, Cin, Cout, B):
     if D <= 64:
         ref = 8.0 * R.pool2(R.conv_dgrad(G, W))
         assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,D,Cout", [(2, 64, 16), (1, 32, 16), (1, 128, 16), (2, 16, 16), (1, 64, 32)])
+def test_output_layer_weight_gradient_from_channel0(B, D, Cout):
+    """gp5_nn_conv3d_wgrad_out0 (dY nonzero in channel 0 only, as gp5_nn_mse_out0
+    writes it; the taps as the MFMA's N dimension at D = 32..128, the full weight
+    gradient at D = 16) agrees with gp5_nn_conv3d_wgrad to float rounding and with the
+    oracle to 1e-4; the other output channels' gradients are exactly 0."""
+    Cin = 16
+    rng = np.random.default_rng(B + D + Cout + 9)
+    L = nn._lib()
+    X = _rb(rng.standard_normal((B, D, D, D, Cin)))
+    G = np.zeros((B, D, D, D, Cout))
+    G[..., 0] = _rb(rng.standard_normal((B, D, D, D)))
+    xb, gb = _buf(X, True), _buf(G, True)
+    ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+    outs = []
+    for f in (L.gp5_nn_conv3d_wgrad_out0, L.gp5_nn_conv3d_wgrad):
+        dW = nn.DevBuf(27 * Cin * Cout * 4)
+        nn._ck(f(xb.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+        outs.append(dW.download(np.float32, (27, Cin, Cout)))
+    scale = np.max(np.abs(outs[1]))
+    assert np.max(np.abs(outs[0] - outs[1])) <= 1e-5 * scale
+    assert not np.any(outs[0][:, :, 1:])
+    if D <= 64:
+        ref = R.conv_wgrad(X, G)
+        assert np.max(np.abs(outs[0] - ref)) <= 1e-4 * np.max(np.abs(ref))
