@@ -285,7 +285,8 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     stride-2 convolutions, 64 folded taps of hi and lo MFMAs for the 8 children of a
     cell: 32/54 of the model count for each; the output layer's weight gradient, whose
     dY has one nonzero channel, runs the taps as the MFMA's N dimension (1024 of the
-    model's 13824 flops per position).  Wall time between device
+    model's 13824 flops per position); the decoder layers' weight gradients sum 64
+    products per coarse cell for the 27 taps of its 8 children (8/27).  Wall time between device
     synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
 
@@ -305,7 +306,12 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     # the output layer's weight gradient from channel 0 (gp5_nn_conv3d_wgrad_out0): one
     # 16x16x32 MFMA per 16 taps and 32 positions, 1024 flops per position against 13824
     out0 = D in (32, 64, 128) and os.environ.get("GP5_WGRAD_OUT0", "1") != "0"
-    wgrad_flops = model_fwd - (layer["c8"] * (1.0 - 1024.0 / 13824.0) if out0 else 0.0)
+    # the decoder layers' weight gradients by output parity (k_wgrad_up2): 64 products
+    # G_a[o] per coarse cell against 216 = 27 taps x 8 children, 8/27 of the model count
+    wup2 = {name for name in ("c5", "c6", "c7")
+            if D // div[name] in (16, 32, 64, 128) and os.environ.get("GP5_WGRAD_UP2", "1") != "0"}
+    wgrad_flops = (sum(f * (8.0 / 27.0 if name in wup2 else 1.0) for name, f in layer.items())
+                   - (layer["c8"] * (1.0 - 1024.0 / 13824.0) if out0 else 0.0))
     train_flops = fwd_flops + dgrad_flops + wgrad_flops
     net = nn.UNet(batch, D, seed=0)
     rng = np.random.default_rng(0)
@@ -338,6 +344,7 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
         "model_forward_flops": model_fwd, "model_forward_TFLOPs": model_fwd / fwd_s / 1e12,
         "model_train_step_flops": model_train, "model_train_TFLOPs": model_train / train_s / 1e12,
         "by_output_parity": sorted(by_parity), "data_gradient_stride2": sorted(stride2),
+        "weight_gradient_by_parity": sorted(wup2),
         "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
                      "flops": "executed MFMA flops"},

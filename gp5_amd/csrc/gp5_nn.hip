@@ -1880,6 +1880,218 @@ __host__ inline size_t wgrad_zs_lds(int D)
     return (size_t)(3 * 16 * wz_chstride((Rb + 2) * (D + 8)) + 16 * wz_chstride(Rb * (D + 8))) * 2;
 }
 
+// Weight gradient of the convolution of a nearest upsampling, by output parity
+// (v_mfma_f32_16x16x32_bf16; X has side Dc = D/2, dY side D, D in {16, 32, 64,
+// 128}).  A fine position p = 2q + a (a in {0,1}^3) reads the parent cell
+// q + o under tap t, o = floor((a + t) / 2) in {a - 1, a}^3, so
+//   dW[t] = sum_a G_a[o(a, t)],   G_a[o] = sum_q X(q + o) dY(2q + a):
+// 64 products per coarse cell against 216 for the 27 taps of its 8 children.
+// A workgroup of 4 waves owns one (16 ci, 16 co) tile pair and Rb = min(Dc,
+// 128/Dc) coarse rows of a run of ZL coarse planes; wave w takes the parities
+// a_z = w >> 1, a_y = w & 1 and keeps the 16 accumulators (s_z, s_y, a_x, s_x),
+// o = a - 1 + s, for the whole walk.  LDS holds a ring of three coarse X slabs
+// (laid out as in k_wgrad_zs) and the two fine dY planes of the current coarse
+// plane split by x parity: row ((a_z * 2Rb + fine row) * 2 + a_x) holds dY at
+// fine x = 2 x_c + a_x in element x_c + 8, elements 0 .. 7 zero.  The x offset
+// moves to dY as in k_wgrad_zs: o_x = -1 reads dY_a at x_c + 1 and o_x = +1 at
+// x_c - 1 (funnel shifts of one aligned read and a neighbouring dword), so every
+// X fragment is one aligned read.  At the end each tap's sum of 8 G's is formed
+// in a fixed order (x parities in registers, then the four waves through LDS)
+// and written to ws[blk][tap][ci][co] for k_wgrad_reduce1.
+constexpr int kWuThreads = 256;
+
+__host__ __device__ constexpr int wu_rows(int Dc) { return Dc < 128 / Dc ? Dc : 128 / Dc; }
+
+// LDS bytes of k_wgrad_up2 for fine cube side D: the staging area, or the 48 KiB
+// of the final combination when that is larger
+__host__ inline size_t wgrad_up2_lds(int D)
+{
+    const int Dc = D / 2, Rb = wu_rows(Dc), XS = Dc + 8;
+    const size_t stage = (size_t)(3 * 16 * wz_chstride((Rb + 2) * XS) + 16 * wz_chstride(8 * Rb * XS)) * 2;
+    return std::max(stage, (size_t)4 * 4 * 3 * 256 * sizeof(float));
+}
+
+__global__ __launch_bounds__(kWuThreads) void k_wgrad_up2(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                          float *ws, int P, int D, int Cin, int Cout, int nzs)
+{
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int az = wave >> 1, ay = wave & 1;
+    const int Dc = D >> 1, Rb = wu_rows(Dc), XS = Dc + 8;
+    const int CSX = wz_chstride((Rb + 2) * XS), CSY = wz_chstride(8 * Rb * XS);
+    const int SLOT = 16 * CSX;  // elements per X slab
+    uint16_t *LX = reinterpret_cast<uint16_t *>(sx);
+    uint16_t *LY = LX + 3 * SLOT;
+    const int nco = Cout / 16;
+    const int cib = (int)blockIdx.y / nco, cob = (int)blockIdx.y % nco;
+    const int nyb = Dc / Rb;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * Dc / nzs, ZL = (zsg + 1) * Dc / nzs - zb;  // coarse rows and planes
+    const int sb = smp * Dc * Dc * Dc;  // coarse position of this sample's origin (X)
+    const int s0 = smp * D * D * D;     // fine position of this sample's origin (dY)
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc(X, (P >> 3) * Cin * 2);
+    const __amdgpu_buffer_rsrc_t yr = make_rsrc(dY, P * Cout * 2);
+    {  // zero the staging area once: the row pads are never written again
+        const int n16 = (3 * SLOT + 16 * CSY) / 8;
+        for (int i = tid; i < n16; i += kWuThreads) reinterpret_cast<uint4 *>(sx)[i] = make_uint4(0, 0, 0, 0);
+    }
+    // X items: (Rb + 2) rows x Dc/2 x-pairs x 2 channel halves (at most 256);
+    // dY items: 2 fine planes x 2Rb fine rows x Dc/2 groups of 4 fine x x 2 halves (at most 512)
+    const int NXI = (Rb + 2) * Dc, NYI = 4 * Rb * Dc;
+    constexpr int MYI = 2;
+    uint4 xv[2], yv[MYI][4];
+    auto load_x = [&](int zz) {
+        const int it = tid;
+        const int pr = it % (Dc / 2), h = (it / (Dc / 2)) & 1, row = it / Dc;
+        const int x = 2 * pr, y = y0 - 1 + row;
+        const bool in = it < NXI && zz >= 0 && zz < Dc && y >= 0 && y < Dc;
+        const int o0 = in ? ((sb + (zz * Dc + y) * Dc + x) * Cin + cib * 16 + 8 * h) * 2 : kOffNone;
+        xv[0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, o0, 0, 0));
+        xv[1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, in ? o0 + Cin * 2 : kOffNone, 0, 0));
+    };
+    auto load_y = [&](int zc) {
+#pragma unroll
+        for (int k = 0; k < MYI; ++k) {
+            const int it = tid + kWuThreads * k;
+            const int x4 = it % (Dc / 2), h = (it / (Dc / 2)) & 1, fr = it / Dc;  // fr = a_z * 2Rb + fine row
+            const int fz = 2 * zc + fr / (2 * Rb), fy = 2 * y0 + fr % (2 * Rb);
+            const bool in = it < NYI && zc < Dc;
+            const int o0 = in ? ((s0 + (fz * D + fy) * D + 4 * x4) * Cout + cob * 16 + 8 * h) * 2 : kOffNone;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                yv[k][j] = __builtin_bit_cast(
+                    uint4, __builtin_amdgcn_raw_buffer_load_b128(yr, in ? o0 + j * Cout * 2 : kOffNone, 0, 0));
+        }
+    };
+    // dword of channel c (0..7 of the half) from two positions: low half from a, high from b
+    auto pair_dw = [](const uint4 &a, const uint4 &b, int c) -> uint32_t {
+        const uint32_t wa = c < 2 ? a.x : (c < 4 ? a.y : (c < 6 ? a.z : a.w));
+        const uint32_t wb = c < 2 ? b.x : (c < 4 ? b.y : (c < 6 ? b.z : b.w));
+        return (c & 1) ? __builtin_amdgcn_perm(wb, wa, 0x07060302u) : __builtin_amdgcn_perm(wb, wa, 0x05040100u);
+    };
+    auto store_x = [&](int slot) {
+        const int it = tid;
+        if (it >= NXI) return;
+        const int pr = it % (Dc / 2), h = (it / (Dc / 2)) & 1, row = it / Dc;
+        uint16_t *p = LX + slot * SLOT + (8 * h) * CSX + row * XS + 2 * pr + 8;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) *reinterpret_cast<uint32_t *>(p + c * CSX) = pair_dw(xv[0], xv[1], c);
+    };
+    auto store_y = [&]() {
+#pragma unroll
+        for (int k = 0; k < MYI; ++k) {
+            const int it = tid + kWuThreads * k;
+            if (it >= NYI) continue;
+            const int x4 = it % (Dc / 2), h = (it / (Dc / 2)) & 1, fr = it / Dc;
+            // fine x 4 x4 + {0, 2} -> parity 0 at coarse x 2 x4, 2 x4 + 1; 4 x4 + {1, 3} -> parity 1
+            uint16_t *p = LY + (8 * h) * CSY + (2 * fr) * XS + 2 * x4 + 8;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                *reinterpret_cast<uint32_t *>(p + c * CSY) = pair_dw(yv[k][0], yv[k][2], c);
+                *reinterpret_cast<uint32_t *>(p + XS + c * CSY) = pair_dw(yv[k][1], yv[k][3], c);
+            }
+        }
+    };
+    __syncthreads();  // the zero fill is complete
+    load_x(zb - 1);
+    store_x(0);
+    load_x(zb);
+    store_x(1);
+    load_x(zb + 1);
+    store_x(2);
+    load_y(zb);
+    store_y();
+    __syncthreads();
+    f32x4 acc[2][2][2][2];  // [s_z][s_y][a_x][s_x]
+#pragma unroll
+    for (int i = 0; i < 16; ++i) (&acc[0][0][0][0])[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nck = Rb * Dc / 32;  // 32-cell chunks of a coarse plane slab (2 or 4)
+    for (int zi = 0; zi < ZL; ++zi) {
+        if (zi + 1 < ZL) {  // in flight during this plane's MFMAs
+            load_x(zb + zi + 2);
+            load_y(zb + zi + 1);
+        }
+        const uint16_t *ys = LY + r * CSY;
+        const uint16_t *xs = LX + r * CSX;
+#pragma unroll 2
+        for (int ck = 0; ck < nck; ++ck) {
+            const int q = 32 * ck + 8 * g;  // slab-local coarse cell of this lane group's 8 cells
+            const int row = q / Dc, x = q - row * Dc;
+            const uint16_t *y0p = ys + ((az * 2 * Rb + 2 * row + ay) * 2) * XS + x + 8;  // a_x = 0
+            const uint16_t *y1p = y0p + XS;                                                // a_x = 1
+            const uint4 c0 = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(y0p, 16));
+            const uint32_t nx0 = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(y0p + 8, 4));
+            const uint4 c1 = *reinterpret_cast<const uint4 *>(__builtin_assume_aligned(y1p, 16));
+            const uint32_t pv1 = *reinterpret_cast<const uint32_t *>(__builtin_assume_aligned(y1p - 2, 4));
+            // a_x = 0: s_x = 0 is o_x = -1 (cells x+1 .. x+8), s_x = 1 is o_x = 0;
+            // a_x = 1: s_x = 0 is o_x = 0, s_x = 1 is o_x = +1 (cells x-1 .. x+6)
+            const bf16x8m b00 = __builtin_bit_cast(
+                bf16x8m, make_uint4(__builtin_amdgcn_alignbit(c0.y, c0.x, 16), __builtin_amdgcn_alignbit(c0.z, c0.y, 16),
+                                    __builtin_amdgcn_alignbit(c0.w, c0.z, 16), __builtin_amdgcn_alignbit(nx0, c0.w, 16)));
+            const bf16x8m b01 = __builtin_bit_cast(bf16x8m, c0);
+            const bf16x8m b10 = __builtin_bit_cast(bf16x8m, c1);
+            const bf16x8m b11 = __builtin_bit_cast(
+                bf16x8m, make_uint4(__builtin_amdgcn_alignbit(c1.x, pv1, 16), __builtin_amdgcn_alignbit(c1.y, c1.x, 16),
+                                    __builtin_amdgcn_alignbit(c1.z, c1.y, 16), __builtin_amdgcn_alignbit(c1.w, c1.z, 16)));
+#pragma unroll
+            for (int sz = 0; sz < 2; ++sz) {
+                const int slot = (zi + az + sz) % 3;  // ring slot of coarse plane z + a_z - 1 + s_z
+#pragma unroll
+                for (int sy = 0; sy < 2; ++sy) {  // slab row of coarse row + a_y - 1 + s_y
+                    const bf16x8m a = *reinterpret_cast<const bf16x8m *>(
+                        __builtin_assume_aligned(xs + slot * SLOT + (row + ay + sy) * XS + x + 8, 16));
+                    acc[sz][sy][0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b00, acc[sz][sy][0][0], 0, 0, 0);
+                    acc[sz][sy][0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b01, acc[sz][sy][0][1], 0, 0, 0);
+                    acc[sz][sy][1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b10, acc[sz][sy][1][0], 0, 0, 0);
+                    acc[sz][sy][1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b11, acc[sz][sy][1][1], 0, 0, 0);
+                }
+            }
+        }
+        if (zi + 1 < ZL) {
+            __syncthreads();  // every wave is done with the slab of plane z-1 and the dY planes of z
+            store_x(zi % 3);  // plane z + 2 takes the slot of plane z - 1
+            store_y();
+            __syncthreads();
+        }
+    }
+    // V[wave][s_z s_y][t_x + 1][ci 16][co 16]: the two x parities combined, in a fixed order
+    // (t_x = -1: s_x = 0 of both; t_x = 0: s_x = 1 of a_x = 0 and s_x = 0 of a_x = 1; t_x = 1: s_x = 1 of both)
+    __syncthreads();  // the staging area is free
+    float *V = reinterpret_cast<float *>(sx);
+#pragma unroll
+    for (int sz = 0; sz < 2; ++sz)
+#pragma unroll
+        for (int sy = 0; sy < 2; ++sy) {
+            const f32x4 v[3] = {acc[sz][sy][0][0] + acc[sz][sy][1][0], acc[sz][sy][0][1] + acc[sz][sy][1][0],
+                                acc[sz][sy][0][1] + acc[sz][sy][1][1]};
+#pragma unroll
+            for (int tx = 0; tx < 3; ++tx)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) V[((wave * 4 + sz * 2 + sy) * 3 + tx) * 256 + (4 * g + i) * 16 + r] = v[tx][i];
+        }
+    __syncthreads();
+    // thread tid owns (ci, co) = (tid / 16, tid % 16) of the tile pair; tap (t_z, t_y, t_x) sums,
+    // over the waves (a_z, a_y) in order, the G with s = (a == 0 ? t >= 0 : t == 1) per axis
+    const int n = 27 * Cin * Cout;
+    float *o = ws + (size_t)blockIdx.x * n + (size_t)(cib * 16 + (tid >> 4)) * Cout + cob * 16 + (tid & 15);
+#pragma unroll
+    for (int tz = -1; tz <= 1; ++tz)
+#pragma unroll
+        for (int ty = -1; ty <= 1; ++ty)
+#pragma unroll
+            for (int tx = 0; tx < 3; ++tx) {
+                float s = 0.f;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const int sz = (w >> 1) == 0 ? (tz >= 0) : (tz == 1);
+                    const int sy = (w & 1) == 0 ? (ty >= 0) : (ty == 1);
+                    s += V[((w * 4 + sz * 2 + sy) * 3 + tx) * 256 + tid];
+                }
+                o[(size_t)(((tz + 1) * 3 + (ty + 1)) * 3 + tx) * Cin * Cout] = s;
+            }
+}
+
 // Sum of the chunk partials in one fixed-order pass: a workgroup owns OPB
 // consecutive outputs, its Q =
 // 256 / OPB thread groups take the chunks c = q, q + Q, q + 2Q, ... (q the
@@ -2988,6 +3200,30 @@ static long long wgrad_zs_plan(int B, int D, int Cin, int Cout, int *nzs_out)
     return per_z * nzs;
 }
 
+// Plan of k_wgrad_up2 for a fine cube side D in {16, 32, 64, 128}: the longest
+// coarse plane run (a power of two, at least 2) that keeps 512 workgroups;
+// returns the number of position blocks (the chunks of the reduction), 0 when
+// the kernel does not apply (or GP5_WGRAD_UP2=0 turns it off).
+static long long wgrad_up2_plan(int B, int D, int Cin, int Cout, int *nzs_out)
+{
+    const char *env = getenv("GP5_WGRAD_UP2");
+    if (env && env[0] == '0') return 0;
+    if (D != 16 && D != 32 && D != 64 && D != 128) return 0;
+    const int Dc = D / 2;
+    const long long per_z = (long long)B * (Dc / wu_rows(Dc)), tiles = (long long)(Cin / 16) * (Cout / 16);
+    int zl = 2;
+    for (int c = Dc; c >= 2; c /= 2)
+        if (per_z * (Dc / c) * tiles >= 512) {
+            zl = c;
+            break;
+        }
+    int nzs = Dc / zl;
+    const char *runs = getenv("GP5_WU2_RUNS");  // coarse plane runs per cube side (measurement)
+    if (runs && atoi(runs) >= 1 && atoi(runs) <= Dc / 2) nzs = atoi(runs);
+    *nzs_out = nzs;
+    return per_z * nzs;
+}
+
 size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
 {
     const long long P = (long long)B * D * D * D;
@@ -2995,10 +3231,12 @@ size_t gp5_nn_wgrad_ws(int B, int D, int Cin, int Cout)
     const long long chunks = (P + chunk - 1) / chunk;
     int nzs = 0;
     const long long zchunks = wgrad_zs_plan(B, D, Cin, Cout, &nzs);
+    const long long uchunks = wgrad_up2_plan(B, D, Cin, Cout, &nzs);
     // channel-major copies of X and dY (blocked by 8 positions when D % 8 == 0), then the partial sums;
-    // or the partial sums of the position blocks of k_wgrad_zs
+    // or the partial sums of the position blocks of k_wgrad_zs or k_wgrad_up2
     const size_t n = (size_t)27 * Cin * Cout * sizeof(float);
-    return std::max((size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)chunks * n, (size_t)zchunks * n);
+    return std::max((size_t)P * (Cin + Cout) * sizeof(uint16_t) + (size_t)chunks * n,
+                    (size_t)std::max(zchunks, uchunks) * n);
 }
 
 // dW = the sum of `chunks` partials of n values each (fixed order)
@@ -3026,6 +3264,20 @@ static int wgrad_run(const void *X, const void *dY, float *dW, float *ws, int B,
     const long long P = (long long)B * D * D * D;
     const long long n = 27LL * Cin * Cout;
     int nzs = 0;
+    const long long ublocks = up ? wgrad_up2_plan(B, D, Cin, Cout, &nzs) : 0;
+    if (ublocks > 0) {  // by output parity
+        const size_t shm = wgrad_up2_lds(D);
+        static size_t shm_up2 = 65536;  // the default limit of dynamic LDS
+        if (shm > shm_up2) {
+            NNCHK(hipFuncSetAttribute((const void *)k_wgrad_up2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                  "wgrad up2 LDS size");
+            shm_up2 = shm;
+        }
+        hipLaunchKernelGGL(k_wgrad_up2, dim3((unsigned)ublocks, (unsigned)((Cin / 16) * (Cout / 16))), dim3(kWuThreads),
+                           shm, nn_stream(), (const uint16_t *)X, (const uint16_t *)dY, ws, (int)P, D, Cin, Cout, nzs);
+        NNCHK(hipGetLastError(), "wgrad up2 launch");
+        return wgrad_reduce(ws, dW, n, (int)ublocks);
+    }
     const long long zblocks = wgrad_zs_plan(B, D, Cin, Cout, &nzs);
     if (zblocks > 0) {
         const size_t shm = wgrad_zs_lds(D);

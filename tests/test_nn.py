@@ -519,6 +519,7 @@ def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B, monkeypatch):
     (the kernels that read the parent cell; the output-parity form, which sums in
     another order, is off here and has its own test)."""
     monkeypatch.setenv("GP5_CONV_UP2P", "0")
+    monkeypatch.setenv("GP5_WGRAD_UP2", "0")
     rng = np.random.default_rng(D * 7 + Cin)
     Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
     U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
@@ -546,6 +547,38 @@ def test_conv_reads_upsampled_input_in_place(D, Cin, Cout, B, monkeypatch):
     assert np.array_equal(grads[0], grads[1])
     gref = R.conv_wgrad(U, G)
     assert np.max(np.abs(grads[0] - gref)) <= 1e-4 * np.max(np.abs(gref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("D,Cin,Cout,B", [(64, 32, 16, 1), (32, 64, 32, 2), (16, 128, 64, 2), (16, 16, 32, 1),
+                                          (128, 16, 16, 1), (32, 16, 48, 1)])
+def test_upsampled_wgrad_by_output_parity(D, Cin, Cout, B, monkeypatch):
+    """gp5_nn_conv3d_wgrad_up at D in {16, 32, 64, 128} sums, per fine parity a and
+    coarse offset o, G_a[o] = sum_q X(q + o) dY(2q + a) and forms each tap from 8 of
+    them (k_wgrad_up2).  It agrees with the oracle on the nearest upsampling to 1e-4,
+    and to float rounding with the kernel that reads each fine position's parent
+    (GP5_WGRAD_UP2=0) and with a single coarse plane run per cube."""
+    rng = np.random.default_rng(D * 11 + Cin + Cout + B)
+    Xh = _rb(rng.standard_normal((B, D // 2, D // 2, D // 2, Cin)))
+    G = _rb(rng.standard_normal((B, D, D, D, Cout)))
+    L = nn._lib()
+    xh, gb = _buf(Xh, True), _buf(G, True)
+    grads = []
+    for up2, runs in (("1", None), ("0", None), ("1", "1")):
+        monkeypatch.setenv("GP5_WGRAD_UP2", up2)
+        if runs:
+            monkeypatch.setenv("GP5_WU2_RUNS", runs)
+        ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, Cin, Cout))
+        dW = nn.DevBuf(27 * Cin * Cout * 4)
+        nn._ck(L.gp5_nn_conv3d_wgrad_up(xh.ptr, gb.ptr, dW.ptr, ws.ptr, B, D, Cin, Cout))
+        grads.append(dW.download(np.float32, (27, Cin, Cout)))
+    scale = np.max(np.abs(grads[1]))
+    assert np.max(np.abs(grads[0] - grads[1])) <= 1e-5 * scale
+    assert np.max(np.abs(grads[2] - grads[1])) <= 1e-5 * scale
+    if D <= 64:
+        U = Xh.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
+        ref = R.conv_wgrad(U, G)
+        assert np.max(np.abs(grads[0] - ref)) <= 1e-4 * np.max(np.abs(ref))
 
 
 @pytest.mark.gpu
