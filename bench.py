@@ -284,8 +284,8 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
     lo MFMAs for two fine positions, and the pooled data gradients of c6 and c7 run as
     stride-2 convolutions, 64 folded taps of hi and lo MFMAs for the 8 children of a
     cell: 32/54 of the model count for each; the output layer's weight gradient, whose
-    dY has one nonzero channel, runs the taps as the MFMA's N dimension (1024 of the
-    model's 13824 flops per position); the decoder layers' weight gradients sum 64
+    dY has one nonzero channel, runs the taps as the MFMA's N dimension and its data
+    gradient the taps as K (1024 of the model's 13824 flops per position each); the decoder layers' weight gradients sum 64
     products per coarse cell for the 27 taps of its 8 children (8/27).  Wall time between device
     synchronisations (the loss stays on the device), so launch costs are included."""
     from gp5_amd import nn
@@ -303,6 +303,10 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
                and nn._pool2p(D // div[name], co)}
     fwd_flops = sum(f * (32.0 / 54.0 if name in by_parity else 1.0) for name, f in layer.items())
     dgrad_flops = sum(f * (32.0 / 54.0 if name in stride2 else 1.0) for name, f in layer.items() if name != "c1")
+    # the output layer's data gradient from channel 0 alone (gp5_nn_conv3d_act_bwd_out0):
+    # one 16x16x32 MFMA per 16 positions, 1024 flops per position against 13824
+    if nn._out0c(D):
+        dgrad_flops -= layer["c8"] * (1.0 - 1024.0 / 13824.0)
     # the output layer's weight gradient from channel 0 (gp5_nn_conv3d_wgrad_out0): one
     # 16x16x32 MFMA per 16 taps and 32 positions, 1024 flops per position against 13824
     out0 = D in (32, 64, 128) and os.environ.get("GP5_WGRAD_OUT0", "1") != "0"

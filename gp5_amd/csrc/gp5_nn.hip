@@ -1418,6 +1418,141 @@ __global__ __launch_bounds__(256) void k_wgrad_o0_reduce(const float *__restrict
     }
 }
 
+// Data gradient of the output layer from channel 0 of its output gradient (one
+// bf16 per position, G0; the loss reads channel 0, so the other channels are
+// zero), with the backward of the leaky ReLU below in the epilogue (as k_conv_zs
+// with ABWD, mirror = 1):
+//   dX[p][n] = sum_tap Wk[26 - tap][n][0] G0[p + off(tap)],
+// one 16x16x32 MFMA per 16 positions and 16 channels with the 27 taps as K (5 of
+// them zero) against 27 16x16x16 MFMAs for the full 16-channel gradient.  A
+// workgroup of 4 waves owns Rb = 256 / D rows of a run of ZL planes and 16
+// channels (16 blockIdx.y ..); LDS holds G0 of planes z-1, z, z+1 in a ring
+// (rows y0 - 1 .. y0 + Rb, 8 zero elements on either side of each row) and a
+// wave-private epilogue scratch.  Lane (r, g) gathers taps 8g .. 8g + 7 of
+// position r of its tile.  The epilogue, the bias partials (row x of bpart for
+// workgroup x, zeros to rows x + gridDim.x, ...) and their order are those of
+// k_conv_zs with ABWD.
+constexpr int kDo0Pad = 8;
+__host__ inline size_t dgrad_o0_lds(int D)
+{
+    const size_t ring = (size_t)3 * (256 / D + 2) * (D + 2 * kDo0Pad) * 2;
+    return std::max((ring + 15) / 16 * 16 + 4 * (size_t)kEpiScratch, (size_t)8 * 256 * sizeof(float));
+}
+
+__global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G0, const uint16_t *__restrict__ Wk,
+                                                  const uint16_t *A, uint16_t *Y, float *Yf, int P, int D, int Cin,
+                                                  int Cout, int nzs, float *bpart)
+{
+    constexpr int MT = 4;
+    typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+    extern __shared__ __align__(16) unsigned char sx[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int nb0 = (int)blockIdx.y;
+    const int Rb = 256 / D, HR = Rb + 2, HS = D + 2 * kDo0Pad;
+    uint16_t *HY = reinterpret_cast<uint16_t *>(sx);  // [3][HR][HS]
+    unsigned char *scr = sx + ((size_t)3 * HR * HS * 2 + 15) / 16 * 16 + wave * kEpiScratch;
+    const int nyb = D / Rb;
+    const int yb = (int)blockIdx.x % nyb, zsg = ((int)blockIdx.x / nyb) % nzs, smp = (int)blockIdx.x / (nyb * nzs);
+    const int y0 = yb * Rb, zb = zsg * D / nzs, ZL = (zsg + 1) * D / nzs - zb;
+    const int s0 = smp * D * D * D;
+    for (int i = tid; i < 3 * HR * HS / 8; i += 256) reinterpret_cast<uint4 *>(HY)[i] = make_uint4(0, 0, 0, 0);
+    // B[k = tap 8g + j][n = r]: the weights of output channel 16 nb0 + r, input channel 0, mirrored taps
+    bf8 b;
+    {
+        uint32_t wv[4];
+#pragma unroll
+        for (int j = 0; j < 8; j += 2) {
+            const int t0 = 8 * g + j, t1 = t0 + 1;
+            const uint32_t lo = t0 < 27 ? Wk[((size_t)(26 - t0) * Cout + 16 * nb0 + r) * Cin] : 0u;
+            const uint32_t hi = t1 < 27 ? Wk[((size_t)(26 - t1) * Cout + 16 * nb0 + r) * Cin] : 0u;
+            wv[j / 2] = lo | (hi << 16);
+        }
+        b = __builtin_bit_cast(bf8, make_uint4(wv[0], wv[1], wv[2], wv[3]));
+    }
+    // this lane's taps 8g + j: ring plane dz + 1 and the element offset within a slab
+    // (row + 1 + dy, x + dx), or none past tap 26
+    int tpl[8], toff[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int t = 8 * g + j;
+        tpl[j] = t < 27 ? t / 9 : -1;
+        toff[j] = t < 27 ? ((t / 3) % 3) * HS + kDo0Pad + t % 3 - 1 : 0;
+    }
+    uint16_t hv[2];  // (Rb + 2) D <= 512 elements of a plane: two per thread
+    auto load_h = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = tid + 256 * k, row = e / D, x = e - row * D;
+            const int y = y0 - 1 + row;
+            const bool in = row < HR && z >= 0 && z < D && y >= 0 && y < D;
+            hv[k] = in ? G0[(size_t)s0 + ((size_t)z * D + y) * D + x] : (uint16_t)0;
+        }
+    };
+    auto store_h = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = tid + 256 * k, row = e / D, x = e - row * D;
+            if (row < HR) HY[(slot * HR + row) * HS + kDo0Pad + x] = hv[k];
+        }
+    };
+    __syncthreads();  // the zero fill is complete
+    load_h(zb - 1);
+    store_h(0);
+    load_h(zb);
+    store_h(1);
+    load_h(zb + 1);
+    store_h(2);
+    __syncthreads();
+    float bs[1][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bs[0][e] = 0.f;
+    for (int zi = 0; zi < ZL; ++zi) {
+        const int z = zb + zi;
+        if (zi + 1 < ZL) load_h(z + 2);  // in flight during this plane's MFMAs
+        const int sb[3] = {(zi % 3) * HR * HS, ((zi + 1) % 3) * HR * HS, ((zi + 2) % 3) * HR * HS};  // planes z-1, z, z+1
+        f32x4 acc[MT][1];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) {
+            const int q = 64 * wave + 16 * t + r, row = q / D, x = q - row * D;
+            const int po = row * HS + x;
+            uint32_t av[4];
+#pragma unroll
+            for (int j = 0; j < 8; j += 2) {
+                const uint32_t lo = tpl[j] >= 0 ? HY[sb[tpl[j]] + toff[j] + po] : 0u;
+                const uint32_t hi = tpl[j + 1] >= 0 ? HY[sb[tpl[j + 1]] + toff[j + 1] + po] : 0u;
+                av[j / 2] = lo | (hi << 16);
+            }
+            acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, make_uint4(av[0], av[1], av[2], av[3])),
+                                                               b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+        conv_epilogue_lds_impl<1, MT, true, true>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, nullptr, 0,
+                                                  const_cast<uint16_t *>(A), nullptr, Y, Yf, scr, bs);
+        if (zi + 1 < ZL) {
+            __syncthreads();  // every wave is done with the slab of plane z-1
+            store_h(zi % 3);  // plane z + 2 takes the slot of plane z - 1
+            __syncthreads();
+        }
+    }
+    // channel c: the lanes with (lane & 1) == c / 8 of the four waves, in wave and lane order
+    __syncthreads();  // every wave is done with the LDS
+    float *red = reinterpret_cast<float *>(sx);  // 8 x 256 floats
+#pragma unroll
+    for (int e = 0; e < 8; ++e) red[e * 256 + tid] = bs[0][e];
+    __syncthreads();
+    if (tid < 16) {
+        const int c = tid;
+        const float *rr = red + (c % 8) * 256;
+        float t = 0.f;
+        for (int w = 0; w < 4; ++w)
+            for (int l = c / 8; l < 64; l += 2) t += rr[64 * w + l];
+        const int co = 16 * nb0 + c;
+        bpart[(size_t)blockIdx.x * Cout + co] = t;
+        for (int row = (int)blockIdx.x + (int)gridDim.x; row < kBiasBlocks; row += (int)gridDim.x)
+            bpart[(size_t)row * Cout + co] = 0.f;
+    }
+}
+
 // Weight gradient: one wave per (16 input channels, 16 output channels, chunk of
 // positions) accumulates all 27 taps at once, so each X and dY element of the
 // chunk is fetched by one wave instead of 27; partial sums go to
@@ -2495,7 +2630,7 @@ __global__ void k_fill0(uint32_t *p, long long n)
 // bf16; partial[b] and partial[gridDim.x + b] the block sums of e^2 and of the
 // channel-0 gradient 2 e / P (in double)
 __global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, const float *base, const float *target,
-                                             float *dYf, uint16_t *H, long long P, double *partial)
+                                             float *dYf, uint16_t *H, int hc, long long P, double *partial)
 {
     __shared__ double s[256], sg[256];
     double a = 0.0, ag = 0.0;
@@ -2513,7 +2648,9 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, co
             dYf[(size_t)p * C] = inv * e;
             for (int c = 1; c < C; ++c) dYf[(size_t)p * C + c] = 0.f;
         }
-        if (H) {  // C % 8 == 0 (checked on the host): 16-byte stores
+        if (H && hc == 1) {  // channel 0 alone, one bf16 per position
+            H[p] = f2bf(inv * e);
+        } else if (H) {  // C % 8 == 0 (checked on the host): 16-byte stores
             uint4 *hrow = reinterpret_cast<uint4 *>(H + (size_t)p * C);
             hrow[0] = make_uint4((uint32_t)f2bf(inv * e), 0u, 0u, 0u);
             for (int c = 1; c < C / 8; ++c) hrow[c] = make_uint4(0u, 0u, 0u, 0u);
@@ -3172,6 +3309,43 @@ int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float 
     return act_bwd_bias_run(Ab, Yf, nullptr, 0.f, D, H, nullptr, (long long)B * D * D * D, Cout, part);
 }
 
+int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+                               int D, int Cin, int Cout)
+{
+    if (!dY0 || !Wk || !Ab || !Yf || !H || !part) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (D < 16 || D > 128 || (D & (D - 1)) || Cin < 1 || Cout < 16 || Cout % 16 || B < 1)
+        return set_error(GP5_ERR_ARG, "conv3d_act_bwd_out0: D a power of two in [16, 128], Cout a multiple of 16");
+    if ((long long)B * D * D * D * Cout * 4 > 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "conv3d_act_bwd_out0: tensors must stay below 2 GiB");
+    const int per_run = B * (D / (256 / D)), gy = Cout / 16;
+    if (per_run > kBiasBlocks) return set_error(GP5_ERR_ARG, "conv3d_act_bwd_out0: batch too large for the bias rows");
+    const size_t shm = dgrad_o0_lds(D);
+    int nzs = 0;
+    const char *rn = getenv("GP5_DO0_RUNS");  // plane runs per cube side (measurement)
+    if (rn && atoi(rn) >= 1 && atoi(rn) <= D / 2) {
+        nzs = atoi(rn);
+    } else {  // fill the device's workgroup slots once
+        static size_t occ_shm = 0;
+        static int slots = 0;
+        if (occ_shm != shm) {
+            int dev = 0, cus = 0, per_cu = 0;
+            NNCHK(hipGetDevice(&dev), "act_bwd_out0 device");
+            NNCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "act_bwd_out0 CU count");
+            NNCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_dgrad_o0, 256, shm),
+                  "act_bwd_out0 occupancy");
+            slots = std::max(1, cus * per_cu);
+            occ_shm = shm;
+        }
+        nzs = std::max(1, std::min(D / 2, slots / (per_run * gy)));
+    }
+    nzs = std::max(1, std::min(nzs, kBiasBlocks / per_run));  // one bias partial row per workgroup
+    hipLaunchKernelGGL(k_dgrad_o0, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(),
+                       (const uint16_t *)dY0, (const uint16_t *)Wk, (const uint16_t *)Ab, (uint16_t *)H, Yf,
+                       B * D * D * D, D, Cin, Cout, nzs, part);
+    NNCHK(hipGetLastError(), "act_bwd_out0 launch");
+    return GP5_OK;
+}
+
 // Plan of k_wgrad_zs for a cube side D (a power of two in [16, 128]): the longest
 // plane run ZL (a power of two, at least 2) that keeps 512 workgroups; returns
 // the number of position blocks (the chunks of the reduction), 0 when the
@@ -3330,15 +3504,21 @@ int gp5_nn_conv3d_wgrad(const void *X, const void *dY, float *dW, float *ws, int
     return wgrad_run(X, dY, dW, ws, B, D, Cin, Cout, 0);
 }
 
-int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+// k_wgrad_o0 and its reduction; dY rows of ldy channels (Cout, or 1: channel 0 alone).
+// Without the kernel's shape the full weight gradient runs (ldy = Cout) or the call
+// fails (ldy = 1).
+static int wgrad_out0_run(const void *X, const void *dY, int ldy, float *dW, float *ws, int B, int D, int Cin, int Cout)
 {
     if (!X || !dY || !dW || !ws) return set_error(GP5_ERR_ARG, "NULL argument");
     int nzs = 0;
     const long long zblocks = (Cin == 16 && Cout % 16 == 0 && Cout >= 16 && (D == 32 || D == 64 || D == 128))
                                   ? wgrad_zs_plan(B, D, Cin, 16, &nzs) : 0;
     const char *env = getenv("GP5_WGRAD_OUT0");
-    if (zblocks <= 0 || (long long)B * D * D * D * Cout * 2 > 0x7fffffffLL || (env && env[0] == '0'))
+    if (zblocks <= 0 || (long long)B * D * D * D * Cout * 2 > 0x7fffffffLL || (ldy > 1 && env && env[0] == '0')) {
+        if (ldy == 1)
+            return set_error(GP5_ERR_ARG, "conv3d_wgrad_out0c: Cin = 16, Cout a multiple of 16, D in {32, 64, 128}");
         return wgrad_run(X, dY, dW, ws, B, D, Cin, Cout, 0);  // every output channel (the others give 0)
+    }
     const int Rb = 256 / D;
     const size_t shm = (size_t)16 * 256 * 2 + (size_t)3 * (Rb + 2) * (D + 2 * kWo0Pad) * 2;
     // D / 16 plane runs (measured at 64^3, batch 8: 4 runs 41.3 us, 8 53.4, 16 73.1, 32
@@ -3349,11 +3529,21 @@ int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws
     const long long blocks = (long long)B * (D / Rb) * runs;
     if (blocks * 27 * 16 > zblocks * 27 * 16 * 16) return set_error(GP5_ERR_ARG, "wgrad out0: workspace");
     hipLaunchKernelGGL(k_wgrad_o0, dim3((unsigned)blocks), dim3(256), std::max<size_t>(shm, 8192), nn_stream(),
-                       (const uint16_t *)X, (const uint16_t *)dY, ws, B * D * D * D, D, Cout, runs);
+                       (const uint16_t *)X, (const uint16_t *)dY, ws, B * D * D * D, D, ldy, runs);
     NNCHK(hipGetLastError(), "wgrad out0 launch");
     hipLaunchKernelGGL(k_wgrad_o0_reduce, dim3(27), dim3(256), 0, nn_stream(), ws, dW, (int)blocks, Cout);
     NNCHK(hipGetLastError(), "wgrad out0 reduce");
     return GP5_OK;
+}
+
+int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
+{
+    return wgrad_out0_run(X, dY, Cout, dW, ws, B, D, Cin, Cout);
+}
+
+int gp5_nn_conv3d_wgrad_out0c(const void *X, const void *dY0, float *dW, float *ws, int B, int D, int Cin, int Cout)
+{
+    return wgrad_out0_run(X, dY0, 1, dW, ws, B, D, Cin, Cout);
 }
 
 int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout)
@@ -3596,17 +3786,19 @@ static int mse_scratch(size_t blocks, double **out)
 // over positions of the gradient rows: channel 0 from the double block sums, 0
 // elsewhere.  The sums finish on the device; the one host copy is the loss
 // (none when acc, a device double the loss is added to, is given instead).
+// hc: channels of the rows of H (C, or 1 for channel 0 alone)
 static int mse_run(const float *Yf, int ldy, int C, const float *base, const float *target, float *dYf, uint16_t *H,
-                   float *db, long long P, double *loss, double *acc = nullptr)
+                   float *db, long long P, double *loss, double *acc = nullptr, int hc = 0)
 {
     if (!Yf || !base || !target || (!loss && !acc)) return set_error(GP5_ERR_ARG, "NULL argument");
-    if (H && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
+    if (hc != 1) hc = C;
+    if (H && hc == C && C % 8) return set_error(GP5_ERR_ARG, "mse_bf16: C must be a multiple of 8");
     if (C < 1 || P < 1) return set_error(GP5_ERR_ARG, "mse: bad size");
     const unsigned blocks = (unsigned)std::min<long long>((P + 255) / 256, 1024);
     double *part = nullptr;
     const int rc = mse_scratch(blocks, &part);
     if (rc != GP5_OK) return rc;
-    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, nn_stream(), Yf, ldy, C, base, target, dYf, H, P, part);
+    hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, nn_stream(), Yf, ldy, C, base, target, dYf, H, hc, P, part);
     NNCHK(hipGetLastError(), "mse");
     hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(256), 0, nn_stream(), part, (int)blocks, P, part + 2 * blocks, db, C, acc);
     NNCHK(hipGetLastError(), "mse");
@@ -3642,6 +3834,13 @@ int gp5_nn_mse_out0_acc(const float *y0, const float *base, const float *target,
 {
     if (!H || !loss_acc) return set_error(GP5_ERR_ARG, "NULL argument");
     return mse_run(y0, 1, C, base, target, nullptr, (uint16_t *)H, db, P, nullptr, loss_acc);
+}
+
+int gp5_nn_mse_out0c(const float *y0, const float *base, const float *target, void *H0, int C, float *db, long long P,
+                     double *loss, double *loss_acc)
+{
+    if (!H0) return set_error(GP5_ERR_ARG, "NULL argument");
+    return mse_run(y0, 1, C, base, target, nullptr, (uint16_t *)H0, db, P, loss, loss_acc, 1);
 }
 
 int gp5_nn_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2, float eps, int t)

@@ -58,6 +58,13 @@ def _up_fused(d: int) -> bool:
     return 8 <= d <= 128 and d & (d - 1) == 0
 
 
+def _out0c(d: int) -> bool:
+    """Whether the output layer's gradient is kept as channel 0 alone (one bf16 per
+    position; gp5_nn_mse_out0c, gp5_nn_conv3d_wgrad_out0c, gp5_nn_conv3d_act_bwd_out0:
+    cube sides 32, 64 and 128); GP5_OUT0_COMPACT=0 keeps the 16-channel rows."""
+    return d in (32, 64, 128) and os.environ.get("GP5_OUT0_COMPACT", "1") != "0"
+
+
 def _lib():
     L = capi.load()
     if getattr(L, "_gp5_nn_sigs", False):
@@ -83,6 +90,9 @@ def _lib():
         "gp5_nn_conv3d_pool": [P, P, P, I, I, I, I, I],
         "gp5_nn_conv3d_act_bwd": [P, P, P, P, P, P, I, I, I, I],
         "gp5_nn_conv3d_wgrad_out0": [P, P, P, P, I, I, I, I],
+        "gp5_nn_conv3d_wgrad_out0c": [P, P, P, P, I, I, I, I],
+        "gp5_nn_conv3d_act_bwd_out0": [P, P, P, P, P, P, I, I, I, I],
+        "gp5_nn_mse_out0c": [P, P, P, P, I, P, LL, ctypes.POINTER(ctypes.c_double), P],
         "gp5_nn_act_bwd_part": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_bias_final_multi": [P, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
@@ -223,7 +233,7 @@ class UNet:
             "u2": (n(D, 32), 2), "a7": (n(D, 16), 2), "d1": (n(D, 16), 2),
             "out": (n(D, 1), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
-            "h_out": (n(D, 16), 2),
+            "h_out": (n(D, 1 if _out0c(D) else 16), 2),  # channel 0 alone, or rows of 16
             "g16a": (n(D, 16), 4), "h16": (n(D, 16), 2),
             "g32a": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
             "g64a": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
@@ -241,6 +251,7 @@ class UNet:
             del sizes["gu32"]
         if _pool_fused(D // 2, 64, 32, B):
             del sizes["gu64"]
+        self._o0c = _out0c(D)
         self.t = {k: DevBuf(e * b) for k, (e, b) in sizes.items()}
         self.t["loss_acc"] = DevBuf(8)  # sum of the losses of backward(read_loss=False) calls
         self.t["loss_acc"].zero()
@@ -465,7 +476,11 @@ class UNet:
         L, t, B, D = _lib(), self.t, self.B, self.D
         loss = ctypes.c_double()
         # loss, the bf16 output gradient and c8's bias gradient in one pass
-        if read_loss:
+        if self._o0c:
+            _ck(L.gp5_nn_mse_out0c(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
+                                   self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss) if read_loss else None,
+                                   None if read_loss else t["loss_acc"].ptr))
+        elif read_loss:
             _ck(L.gp5_nn_mse_out0(t["out"].ptr, t["base"].ptr, t["target"].ptr, t["h_out"].ptr, 16,
                                   self.w["c8"]["bg"].ptr, B * D ** 3, ctypes.byref(loss)))
         else:
@@ -477,13 +492,15 @@ class UNet:
         # before the join at the end.
         self._overlapping = self._side_setup()
         # c8, c7
-        e8 = self.w["c8"]  # h_out is nonzero in channel 0 only
-        self._side("c8", lambda: _ck(L.gp5_nn_conv3d_wgrad_out0(t["d1"].ptr, t["h_out"].ptr, e8["Wg"].ptr, self.ws.ptr,
-                                                                B, D, e8["ci"], e8["co"])))
+        e8 = self.w["c8"]  # h_out is nonzero in channel 0 only (or holds channel 0 alone)
+        wg8 = L.gp5_nn_conv3d_wgrad_out0c if self._o0c else L.gp5_nn_conv3d_wgrad_out0
+        self._side("c8", lambda: _ck(wg8(t["d1"].ptr, t["h_out"].ptr, e8["Wg"].ptr, self.ws.ptr, B, D, e8["ci"],
+                                         e8["co"])))
         # c8's data gradient g16a = dd1 = de1 (skip part) and c7's activation backward
         # dz7 = dd1 * act'(a7) (h16, c7's bias partials) in one pass where the shape allows
         e = self.w["c8"]
-        _ck(L.gp5_nn_conv3d_act_bwd(t["h_out"].ptr, e["Wb"].ptr, t["a7"].ptr, t["g16a"].ptr, t["h16"].ptr,
+        act_bwd8 = L.gp5_nn_conv3d_act_bwd_out0 if self._o0c else L.gp5_nn_conv3d_act_bwd
+        _ck(act_bwd8(t["h_out"].ptr, e["Wb"].ptr, t["a7"].ptr, t["g16a"].ptr, t["h16"].ptr,
                                     self.bpart["c7"].ptr, B, D, e["co"], e["ci"]))
         self._side("c7", lambda: self._wgrad_up("c7", "d2", "u2", t["h16"], D))
         if _pool_fused(D, 32, 16, B):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)

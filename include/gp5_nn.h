@@ -79,6 +79,15 @@ int gp5_nn_conv3d_pool(const void *X, const void *Wk, float *Yh, int B, int D, i
  * run).  Cout / 4 must divide 256. */
 int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
                           int D, int Cin, int Cout);
+/* gp5_nn_conv3d_act_bwd for the output layer's gradient, given as channel 0 alone
+ * (dY0: one bf16 per position, gp5_nn_mse_out0c's H0; the loss reads channel 0, so
+ * the other Cin - 1 channels are zero): Yf[p][n] = sum_tap Wk[26 - tap][n][0]
+ * dY0[p + off(tap)], then H and the bias partials as gp5_nn_conv3d_act_bwd gives
+ * them.  One 16x16x32 MFMA per 16 positions and 16 channels with the 27 taps as K;
+ * the result agrees with gp5_nn_conv3d_act_bwd on the full rows to float rounding.
+ * D a power of two in [16, 128], Cout a multiple of 16. */
+int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+                               int D, int Cin, int Cout);
 /* The output layer's form: no activation, and the float output Yf0 keeps only
  * channel 0 (one float per position, B*D^3 floats); the other channels are
  * computed (the MFMA tiles are 16 wide) and dropped. */
@@ -102,6 +111,10 @@ int gp5_nn_conv3d_wgrad_up(const void *Xh, const void *dY, float *dW, float *ws,
  * are the MFMA's N dimension (one 16x16x32 MFMA per 16 taps and 32 positions instead
  * of 27); otherwise the full weight gradient runs (its other columns come out 0). */
 int gp5_nn_conv3d_wgrad_out0(const void *X, const void *dY, float *dW, float *ws, int B, int D, int Cin, int Cout);
+/* gp5_nn_conv3d_wgrad_out0 with dY given as channel 0 alone (dY0: one bf16 per
+ * position); the same result bit for bit.  Cin = 16, Cout a multiple of 16 and
+ * D = 32, 64 or 128 (other shapes are refused). */
+int gp5_nn_conv3d_wgrad_out0c(const void *X, const void *dY0, float *dW, float *ws, int B, int D, int Cin, int Cout);
 /* db[c] = sum over positions of dY[p][c] (float). */
 int gp5_nn_bias_grad(const float *dY, float *db, long long P, int C);
 
@@ -161,6 +174,12 @@ int gp5_nn_mse_out0(const float *y0, const float *base, const float *target, voi
  * its losses when it needs them (one loss per call, added in call order). */
 int gp5_nn_mse_out0_acc(const float *y0, const float *base, const float *target, void *H, int C, float *db,
                         long long P, double *loss_acc);
+/* gp5_nn_mse_out0 that writes channel 0 of the gradient alone (H0: one bf16 per
+ * position, the values gp5_nn_mse_out0 writes to channel 0 of its rows); db
+ * receives C floats as there.  The loss goes to *loss (host) and/or is added to
+ * loss_acc[0] (device), whichever is not NULL. */
+int gp5_nn_mse_out0c(const float *y0, const float *base, const float *target, void *H0, int C, float *db, long long P,
+                     double *loss, double *loss_acc);
 /* Droplet input channels (one cube of side D on the scaled box [-ub, ub]^3, C-ordered
  * [D][D][D][16] bf16): 0 = 0.5 (1 - tanh((r - 1) s)), 1 = a0 u_i^2 + a1 u_j^2 + a2 u_k^2,
  * 2..6 = c[0..4], 7 = r / ub, 8..15 = 0; base (float, D^3) = channel 0.  float32

@@ -143,6 +143,15 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     # conv3d_act_bwd: every pointer, and Cout / 4 dividing 256
     assert L.gp5_nn_conv3d_act_bwd(fake, fake, None, fake, fake, fake, 1, 16, 16, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_act_bwd(fake, fake, fake, fake, fake, fake, 1, 16, 16, 48) == capi.GP5_ERR_ARG
+    # conv3d_act_bwd_out0: every pointer, D a power of two in [16, 128], Cout a multiple of 16
+    assert L.gp5_nn_conv3d_act_bwd_out0(fake, fake, fake, fake, None, fake, 1, 32, 16, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_act_bwd_out0(fake, fake, fake, fake, fake, fake, 1, 24, 16, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_act_bwd_out0(fake, fake, fake, fake, fake, fake, 1, 256, 16, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_act_bwd_out0(fake, fake, fake, fake, fake, fake, 1, 32, 16, 8) == capi.GP5_ERR_ARG
+    # wgrad_out0c: no full-row fallback, so shapes outside the kernel's are refused
+    assert L.gp5_nn_conv3d_wgrad_out0c(fake, fake, fake, fake, 1, 16, 16, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_wgrad_out0c(fake, fake, fake, fake, 1, 32, 32, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_conv3d_wgrad_out0c(fake, None, fake, fake, 1, 32, 16, 16) == capi.GP5_ERR_ARG
     # act_bwd_part needs its partial buffer; bias_final_multi 1..16 complete segments
     assert L.gp5_nn_act_bwd_part(fake, fake, None, 0.0, fake, None, 1, 8, 16) == capi.GP5_ERR_ARG
     bseg = nn.BiasSeg(fake, None, 16)
@@ -155,6 +164,9 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_mse_out0(fake, fake, fake, None, 16, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_out0(fake, fake, fake, fake, 12, None, 64, ctypes.byref(loss)) == capi.GP5_ERR_ARG
     assert L.gp5_nn_mse_out0_acc(fake, fake, fake, fake, 16, None, 64, None) == capi.GP5_ERR_ARG
+    # mse_out0c: H0 required, and a loss destination
+    assert L.gp5_nn_mse_out0c(fake, fake, fake, None, 16, None, 64, ctypes.byref(loss), None) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_mse_out0c(fake, fake, fake, fake, 16, None, 64, None, None) == capi.GP5_ERR_ARG
     # streams and events: NULL handles out and events in
     assert L.gp5_nn_stream_create(None) == capi.GP5_ERR_ARG
     assert L.gp5_nn_event_create(None) == capi.GP5_ERR_ARG

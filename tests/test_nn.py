@@ -702,6 +702,67 @@ def test_conv_act_bwd_equals_the_two_passes(B, D, Cin, Cout):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,D", [(1, 32), (2, 64), (1, 128), (2, 16)])
+def test_output_gradient_as_channel0(B, D):
+    """The output layer's gradient kept as channel 0 alone (one bf16 per position):
+    gp5_nn_mse_out0c writes channel 0 of gp5_nn_mse_out0's rows with the same loss
+    and bias gradient (bit for bit); gp5_nn_conv3d_wgrad_out0c gives
+    gp5_nn_conv3d_wgrad_out0's result bit for bit (and refuses D = 16);
+    gp5_nn_conv3d_act_bwd_out0 (the 27 taps as the MFMA's K) agrees with
+    gp5_nn_conv3d_act_bwd on the full rows to float rounding (Yf, the bias gradient
+    from its partials), H to one bf16 step, and Yf with the oracle to 1e-4."""
+    rng = np.random.default_rng(B * 5 + D)
+    L = nn._lib()
+    P = B * D ** 3
+    y0 = rng.standard_normal(P).astype(np.float32)
+    base = rng.standard_normal(P).astype(np.float32)
+    tgt = (base + rng.standard_normal(P)).astype(np.float32)
+    yb, bb, tb = _buf(y0), _buf(base), _buf(tgt)
+    H, H0 = nn.DevBuf(P * 16 * 2), nn.DevBuf(P * 2)
+    db = [nn.DevBuf(16 * 4), nn.DevBuf(16 * 4)]
+    loss = [ctypes.c_double(), ctypes.c_double()]
+    nn._ck(L.gp5_nn_mse_out0(yb.ptr, bb.ptr, tb.ptr, H.ptr, 16, db[0].ptr, P, ctypes.byref(loss[0])))
+    nn._ck(L.gp5_nn_mse_out0c(yb.ptr, bb.ptr, tb.ptr, H0.ptr, 16, db[1].ptr, P, ctypes.byref(loss[1]), None))
+    hf, h0 = H.download(np.uint16, (P, 16)), H0.download(np.uint16, (P,))
+    assert np.array_equal(hf[:, 0], h0) and not np.any(hf[:, 1:])
+    assert loss[0].value == loss[1].value
+    assert np.array_equal(db[0].download(np.float32, (16,)), db[1].download(np.float32, (16,)))
+    X = _rb(rng.standard_normal((B, D, D, D, 16)))
+    xb = _buf(X, True)
+    ws = nn.DevBuf(L.gp5_nn_wgrad_ws(B, D, 16, 16))
+    dW = [nn.DevBuf(27 * 256 * 4), nn.DevBuf(27 * 256 * 4)]
+    if D == 16:
+        assert L.gp5_nn_conv3d_wgrad_out0c(xb.ptr, H0.ptr, dW[1].ptr, ws.ptr, B, D, 16, 16) != 0
+    else:
+        nn._ck(L.gp5_nn_conv3d_wgrad_out0(xb.ptr, H.ptr, dW[0].ptr, ws.ptr, B, D, 16, 16))
+        nn._ck(L.gp5_nn_conv3d_wgrad_out0c(xb.ptr, H0.ptr, dW[1].ptr, ws.ptr, B, D, 16, 16))
+        assert np.array_equal(dW[0].download(np.float32, (27, 16, 16)), dW[1].download(np.float32, (27, 16, 16)))
+    W = _rb(rng.standard_normal((27, 16, 16)) * 0.1)  # [27][ci][co] of the output layer
+    Ab = _rb(rng.standard_normal((B, D, D, D, 16)))
+    wb, ab = _buf(W, True), _buf(Ab, True)
+    out = []
+    for f, g in ((L.gp5_nn_conv3d_act_bwd, H), (L.gp5_nn_conv3d_act_bwd_out0, H0)):
+        Yf, Hb = nn.DevBuf(P * 16 * 4), nn.DevBuf(P * 16 * 2)
+        part, dbb = nn.DevBuf(L.gp5_nn_bias_part_floats(16) * 4), nn.DevBuf(16 * 4)
+        nn._ck(f(g.ptr, wb.ptr, ab.ptr, Yf.ptr, Hb.ptr, part.ptr, B, D, 16, 16))
+        seg = (nn.BiasSeg * 1)(nn.BiasSeg(part.ptr, dbb.ptr, 16))
+        nn._ck(L.gp5_nn_bias_final_multi(ctypes.cast(seg, ctypes.c_void_p), 1))
+        out.append((Yf.download(np.float32, (P, 16)), nn.from_bf16(Hb.download(np.uint16, (P, 16))),
+                    dbb.download(np.float32, (16,))))
+    scale = np.max(np.abs(out[0][0]))
+    assert np.max(np.abs(out[1][0] - out[0][0])) <= 1e-5 * scale
+    assert np.all(np.abs(out[1][1] - out[0][1]) <= 2.0 ** -7 * np.abs(out[0][1]) + 1e-6 * scale)
+    ov = out[0][0].astype(np.float64) * np.where(Ab.reshape(P, 16) > 0, 1.0, 0.1)
+    tol = 1e-5 * np.abs(ov).sum(axis=0) + 1e-6
+    assert np.all(np.abs(out[1][2] - ov.sum(axis=0)) <= tol)
+    if D <= 64:
+        G = np.zeros((B, D, D, D, 16))
+        G[..., 0] = nn.from_bf16(h0).reshape(B, D, D, D)
+        ref = R.conv_dgrad(G, W).reshape(P, 16)
+        assert np.max(np.abs(out[1][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B,skipped", [(64, 32, 16, 1, True), (32, 32, 32, 2, False),
                                                   (128, 32, 16, 1, True), (32, 32, 16, 1, True),
                                                   (32, 64, 32, 2, True), (64, 64, 16, 1, False),
