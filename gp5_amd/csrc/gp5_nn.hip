@@ -1420,8 +1420,7 @@ __global__ __launch_bounds__(256) void k_wgrad_o0_reduce(const float *__restrict
 
 // Data gradient of the output layer from channel 0 of its output gradient (one
 // bf16 per position, G0; the loss reads channel 0, so the other channels are
-// zero), with the backward of the leaky ReLU below in the epilogue (as k_conv_zs
-// with ABWD, mirror = 1):
+// zero), with the backward of the leaky ReLU below in the epilogue (mirror = 1):
 //   dX[p][n] = sum_tap Wk[26 - tap][n][0] G0[p + off(tap)],
 // one 16x16x32 MFMA per 16 positions and 16 channels with the 27 taps as K (5 of
 // them zero) against 27 16x16x16 MFMAs for the full 16-channel gradient.  A
@@ -1429,9 +1428,11 @@ __global__ __launch_bounds__(256) void k_wgrad_o0_reduce(const float *__restrict
 // channels (16 blockIdx.y ..); LDS holds G0 of planes z-1, z, z+1 in a ring
 // (rows y0 - 1 .. y0 + Rb, 8 zero elements on either side of each row) and a
 // wave-private epilogue scratch.  Lane (r, g) gathers taps 8g .. 8g + 7 of
-// position r of its tile.  The epilogue, the bias partials (row x of bpart for
-// workgroup x, zeros to rows x + gridDim.x, ...) and their order are those of
-// k_conv_zs with ABWD.
+// position r of its tile.  The epilogue writes Gd = bf16(dX) and H =
+// bf16(dX act'(A)) (A: the activation of the layer below, read one plane ahead)
+// and adds dX act'(A) to this lane's bias sums in the order of
+// conv_epilogue_lds_impl with ABWD; the bias partials (row x of bpart for
+// workgroup x, zeros to rows x + gridDim.x, ...) are those of k_conv_zs with ABWD.
 constexpr int kDo0Pad = 8;
 __host__ inline size_t dgrad_o0_lds(int D)
 {
@@ -1440,8 +1441,8 @@ __host__ inline size_t dgrad_o0_lds(int D)
 }
 
 __global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G0, const uint16_t *__restrict__ Wk,
-                                                  const uint16_t *A, uint16_t *Y, float *Yf, int P, int D, int Cin,
-                                                  int Cout, int nzs, float *bpart)
+                                                  const uint16_t *__restrict__ A, uint16_t *Y, uint16_t *Gd, int P,
+                                                  int D, int Cin, int Cout, int nzs, float *bpart)
 {
     constexpr int MT = 4;
     typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -1496,6 +1497,12 @@ __global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G
             if (row < HR) HY[(slot * HR + row) * HS + kDo0Pad + x] = hv[k];
         }
     };
+    // epilogue rows: lane's position lane / 2 of the 32 of a tile pair, channels cb + c8 .. + 7
+    const int epos = lane / 2, c8 = 8 * (lane & 1), cb = 16 * nb0;
+    auto eoff = [&](int z, int h) {  // element offset of the lane's row piece, tile pair h of plane z
+        return (size_t)(s0 + (z * D + y0) * D + 64 * wave + 32 * h + epos) * Cout + cb + c8;
+    };
+    uint4 acur[2], anext[2];  // A of this plane (and of the next, in flight)
     __syncthreads();  // the zero fill is complete
     load_h(zb - 1);
     store_h(0);
@@ -1503,13 +1510,20 @@ __global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G
     store_h(1);
     load_h(zb + 1);
     store_h(2);
-    __syncthreads();
-    float bs[1][8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) bs[0][e] = 0.f;
+    for (int h = 0; h < 2; ++h) acur[h] = *reinterpret_cast<const uint4 *>(A + eoff(zb, h));
+    __syncthreads();
+    float bs[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bs[e] = 0.f;
+    float *T = reinterpret_cast<float *>(scr);
     for (int zi = 0; zi < ZL; ++zi) {
         const int z = zb + zi;
-        if (zi + 1 < ZL) load_h(z + 2);  // in flight during this plane's MFMAs
+        if (zi + 1 < ZL) {  // in flight during this plane's MFMAs and epilogue
+            load_h(z + 2);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) anext[h] = *reinterpret_cast<const uint4 *>(A + eoff(z + 1, h));
+        }
         const int sb[3] = {(zi % 3) * HR * HS, ((zi + 1) % 3) * HR * HS, ((zi + 2) % 3) * HR * HS};  // planes z-1, z, z+1
         f32x4 acc[MT][1];
 #pragma unroll
@@ -1526,9 +1540,34 @@ __global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G
             acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, make_uint4(av[0], av[1], av[2], av[3])),
                                                                b, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
         }
-        conv_epilogue_lds_impl<1, MT, true, true>(acc, s0 + (z * D + y0) * D + 64 * wave, nb0, P, Cout, nullptr, 0,
-                                                  const_cast<uint16_t *>(A), nullptr, Y, Yf, scr, bs);
+        // epilogue, a tile pair at a time through the wave's scratch (D[m = 4g + i][n = r])
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) T[(16 * tt + 4 * g + i) * kEpiStride + r] = acc[2 * h + tt][0][i];
+            __builtin_amdgcn_wave_barrier();
+            const float4 lo = *reinterpret_cast<const float4 *>(T + epos * kEpiStride + c8);
+            const float4 hi = *reinterpret_cast<const float4 *>(T + epos * kEpiStride + c8 + 4);
+            const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            const uint16_t *ah = reinterpret_cast<const uint16_t *>(&acur[h]);
+            uint16_t hb[8], gb[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float ov = v[e] * (bf2f(ah[e]) > 0.f ? 1.f : kLeak);
+                hb[e] = f2bf(ov);
+                gb[e] = f2bf(v[e]);
+                bs[e] += ov;
+            }
+            const size_t o = eoff(z, h);
+            *reinterpret_cast<uint4 *>(Y + o) = __builtin_bit_cast(uint4, hb);
+            *reinterpret_cast<uint4 *>(Gd + o) = __builtin_bit_cast(uint4, gb);
+            __builtin_amdgcn_wave_barrier();  // the scratch is rewritten by the next pair
+        }
         if (zi + 1 < ZL) {
+            acur[0] = anext[0];
+            acur[1] = anext[1];
             __syncthreads();  // every wave is done with the slab of plane z-1
             store_h(zi % 3);  // plane z + 2 takes the slot of plane z - 1
             __syncthreads();
@@ -1538,7 +1577,7 @@ __global__ __launch_bounds__(256) void k_dgrad_o0(const uint16_t *__restrict__ G
     __syncthreads();  // every wave is done with the LDS
     float *red = reinterpret_cast<float *>(sx);  // 8 x 256 floats
 #pragma unroll
-    for (int e = 0; e < 8; ++e) red[e * 256 + tid] = bs[0][e];
+    for (int e = 0; e < 8; ++e) red[e * 256 + tid] = bs[e];
     __syncthreads();
     if (tid < 16) {
         const int c = tid;
@@ -2328,12 +2367,15 @@ __device__ __forceinline__ void act_bwd_bias_pass4(const uint16_t *__restrict__ 
 // gradient is dA + scale * G at the parent cell (G has side D/2: the adjoint of
 // pooling added on the fly, the same fused multiply-add as k_up2_add4).
 // (one instance per form, so that each gets its own register allocation: in one
-// kernel the unrolled form slowed the G form 47.2 -> 50.5 us)
-template <bool HAS_G>
-__global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__restrict__ A, const float *__restrict__ dA,
+// kernel the unrolled form slowed the G form 47.2 -> 50.5 us).  DBF (with G): dA
+// is bf16.
+template <bool HAS_G, bool DBF = false>
+__global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__restrict__ A, const void *__restrict__ dAv,
                                                            uint16_t *__restrict__ H, float *part, long long P, int C,
                                                            const float *__restrict__ G, float scale, int D)
 {
+    static_assert(HAS_G || !DBF, "bf16 dA with the up-add form only");
+    const float *__restrict__ dA = reinterpret_cast<const float *>(dAv);
     __shared__ float4 s[256];
     const int C4 = C / 4, per_pos = 256 / C4;
     const int c4 = threadIdx.x % C4, sub = threadIdx.x / C4;
@@ -2346,7 +2388,14 @@ __global__ __launch_bounds__(256) void k_act_bwd_cast_bias(const uint16_t *__res
         for (long long p = p0 + sub; p < p1; p += per_pos) {
             const long long i = p * C4 + c4;
             const uint2 av = reinterpret_cast<const uint2 *>(A)[i];
-            float4 d = reinterpret_cast<const float4 *>(dA)[i];
+            float4 d;
+            if constexpr (DBF) {
+                const uint2 dv = reinterpret_cast<const uint2 *>(dAv)[i];
+                d = make_float4(__uint_as_float(dv.x << 16), __uint_as_float(dv.x & 0xffff0000u),
+                                __uint_as_float(dv.y << 16), __uint_as_float(dv.y & 0xffff0000u));
+            } else {
+                d = reinterpret_cast<const float4 *>(dA)[i];
+            }
             const int pp = (int)p, Dh = D / 2;  // P = B D^3 < 2^31
             const int x = pp % D, y = (pp / D) % D, z = (pp / (D * D)) % D, b = pp / (D * D * D);
             const float4 v = reinterpret_cast<const float4 *>(G)[(((b * Dh + z / 2) * Dh + y / 2) * Dh + x / 2) * C4 + c4];
@@ -3294,8 +3343,8 @@ int gp5_nn_conv3d_up(const void *Xh, const void *Wk, const float *bias, void *A,
     return conv3d_run(Xh, Wk, bias, A, skip, Y, Yf, B, D, Cin, Cout, mirror, act, 1);
 }
 
-static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
-                            long long P, int C, float *out_part = nullptr);
+static int act_bwd_bias_run(const void *A, const void *dA, const float *G, float scale, int D, void *H, float *db,
+                            long long P, int C, float *out_part = nullptr, bool dbf = false);
 
 int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
                           int D, int Cin, int Cout)
@@ -3309,10 +3358,10 @@ int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float 
     return act_bwd_bias_run(Ab, Yf, nullptr, 0.f, D, H, nullptr, (long long)B * D * D * D, Cout, part);
 }
 
-int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, void *Gd, void *H, float *part, int B,
                                int D, int Cin, int Cout)
 {
-    if (!dY0 || !Wk || !Ab || !Yf || !H || !part) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (!dY0 || !Wk || !Ab || !Gd || !H || !part) return set_error(GP5_ERR_ARG, "NULL argument");
     if (D < 16 || D > 128 || (D & (D - 1)) || Cin < 1 || Cout < 16 || Cout % 16 || B < 1)
         return set_error(GP5_ERR_ARG, "conv3d_act_bwd_out0: D a power of two in [16, 128], Cout a multiple of 16");
     if ((long long)B * D * D * D * Cout * 4 > 0x7fffffffLL)
@@ -3340,8 +3389,8 @@ int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, 
     }
     nzs = std::max(1, std::min(nzs, kBiasBlocks / per_run));  // one bias partial row per workgroup
     hipLaunchKernelGGL(k_dgrad_o0, dim3((unsigned)(per_run * nzs), (unsigned)gy), dim3(256), shm, nn_stream(),
-                       (const uint16_t *)dY0, (const uint16_t *)Wk, (const uint16_t *)Ab, (uint16_t *)H, Yf,
-                       B * D * D * D, D, Cin, Cout, nzs, part);
+                       (const uint16_t *)dY0, (const uint16_t *)Wk, (const uint16_t *)Ab, (uint16_t *)H,
+                       (uint16_t *)Gd, B * D * D * D, D, Cin, Cout, nzs, part);
     NNCHK(hipGetLastError(), "act_bwd_out0 launch");
     return GP5_OK;
 }
@@ -3641,17 +3690,21 @@ int gp5_nn_act_bwd_cast(const void *A, const float *dA, float *dPre, void *H, lo
 
 // db set: the bias gradient is finished here (partials in the shared scratch);
 // else the partials go to the caller's `part` (kBiasBlocks * C floats)
-static int act_bwd_bias_run(const void *A, const float *dA, const float *G, float scale, int D, void *H, float *db,
-                            long long P, int C, float *out_part)
+static int act_bwd_bias_run(const void *A, const void *dA, const float *G, float scale, int D, void *H, float *db,
+                            long long P, int C, float *out_part, bool dbf)
 {
     if (!A || !dA || !H || (!db && !out_part)) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (dbf && !G) return set_error(GP5_ERR_ARG, "act_bwd: a bf16 dA needs G");
     if (C < 4 || C % 4 || 256 % (C / 4)) return set_error(GP5_ERR_ARG, "act_bwd_cast_bias: C / 4 must divide 256");
     float *part = out_part;
     if (!part) {
         const int rc = bias_scratch((size_t)kBiasBlocks * C, &part);
         if (rc != GP5_OK) return rc;
     }
-    if (G)
+    if (G && dbf)
+        hipLaunchKernelGGL((k_act_bwd_cast_bias<true, true>), dim3(kBiasBlocks), dim3(256), 0, nn_stream(),
+                           (const uint16_t *)A, dA, (uint16_t *)H, part, P, C, G, scale, D);
+    else if (G)
         hipLaunchKernelGGL(k_act_bwd_cast_bias<true>, dim3(kBiasBlocks), dim3(256), 0, nn_stream(), (const uint16_t *)A,
                            dA, (uint16_t *)H, part, P, C, G, scale, D);
     else
@@ -3677,6 +3730,16 @@ int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float
     if ((long long)B * D * D * D * C >= 0x7fffffffLL)
         return set_error(GP5_ERR_ARG, "up2_add_act_bwd_cast_bias: tensors must stay below 2^31 elements");
     return act_bwd_bias_run(A, dA, G, scale, D, H, db, (long long)B * D * D * D, C);
+}
+
+int gp5_nn_act_bwd_part_bf16(const void *A, const void *dA, const float *G, float scale, void *H, float *part, int B,
+                             int D, int C)
+{
+    if (!part || !G) return set_error(GP5_ERR_ARG, "NULL argument");
+    if (B < 1 || D < 2 || D % 2) return set_error(GP5_ERR_ARG, "act_bwd_part_bf16: D must be even");
+    if ((long long)B * D * D * D * C >= 0x7fffffffLL)
+        return set_error(GP5_ERR_ARG, "act_bwd_part_bf16: tensors must stay below 2^31 elements");
+    return act_bwd_bias_run(A, dA, G, scale, D, H, nullptr, (long long)B * D * D * D, C, part, true);
 }
 
 size_t gp5_nn_bias_part_floats(int C)

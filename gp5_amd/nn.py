@@ -97,6 +97,7 @@ def _lib():
         "gp5_nn_bias_final_multi": [P, I],
         "gp5_nn_act_bwd_cast_bias": [P, P, P, P, LL, I],
         "gp5_nn_up2_add_act_bwd_cast_bias": [P, P, P, F, P, P, I, I, I],
+        "gp5_nn_act_bwd_part_bf16": [P, P, P, F, P, P, I, I, I],
         "gp5_nn_mse": [P, I, P, P, P, LL, ctypes.POINTER(ctypes.c_double)],
         "gp5_nn_adam": [P, P, P, P, LL, F, F, F, F, I],
         "gp5_nn_relayout": [P, P, P, I, I],
@@ -234,7 +235,7 @@ class UNet:
             "out": (n(D, 1), 4), "base": (n(D, 1), 4), "target": (n(D, 1), 4),
             # gradients (float) and their bf16 copies
             "h_out": (n(D, 1 if _out0c(D) else 16), 2),  # channel 0 alone, or rows of 16
-            "g16a": (n(D, 16), 4), "h16": (n(D, 16), 2),
+            "g16a": (n(D, 16), 2 if _out0c(D) else 4), "h16": (n(D, 16), 2),  # bf16 from the channel-0 form
             "g32a": (n(D // 2, 32), 4), "h32": (n(D // 2, 32), 2),
             "g64a": (n(D // 4, 64), 4), "h64": (n(D // 4, 64), 2),
             "g128": (n(D // 8, 128), 4), "h128": (n(D // 8, 128), 2),
@@ -406,16 +407,28 @@ class UNet:
                 self._ev[name, kind] = ev
         return True
 
+    def _fork(self, name):
+        """Marks the point of the main stream after which layer name's weight
+        gradient may run (its inputs are queued): _side(name, ...) waits for this
+        point, so the main stream's next pass can be queued before the side
+        stream's calls, which cost the host more than a launch."""
+        if self._overlapping:
+            _ck(_lib().gp5_nn_event_record(self._ev[name, "dz"], self._main))
+            self._forked.add(name)
+
     def _side(self, name, *fns):
         """Queues fns (weight gradients) on the side stream behind the work queued
-        so far on the main stream, which has produced their inputs; the
-        data-gradient passes that follow on the main stream run beside them."""
+        on the main stream up to _fork(name) (or so far, without one), which has
+        produced their inputs; the data-gradient passes on the main stream run
+        beside them."""
         if not self._overlapping:
             for fn in fns:
                 fn()
             return
         L = _lib()
-        _ck(L.gp5_nn_event_record(self._ev[name, "dz"], self._main))
+        if name not in self._forked:
+            _ck(L.gp5_nn_event_record(self._ev[name, "dz"], self._main))
+        self._forked.discard(name)
         _ck(L.gp5_nn_stream_wait(self._aux, self._ev[name, "dz"]))
         _ck(L.gp5_nn_set_stream(self._aux))
         try:
@@ -448,12 +461,13 @@ class UNet:
         except Exception:
             pass
 
-    def _up_add_act_bwd(self, name, a, g, gh, h, D):
+    def _up_add_act_bwd(self, name, a, g, gh, h, D, g_bf16=False):
         """_act_bwd with the incoming gradient t[g] + t[gh] / 8 at the parent cell
         (t[gh] of side D/2: the adjoint of the pooling that fed the next level), in
-        one pass that does not write t[g] back."""
-        _ck(_lib().gp5_nn_act_bwd_part(self.t[a].ptr, self.t[g].ptr, self.t[gh].ptr, 0.125, self.t[h].ptr,
-                                       self.bpart[name].ptr, self.B, D, self.w[name]["co"]))
+        one pass that does not write t[g] back (t[g] in bf16 with g_bf16)."""
+        f = _lib().gp5_nn_act_bwd_part_bf16 if g_bf16 else _lib().gp5_nn_act_bwd_part
+        _ck(f(self.t[a].ptr, self.t[g].ptr, self.t[gh].ptr, 0.125, self.t[h].ptr, self.bpart[name].ptr, self.B, D,
+              self.w[name]["co"]))
 
     def _bias_final(self):
         """The bias gradients of c1..c7 from their block partials, one launch."""
@@ -491,52 +505,63 @@ class UNet:
         # measured slower); nothing on the main stream overwrites what they read
         # before the join at the end.
         self._overlapping = self._side_setup()
+        self._forked = set()
+        # Each weight gradient is queued on the side stream after the main stream's
+        # next pass (_fork marks where its inputs are ready), so that the side
+        # stream's host calls do not hold up the data-gradient chain.
         # c8, c7
         e8 = self.w["c8"]  # h_out is nonzero in channel 0 only (or holds channel 0 alone)
         wg8 = L.gp5_nn_conv3d_wgrad_out0c if self._o0c else L.gp5_nn_conv3d_wgrad_out0
-        self._side("c8", lambda: _ck(wg8(t["d1"].ptr, t["h_out"].ptr, e8["Wg"].ptr, self.ws.ptr, B, D, e8["ci"],
-                                         e8["co"])))
-        # c8's data gradient g16a = dd1 = de1 (skip part) and c7's activation backward
+        self._fork("c8")
+        # c8's data gradient g16a = dd1 = de1 (skip part; bf16 in the channel-0 form) and c7's activation backward
         # dz7 = dd1 * act'(a7) (h16, c7's bias partials) in one pass where the shape allows
         e = self.w["c8"]
         act_bwd8 = L.gp5_nn_conv3d_act_bwd_out0 if self._o0c else L.gp5_nn_conv3d_act_bwd
         _ck(act_bwd8(t["h_out"].ptr, e["Wb"].ptr, t["a7"].ptr, t["g16a"].ptr, t["h16"].ptr,
                                     self.bpart["c7"].ptr, B, D, e["co"], e["ci"]))
-        self._side("c7", lambda: self._wgrad_up("c7", "d2", "u2", t["h16"], D))
+        self._side("c8", lambda: _ck(wg8(t["d1"].ptr, t["h_out"].ptr, e8["Wg"].ptr, self.ws.ptr, B, D, e8["ci"],
+                                         e8["co"])))
+        self._fork("c7")
         if _pool_fused(D, 32, 16, B):  # gradient of u2 summed over each cell's children: dd2 = de2 (skip part)
             e = self.w["c7"]
             _ck(L.gp5_nn_conv3d_pool(t["h16"].ptr, e["Wb"].ptr, t["g32a"].ptr, B, D, e["co"], e["ci"], 1))
         else:
             self._dgrad("c7", t["h16"], t["gu32"], D)               # gradient of u2 (side D, 32 ch)
             _ck(L.gp5_nn_pool2_f32(t["gu32"].ptr, t["g32a"].ptr, B, D, 32, 8.0))  # dd2 = de2 (skip part)
+        self._side("c7", lambda: self._wgrad_up("c7", "d2", "u2", t["h16"], D))
         # c6
         self._act_bwd("c6", "a6", "g32a", "h32", D // 2)
-        self._side("c6", lambda: self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2))
+        self._fork("c6")
         if _pool_fused(D // 2, 64, 32, B):  # dd3 = de3 (skip part)
             e = self.w["c6"]
             _ck(L.gp5_nn_conv3d_pool(t["h32"].ptr, e["Wb"].ptr, t["g64a"].ptr, B, D // 2, e["co"], e["ci"], 1))
         else:
             self._dgrad("c6", t["h32"], t["gu64"], D // 2)
             _ck(L.gp5_nn_pool2_f32(t["gu64"].ptr, t["g64a"].ptr, B, D // 2, 64, 8.0))  # dd3 = de3 (skip part)
+        self._side("c6", lambda: self._wgrad_up("c6", "d3", "u3", t["h32"], D // 2))
         # c5
         self._act_bwd("c5", "a5", "g64a", "h64", D // 4)
-        self._side("c5", lambda: self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4))
+        self._fork("c5")
         self._dgrad("c5", t["h64"], t["gu128"], D // 4)
         _ck(L.gp5_nn_pool2_f32(t["gu128"].ptr, t["g128"].ptr, B, D // 4, 128, 8.0))  # de4
+        self._side("c5", lambda: self._wgrad_up("c5", "e4", "u4", t["h64"], D // 4))
         # c4
         self._act_bwd("c4", "e4", "g128", "h128", D // 8)
-        self._side("c4", lambda: self._wgrad("c4", t["p3"], t["h128"], D // 8))
+        self._fork("c4")
         self._dgrad("c4", t["h128"], t["gp64"], D // 8)
+        self._side("c4", lambda: self._wgrad("c4", t["p3"], t["h128"], D // 8))
         # c3 (de3 total = dd3 + the adjoint of the pooling into c4)
         self._up_add_act_bwd("c3", "e3", "g64a", "gp64", "h64e", D // 4)
-        self._side("c3", lambda: self._wgrad("c3", t["p2"], t["h64e"], D // 4))
+        self._fork("c3")
         self._dgrad("c3", t["h64e"], t["gp32"], D // 4)
+        self._side("c3", lambda: self._wgrad("c3", t["p2"], t["h64e"], D // 4))
         # c2 (de2 total)
         self._up_add_act_bwd("c2", "e2", "g32a", "gp32", "h32e", D // 2)
-        self._side("c2", lambda: self._wgrad("c2", t["p1"], t["h32e"], D // 2))
+        self._fork("c2")
         self._dgrad("c2", t["h32e"], t["gp16"], D // 2)
+        self._side("c2", lambda: self._wgrad("c2", t["p1"], t["h32e"], D // 2))
         # c1 (de1 total)
-        self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16e", D)
+        self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16e", D, g_bf16=self._o0c)
         self._side("c1", lambda: self._wgrad("c1", t["X"], t["h16e"], D))
         self._bias_final()
         self._after_side("c1")  # every weight gradient is done before anything reads them

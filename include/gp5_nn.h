@@ -81,12 +81,13 @@ int gp5_nn_conv3d_act_bwd(const void *dY, const void *Wk, const void *Ab, float 
                           int D, int Cin, int Cout);
 /* gp5_nn_conv3d_act_bwd for the output layer's gradient, given as channel 0 alone
  * (dY0: one bf16 per position, gp5_nn_mse_out0c's H0; the loss reads channel 0, so
- * the other Cin - 1 channels are zero): Yf[p][n] = sum_tap Wk[26 - tap][n][0]
- * dY0[p + off(tap)], then H and the bias partials as gp5_nn_conv3d_act_bwd gives
- * them.  One 16x16x32 MFMA per 16 positions and 16 channels with the 27 taps as K;
- * the result agrees with gp5_nn_conv3d_act_bwd on the full rows to float rounding.
- * D a power of two in [16, 128], Cout a multiple of 16. */
-int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, float *Yf, void *H, float *part, int B,
+ * the other Cin - 1 channels are zero): the data gradient v[p][n] = sum_tap
+ * Wk[26 - tap][n][0] dY0[p + off(tap)] goes out as Gd = bf16(v) (Cout channels),
+ * H = bf16(v act'(Ab)) and the bias partials of v act'(Ab) as gp5_nn_conv3d_act_bwd
+ * gives them.  One 16x16x32 MFMA per 16 positions and 16 channels with the 27 taps
+ * as K; v agrees with gp5_nn_conv3d_act_bwd's Yf to float rounding.  D a power of
+ * two in [16, 128], Cout a multiple of 16. */
+int gp5_nn_conv3d_act_bwd_out0(const void *dY0, const void *Wk, const void *Ab, void *Gd, void *H, float *part, int B,
                                int D, int Cin, int Cout);
 /* The output layer's form: no activation, and the float output Yf0 keeps only
  * channel 0 (one float per position, B*D^3 floats); the other channels are
@@ -144,6 +145,10 @@ int gp5_nn_up2_add_act_bwd_cast_bias(const void *A, const float *dA, const float
 size_t gp5_nn_bias_part_floats(int C);
 int gp5_nn_act_bwd_part(const void *A, const float *dA, const float *G, float scale, void *H, float *part, int B, int D,
                         int C);
+/* gp5_nn_act_bwd_part with G (required) and dA in bf16 (gp5_nn_conv3d_act_bwd_out0's
+ * Gd): the same arithmetic on the bf16 values. */
+int gp5_nn_act_bwd_part_bf16(const void *A, const void *dA, const float *G, float scale, void *H, float *part, int B,
+                             int D, int C);
 typedef struct gp5_nn_bias_seg {
     const float *part; /* partials of gp5_nn_act_bwd_part */
     float *db;         /* C floats: the bias gradient */

@@ -152,6 +152,10 @@ def test_network_entry_points_reject_bad_shapes_on_the_host():
     assert L.gp5_nn_conv3d_wgrad_out0c(fake, fake, fake, fake, 1, 16, 16, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_wgrad_out0c(fake, fake, fake, fake, 1, 32, 32, 16) == capi.GP5_ERR_ARG
     assert L.gp5_nn_conv3d_wgrad_out0c(fake, None, fake, fake, 1, 32, 16, 16) == capi.GP5_ERR_ARG
+    # act_bwd_part_bf16 needs G and its partial buffer
+    assert L.gp5_nn_act_bwd_part_bf16(fake, fake, None, 0.125, fake, fake, 1, 8, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_act_bwd_part_bf16(fake, fake, fake, 0.125, fake, None, 1, 8, 16) == capi.GP5_ERR_ARG
+    assert L.gp5_nn_act_bwd_part_bf16(fake, fake, fake, 0.125, fake, fake, 1, 7, 16) == capi.GP5_ERR_ARG
     # act_bwd_part needs its partial buffer; bias_final_multi 1..16 complete segments
     assert L.gp5_nn_act_bwd_part(fake, fake, None, 0.0, fake, None, 1, 8, 16) == capi.GP5_ERR_ARG
     bseg = nn.BiasSeg(fake, None, 16)

@@ -709,8 +709,9 @@ def test_output_gradient_as_channel0(B, D):
     and bias gradient (bit for bit); gp5_nn_conv3d_wgrad_out0c gives
     gp5_nn_conv3d_wgrad_out0's result bit for bit (and refuses D = 16);
     gp5_nn_conv3d_act_bwd_out0 (the 27 taps as the MFMA's K) agrees with
-    gp5_nn_conv3d_act_bwd on the full rows to float rounding (Yf, the bias gradient
-    from its partials), H to one bf16 step, and Yf with the oracle to 1e-4."""
+    gp5_nn_conv3d_act_bwd on the full rows: its bf16 data gradient and H to one bf16
+    step, the bias gradient from its partials to float rounding, the data gradient
+    with the oracle to bf16 rounding."""
     rng = np.random.default_rng(B * 5 + D)
     L = nn._lib()
     P = B * D ** 3
@@ -741,16 +742,18 @@ def test_output_gradient_as_channel0(B, D):
     Ab = _rb(rng.standard_normal((B, D, D, D, 16)))
     wb, ab = _buf(W, True), _buf(Ab, True)
     out = []
-    for f, g in ((L.gp5_nn_conv3d_act_bwd, H), (L.gp5_nn_conv3d_act_bwd_out0, H0)):
-        Yf, Hb = nn.DevBuf(P * 16 * 4), nn.DevBuf(P * 16 * 2)
+    for f, g, gbytes in ((L.gp5_nn_conv3d_act_bwd, H, 4), (L.gp5_nn_conv3d_act_bwd_out0, H0, 2)):
+        Yf, Hb = nn.DevBuf(P * 16 * gbytes), nn.DevBuf(P * 16 * 2)
         part, dbb = nn.DevBuf(L.gp5_nn_bias_part_floats(16) * 4), nn.DevBuf(16 * 4)
         nn._ck(f(g.ptr, wb.ptr, ab.ptr, Yf.ptr, Hb.ptr, part.ptr, B, D, 16, 16))
         seg = (nn.BiasSeg * 1)(nn.BiasSeg(part.ptr, dbb.ptr, 16))
         nn._ck(L.gp5_nn_bias_final_multi(ctypes.cast(seg, ctypes.c_void_p), 1))
-        out.append((Yf.download(np.float32, (P, 16)), nn.from_bf16(Hb.download(np.uint16, (P, 16))),
-                    dbb.download(np.float32, (16,))))
+        yf = (Yf.download(np.float32, (P, 16)) if gbytes == 4
+              else nn.from_bf16(Yf.download(np.uint16, (P, 16))))
+        out.append((yf, nn.from_bf16(Hb.download(np.uint16, (P, 16))), dbb.download(np.float32, (16,))))
     scale = np.max(np.abs(out[0][0]))
-    assert np.max(np.abs(out[1][0] - out[0][0])) <= 1e-5 * scale
+    step = 2.0 ** -7 * np.abs(out[0][0]) + 1e-6 * scale  # one bf16 step of the float gradient
+    assert np.all(np.abs(out[1][0] - out[0][0]) <= step)
     assert np.all(np.abs(out[1][1] - out[0][1]) <= 2.0 ** -7 * np.abs(out[0][1]) + 1e-6 * scale)
     ov = out[0][0].astype(np.float64) * np.where(Ab.reshape(P, 16) > 0, 1.0, 0.1)
     tol = 1e-5 * np.abs(ov).sum(axis=0) + 1e-6
@@ -759,7 +762,30 @@ def test_output_gradient_as_channel0(B, D):
         G = np.zeros((B, D, D, D, 16))
         G[..., 0] = nn.from_bf16(h0).reshape(B, D, D, D)
         ref = R.conv_dgrad(G, W).reshape(P, 16)
-        assert np.max(np.abs(out[1][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+        assert np.all(np.abs(out[1][0] - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-4 * np.max(np.abs(ref)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("B,D,C", [(1, 64, 16), (2, 32, 32)])
+def test_act_bwd_part_with_bf16_gradient(B, D, C):
+    """gp5_nn_act_bwd_part_bf16 gives, for a bf16 dA, what gp5_nn_act_bwd_part gives
+    for the same values in float (H and the bias partials bit for bit); it needs G."""
+    rng = np.random.default_rng(B + D + C)
+    L = nn._lib()
+    P = B * D ** 3
+    dA = _rb(rng.standard_normal((P, C)))
+    A = _rb(rng.standard_normal((P, C)))
+    G = rng.standard_normal((B * (D // 2) ** 3, C)).astype(np.float32)
+    ab, gb = _buf(A, True), _buf(G)
+    out = []
+    for f, d in ((L.gp5_nn_act_bwd_part, _buf(dA)), (L.gp5_nn_act_bwd_part_bf16, _buf(dA, True))):
+        H, part = nn.DevBuf(P * C * 2), nn.DevBuf(L.gp5_nn_bias_part_floats(C) * 4)
+        nn._ck(f(ab.ptr, d.ptr, gb.ptr, 0.125, H.ptr, part.ptr, B, D, C))
+        out.append((H.download(np.uint16, (P, C)), part.download(np.float32, (L.gp5_nn_bias_part_floats(C),))))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    H = nn.DevBuf(P * C * 2)
+    part = nn.DevBuf(L.gp5_nn_bias_part_floats(C) * 4)
+    assert L.gp5_nn_act_bwd_part_bf16(ab.ptr, ab.ptr, None, 0.125, H.ptr, part.ptr, B, D, C) != 0
 
 
 @pytest.mark.gpu
