@@ -351,7 +351,10 @@ def network_mfma(batch: int = 8, steps: int = 10, D: int = 64):
         "weight_gradient_by_parity": sorted(wup2),
         "roofline": {"bound": "mfma", "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "forward_frac": fwd_tf / MFMA_BF16_PEAK_TFLOPS, "train_frac": train_tf / MFMA_BF16_PEAK_TFLOPS,
-                     "flops": "executed MFMA flops"},
+                     "flops": "executed MFMA flops",
+                     # the same times against the direct 3x3x3 convolution's flop count
+                     "model_forward_frac": model_fwd / fwd_s / 1e12 / MFMA_BF16_PEAK_TFLOPS,
+                     "model_train_frac": model_train / train_s / 1e12 / MFMA_BF16_PEAK_TFLOPS},
     }
 
 

@@ -261,6 +261,9 @@ class UNet:
                  [(D, C0, 16), (D // 2, 16, 32), (D // 4, 32, 64), (D // 8, 64, 128), (D // 4, 128, 64),
                   (D // 2, 64, 32), (D, 32, 16), (D, 16, 16)])
         self.ws = DevBuf(ws)
+        # c1's weight gradient runs last, on the main stream, beside the side stream's
+        # tail: its own workspace
+        self.ws1 = DevBuf(L.gp5_nn_wgrad_ws(B, D, C0, 16))
         # bias-gradient block partials of c1..c7, finished together at the end of backward
         self.bpart = {name: DevBuf(L.gp5_nn_bias_part_floats(co) * 4) for name, ci, co in SHAPES if name != "c8"}
         self._bsegs = None
@@ -369,9 +372,10 @@ class UNet:
         out = self.t["out"].download(np.float32, (B, D, D, D))
         return self.t["base"].download(np.float32, (B, D, D, D)) + out
 
-    def _wgrad(self, name, x, dz_bf, D):
+    def _wgrad(self, name, x, dz_bf, D, ws=None):
         e = self.w[name]
-        _ck(_lib().gp5_nn_conv3d_wgrad(x.ptr, dz_bf.ptr, e["Wg"].ptr, self.ws.ptr, self.B, D, e["ci"], e["co"]))
+        _ck(_lib().gp5_nn_conv3d_wgrad(x.ptr, dz_bf.ptr, e["Wg"].ptr, (ws or self.ws).ptr, self.B, D, e["ci"],
+                                       e["co"]))
 
     def _wgrad_up(self, name, src, tmp, dz_bf, D):
         """Weight gradient of _conv_up: the input is the upsampling of src."""
@@ -562,9 +566,14 @@ class UNet:
         self._side("c2", lambda: self._wgrad("c2", t["p1"], t["h32e"], D // 2))
         # c1 (de1 total)
         self._up_add_act_bwd("c1", "e1", "g16a", "gp16", "h16e", D, g_bf16=self._o0c)
-        self._side("c1", lambda: self._wgrad("c1", t["X"], t["h16e"], D))
-        self._bias_final()
-        self._after_side("c1")  # every weight gradient is done before anything reads them
+        # c1's weight gradient: on the main stream (nothing else is left to overlap it
+        # with there, and a fork and join would only add their latency), in its own
+        # workspace while the side stream may still run c2's; the bias gradients (every
+        # partial is written once c1's activation backward is) beside it on the side stream
+        self._fork("c1")
+        self._wgrad("c1", t["X"], t["h16e"], D, ws=self.ws1)
+        self._side("c1", self._bias_final)
+        self._after_side("c1")  # every gradient is done before anything reads them
         return float(loss.value) if read_loss else None
 
     def take_loss_sum(self) -> float:
