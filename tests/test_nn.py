@@ -442,6 +442,48 @@ def test_conv_z_sliding_path(D, Cin, Cout, B, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("D,Cout,B,up", [(64, 16, 1, 0), (16, 16, 2, 0), (32, 32, 1, 0), (64, 16, 1, 1)])
+def test_conv_z_sliding_variants_agree(D, Cout, B, up, monkeypatch):
+    """Every variant of the z-sliding kernel with 16 input channels (GP5_CONV_ZSV: bit
+    0 wave-private epilogue barriers, bit 1 fragment reads one tap ahead) gives bit
+    for bit what the direct kernel gives (forward with bias, activation and both
+    outputs, and mirrored; up = 1 reads the half-side input upsampled, against the
+    direct kernel on the upsampled input)."""
+    Cin = 16
+    rng = np.random.default_rng(D + Cout + B + up)
+    Ds = D // 2 if up else D
+    X = _rb(rng.standard_normal((B, Ds, Ds, Ds, Cin)))
+    W = _rb(rng.standard_normal((27, Cin, Cout)) * 0.1)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    L = nn._lib()
+    U = X.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3) if up else X
+    xb, ub = _buf(X, True), _buf(U, True)
+    wf, wb, bb = _buf(np.ascontiguousarray(W.transpose(0, 2, 1)), True), _buf(W, True), _buf(b)
+    shape = (B, D, D, D, Cout)
+    outs = []
+    for zs, v in (("0", "3"), ("1", "0"), ("1", "1"), ("1", "2"), ("1", "3")):
+        monkeypatch.setenv("GP5_CONV_ZS", zs)
+        monkeypatch.setenv("GP5_CONV_LDS", zs)
+        monkeypatch.setenv("GP5_CONV_UP2P", "0")
+        monkeypatch.setenv("GP5_CONV_ZSV", v)
+        # the direct kernel (zs = 0) on the upsampled input itself
+        f, x = (L.gp5_nn_conv3d_up, xb) if up and zs == "1" else (L.gp5_nn_conv3d, ub)
+        Yf, A = nn.DevBuf(B * D ** 3 * Cout * 4), nn.DevBuf(B * D ** 3 * Cout * 2)
+        nn._ck(f(x.ptr, wf.ptr, bb.ptr, A.ptr, None, None, Yf.ptr, B, D, Cin, Cout, 0, 1))
+        got = [Yf.download(np.float32, shape), A.download(np.uint16, shape)]
+        if not up and Cout == 16:  # mirrored: the data gradient of a 16 -> 16 layer
+            dX = nn.DevBuf(B * D ** 3 * Cin * 4)
+            nn._ck(L.gp5_nn_conv3d(xb.ptr, wb.ptr, None, None, None, None, dX.ptr, B, D, Cin, Cin, 1, 0))
+            got.append(dX.download(np.float32, (B, D, D, D, Cin)))
+        outs.append(got)
+    for o in outs[1:]:
+        for a, c in zip(outs[0], o):
+            assert np.array_equal(a, c)
+    ref = R.lrelu(R.conv(U, W, b.astype(np.float64)))
+    assert np.max(np.abs(outs[1][0] - ref)) <= 1e-4 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("D,Cin,Cout,B,up,mirror", [(8, 64, 128, 8, 0, 0), (8, 128, 64, 8, 0, 1),
                                                     (16, 64, 32, 2, 0, 1), (16, 128, 64, 1, 1, 0),
                                                     (8, 64, 32, 2, 0, 0)])
