@@ -808,6 +808,38 @@ def test_output_gradient_as_channel0(B, D):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,D,C", [(1, 32, 32), (2, 16, 64)])
+def test_channel0_data_gradient_over_channel_blocks(B, D, C):
+    """gp5_nn_conv3d_act_bwd_out0 with C data-gradient channels (C / 16 blocks of 16 as
+    grid rows, each with its own bias-partial columns) against gp5_nn_conv3d_act_bwd on
+    the full 16-channel rows: the bf16 data gradient and H to one bf16 step, the bias
+    gradient from the partials to float rounding."""
+    rng = np.random.default_rng(B + D + C + 3)
+    L = nn._lib()
+    P = B * D ** 3
+    g0 = _rb(rng.standard_normal(P))
+    Gf = np.zeros((P, 16))
+    Gf[:, 0] = g0
+    W = _rb(rng.standard_normal((27, C, 16)) * 0.1)  # [27][ci][co]: ci = C data-gradient channels
+    Ab = _rb(rng.standard_normal((P, C)))
+    g0b, gfb, wb, ab = _buf(g0, True), _buf(Gf, True), _buf(W, True), _buf(Ab, True)
+    out = []
+    for f, g, gbytes in ((L.gp5_nn_conv3d_act_bwd, gfb, 4), (L.gp5_nn_conv3d_act_bwd_out0, g0b, 2)):
+        Yf, Hb = nn.DevBuf(P * C * gbytes), nn.DevBuf(P * C * 2)
+        part, dbb = nn.DevBuf(L.gp5_nn_bias_part_floats(C) * 4), nn.DevBuf(C * 4)
+        nn._ck(f(g.ptr, wb.ptr, ab.ptr, Yf.ptr, Hb.ptr, part.ptr, B, D, 16, C))
+        seg = (nn.BiasSeg * 1)(nn.BiasSeg(part.ptr, dbb.ptr, C))
+        nn._ck(L.gp5_nn_bias_final_multi(ctypes.cast(seg, ctypes.c_void_p), 1))
+        yf = Yf.download(np.float32, (P, C)) if gbytes == 4 else nn.from_bf16(Yf.download(np.uint16, (P, C)))
+        out.append((yf, nn.from_bf16(Hb.download(np.uint16, (P, C))), dbb.download(np.float32, (C,))))
+    scale = np.max(np.abs(out[0][0]))
+    assert np.all(np.abs(out[1][0] - out[0][0]) <= 2.0 ** -7 * np.abs(out[0][0]) + 1e-6 * scale)
+    assert np.all(np.abs(out[1][1] - out[0][1]) <= 2.0 ** -7 * np.abs(out[0][1]) + 1e-6 * scale)
+    ov = out[0][0].astype(np.float64) * np.where(Ab > 0, 1.0, 0.1)
+    assert np.all(np.abs(out[1][2] - ov.sum(axis=0)) <= 1e-5 * np.abs(ov).sum(axis=0) + 1e-6)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("B,D,C", [(1, 64, 16), (2, 32, 32)])
 def test_act_bwd_part_with_bf16_gradient(B, D, C):
     """gp5_nn_act_bwd_part_bf16 gives, for a bf16 dA, what gp5_nn_act_bwd_part gives
