@@ -1399,13 +1399,22 @@ __global__ __launch_bounds__(256) void k_wgrad_o0(const uint16_t *__restrict__ X
 
 // dW[t][ci][co] = sum over the blocks of part[blk][t][ci] for co = 0, 0 for the other
 // output channels.  Workgroup t: thread (ci = tid % 16, k = tid / 16) adds blocks k,
-// k + 16, ... in order, then a fixed tree over k.
+// k + 16, ... in order (four loads in flight), then a fixed tree over k.
 __global__ __launch_bounds__(256) void k_wgrad_o0_reduce(const float *__restrict__ part, float *dW, int blocks, int Cout)
 {
     __shared__ float s[256];
     const int t = blockIdx.x, ci = threadIdx.x & 15, k = threadIdx.x >> 4;
     float v = 0.f;
-    for (int b = k; b < blocks; b += 16) v += part[((size_t)b * 27 + t) * 16 + ci];
+    int b = k;
+    for (; b + 48 < blocks; b += 64) {  // four loads in flight, added in the same order
+        const float p0 = part[((size_t)b * 27 + t) * 16 + ci], p1 = part[((size_t)(b + 16) * 27 + t) * 16 + ci],
+                    p2 = part[((size_t)(b + 32) * 27 + t) * 16 + ci], p3 = part[((size_t)(b + 48) * 27 + t) * 16 + ci];
+        v += p0;
+        v += p1;
+        v += p2;
+        v += p3;
+    }
+    for (; b < blocks; b += 16) v += part[((size_t)b * 27 + t) * 16 + ci];
     s[threadIdx.x] = v;
     __syncthreads();
     for (int w = 8; w > 0; w >>= 1) {
@@ -3570,11 +3579,12 @@ static int wgrad_out0_run(const void *X, const void *dY, int ldy, float *dW, flo
     }
     const int Rb = 256 / D;
     const size_t shm = (size_t)16 * 256 * 2 + (size_t)3 * (Rb + 2) * (D + 2 * kWo0Pad) * 2;
-    // D / 16 plane runs (measured at 64^3, batch 8: 4 runs 41.3 us, 8 53.4, 16 73.1, 32
-    // 110.8, the full weight gradient 58.2); the partials, 27 x 16 floats per block, fit
+    // D / 8 plane runs (measured at 64^3, batch 8, with the reduction's four loads in
+    // flight: 4 runs 35.1 us, 8 33.4; with one load at a time 4 runs 41.3 us, 8 53.4, 16
+    // 73.1; the full weight gradient 58.2); the partials, 27 x 16 floats per block, fit
     // the plan's workspace of 27 x 16 x 16 per block
     const char *rn = getenv("GP5_WO0_RUNS");  // measurement
-    const int runs = std::max(1, std::min(D / 2, rn && atoi(rn) > 0 ? atoi(rn) : D / 16));
+    const int runs = std::max(1, std::min(D / 2, rn && atoi(rn) > 0 ? atoi(rn) : D / 8));
     const long long blocks = (long long)B * (D / Rb) * runs;
     if (blocks * 27 * 16 > zblocks * 27 * 16 * 16) return set_error(GP5_ERR_ARG, "wgrad out0: workspace");
     hipLaunchKernelGGL(k_wgrad_o0, dim3((unsigned)blocks), dim3(256), std::max<size_t>(shm, 8192), nn_stream(),
