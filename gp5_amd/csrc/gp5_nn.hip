@@ -2693,9 +2693,9 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, co
     __shared__ double s[256], sg[256];
     double a = 0.0, ag = 0.0;
     const float inv = 2.0f / (float)P;
-    for (long long p = (long long)blockIdx.x * 256 + threadIdx.x; p < P; p += (long long)gridDim.x * 256) {
-        const float y = base[p] + Yf[(size_t)p * ldy];
-        const float e = y - target[p];
+    const long long st = (long long)gridDim.x * 256;
+    // position p's error e: the sums, then the gradient row (float and/or bf16)
+    auto put = [&](long long p, float e) {
         a += (double)e * e;
         ag += (double)(inv * e);
         if (dYf && C % 4 == 0) {  // the row as 16-byte stores (rows are 16-byte aligned)
@@ -2713,7 +2713,21 @@ __global__ __launch_bounds__(256) void k_mse(const float *Yf, int ldy, int C, co
             hrow[0] = make_uint4((uint32_t)f2bf(inv * e), 0u, 0u, 0u);
             for (int c = 1; c < C / 8; ++c) hrow[c] = make_uint4(0u, 0u, 0u, 0u);
         }
+    };
+    long long p0 = (long long)blockIdx.x * 256 + threadIdx.x;
+    // four positions' loads in flight, then their sums in position order
+    for (; p0 + 3 * st < P; p0 += 4 * st) {
+        float y[4], t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long p = p0 + k * st;
+            y[k] = base[p] + Yf[(size_t)p * ldy];
+            t[k] = target[p];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) put(p0 + k * st, y[k] - t[k]);
     }
+    for (long long p = p0; p < P; p += st) put(p, base[p] + Yf[(size_t)p * ldy] - target[p]);
     s[threadIdx.x] = a;
     sg[threadIdx.x] = ag;
     __syncthreads();
