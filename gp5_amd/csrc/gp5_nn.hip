@@ -2814,10 +2814,8 @@ __global__ __launch_bounds__(256) void k_adam_multi(AdamMulti a)
         float pi = p[i];
         pi -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
         p[i] = pi;
-        if (wf || wb) {
-            const int co = (int)(i % Cout);
-            const int ci = (int)((i / Cout) % Cin);
-            const int tap = (int)(i / ((long long)Cin * Cout));
+        if (wf || wb) {  // a weight tensor: 27 Cin Cout < 2^31 elements, 32-bit index arithmetic
+            const int ii = (int)i, co = ii % Cout, ci = (ii / Cout) % Cin, tap = ii / (Cin * Cout);
             const uint16_t h = f2bf(pi);
             if (wb) wb[i] = h;
             if (wf) wf[((size_t)tap * Cout + co) * Cin + ci] = h;
@@ -3956,7 +3954,7 @@ int gp5_nn_adam_multi(const gp5_nn_adam_seg *segs, int nseg, float lr, float b1,
             snprintf(buf, sizeof buf, "adam_multi: tensor %d is incomplete", k);
             return set_error(GP5_ERR_ARG, buf);
         }
-        if ((sg.wf || sg.wb) && (sg.cin < 1 || sg.cout < 1 || 27LL * sg.cin * sg.cout != sg.n)) {
+        if ((sg.wf || sg.wb) && (sg.cin < 1 || sg.cout < 1 || 27LL * sg.cin * sg.cout != sg.n || sg.n > 0x7fffffffLL)) {
             snprintf(buf, sizeof buf, "adam_multi: tensor %d has %lld values, not 27 x %d x %d", k, sg.n, sg.cin, sg.cout);
             return set_error(GP5_ERR_ARG, buf);
         }
