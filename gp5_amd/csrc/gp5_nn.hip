@@ -2555,6 +2555,34 @@ __global__ void k_pool2(const T *X, T *Y, int B, int D, int C, float scale)
     }
 }
 
+// k_pool2 for float with C a multiple of 4 and B D^3 C < 2^31: four channels per thread
+// and 32-bit index arithmetic, the same sums in the same order (children k = 0..7)
+__global__ void k_pool2_f32x4(const float *__restrict__ X, float *__restrict__ Y, int B, int D, int C, float scale)
+{
+    const int Dh = D / 2, C4 = C / 4;
+    const int n = B * Dh * Dh * Dh * C4;
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += (int)(gridDim.x * blockDim.x)) {
+        const int c4 = i % C4, p = i / C4;
+        const int x = p % Dh, y = (p / Dh) % Dh, z = (p / (Dh * Dh)) % Dh, b = p / (Dh * Dh * Dh);
+        float4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = ((b * D + 2 * z + (k >> 2)) * D + 2 * y + ((k >> 1) & 1)) * D + 2 * x + (k & 1);
+            v[k] = reinterpret_cast<const float4 *>(X)[(size_t)q * C4 + c4];
+        }
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            s.x += v[k].x;
+            s.y += v[k].y;
+            s.z += v[k].z;
+            s.w += v[k].w;
+        }
+        const float f = 0.125f * scale;
+        reinterpret_cast<float4 *>(Y)[i] = make_float4(s.x * f, s.y * f, s.z * f, s.w * f);
+    }
+}
+
 // Y (side 2D) = scale * X at the parent
 template <typename T>
 __global__ void k_up2(const T *X, T *Y, int B, int D, int C, float scale)
@@ -3680,7 +3708,10 @@ int gp5_nn_pool2_f32(const float *X, float *Y, int B, int D, int C, float scale)
 {
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
     const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * C;
-    hipLaunchKernelGGL(k_pool2<float>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
+    if (C % 4 == 0 && (long long)B * D * D * D * C < 0x7fffffffLL)
+        hipLaunchKernelGGL(k_pool2_f32x4, dim3(grid_for(n / 4)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
+    else
+        hipLaunchKernelGGL(k_pool2<float>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
     NNCHK(hipGetLastError(), "pool2");
     return GP5_OK;
 }

@@ -808,6 +808,24 @@ def test_output_gradient_as_channel0(B, D):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B,D,C", [(2, 16, 128), (1, 8, 6), (3, 6, 20)])
+def test_pool2_f32_adds_the_children_in_order(B, D, C):
+    """gp5_nn_pool2_f32 (four channels per thread when C is a multiple of 4, one
+    otherwise) gives bit for bit the float32 sum of the 8 children in the order
+    k = 0..7 (z, y, x bits) times 0.125 * scale."""
+    rng = np.random.default_rng(B * 100 + D + C)
+    L = nn._lib()
+    X = rng.standard_normal((B, D, D, D, C)).astype(np.float32)
+    xb, yb = _buf(X), nn.DevBuf(B * D ** 3 * C // 2)
+    nn._ck(L.gp5_nn_pool2_f32(xb.ptr, yb.ptr, B, D, C, 8.0))
+    got = yb.download(np.float32, (B, D // 2, D // 2, D // 2, C))
+    s = np.zeros((B, D // 2, D // 2, D // 2, C), np.float32)
+    for k in range(8):
+        s = s + X[:, (k >> 2)::2, ((k >> 1) & 1)::2, (k & 1)::2, :]
+    assert np.array_equal(got, s * np.float32(0.125 * 8.0))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("B,D,C", [(1, 32, 32), (2, 16, 64)])
 def test_channel0_data_gradient_over_channel_blocks(B, D, C):
     """gp5_nn_conv3d_act_bwd_out0 with C data-gradient channels (C / 16 blocks of 16 as
