@@ -2485,19 +2485,21 @@ __global__ __launch_bounds__(256) void k_bias_final(const float *part, float *db
 
 // bf16 pooling and upsampling, 4 channels (8 bytes) per thread; same arithmetic
 // as the scalar templates below (the 8 children summed in order, then scaled)
+// (I: the index type; 32 bits when B D^3 C < 2^31)
+template <typename I>
 __global__ void k_pool2_bf16x4(const uint16_t *__restrict__ X, uint16_t *__restrict__ Y, int B, int D, int C,
                                float scale)
 {
     const int Dh = D / 2, C4 = C / 4;
-    const long long n = (long long)B * Dh * Dh * Dh * C4;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const I n = (I)B * Dh * Dh * Dh * C4;
+    for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (I)gridDim.x * blockDim.x) {
         const int c4 = (int)(i % C4);
-        const long long p = i / C4;
-        const int x = (int)(p % Dh), y = (int)((p / Dh) % Dh), z = (int)((p / ((long long)Dh * Dh)) % Dh);
-        const long long b = p / ((long long)Dh * Dh * Dh);
+        const I p = i / C4;
+        const int x = (int)(p % Dh), y = (int)((p / Dh) % Dh), z = (int)((p / ((I)Dh * Dh)) % Dh);
+        const I b = p / ((I)Dh * Dh * Dh);
         float s[4] = {0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < 8; ++k) {
-            const long long q = ((b * D + 2 * z + (k >> 2)) * D + 2 * y + ((k >> 1) & 1)) * D + 2 * x + (k & 1);
+            const I q = ((b * D + 2 * z + (k >> 2)) * D + 2 * y + ((k >> 1) & 1)) * D + 2 * x + (k & 1);
             const bf16x4 v = ld4(X + (size_t)q * C + 4 * c4);
 #pragma unroll
             for (int e = 0; e < 4; ++e) s[e] += bf2f((uint16_t)v[e]);
@@ -3688,8 +3690,12 @@ int gp5_nn_pool2_bf16(const void *X, void *Y, int B, int D, int C, float scale)
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
     if (C % 4) return set_error(GP5_ERR_ARG, "pool2: C must be a multiple of 4");
     const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * (C / 4);
-    hipLaunchKernelGGL(k_pool2_bf16x4, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)X, (uint16_t *)Y, B, D,
-                       C, scale);
+    if ((long long)B * D * D * D * C < 0x7fffffffLL)
+        hipLaunchKernelGGL(k_pool2_bf16x4<int>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)X,
+                           (uint16_t *)Y, B, D, C, scale);
+    else
+        hipLaunchKernelGGL(k_pool2_bf16x4<long long>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), (const uint16_t *)X,
+                           (uint16_t *)Y, B, D, C, scale);
     NNCHK(hipGetLastError(), "pool2");
     return GP5_OK;
 }
