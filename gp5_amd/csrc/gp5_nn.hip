@@ -3714,7 +3714,8 @@ int gp5_nn_pool2_f32(const float *X, float *Y, int B, int D, int C, float scale)
 {
     if (D % 2) return set_error(GP5_ERR_ARG, "pool of an odd side");
     const long long n = (long long)B * (D / 2) * (D / 2) * (D / 2) * C;
-    if (C % 4 == 0 && (long long)B * D * D * D * C < 0x7fffffffLL)
+    const bool a16 = ((uintptr_t)X | (uintptr_t)Y) % 16 == 0;  // the four-channel form's 16-byte accesses
+    if (C % 4 == 0 && a16 && (long long)B * D * D * D * C < 0x7fffffffLL)
         hipLaunchKernelGGL(k_pool2_f32x4, dim3(grid_for(n / 4)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
     else
         hipLaunchKernelGGL(k_pool2<float>, dim3(grid_for(n)), dim3(256), 0, nn_stream(), X, Y, B, D, C, scale);
